@@ -1,0 +1,134 @@
+/*
+ * opentbl.h -- C ABI of libopentbl_amd.so, the MI355X RK3 substep of the
+ * OpenTBL zero-pressure-gradient boundary-layer DNS.
+ *
+ * One otbl_ctx holds one box: sizes, FP64 coefficients, every field resident
+ * in device memory (HBM), the HIP stream the work is enqueued on and the
+ * captured substep graphs.  All entry points return OTBL_OK (0) or a non-zero
+ * status; otbl_last_error() gives the message.  Host arrays are plain
+ * doubles; complex fields use the reference's planes layout: the complex*16
+ * array (0:nz2, R, 1:nx) seen as doubles, i.e. real index kr = 0..nz1-1
+ * fastest (2k = Re of mode k, 2k+1 = Im), then row j = 1..R, then plane
+ * i = 1..nx, with R = ny+1 for u, w and R = ny for v, p.
+ *
+ * The reference's plane<->pencil transposes (change.f90:224-275, chp2x and
+ * chx2p) have no counterpart: every x-line kernel reads and writes the
+ * planes layout directly.
+ */
+#ifndef OPENTBL_H
+#define OPENTBL_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct otbl_ctx otbl_ctx;
+
+enum { OTBL_OK = 0, OTBL_EINVAL = 1, OTBL_EHIP = 2, OTBL_ENOMEM = 3, OTBL_ESTATE = 4 };
+enum { OTBL_FP64 = 0, OTBL_MIXED = 1 };
+/* dt every step: recomputed from the CFL limits at substep 1 of every step
+ * (bl_1.F90:91, rhsp.F90:214-262).  Frozen: computed once, then kept. */
+enum { OTBL_DT_EVERY_STEP = 0, OTBL_DT_FROZEN = 1 };
+/* field ids: state the reference keeps between substeps (main_val) */
+enum { OTBL_U = 0, OTBL_V, OTBL_W, OTBL_P, OTBL_RHSUPA, OTBL_RHSVPA, OTBL_RHSWPA,
+       OTBL_NFIELDS };
+
+typedef struct otbl_params {
+  int nx, ny, nz;            /* grid: mod.F90:22 (parameter nx, ny, nz) */
+  int xout;                  /* recycling plane: mod.F90:23 (xout) */
+  double re, ax, az, cfl;    /* hrem_1.dat values read at bl_1.F90:239-244 */
+  int precision;             /* OTBL_FP64 or OTBL_MIXED */
+  int dt_mode;               /* OTBL_DT_EVERY_STEP or OTBL_DT_FROZEN */
+} otbl_params;
+
+/* Box setup.  Replaces iniciap_1 (bl_1.F90:179-367: sizes, wavenumbers),
+ * alloavar/alloabuff (alloa.F90:206-327: field and buffer allocation, here
+ * in device memory) and coef (coeft.f90:10-1234: every compact-scheme
+ * coefficient in FP64, factorised on the GPU).  y has ny+2 entries,
+ * y(0:ny+1) as getstartzy reads it (getstartzy-bg.F90:180-295). */
+int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out);
+int otbl_destroy(otbl_ctx *ctx);
+const char *otbl_last_error(void);
+
+/* State transfer (the restart read of getstartzy-bg.F90:180-295 and the
+ * write of escru.F90:9-251 without the file format). */
+int otbl_set_field(otbl_ctx *ctx, int id, const double *host);
+int otbl_get_field(otbl_ctx *ctx, int id, double *host);
+size_t otbl_field_len(const otbl_ctx *ctx, int id); /* doubles */
+/* Inflow profiles u0(1:ny+1), v0(1:ny) and the running mean um(1:ny+1)
+ * (genmod, mod.F90:86-93), then inlet_retheta (bl_1.F90:372-396) and
+ * magicnumber (boun.f90:102-143). */
+int otbl_set_inflow(otbl_ctx *ctx, const double *u0, const double *v0, const double *um);
+int otbl_get_um(otbl_ctx *ctx, double *um);
+/* genflu diagnostics: rthin, din, rthout, dout, gamma(1), gamma(2) */
+int otbl_get_genflu(otbl_ctx *ctx, double *out6);
+/* energies (rhsp.F90:493-509, stotf.F90:395-440) of the last substep 1 */
+int otbl_get_ener(otbl_ctx *ctx, double *ener15);
+int otbl_set_time(otbl_ctx *ctx, double tiempo, double timeinit, double dt);
+int otbl_get_time(otbl_ctx *ctx, double *tiempo, double *dt);
+/* Coefficient arrays after setup, by their reference names (inxu, dcbx,
+ * vyui, ayp, kmod, ...), in Fortran order; rows/cols give the shape. */
+int otbl_get_coef(otbl_ctx *ctx, const char *name, double *out, int *rows, int *cols);
+
+/* ---- the RK3 substep (bl_1.F90:94-125) ---- */
+/* rhsp (rhsp.F90:22-489): right-hand side, CFL dt (substep 1), genflu,
+ * pressure-gradient predictor, boundary conditions, explicit RK3
+ * combination and the implicit y-viscous solve.  m = 1..3. */
+int otbl_rhsp(otbl_ctx *ctx, int m);
+/* outflow_correction (boun.f90:63-93), FP64 net-flux sum on the GPU */
+int otbl_outflow_correction(otbl_ctx *ctx);
+/* pois (pois.f90:11-189) with an explicit varstep, and with the
+ * varstep = 0.5/dt/rkdv(m) of bl_1.F90:117 evaluated on the GPU */
+int otbl_pois(otbl_ctx *ctx, double varstep);
+int otbl_pois_m(otbl_ctx *ctx, int m);
+/* one substep: rhsp, outflow_correction, pois (bl_1.F90:98-119) */
+int otbl_substep(otbl_ctx *ctx, int m);
+/* nsteps whole steps: 3 substeps each, then tiempo += dt (bl_1.F90:85-128) */
+int otbl_step(otbl_ctx *ctx, int nsteps);
+/* 1: each substep runs as one captured hipGraph (captured on first use);
+ * 0: kernels are launched one by one. */
+int otbl_use_graphs(otbl_ctx *ctx, int on);
+int otbl_sync(otbl_ctx *ctx);
+
+/* ---- operators on device buffers of this box's size (lip.f90, ffts.f90) ----
+ * in/out are device pointers to planes-layout fields (R rows per plane). */
+/* interpxx (lip.f90:248-300): which_u=1 -> u coefficients (inxu, cofiux,
+ * loguv=1), 0 -> v/w coefficients (inxv, cofivx, loguv=0) */
+int otbl_interpxx(otbl_ctx *ctx, const double *gen, double *geni, int R, int which_u);
+/* difvisxx (lip.f90:52-142): convective d/dx of prod plus rex d2/dx2 of vel */
+int otbl_difvisxx(otbl_ctx *ctx, const double *prod, const double *vel, double *out, int R,
+                  int which_u);
+/* interpyy (lip.f90:359-440), spectral rows: loguv 0 = u/w rows (ny+1) to v
+ * rows, 1 = v rows to u rows; output has ny+1 rows per plane (row ny+1 is
+ * not written) */
+int otbl_interpyy(otbl_ctx *ctx, const double *gen, double *geni, int loguv, int nplanes);
+/* differzy (lip.f90:152-232): which 0 = v rows -> u rows (dcyu), 1 = u rows
+ * -> v rows (dcyv); in and out have ny+1 rows per plane */
+int otbl_differzy(otbl_ctx *ctx, const double *geni, double *res, int which, int nplanes);
+/* vistyy (lip.f90:447-513) and implzy (lip.f90:528-597) */
+int otbl_vistyy(otbl_ctx *ctx, const double *u, double *res, int R, int nplanes);
+int otbl_implzy(otbl_ctx *ctx, double *u, const double *res, int R, double rkk, int nplanes);
+/* fourxz (ffts.f90:27-67) on nrows spectral rows (nz1 doubles each) and
+ * physical rows (nz doubles each): iopt >= 0 c2r, iopt < 0 r2c */
+int otbl_fourxz(otbl_ctx *ctx, double *fou, double *phys, int iopt, int nrows);
+/* cosftx (ffts.f90:147-181) along x over planes 2..nx of a v-type field,
+ * rows 1..ny: iopt < 0 DCT-II x 1/(nx-1), iopt > 0 DCT-III x 0.5 */
+int otbl_cosftx(otbl_ctx *ctx, const double *in, double *out, int iopt);
+
+/* ---- device memory helpers (plain pointers, no framework types) ---- */
+int otbl_dev_alloc(size_t bytes, void **ptr);
+int otbl_dev_free(void *ptr);
+int otbl_h2d(void *dst, const void *src, size_t bytes);
+int otbl_d2h(void *dst, const void *src, size_t bytes);
+
+/* ---- measurement ----
+ * With kprof on, every kernel launch is bracketed by HIP events on the
+ * context's stream; the report lists name, launches, total ms.  Kernels run
+ * one by one (no graphs) while kprof is on. */
+int otbl_kprof(otbl_ctx *ctx, int on);
+int otbl_kprof_report(otbl_ctx *ctx, char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

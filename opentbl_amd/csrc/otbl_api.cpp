@@ -1,0 +1,643 @@
+// C ABI of libopentbl_amd: box setup, state transfer, the RK3 substep (eager
+// or as captured hipGraphs), operator entry points and kernel timing.
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "otbl_internal.h"
+
+namespace otbl {
+
+static thread_local std::string g_err;
+
+void fail_hip(hipError_t e, const char *what) {
+  throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
+}
+
+void kbegin(Box &b, const char *name) {
+  if (!b.kprof) return;
+  hipEvent_t e0, e1;
+  OTBL_HIP(hipEventCreate(&e0));
+  OTBL_HIP(hipEventCreate(&e1));
+  OTBL_HIP(hipEventRecord(e0, b.s));
+  b.pending.push_back({name, {e0, e1}});
+}
+
+void kend(Box &b) {
+  OTBL_HIP(hipGetLastError());
+  if (!b.kprof) return;
+  OTBL_HIP(hipEventRecord(b.pending.back().second.second, b.s));
+}
+
+static void drain_kprof(Box &b) {
+  if (b.pending.empty()) return;
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  for (auto &p : b.pending) {
+    float ms = 0.f;
+    OTBL_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    auto &st = b.kstat[p.first];
+    st.n += 1;
+    st.ms += ms;
+    OTBL_HIP(hipEventDestroy(p.second.first));
+    OTBL_HIP(hipEventDestroy(p.second.second));
+  }
+  b.pending.clear();
+}
+
+static int small_factors(int n, int *fac) {
+  int nf = 0;
+  const int pr[6] = {8, 4, 2, 3, 5, 7};
+  for (int q = 0; q < 6; ++q)
+    while (n % pr[q] == 0) {
+      fac[nf++] = pr[q];
+      n /= pr[q];
+    }
+  return n == 1 ? nf : -1;
+}
+
+template <class T> static void dalloc(T *&p, size_t n) {
+  OTBL_HIP(hipMalloc((void **)&p, n * sizeof(T) + 64));
+  OTBL_HIP(hipMemset(p, 0, n * sizeof(T) + 64));
+}
+
+static void put_ds(Box &b, size_t off, const void *src, size_t bytes) {
+  OTBL_HIP(hipMemcpy((char *)b.ds + off, src, bytes, hipMemcpyHostToDevice));
+}
+static void get_ds(Box &b, size_t off, void *dst, size_t bytes) {
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  OTBL_HIP(hipMemcpy(dst, (char *)b.ds + off, bytes, hipMemcpyDeviceToHost));
+}
+
+// Coefficient arrays, in upload order, with their device slot.  cols: the
+// Fortran extent ('X' nx, 'Y' ny, 'U' ny+1, 'P' ny-1, 'M' nx-1, 'K' nz2+1,
+// 'G' ny+2); first: the first Fortran index exported.
+struct CoefSlot {
+  const char *name;
+  std::vector<double> HostCoef::*vec;
+  const double *DevCoef::*dev;
+  int rows;
+  int first;
+  char cols;
+};
+
+static const CoefSlot kSlots[] = {
+    {"inxu", &HostCoef::inxu, &DevCoef::inxu, 3, 1, 'X'},       {"inxv", &HostCoef::inxv, &DevCoef::inxv, 3, 1, 'X'},
+    {"inyu", &HostCoef::inyu, &DevCoef::inyu, 3, 1, 'Y'},       {"inyv", &HostCoef::inyv, &DevCoef::inyv, 3, 1, 'Y'},
+    {"dcxu", &HostCoef::dcxu, &DevCoef::dcxu, 3, 1, 'X'},       {"dcxv", &HostCoef::dcxv, &DevCoef::dcxv, 3, 1, 'X'},
+    {"dcyu", &HostCoef::dcyu, &DevCoef::dcyu, 3, 1, 'U'},       {"dcyv", &HostCoef::dcyv, &DevCoef::dcyv, 3, 1, 'Y'},
+    {"vixu", &HostCoef::vixu, &DevCoef::vixu, 3, 1, 'X'},       {"vixv", &HostCoef::vixv, &DevCoef::vixv, 3, 1, 'X'},
+    {"viyu", &HostCoef::viyu, &DevCoef::viyu, 3, 1, 'U'},       {"viyv", &HostCoef::viyv, &DevCoef::viyv, 3, 1, 'Y'},
+    {"vyui", &HostCoef::vyui, &DevCoef::vyui, 3, 1, 'U'},       {"vyvi", &HostCoef::vyvi, &DevCoef::vyvi, 3, 1, 'Y'},
+    {"cofiux", &HostCoef::cofiux, &DevCoef::cofiux, 2, 1, 'X'}, {"cofivx", &HostCoef::cofivx, &DevCoef::cofivx, 2, 1, 'X'},
+    {"cofiuy", &HostCoef::cofiuy, &DevCoef::cofiuy, 2, 1, 'Y'}, {"cofivy", &HostCoef::cofivy, &DevCoef::cofivy, 2, 1, 'Y'},
+    {"cofcxu", &HostCoef::cofcxu, &DevCoef::cofcxu, 2, 1, 'X'}, {"cofcxv", &HostCoef::cofcxv, &DevCoef::cofcxv, 2, 1, 'X'},
+    {"cofcyu", &HostCoef::cofcyu, &DevCoef::cofcyu, 2, 1, 'Y'}, {"cofcyv", &HostCoef::cofcyv, &DevCoef::cofcyv, 2, 1, 'Y'},
+    {"cofvxu", &HostCoef::cofvxu, &DevCoef::cofvxu, 3, 1, 'X'}, {"cofvxv", &HostCoef::cofvxv, &DevCoef::cofvxv, 3, 1, 'X'},
+    {"cofvyu", &HostCoef::cofvyu, &DevCoef::cofvyu, 3, 1, 'Y'}, {"cofvyv", &HostCoef::cofvyv, &DevCoef::cofvyv, 3, 1, 'Y'},
+    {"ayp", &HostCoef::ayp, &DevCoef::ayp, 3, 1, 'P'},
+    {"y", &HostCoef::y, &DevCoef::y, 1, 0, 'G'},             {"dy", &HostCoef::dy, &DevCoef::dy, 1, 0, 'U'},
+    {"idy", &HostCoef::idy, &DevCoef::idy, 1, 0, 'U'},       {"idyy", &HostCoef::idyy, &DevCoef::idyy, 1, 1, 'Y'},
+    {"dymin", &HostCoef::dymin, &DevCoef::dymin, 1, 1, 'Y'}, {"hy", &HostCoef::hy, &DevCoef::hy, 1, 0, 'U'},
+    {"kaz", &HostCoef::kaz, &DevCoef::kaz, 1, 0, 'K'},       {"kaz2", &HostCoef::kaz2, &DevCoef::kaz2, 1, 0, 'K'},
+    {"kvis", &HostCoef::kvis, &DevCoef::kvis, 1, 0, 'K'},    {"kmod", &HostCoef::kmod, &DevCoef::kmod, 1, 1, 'M'},
+    {"vmagic", &HostCoef::vmagic, &DevCoef::vmagic, 1, 1, 'X'},
+};
+
+static int slot_cols(const Box &b, char c) {
+  switch (c) {
+  case 'X': return b.nx;
+  case 'Y': return b.ny;
+  case 'U': return b.ny + 1;
+  case 'P': return b.ny - 1;
+  case 'M': return b.nx - 1;
+  case 'K': return b.nz2 + 1;
+  default: return b.ny + 2;
+  }
+}
+
+static size_t fixed_len(const char *nm) {
+  if (!strcmp(nm, "dcbx")) return 20;
+  if (!strcmp(nm, "ccon")) return 8;
+  return 16;
+}
+
+static void upload_coef(Box &b) {
+  HostCoef &hc = b.hc;
+  size_t total = 0;
+  for (const auto &sl : kSlots) total += (hc.*(sl.vec)).size() + 8;
+  total += 7 * 24;
+  if (!b.coefbuf) dalloc(b.coefbuf, total);
+  std::vector<double> host(total, 0.0);
+  size_t off = 0;
+  for (const auto &sl : kSlots) {
+    const auto &v = hc.*(sl.vec);
+    std::copy(v.begin(), v.end(), host.begin() + off);
+    b.dc.*(sl.dev) = b.coefbuf + off;
+    off += v.size() + 8;
+  }
+  struct { double *src; size_t n; const double *DevCoef::*dev; } fx[7] = {
+      {hc.inbx, 16, &DevCoef::inbx}, {hc.inby, 16, &DevCoef::inby}, {hc.dcbx, 20, &DevCoef::dcbx},
+      {hc.dcby, 16, &DevCoef::dcby}, {hc.ccon, 8, &DevCoef::ccon},  {hc.cofvbx, 16, &DevCoef::cofvbx},
+      {hc.cofvby, 16, &DevCoef::cofvby}};
+  for (auto &f : fx) {
+    std::copy(f.src, f.src + f.n, host.begin() + off);
+    b.dc.*(f.dev) = b.coefbuf + off;
+    off += 24;
+  }
+  OTBL_HIP(hipMemcpy(b.coefbuf, host.data(), total * sizeof(double), hipMemcpyHostToDevice));
+}
+
+static void build_dct(Box &b) {
+  const int N = b.nx - 1;
+  std::vector<double> c2((size_t)N * N), c3((size_t)N * N);
+  const double pi = b.hc.pi;
+  // cos(pi t / (2N)) with t reduced mod 4N: the FFTW REDFT10 / REDFT01
+  // definitions (ffts.f90:136-137)
+  std::vector<double> tab(4 * (size_t)N);
+  for (int t = 0; t < 4 * N; ++t) tab[t] = std::cos(pi * t / (2.0 * N));
+  for (int q = 0; q < N; ++q)
+    for (int n = 0; n < N; ++n) {
+      c2[(size_t)q * N + n] = 2.0 / N * tab[((long)q * (2 * n + 1)) % (4 * N)];
+      c3[(size_t)q * N + n] = 0.5 * (n == 0 ? 1.0 : 2.0 * tab[((long)n * (2 * q + 1)) % (4 * N)]);
+    }
+  dalloc(b.dct2, (size_t)N * N);
+  dalloc(b.dct3, (size_t)N * N);
+  OTBL_HIP(hipMemcpy(b.dct2, c2.data(), c2.size() * sizeof(double), hipMemcpyHostToDevice));
+  OTBL_HIP(hipMemcpy(b.dct3, c3.data(), c3.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+static void free_box(Box *b) {
+  if (!b) return;
+  for (int q = 0; q < 4; ++q)
+    if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
+  double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
+                    b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
+                    b->implv, b->um, b->u0, b->v0, b->gwork, b->tw};
+  for (double *p : bufs)
+    if (p) (void)hipFree(p);
+  for (int f = 0; f < OTBL_NFIELDS; ++f)
+    if (b->f[f]) (void)hipFree(b->f[f]);
+  if (b->ds) (void)hipFree(b->ds);
+  if (b->vmbits) (void)hipFree(b->vmbits);
+  for (auto &p : b->pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  if (b->s) (void)hipStreamDestroy(b->s);
+  delete b;
+}
+
+void k_setup_attrs(Box &b);
+
+static void enqueue_substep(Box &b, int m) {
+  k_substep_rhsp(b, m);
+  k_outflow(b);
+  k_pois(b, m, 0.0, true);
+}
+
+static void run_substep(Box &b, int m) {
+  if (!b.use_graphs || b.kprof) {
+    enqueue_substep(b, m);
+    return;
+  }
+  if (!b.gexec[m]) {
+    hipGraph_t g;
+    OTBL_HIP(hipStreamBeginCapture(b.s, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_substep(b, m);
+    } catch (...) {
+      hipGraph_t dummy;
+      (void)hipStreamEndCapture(b.s, &dummy);
+      throw;
+    }
+    OTBL_HIP(hipStreamEndCapture(b.s, &g));
+    OTBL_HIP(hipGraphInstantiate(&b.gexec[m], g, nullptr, nullptr, 0));
+    OTBL_HIP(hipGraphDestroy(g));
+  }
+  OTBL_HIP(hipGraphLaunch(b.gexec[m], b.s));
+}
+
+} // namespace otbl
+
+using namespace otbl;
+
+struct otbl_ctx {
+  Box *b;
+};
+
+#define API_BEGIN try {
+#define API_END                                                                \
+  return OTBL_OK;                                                              \
+  }                                                                            \
+  catch (const std::invalid_argument &e) {                                     \
+    g_err = e.what();                                                          \
+    return OTBL_EINVAL;                                                        \
+  }                                                                            \
+  catch (const std::bad_alloc &e) {                                            \
+    g_err = "out of memory";                                                   \
+    return OTBL_ENOMEM;                                                        \
+  }                                                                            \
+  catch (const std::exception &e) {                                            \
+    g_err = e.what();                                                          \
+    return OTBL_EHIP;                                                          \
+  }
+
+static Box &box_of(otbl_ctx *ctx) {
+  if (!ctx || !ctx->b) throw std::invalid_argument("null otbl_ctx");
+  return *ctx->b;
+}
+
+static void need(bool ok, const char *msg) {
+  if (!ok) throw std::invalid_argument(msg);
+}
+
+extern "C" {
+
+const char *otbl_last_error(void) { return g_err.c_str(); }
+
+int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
+  Box *b = nullptr;
+  API_BEGIN
+  need(prm && y && out, "otbl_create: null argument");
+  *out = nullptr;
+  need(prm->nx >= 8, "nx must be >= 8");
+  need(prm->ny >= 24, "ny must be >= 24 (the free-stream average spans rows ny-19..ny-4)");
+  need(prm->nz >= 6 && prm->nz % 2 == 0, "nz must be even and >= 6");
+  need(prm->xout >= 2 && prm->xout <= prm->nx - 1, "xout must lie in 2..nx-1 (bl_1.F90:287-297)");
+  need(prm->re > 0 && prm->ax > 0 && prm->az > 0 && prm->cfl > 0, "re, ax, az, cfl must be > 0");
+  need(prm->precision == OTBL_FP64, "only OTBL_FP64 is available in this build");
+  need(prm->dt_mode == OTBL_DT_EVERY_STEP || prm->dt_mode == OTBL_DT_FROZEN, "bad dt_mode");
+  b = new Box();
+  b->nx = prm->nx; b->ny = prm->ny; b->nz = prm->nz;
+  b->nz1 = 2 * (prm->nz / 3);
+  b->nz2 = b->nz1 / 2 - 1;
+  b->mz = b->nz2 + 1;
+  b->xout = prm->xout;
+  b->precision = prm->precision;
+  b->dt_mode = prm->dt_mode;
+  b->re = prm->re; b->ax = prm->ax; b->az = prm->az; b->cfl = prm->cfl;
+  b->nfac = small_factors(b->nz, b->fac);
+  need(b->nfac > 0, "nz must factor into 2, 3, 5 and 7");
+  need((size_t)4 * b->nz * 16 <= 160 * 1024, "nz too large for the FP64 LDS transform");
+  b->pu = (size_t)b->nz1 * (b->ny + 1);
+  b->pv = (size_t)b->nz1 * b->ny;
+  b->nu = b->pu * b->nx;
+  b->nv = b->pv * b->nx;
+  OTBL_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+  build_coef(b->hc, b->nx, b->ny, b->nz, b->re, b->ax, b->az, b->cfl, y);
+  upload_coef(*b);
+  for (int f = 0; f < OTBL_NFIELDS; ++f)
+    dalloc(b->f[f], (f == OTBL_V || f == OTBL_P || f == OTBL_RHSVPA) ? b->nv : b->nu);
+  dalloc(b->resu, b->nu); dalloc(b->resv, b->nv); dalloc(b->resw, b->nu);
+  dalloc(b->rhsu, b->nu); dalloc(b->rhsv, b->nv); dalloc(b->rhsw, b->nu);
+  dalloc(b->wki1, b->nu); dalloc(b->wki2, b->nv); dalloc(b->wki3, b->nu);
+  dalloc(b->ya, b->nu); dalloc(b->yb, b->nu); dalloc(b->yc, b->nu);
+  dalloc(b->rest, b->nv); dalloc(b->rest2, b->nv);
+  dalloc(b->implu, 3 * (size_t)(b->ny + 2) + 8);
+  dalloc(b->implv, 3 * (size_t)(b->ny + 2) + 8);
+  dalloc(b->um, b->ny + 2); dalloc(b->u0, b->ny + 2); dalloc(b->v0, b->ny + 2);
+  dalloc(b->gwork, (size_t)b->nz1 * (b->ny + 1) + 16 * (size_t)(b->ny + 2));
+  OTBL_HIP(hipMalloc((void **)&b->ds, sizeof(DevScalars)));
+  OTBL_HIP(hipMemset(b->ds, 0, sizeof(DevScalars)));
+  dalloc(b->vmbits, b->ny + 1);
+  {
+    std::vector<double> tw(2 * (size_t)b->nz);
+    for (int t = 0; t < b->nz; ++t) {
+      tw[2 * t] = std::cos(2.0 * b->hc.pi * t / b->nz);
+      tw[2 * t + 1] = std::sin(2.0 * b->hc.pi * t / b->nz);
+    }
+    dalloc(b->tw, tw.size());
+    OTBL_HIP(hipMemcpy(b->tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  build_dct(*b);
+  k_setup_attrs(*b);
+  k_factor_coef(*b, b->coefbuf);
+  OTBL_HIP(hipStreamSynchronize(b->s));
+  *out = new otbl_ctx{b};
+  b = nullptr;
+  return OTBL_OK;
+  }
+  catch (const std::invalid_argument &e) {
+    free_box(b);
+    g_err = e.what();
+    return OTBL_EINVAL;
+  }
+  catch (const std::exception &e) {
+    free_box(b);
+    g_err = e.what();
+    return OTBL_EHIP;
+  }
+}
+
+int otbl_destroy(otbl_ctx *ctx) {
+  if (!ctx) return OTBL_OK;
+  if (ctx->b) {
+    (void)hipStreamSynchronize(ctx->b->s);
+    free_box(ctx->b);
+  }
+  delete ctx;
+  return OTBL_OK;
+}
+
+size_t otbl_field_len(const otbl_ctx *ctx, int id) {
+  if (!ctx || !ctx->b || id < 0 || id >= OTBL_NFIELDS) return 0;
+  const Box &b = *ctx->b;
+  return (id == OTBL_V || id == OTBL_P || id == OTBL_RHSVPA) ? b.nv : b.nu;
+}
+
+int otbl_set_field(otbl_ctx *ctx, int id, const double *host) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(id >= 0 && id < OTBL_NFIELDS && host, "otbl_set_field: bad id or null pointer");
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  OTBL_HIP(hipMemcpy(b.f[id], host, otbl_field_len(ctx, id) * sizeof(double), hipMemcpyHostToDevice));
+  API_END
+}
+
+int otbl_get_field(otbl_ctx *ctx, int id, double *host) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(id >= 0 && id < OTBL_NFIELDS && host, "otbl_get_field: bad id or null pointer");
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  OTBL_HIP(hipMemcpy(host, b.f[id], otbl_field_len(ctx, id) * sizeof(double), hipMemcpyDeviceToHost));
+  API_END
+}
+
+int otbl_set_inflow(otbl_ctx *ctx, const double *u0, const double *v0, const double *um) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(u0 && v0 && um, "otbl_set_inflow: null pointer");
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  double rthin = 0, din = 0;
+  inlet_setup(b.hc, u0, &rthin, &din);
+  OTBL_HIP(hipMemcpy(b.u0, u0, (b.ny + 1) * sizeof(double), hipMemcpyHostToDevice));
+  OTBL_HIP(hipMemcpy(b.v0, v0, b.ny * sizeof(double), hipMemcpyHostToDevice));
+  OTBL_HIP(hipMemcpy(b.um, um, (b.ny + 1) * sizeof(double), hipMemcpyHostToDevice));
+  OTBL_HIP(hipMemcpy((void *)b.dc.vmagic, b.hc.vmagic.data(), b.hc.vmagic.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+  put_ds(b, offsetof(DevScalars, rthin), &rthin, sizeof(double));
+  put_ds(b, offsetof(DevScalars, din), &din, sizeof(double));
+  API_END
+}
+
+int otbl_get_um(otbl_ctx *ctx, double *um) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  OTBL_HIP(hipMemcpy(um, b.um, (b.ny + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  API_END
+}
+
+int otbl_get_genflu(otbl_ctx *ctx, double *o) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  DevScalars d;
+  get_ds(b, 0, &d, sizeof d);
+  o[0] = d.rthin; o[1] = d.din; o[2] = d.rthout; o[3] = d.dout; o[4] = d.gamma1; o[5] = d.gamma2;
+  API_END
+}
+
+int otbl_get_ener(otbl_ctx *ctx, double *e) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  DevScalars d;
+  get_ds(b, 0, &d, sizeof d);
+  for (int q = 0; q < 15; ++q) e[q] = d.ener[q + 1];
+  API_END
+}
+
+int otbl_set_time(otbl_ctx *ctx, double tiempo, double timeinit, double dt) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  int valid = dt > 0.0 ? 1 : 0;
+  put_ds(b, offsetof(DevScalars, tiempo), &tiempo, sizeof(double));
+  put_ds(b, offsetof(DevScalars, timeinit), &timeinit, sizeof(double));
+  put_ds(b, offsetof(DevScalars, dt), &dt, sizeof(double));
+  put_ds(b, offsetof(DevScalars, dt_valid), &valid, sizeof(int));
+  API_END
+}
+
+int otbl_get_time(otbl_ctx *ctx, double *tiempo, double *dt) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  DevScalars d;
+  get_ds(b, 0, &d, sizeof d);
+  if (tiempo) *tiempo = d.tiempo;
+  if (dt) *dt = d.dt;
+  API_END
+}
+
+int otbl_get_coef(otbl_ctx *ctx, const char *name, double *out, int *rows, int *cols) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(name && rows && cols, "otbl_get_coef: null argument");
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  for (const auto &sl : kSlots) {
+    if (strcmp(sl.name, name)) continue;
+    const auto &hv = b.hc.*(sl.vec);
+    std::vector<double> dv(hv.size());
+    OTBL_HIP(hipMemcpy(dv.data(), b.dc.*(sl.dev), dv.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int r = sl.rows, first = sl.first, n = slot_cols(b, sl.cols);
+    *rows = r;
+    *cols = n;
+    if (out)
+      for (int q = 0; q < r * n; ++q) out[q] = dv[(size_t)r * first + q];
+    return OTBL_OK;
+  }
+  const size_t n = fixed_len(name);
+  const double *src = nullptr;
+  int r = 4, c = 4;
+  if (!strcmp(name, "inbx")) src = b.dc.inbx;
+  else if (!strcmp(name, "inby")) src = b.dc.inby;
+  else if (!strcmp(name, "dcbx")) { src = b.dc.dcbx; r = 5; }
+  else if (!strcmp(name, "dcby")) src = b.dc.dcby;
+  else if (!strcmp(name, "ccon")) { src = b.dc.ccon; r = 2; }
+  else if (!strcmp(name, "cofvbx")) src = b.dc.cofvbx;
+  else if (!strcmp(name, "cofvby")) src = b.dc.cofvby;
+  need(src != nullptr, "otbl_get_coef: unknown coefficient name");
+  *rows = r;
+  *cols = c;
+  if (out) OTBL_HIP(hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost));
+  API_END
+}
+
+int otbl_rhsp(otbl_ctx *ctx, int m) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(m >= 1 && m <= 3, "substep m must be 1..3");
+  k_substep_rhsp(b, m);
+  API_END
+}
+
+int otbl_outflow_correction(otbl_ctx *ctx) {
+  API_BEGIN
+  k_outflow(box_of(ctx));
+  API_END
+}
+
+int otbl_pois(otbl_ctx *ctx, double varstep) {
+  API_BEGIN
+  k_pois(box_of(ctx), 0, varstep, false);
+  API_END
+}
+
+int otbl_pois_m(otbl_ctx *ctx, int m) {
+  API_BEGIN
+  need(m >= 1 && m <= 3, "substep m must be 1..3");
+  k_pois(box_of(ctx), m, 0.0, true);
+  API_END
+}
+
+int otbl_substep(otbl_ctx *ctx, int m) {
+  API_BEGIN
+  need(m >= 1 && m <= 3, "substep m must be 1..3");
+  run_substep(box_of(ctx), m);
+  API_END
+}
+
+int otbl_step(otbl_ctx *ctx, int nsteps) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(nsteps >= 0, "nsteps must be >= 0");
+  for (int s = 0; s < nsteps; ++s) {
+    for (int m = 1; m <= 3; ++m) run_substep(b, m);
+    k_end_step(b);
+  }
+  API_END
+}
+
+int otbl_use_graphs(otbl_ctx *ctx, int on) {
+  API_BEGIN
+  box_of(ctx).use_graphs = on != 0;
+  API_END
+}
+
+int otbl_sync(otbl_ctx *ctx) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  drain_kprof(b);
+  API_END
+}
+
+int otbl_interpxx(otbl_ctx *ctx, const double *gen, double *geni, int R, int which_u) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(R == b.ny || R == b.ny + 1, "R must be ny or ny+1");
+  k_interpxx(b, gen, geni, R, which_u);
+  API_END
+}
+
+int otbl_difvisxx(otbl_ctx *ctx, const double *prod, const double *vel, double *out, int R, int which_u) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(R == b.ny || R == b.ny + 1, "R must be ny or ny+1");
+  k_difvisxx(b, prod, vel, out, R, which_u, 0);
+  API_END
+}
+
+int otbl_interpyy(otbl_ctx *ctx, const double *gen, double *geni, int loguv, int nplanes) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(nplanes >= 1 && nplanes <= b.nx, "nplanes out of range");
+  k_interpyy(b, gen, geni, loguv ? 1 : 0, 1, nplanes, loguv ? b.pv : b.pu, b.pu);
+  API_END
+}
+
+int otbl_differzy(otbl_ctx *ctx, const double *geni, double *res, int which, int nplanes) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(nplanes >= 1 && nplanes <= b.nx, "nplanes out of range");
+  k_differzy(b, geni, res, which ? 1 : 0, nplanes, b.pu, b.pu);
+  API_END
+}
+
+int otbl_vistyy(otbl_ctx *ctx, const double *u, double *res, int R, int nplanes) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(R == b.ny || R == b.ny + 1, "R must be ny or ny+1");
+  need(nplanes >= 1 && nplanes <= b.nx, "nplanes out of range");
+  k_vistyy(b, u, res, R, nplanes);
+  API_END
+}
+
+int otbl_implzy(otbl_ctx *ctx, double *u, const double *res, int R, double rkk, int nplanes) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(R == b.ny || R == b.ny + 1, "R must be ny or ny+1");
+  need(nplanes >= 1 && nplanes <= b.nx, "nplanes out of range");
+  k_implzy_op(b, u, res, R, rkk, nplanes);
+  API_END
+}
+
+int otbl_fourxz(otbl_ctx *ctx, double *fou, double *phys, int iopt, int nrows) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(nrows >= 1, "nrows must be >= 1");
+  k_fourxz(b, fou, phys, iopt, nrows);
+  API_END
+}
+
+int otbl_cosftx(otbl_ctx *ctx, const double *in, double *out, int iopt) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(in != out, "otbl_cosftx works out of place");
+  k_cosftx(b, in, out, iopt);
+  API_END
+}
+
+int otbl_dev_alloc(size_t bytes, void **ptr) {
+  API_BEGIN
+  need(ptr != nullptr, "null pointer");
+  OTBL_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  API_END
+}
+
+int otbl_dev_free(void *ptr) {
+  API_BEGIN
+  if (ptr) OTBL_HIP(hipFree(ptr));
+  API_END
+}
+
+int otbl_h2d(void *dst, const void *src, size_t bytes) {
+  API_BEGIN
+  OTBL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  API_END
+}
+
+int otbl_d2h(void *dst, const void *src, size_t bytes) {
+  API_BEGIN
+  OTBL_HIP(hipDeviceSynchronize());
+  OTBL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  API_END
+}
+
+int otbl_kprof(otbl_ctx *ctx, int on) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  drain_kprof(b);
+  b.kprof = on != 0;
+  if (on) b.kstat.clear();
+  API_END
+}
+
+int otbl_kprof_report(otbl_ctx *ctx, char *buf, size_t len) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  drain_kprof(b);
+  std::string s;
+  char line[256];
+  for (auto &kv : b.kstat) {
+    snprintf(line, sizeof line, "%s %d %.6f\n", kv.first.c_str(), kv.second.n, kv.second.ms);
+    s += line;
+  }
+  need(buf && len > s.size(), "otbl_kprof_report: buffer too small");
+  memcpy(buf, s.c_str(), s.size() + 1);
+  API_END
+}
+
+} // extern "C"

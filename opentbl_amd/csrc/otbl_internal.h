@@ -1,0 +1,129 @@
+// Internal declarations of libopentbl_amd: host coefficient set, device
+// coefficient pointers, the box context and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "opentbl.h"
+
+namespace otbl {
+
+// (r, n) Fortran arrays are stored with column i (1-based) at offset r*i, so
+// a (3, 1:n) array has 3*(n+1) doubles and element (q, i) is a[3*i + q - 1].
+struct HostCoef {
+  int nx = 0, ny = 0, nz = 0, nz1 = 0, nz2 = 0;
+  double re = 0, ax = 0, az = 0, cfl = 0, pi = 0;
+  std::vector<double> y, dy, dyi, idy, idyy, dymin, x, dx, dxi, hy;
+  std::vector<double> kaz, kaz2, kvis, kmod, vmagic;
+  double idx = 0, idxx = 0, dxmin = 0, dzmin = 0, dtret = 0, rex = 0;
+  double rkc[4] = {0}, rkd[4] = {0}, rkcv[4] = {0}, rkdv[4] = {0};
+  // tridiagonal left-hand sides (unfactored here; factorised on the GPU)
+  std::vector<double> inxu, inxv, inyu, inyv, dcxu, dcxv, dcyu, dcyv;
+  std::vector<double> vixu, vixv, viyu, viyv, vyui, vyvi;
+  // explicit right-hand sides
+  std::vector<double> cofiux, cofivx, cofiuy, cofivy, cofcxu, cofcxv, cofcyu, cofcyv;
+  std::vector<double> cofvxu, cofvxv, cofvyu, cofvyv, ayp;
+  double inbx[16] = {0}, inby[16] = {0}, dcbx[20] = {0}, dcby[16] = {0}, ccon[8] = {0};
+  double cofvbx[16] = {0}, cofvby[16] = {0};
+};
+
+// Builds every coefficient of the RK3 step (coeft.f90:10-1234).
+void build_coef(HostCoef &hc, int nx, int ny, int nz, double re, double ax, double az,
+                double cfl, const double *y);
+// inlet_retheta (bl_1.F90:372-396) and magicnumber (boun.f90:102-143)
+void inlet_setup(HostCoef &hc, const double *u0, double *rthin, double *din);
+
+// Device coefficient pointers (all FP64, same (r, n+1) layout as HostCoef).
+struct DevCoef {
+  const double *inxu, *inxv, *inyu, *inyv, *dcxu, *dcxv, *dcyu, *dcyv;
+  const double *vixu, *vixv, *viyu, *viyv, *vyui, *vyvi;
+  const double *cofiux, *cofivx, *cofiuy, *cofivy, *cofcxu, *cofcxv, *cofcyu, *cofcyv;
+  const double *cofvxu, *cofvxv, *cofvyu, *cofvyv, *ayp;
+  const double *inbx, *inby, *dcbx, *dcby, *ccon, *cofvbx, *cofvby;
+  const double *y, *dy, *idy, *idyy, *dymin, *hy, *kaz, *kaz2, *kvis, *kmod, *vmagic;
+};
+
+// Scalars that live on the device so a substep never waits for the host.
+struct DevScalars {
+  double dt, tiempo, timeinit;
+  double var1, var2, var3, var4, rkk, varstep;
+  double rthin, din, rthout, dout, utauout, gamma1, gamma2, timec;
+  double sumax;
+  unsigned long long umax_bits, wmax_bits; // |u|, |w_x| maxima (IEEE order)
+  int dt_valid, pad;
+  double ener[16];
+};
+
+struct Box {
+  int nx, ny, nz, nz1, nz2, mz, xout, precision, dt_mode;
+  double re, ax, az, cfl;
+  size_t nu, nv;     // doubles per u-type / v-type field
+  size_t pu, pv;     // doubles per u-type / v-type plane
+  hipStream_t s = nullptr;
+  HostCoef hc;
+  DevCoef dc{};
+  double *coefbuf = nullptr;
+  // persistent state
+  double *f[OTBL_NFIELDS] = {nullptr};
+  // work fields
+  double *resu = nullptr, *resv = nullptr, *resw = nullptr;
+  double *rhsu = nullptr, *rhsv = nullptr, *rhsw = nullptr;
+  double *wki1 = nullptr, *wki2 = nullptr, *wki3 = nullptr;
+  double *ya = nullptr, *yb = nullptr, *yc = nullptr; // y-interpolated / product work
+  double *rest = nullptr, *rest2 = nullptr;
+  // Poisson x transforms (DCT matrices, (nx-1)^2 each)
+  double *dct2 = nullptr, *dct3 = nullptr;
+  // implicit y LU (dt dependent), u-type and v-type: a1, a2, a3 rows
+  double *implu = nullptr, *implv = nullptr;
+  // genflu / inflow
+  double *um = nullptr, *u0 = nullptr, *v0 = nullptr, *gwork = nullptr;
+  DevScalars *ds = nullptr;
+  unsigned long long *vmbits = nullptr; // per-row max |v_x|
+  // FFT twiddles e^{2 pi i t / nz}
+  double *tw = nullptr;
+  int nfac = 0, fac[16] = {0};
+  // graphs
+  bool use_graphs = false;
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  // measurement
+  bool kprof = false;
+  struct KStat { int n = 0; double ms = 0.0; };
+  std::map<std::string, KStat> kstat;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+};
+
+// kernel launchers (otbl_kernels.hip); each enqueues on b.s
+void k_factor_coef(Box &b, double *coefbuf_rw);
+void k_interpxx(Box &b, const double *gen, double *geni, int R, int which_u);
+void k_difvisxx(Box &b, const double *prod, const double *vel, double *out, int R,
+                int which_u, int mode);
+void k_interpyy(Box &b, const double *gen, double *geni, int loguv, int i0, int nplanes,
+                size_t in_pstride, size_t out_pstride);
+void k_differzy(Box &b, const double *geni, double *res, int which, int nplanes,
+                size_t in_pstride, size_t out_pstride);
+void k_vistyy(Box &b, const double *u, double *res, int R, int nplanes);
+void k_implzy_op(Box &b, double *u, const double *res, int R, double rkk, int nplanes);
+void k_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows);
+void k_cosftx(Box &b, const double *in, double *out, int iopt);
+void k_substep_rhsp(Box &b, int m);
+void k_outflow(Box &b);
+void k_pois(Box &b, int m, double varstep, bool use_m);
+void k_end_step(Box &b);
+void k_set_inflow_dev(Box &b);
+
+// launch bookkeeping (otbl_api.cpp)
+void kbegin(Box &b, const char *name);
+void kend(Box &b);
+[[noreturn]] void fail_hip(hipError_t e, const char *what);
+
+} // namespace otbl
+
+#define OTBL_HIP(call)                                                         \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) otbl::fail_hip(e_, #call);                           \
+  } while (0)

@@ -1,0 +1,1345 @@
+// HIP kernels of the RK3 substep (FP64 path) for gfx950.
+//
+// Data layout (the reference's planes): a complex field (0:nz2, R, 1:nx) as
+// doubles, real index kr = 0..nz1-1 fastest, then row j = 1..R, then plane i.
+// Element (kr, j, i) sits at (i-1)*P + (j-1)*nz1 + kr with P = R*nz1.
+//
+// Line solves map one thread to one line:
+//  * x lines (lip.f90 interpxx/difvisxx): a line is a fixed (kr, j); the
+//    thread marches over planes, so a wavefront reads 64 consecutive doubles
+//    of one plane per step (coalesced), no plane<->pencil transpose.
+//  * y lines (interpyy, differzy, vistyy, implzy, soltriy): a line is a fixed
+//    (kr, i); the thread marches over rows, again reading consecutive kr.
+// The z transforms work on one (j, i) row per workgroup in LDS (mixed-radix
+// Stockham, radices 8/4/2/3/5/7), two real signals per complex transform.
+#include "otbl_internal.h"
+
+#include <cmath>
+#include <cstdio>
+
+namespace otbl {
+
+#define C3(a, r, i) (a)[3 * (size_t)(i) + (r)-1]
+#define C2(a, r, i) (a)[2 * (size_t)(i) + (r)-1]
+#define M4(a, r, c) (a)[4 * ((c)-1) + (r)-1]
+#define M5(a, r, c) (a)[5 * ((c)-1) + (r)-1]
+#define M2(a, r, c) (a)[2 * ((c)-1) + (r)-1]
+
+static constexpr int TPB = 256;
+
+static inline unsigned nblk(size_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
+
+#define LAUNCH(b, name, grid, block, shm, kern, ...)                          \
+  do {                                                                         \
+    kbegin(b, name);                                                           \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shm, (b).s, __VA_ARGS__); \
+    kend(b);                                                                   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// coefficient factorisation (coefinv, coeft.f90:1338-1353) in FP64
+// ---------------------------------------------------------------------------
+struct FactorJob {
+  double *a[12];
+  int n[12];
+};
+
+__global__ void factor_kernel(FactorJob job) {
+  int t = threadIdx.x;
+  if (t >= 12) return;
+  double *a = job.a[t];
+  int neq = job.n[t];
+  for (int j = 2; j <= neq; ++j) {
+    double d = 1.0 / C3(a, 2, j - 1);
+    C3(a, 2, j - 1) = d;
+    C3(a, 1, j) = -C3(a, 1, j) * d;
+    C3(a, 2, j) = C3(a, 2, j) + C3(a, 1, j) * C3(a, 3, j - 1);
+  }
+  C3(a, 2, neq) = 1.0 / C3(a, 2, neq);
+}
+
+void k_factor_coef(Box &b, double *) {
+  FactorJob job;
+  const DevCoef &c = b.dc;
+  double *arr[12] = {(double *)c.inxu, (double *)c.inxv, (double *)c.inyu, (double *)c.inyv,
+                     (double *)c.dcxu, (double *)c.dcxv, (double *)c.dcyu, (double *)c.dcyv,
+                     (double *)c.vixu, (double *)c.vixv, (double *)c.viyu, (double *)c.viyv};
+  int n[12] = {b.nx, b.nx, b.ny, b.ny, b.nx, b.nx, b.ny + 1, b.ny, b.nx, b.nx, b.ny + 1, b.ny};
+  for (int q = 0; q < 12; ++q) {
+    job.a[q] = arr[q];
+    job.n[q] = n[q];
+  }
+  LAUNCH(b, "factor_coef", 1, 64, 0, factor_kernel, job);
+}
+
+// ---------------------------------------------------------------------------
+// x-line operators
+// ---------------------------------------------------------------------------
+// interpxx (lip.f90:248-300): midpoint interpolation along x.
+__global__ void interpxx_kernel(const double *__restrict__ gen, double *__restrict__ geni,
+                                const double *__restrict__ icon, const double *__restrict__ econ,
+                                const double *__restrict__ inbx, int chi, int chi2, int nx,
+                                size_t P) {
+  size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= P) return;
+  const double *g = gen + l;
+  double *o = geni + l;
+  double g1 = g[0], g2 = g[P], g3 = g[2 * P], g4 = g[3 * P];
+  const double gl0 = g[(size_t)(nx - 1) * P], gl1 = g[(size_t)(nx - 2) * P],
+               gl2 = g[(size_t)(nx - 3) * P], gl3 = g[(size_t)(nx - 4) * P];
+  double y = M4(inbx, chi2, 1) * g1 + M4(inbx, chi2, 2) * g2 + M4(inbx, chi2, 3) * g3 +
+             M4(inbx, chi2, 4) * g4;
+  o[0] = y;
+  double gi = g2; // g(i)
+  for (int i = 2; i <= nx - 1; ++i) {
+    double gn = g[(size_t)i * P]; // g(i+1)
+    double rhs = C2(econ, 1, i) * gi + C2(econ, 2, i) * gn;
+    y = rhs + C3(icon, 1, i) * y;
+    o[(size_t)(i - 1) * P] = y;
+    gi = gn;
+  }
+  {
+    double rhs = M4(inbx, chi, 1) * gl0 + M4(inbx, chi, 2) * gl1 + M4(inbx, chi, 3) * gl2 +
+                 M4(inbx, chi, 4) * gl3;
+    y = rhs + C3(icon, 1, nx) * y;
+  }
+  double x = y * C3(icon, 2, nx);
+  o[(size_t)(nx - 1) * P] = x;
+  for (int i = nx - 1; i >= 1; --i) {
+    double yi = o[(size_t)(i - 1) * P];
+    x = (yi - C3(icon, 3, i) * x) * C3(icon, 2, i);
+    o[(size_t)(i - 1) * P] = x;
+  }
+}
+
+void k_interpxx(Box &b, const double *gen, double *geni, int R, int which_u) {
+  size_t P = (size_t)b.nz1 * R;
+  const double *icon = which_u ? b.dc.inxu : b.dc.inxv;
+  const double *econ = which_u ? b.dc.cofiux : b.dc.cofivx;
+  int chi = which_u ? 2 : 1;
+  int chi2 = which_u ? chi + 1 : chi + 3;
+  LAUNCH(b, "interpxx", nblk(P), TPB, 0, interpxx_kernel, gen, geni, icon, econ, b.dc.inbx, chi,
+         chi2, b.nx, P);
+}
+
+// difvisxx (lip.f90:52-142): convective d/dx of the product plus rex times
+// the compact d2/dx2 of the velocity.  mode 0: out = result; mode 1:
+// out += result except on the last plane, where out = result (the outflow
+// handling of rhsp.F90:200,204).  sc, sv: scratch fields for the two
+// forward sweeps.
+__global__ void difvisxx_kernel(const double *__restrict__ gp, const double *__restrict__ q,
+                                double *__restrict__ out, double *__restrict__ sc,
+                                double *__restrict__ sv, const double *__restrict__ icon,
+                                const double *__restrict__ iconv, const double *__restrict__ econ,
+                                const double *__restrict__ econv, const double *__restrict__ dcbx,
+                                const double *__restrict__ cofvbx, int chv, int ic, int nx,
+                                size_t P, double rex, double idx, int mode) {
+  size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= P) return;
+  const double *g = gp + l;
+  const double *v = q + l;
+  double *c = sc + l, *s = sv + l, *o = out + l;
+  const int chv1 = chv + 2;
+  double g1 = g[0], g2 = g[P], g3 = g[2 * P], g4 = g[3 * P];
+  double q1 = v[0], q2 = v[P], q3 = v[2 * P], q4 = v[3 * P];
+  double yc = M5(dcbx, chv, 1) * g1 + M5(dcbx, chv, 2) * g2 + M5(dcbx, chv, 3) * g3 +
+              M5(dcbx, chv, 4) * g4;
+  double yv = M4(cofvbx, ic, 1) * q1 + M4(cofvbx, ic, 2) * q2 + M4(cofvbx, ic, 3) * q3 +
+              M4(cofvbx, ic, 4) * q4;
+  c[0] = yc;
+  s[0] = yv;
+  double gim1 = g1, qim1 = q1, qi = q2;
+  for (int i = 2; i <= nx - 1; ++i) {
+    double gi = g[(size_t)(i - 1) * P];
+    double qn = v[(size_t)i * P];
+    double rc = C2(econ, 1, i) * gim1 + C2(econ, 2, i) * gi;
+    double rv = C3(econv, 1, i) * qn + C3(econv, 2, i) * qi + C3(econv, 3, i) * qim1;
+    yc = rc + C3(icon, 1, i) * yc;
+    yv = rv + C3(iconv, 1, i) * yv;
+    c[(size_t)(i - 1) * P] = yc;
+    s[(size_t)(i - 1) * P] = yv;
+    gim1 = gi;
+    qim1 = qi;
+    qi = qn;
+  }
+  double gn = g[(size_t)(nx - 1) * P], gn1 = g[(size_t)(nx - 2) * P], gn2 = g[(size_t)(nx - 3) * P],
+         gn3 = g[(size_t)(nx - 4) * P];
+  double qn = qi, qn1 = qim1, qn2 = v[(size_t)(nx - 3) * P], qn3 = v[(size_t)(nx - 4) * P];
+  double rc = M5(dcbx, chv1, 1) * gn + M5(dcbx, chv1, 2) * gn1 + M5(dcbx, chv1, 3) * gn2 +
+              M5(dcbx, chv1, 4) * gn3;
+  double rv = M4(cofvbx, ic + 1, 1) * qn + M4(cofvbx, ic + 1, 2) * qn1 +
+              M4(cofvbx, ic + 1, 3) * qn2 + M4(cofvbx, ic + 1, 4) * qn3;
+  yc = rc + C3(icon, 1, nx) * yc;
+  yv = rv + C3(iconv, 1, nx) * yv;
+  double xc = yc * C3(icon, 2, nx);
+  double xv = yv * C3(iconv, 2, nx);
+  // outflow rows: U_inf du/dx convective, no viscous term (lip.f90:105,132)
+  double last = 1.0 * (qn - qn1) * idx + rex * 0.0;
+  size_t on = (size_t)(nx - 1) * P;
+  o[on] = last;
+  for (int i = nx - 1; i >= 1; --i) {
+    size_t oi = (size_t)(i - 1) * P;
+    xc = (c[oi] - C3(icon, 3, i) * xc) * C3(icon, 2, i);
+    xv = (s[oi] - C3(iconv, 3, i) * xv) * C3(iconv, 2, i);
+    double r = xc + rex * xv;
+    if (mode == 1) r = o[oi] + r;
+    o[oi] = r;
+  }
+}
+
+void k_difvisxx(Box &b, const double *prod, const double *vel, double *out, int R, int which_u,
+                int mode) {
+  size_t P = (size_t)b.nz1 * R;
+  const DevCoef &c = b.dc;
+  int vel_u = which_u; // viscous coefficient set: u (ic=3) or v/w (ic=1)
+  LAUNCH(b, "difvisxx", nblk(P), TPB, 0, difvisxx_kernel, prod, vel, out, b.ya, b.yb,
+         which_u ? c.dcxu : c.dcxv, which_u ? c.vixu : c.vixv, which_u ? c.cofcxu : c.cofcxv,
+         which_u ? c.cofvxu : c.cofvxv, c.dcbx, c.cofvbx, which_u ? 1 : 2, vel_u ? 3 : 1, b.nx,
+         P, b.hc.rex, b.hc.idx, mode);
+}
+
+// ---------------------------------------------------------------------------
+// y-line operators.  Column c = (kr, plane); rows at stride nz1.
+// ---------------------------------------------------------------------------
+// Tridiagonal sweep helpers on a column already holding right-hand sides
+// rows 1..n in `r` (stride s).
+__device__ inline void tri_solve_col(double *r, size_t s, const double *icon, int n) {
+  double y = r[0];
+  for (int j = 2; j <= n; ++j) {
+    y = r[(size_t)(j - 1) * s] + C3(icon, 1, j) * y;
+    r[(size_t)(j - 1) * s] = y;
+  }
+  double x = y * C3(icon, 2, n);
+  r[(size_t)(n - 1) * s] = x;
+  for (int j = n - 1; j >= 1; --j) {
+    x = (r[(size_t)(j - 1) * s] - C3(icon, 3, j) * x) * C3(icon, 2, j);
+    r[(size_t)(j - 1) * s] = x;
+  }
+}
+
+// interpyy (lip.f90:359-440) on spectral rows.
+__global__ void interpyy_kernel(const double *__restrict__ gen, double *__restrict__ geni,
+                                const double *__restrict__ icon, const double *__restrict__ econ,
+                                const double *__restrict__ inby, int loguv, int ny, int nz1,
+                                int i0, int nplanes, size_t ips, size_t ops) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * nplanes) return;
+  int kr = (int)(cidx % nz1);
+  size_t ip = (size_t)(i0 - 1) + cidx / nz1;
+  const double *g = gen + ip * ips + kr;
+  double *o = geni + ip * ops + kr;
+  const size_t s = nz1;
+#define G(j) g[(size_t)((j)-1) * s]
+  if (loguv == 0) {
+    double top = M4(inby, 1, 1) * G(ny + 1) + M4(inby, 1, 2) * G(ny) + M4(inby, 1, 3) * G(ny - 1) +
+                 M4(inby, 1, 4) * G(ny - 2);
+    o[0] = 0.0;
+    for (int j = 2; j <= ny - 1; ++j) o[(size_t)(j - 1) * s] = C2(econ, 1, j) * G(j) + C2(econ, 2, j) * G(j + 1);
+    o[(size_t)(ny - 1) * s] = top;
+  } else {
+    double bot = M4(inby, 3, 1) * G(1) + M4(inby, 3, 2) * G(2) + M4(inby, 3, 3) * G(3) +
+                 M4(inby, 3, 4) * G(4);
+    double top = M4(inby, 4, 1) * G(ny) + M4(inby, 4, 2) * G(ny - 1) + M4(inby, 4, 3) * G(ny - 2) +
+                 M4(inby, 4, 4) * G(ny - 3);
+    for (int j = ny - 1; j >= 2; --j) o[(size_t)(j - 1) * s] = C2(econ, 2, j) * G(j) + C2(econ, 1, j) * G(j - 1);
+    o[0] = bot;
+    o[(size_t)(ny - 1) * s] = top;
+  }
+#undef G
+  tri_solve_col(o, s, icon, ny);
+}
+
+void k_interpyy(Box &b, const double *gen, double *geni, int loguv, int i0, int nplanes,
+                size_t ips, size_t ops) {
+  size_t n = (size_t)b.nz1 * nplanes;
+  LAUNCH(b, "interpyy", nblk(n), TPB, 0, interpyy_kernel, gen, geni,
+         loguv == 0 ? b.dc.inyu : b.dc.inyv, loguv == 0 ? b.dc.cofiuy : b.dc.cofivy, b.dc.inby,
+         loguv, b.ny, b.nz1, i0, nplanes, ips, ops);
+}
+
+// differzy (lip.f90:152-232).  which 0: v rows -> u rows (dcyu, cofcyu,
+// chv=3, fwb1=-1, fwb2=0, jee=ny+1); which 1: u rows -> v rows (dcyv,
+// cofcyv, chv=1, fwb1=0, fwb2=1, jee=ny).
+__device__ inline void differzy_col(const double *g, double *o, size_t s, int which, int ny,
+                                    const DevCoef &c) {
+#define G(j) g[(size_t)((j)-1) * s]
+  if (which == 0) {
+    const int jee = ny + 1;
+    double bot = M4(c.dcby, 3, 1) * G(1) + M4(c.dcby, 3, 2) * G(2) + M4(c.dcby, 3, 3) * G(3) +
+                 M4(c.dcby, 3, 4) * G(4);
+    double top = M4(c.dcby, 4, 1) * G(ny) + M4(c.dcby, 4, 2) * G(ny - 1) +
+                 M4(c.dcby, 4, 3) * G(ny - 2) + M4(c.dcby, 4, 4) * G(ny - 3);
+    for (int j = jee - 1; j >= 2; --j)
+      o[(size_t)(j - 1) * s] = C2(c.cofcyu, 2, j) * G(j) + C2(c.cofcyu, 1, j) * G(j - 1);
+    o[0] = bot;
+    o[(size_t)(jee - 1) * s] = top;
+    tri_solve_col(o, s, c.dcyu, jee);
+  } else {
+    const int jee = ny;
+    double bot = M4(c.dcby, 1, 1) * G(1) + M4(c.dcby, 1, 2) * G(2) + M4(c.dcby, 1, 3) * G(3) +
+                 M4(c.dcby, 1, 4) * G(4);
+    double top = M4(c.dcby, 2, 1) * G(ny) + M4(c.dcby, 2, 2) * G(ny - 1) +
+                 M4(c.dcby, 2, 3) * G(ny - 2) + M4(c.dcby, 2, 4) * G(ny - 3);
+    for (int j = 2; j <= jee - 1; ++j)
+      o[(size_t)(j - 1) * s] = C2(c.cofcyv, 2, j) * G(j + 1) + C2(c.cofcyv, 1, j) * G(j);
+    o[0] = bot;
+    o[(size_t)(jee - 1) * s] = top;
+    tri_solve_col(o, s, c.dcyv, jee);
+  }
+#undef G
+}
+
+__global__ void differzy_kernel(const double *__restrict__ gen, double *__restrict__ res, DevCoef c,
+                                int which, int ny, int nz1, int nplanes, size_t ips, size_t ops) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * nplanes) return;
+  int kr = (int)(cidx % nz1);
+  size_t ip = cidx / nz1;
+  differzy_col(gen + ip * ips + kr, res + ip * ops + kr, nz1, which, ny, c);
+}
+
+void k_differzy(Box &b, const double *geni, double *res, int which, int nplanes, size_t ips,
+                size_t ops) {
+  size_t n = (size_t)b.nz1 * nplanes;
+  LAUNCH(b, "differzy", nblk(n), TPB, 0, differzy_kernel, geni, res, b.dc, which, b.ny, b.nz1,
+         nplanes, ips, ops);
+}
+
+// vistyy (lip.f90:447-513): compact d2/dy2 of one column into o.
+__device__ inline void vistyy_col(const double *u, double *o, size_t s, int jee, int ny,
+                                  const DevCoef &c) {
+  const double *icon = (jee == ny) ? c.viyv : c.viyu;
+  const double *econ = (jee == ny) ? c.cofvyv : c.cofvyu;
+  const int ic = (jee == ny) ? 1 : 3;
+#define U(j) u[(size_t)((j)-1) * s]
+  o[0] = M4(c.cofvby, ic, 1) * U(1) + M4(c.cofvby, ic, 2) * U(2) + M4(c.cofvby, ic, 3) * U(3) +
+         M4(c.cofvby, ic, 4) * U(4);
+  for (int j = 2; j <= jee - 1; ++j)
+    o[(size_t)(j - 1) * s] = C3(econ, 1, j) * U(j + 1) + C3(econ, 2, j) * U(j) + C3(econ, 3, j) * U(j - 1);
+  o[(size_t)(jee - 1) * s] = M4(c.cofvby, ic + 1, 1) * U(jee) + M4(c.cofvby, ic + 1, 2) * U(jee - 1) +
+                             M4(c.cofvby, ic + 1, 3) * U(jee - 2) + M4(c.cofvby, ic + 1, 4) * U(jee - 3);
+#undef U
+  tri_solve_col(o, s, icon, jee);
+}
+
+__global__ void vistyy_kernel(const double *__restrict__ u, double *__restrict__ res, DevCoef c,
+                              int jee, int ny, int nz1, int nplanes) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * nplanes) return;
+  int kr = (int)(cidx % nz1);
+  size_t ip = cidx / nz1;
+  size_t ps = (size_t)nz1 * jee;
+  vistyy_col(u + ip * ps + kr, res + ip * ps + kr, nz1, jee, ny, c);
+}
+
+void k_vistyy(Box &b, const double *u, double *res, int R, int nplanes) {
+  size_t n = (size_t)b.nz1 * nplanes;
+  LAUNCH(b, "vistyy", nblk(n), TPB, 0, vistyy_kernel, u, res, b.dc, R, b.ny, b.nz1, nplanes);
+}
+
+// Implicit y-viscous factors for the current dt (implzy's a1, a2, a3 and
+// their LU, lip.f90:545-568), computed once per substep in FP64.
+// layout per set (n = jee): f[0..n] a1 factored, f[n+1..2n+1] a2 (inverse
+// diagonal), f[2n+2..3n+2] a3, f[3n+3] = a1(2) and f[3n+4] = a3(jee-1)
+// before they are zeroed.
+__device__ void impl_factor(double *f, const double *icon, const double *econ, int jee, double v1) {
+  double *a1 = f, *a2 = f + (jee + 1), *a3 = f + 2 * (jee + 1);
+  for (int j = 2; j <= jee - 1; ++j) {
+    a1[j] = C3(icon, 1, j) + v1 * C3(econ, 3, j);
+    a2[j] = C3(icon, 2, j) + v1 * C3(econ, 2, j);
+    a3[j] = C3(icon, 3, j) + v1 * C3(econ, 1, j);
+  }
+  f[3 * (jee + 1)] = a1[2];
+  f[3 * (jee + 1) + 1] = a3[jee - 1];
+  a1[2] = 0.0;
+  a3[jee - 1] = 0.0;
+  for (int j = 3; j <= jee - 1; ++j) {
+    double d = 1.0 / a2[j - 1];
+    a2[j - 1] = d;
+    a1[j] = -a1[j] * d;
+    a2[j] = a2[j] + a1[j] * a3[j - 1];
+  }
+  a2[jee - 1] = 1.0 / a2[jee - 1];
+}
+
+__global__ void impl_factor_kernel(double *fu, double *fv, DevCoef c, int ny, const DevScalars *ds,
+                                   double rkk_explicit, int use_explicit) {
+  double rkk = use_explicit ? rkk_explicit : ds->rkk;
+  if (threadIdx.x == 0) impl_factor(fu, c.vyui, c.cofvyu, ny + 1, rkk);
+  if (threadIdx.x == 1) impl_factor(fv, c.vyvi, c.cofvyv, ny, rkk);
+}
+
+// implzy (lip.f90:528-597): interior rows 2..jee-1 of u from res; rows 1 and
+// jee of u are the boundary values already in place.
+__device__ inline void implzy_col(double *u, const double *res, size_t s, int jee,
+                                  const double *icon, const double *f) {
+  const double *a1 = f, *a2 = f + (jee + 1), *a3 = f + 2 * (jee + 1);
+  const double a1b = f[3 * (jee + 1)], a3b = f[3 * (jee + 1) + 1];
+#define R(j) res[(size_t)((j)-1) * s]
+#define UU(j) u[(size_t)((j)-1) * s]
+  double rm = R(1), r0 = R(2);
+  double u1 = UU(1), ujee = UU(jee);
+  double y = 0.0;
+  for (int j = 2; j <= jee - 1; ++j) {
+    double rp = R(j + 1);
+    double rhs = C3(icon, 1, j) * rm + C3(icon, 2, j) * r0 + C3(icon, 3, j) * rp;
+    if (j == 2) rhs -= a1b * u1;
+    if (j == jee - 1) rhs -= a3b * ujee;
+    y = (j == 2) ? rhs : rhs + a1[j] * y;
+    UU(j) = y;
+    rm = r0;
+    r0 = rp;
+  }
+  double x = y * a2[jee - 1];
+  UU(jee - 1) = x;
+  for (int j = jee - 2; j >= 2; --j) {
+    x = (UU(j) - a3[j] * x) * a2[j];
+    UU(j) = x;
+  }
+#undef R
+#undef UU
+}
+
+__global__ void implzy_kernel(double *u, const double *res, DevCoef c, const double *f, int jee,
+                              int ny, int nz1, int nplanes) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * nplanes) return;
+  int kr = (int)(cidx % nz1);
+  size_t ip = cidx / nz1;
+  size_t ps = (size_t)nz1 * jee;
+  implzy_col(u + ip * ps + kr, res + ip * ps + kr, nz1, jee, jee == ny + 1 ? c.vyui : c.vyvi, f);
+}
+
+void k_implzy_op(Box &b, double *u, const double *res, int R, double rkk, int nplanes) {
+  LAUNCH(b, "impl_factor", 1, 64, 0, impl_factor_kernel, b.implu, b.implv, b.dc, b.ny, b.ds, rkk, 1);
+  size_t n = (size_t)b.nz1 * nplanes;
+  LAUNCH(b, "implzy", nblk(n), TPB, 0, implzy_kernel, u, res, b.dc, R == b.ny + 1 ? b.implu : b.implv,
+         R, b.ny, b.nz1, nplanes);
+}
+
+// ---------------------------------------------------------------------------
+// z transforms in LDS
+// ---------------------------------------------------------------------------
+struct FFTPlan {
+  int n, nfac;
+  int fac[12];
+};
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// One Stockham stage of radix R (decimation in time, self-sorting).
+template <int R>
+__device__ inline void fft_stage(const double2 *__restrict__ a, double2 *__restrict__ b, int n,
+                                 int Ns, const double2 *__restrict__ tw, double sgn) {
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  const int rstep = n / R;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    int k = j % Ns;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double2 x = a[j + r * nb];
+      double2 w = tw[(k * r * tstep) % n];
+      w.y *= sgn;
+      v[r] = (r == 0) ? x : cmul(x, w);
+    }
+    int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      double2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        double2 w = tw[((r * q) % R) * rstep];
+        w.y *= sgn;
+        double2 t = cmul(v[r], w);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
+      b[idxD + q * Ns] = acc;
+    }
+  }
+}
+
+// Complex transform of length plan.n in LDS; sgn = -1 forward (FFTW sign),
+// +1 backward (unnormalised).  Returns the buffer holding the result.
+__device__ double2 *lds_fft(double2 *a, double2 *t, const FFTPlan &p, const double2 *tw, double sgn) {
+  int Ns = 1;
+  for (int s = 0; s < p.nfac; ++s) {
+    int R = p.fac[s];
+    switch (R) {
+    case 8: fft_stage<8>(a, t, p.n, Ns, tw, sgn); break;
+    case 4: fft_stage<4>(a, t, p.n, Ns, tw, sgn); break;
+    case 2: fft_stage<2>(a, t, p.n, Ns, tw, sgn); break;
+    case 3: fft_stage<3>(a, t, p.n, Ns, tw, sgn); break;
+    case 5: fft_stage<5>(a, t, p.n, Ns, tw, sgn); break;
+    case 7: fft_stage<7>(a, t, p.n, Ns, tw, sgn); break;
+    }
+    __syncthreads();
+    double2 *tmp = a;
+    a = t;
+    t = tmp;
+    Ns *= R;
+  }
+  return a;
+}
+
+// Hermitian-extended spectrum of two real signals x = c2r(X), y = c2r(Y)
+// packed as z = x + i y.  X, Y rows hold modes 0..nz2 (re, im pairs);
+// Im of mode 0 is ignored as in a real inverse transform.
+__device__ inline void pack_pair(double2 *z, const double *X, const double *Y, int n, int nz2) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) z[k] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int k = threadIdx.x; k <= nz2; k += blockDim.x) {
+    double xr = X ? X[2 * k] : 0.0, xi = X ? X[2 * k + 1] : 0.0;
+    double yr = Y ? Y[2 * k] : 0.0, yi = Y ? Y[2 * k + 1] : 0.0;
+    if (k == 0) {
+      z[0] = make_double2(xr, yr);
+    } else {
+      z[k] = make_double2(xr - yi, xi + yr);
+      z[n - k] = make_double2(xr + yi, -xi + yr);
+    }
+  }
+  __syncthreads();
+}
+
+// Split the transform Z of z = a + i b into the spectra of a and b, modes
+// 0..nz2, scaled by 1/n (the r2c scaling of ffts.f90:51).
+__device__ inline void unpack_pair(const double2 *Z, double *A, double *B, int n, int nz2) {
+  const double sc = 1.0 / n;
+  for (int k = threadIdx.x; k <= nz2; k += blockDim.x) {
+    double2 z = Z[k];
+    double2 w = Z[k == 0 ? 0 : n - k];
+    if (A) {
+      A[2 * k] = 0.5 * (z.x + w.x) * sc;
+      A[2 * k + 1] = (k == 0) ? 0.0 : 0.5 * (z.y - w.y) * sc;
+    }
+    if (B) {
+      B[2 * k] = 0.5 * (z.y + w.y) * sc;
+      B[2 * k + 1] = (k == 0) ? 0.0 : -0.5 * (z.x - w.x) * sc;
+    }
+  }
+}
+
+__device__ inline void block_atomic_max(double v, unsigned long long *dst) {
+  // v >= 0: IEEE bit patterns of non-negative doubles order like integers
+  __shared__ double red[TPB / 64];
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int q = 1; q < (int)(blockDim.x / 64); ++q) m = fmax(m, red[q]);
+    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+  __syncthreads();
+}
+
+// The buffers of bufs[4] other than x and y (first two found).
+__device__ inline void pick_free(double2 *const bufs[4], const double2 *x, const double2 *y,
+                                 double2 *out[2]) {
+  int c = 0;
+  for (int q = 0; q < 4 && c < 2; ++q)
+    if (bufs[q] != x && bufs[q] != y) out[c++] = bufs[q];
+}
+
+// Plane loop 1 of rhsp (rhsp.F90:120-153) for one (j, i) row: with u, u_x,
+// w_x, v_x and u interpolated to v rows (uy) in spectral space, form
+// r2c(u_x^2) -> wki1, r2c(u w_x) -> wki3, r2c(v_x uy) -> wki2 (rows <= ny),
+// and the maxima |u|, |w_x|, |v_x|(j) for the CFL time step.
+__global__ void products1_kernel(const double *__restrict__ u, double *wki1, double *wki2,
+                                 double *wki3, const double *__restrict__ uy, FFTPlan plan,
+                                 const double2 *__restrict__ tw, int ny, int nz1, int nz2,
+                                 DevScalars *ds, unsigned long long *vmbits) {
+  extern __shared__ double2 sh[];
+  const int n = plan.n;
+  double2 *A = sh, *B = sh + n, *Cc = sh + 2 * n, *T = sh + 3 * n;
+  const int j = blockIdx.x + 1, i = blockIdx.y + 1;
+  const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
+  const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
+  const bool vrow = j <= ny;
+  double2 *bufs[4] = {A, B, Cc, T};
+  pack_pair(A, u + ou, wki1 + ou, n, nz2);
+  double2 *rA = lds_fft(A, T, plan, tw, 1.0);
+  pack_pair(B, wki3 + ou, vrow ? wki2 + ov : nullptr, n, nz2);
+  double2 *fr[2];
+  pick_free(bufs, rA, B, fr);
+  double2 *rB = lds_fft(B, fr[0], plan, tw, 1.0);
+  double2 *rC = nullptr;
+  if (vrow) {
+    pick_free(bufs, rA, rB, fr);
+    pack_pair(fr[0], uy + ou, nullptr, n, nz2);
+    rC = lds_fft(fr[0], fr[1], plan, tw, 1.0);
+  }
+  double um = 0.0, wm = 0.0, vm = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    double2 a = rA[q], b2 = rB[q];
+    double uu = a.x, ux = a.y, wx = b2.x, vx = b2.y;
+    um = fmax(um, fabs(uu));
+    wm = fmax(wm, fabs(wx));
+    rA[q] = make_double2(ux * ux, uu * wx);
+    if (vrow) {
+      vm = fmax(vm, fabs(vx));
+      rB[q] = make_double2(vx * rC[q].x, 0.0);
+    }
+  }
+  __syncthreads();
+  block_atomic_max(um, &ds->umax_bits);
+  block_atomic_max(wm, &ds->wmax_bits);
+  if (vrow) block_atomic_max(vm, &vmbits[j - 1]);
+  // forward transforms; buffers other than rA, rB are free now
+  pick_free(bufs, rA, rB, fr);
+  double2 *FA = lds_fft(rA, fr[0], plan, tw, -1.0);
+  unpack_pair(FA, wki1 + ou, wki3 + ou, n, nz2);
+  if (vrow) {
+    __syncthreads();
+    pick_free(bufs, rB, rB, fr);
+    double2 *FB = lds_fft(rB, fr[0], plan, tw, -1.0);
+    unpack_pair(FB, wki2 + ov, nullptr, n, nz2);
+  }
+}
+
+// Plane loop 2 of rhsp (rhsp.F90:367-415) for one (j, i) row, j <= ny:
+// vw = r2c(v * w_v) -> ya (w interpolated to v rows is read from ya),
+// vv = r2c(v_u^2) -> yb (v interpolated to u rows is read from yb),
+// ww = r2c(w^2) -> yc.
+__global__ void products2_kernel(const double *__restrict__ resv, const double *__restrict__ resw,
+                                 double *ya, double *yb, double *yc, FFTPlan plan,
+                                 const double2 *__restrict__ tw, int ny, int nz1, int nz2) {
+  extern __shared__ double2 sh[];
+  const int n = plan.n;
+  double2 *A = sh, *B = sh + n, *T = sh + 2 * n, *T2 = sh + 3 * n;
+  const int j = blockIdx.x + 1, i = blockIdx.y + 2;
+  const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
+  const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
+  pack_pair(A, resv + ov, ya + ou, n, nz2);
+  double2 *rA = lds_fft(A, T, plan, tw, 1.0);
+  double2 *fB = (rA == A) ? T : A;
+  pack_pair(B, yb + ou, resw + ou, n, nz2);
+  double2 *rB = lds_fft(B, T2, plan, tw, 1.0);
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    double2 a = rA[q], b2 = rB[q];
+    rA[q] = make_double2(a.x * a.y, b2.x * b2.x);
+    rB[q] = make_double2(b2.y * b2.y, 0.0);
+  }
+  __syncthreads();
+  double2 *FA = lds_fft(rA, fB, plan, tw, -1.0);
+  unpack_pair(FA, ya + ou, yb + ou, n, nz2);
+  double2 *spareB = (rB == B) ? T2 : B;
+  __syncthreads();
+  double2 *FB = lds_fft(rB, spareB, plan, tw, -1.0);
+  unpack_pair(FB, yc + ou, nullptr, n, nz2);
+}
+
+// fourxz (ffts.f90:27-67) on independent rows, for the operator-level API.
+__global__ void fourxz_kernel(double *fou, double *phys, int iopt, FFTPlan plan,
+                              const double2 *__restrict__ tw, int nz1, int nz2) {
+  extern __shared__ double2 sh[];
+  const int n = plan.n;
+  double2 *A = sh, *T = sh + n;
+  size_t r = blockIdx.x;
+  double *F = fou + r * nz1;
+  double *Ph = phys + r * n;
+  if (iopt >= 0) {
+    pack_pair(A, F, nullptr, n, nz2);
+    double2 *R = lds_fft(A, T, plan, tw, 1.0);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) Ph[q] = R[q].x;
+  } else {
+    for (int q = threadIdx.x; q < n; q += blockDim.x) A[q] = make_double2(Ph[q], 0.0);
+    __syncthreads();
+    double2 *R = lds_fft(A, T, plan, tw, -1.0);
+    unpack_pair(R, F, nullptr, n, nz2);
+  }
+}
+
+void k_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows) {
+  FFTPlan plan;
+  plan.n = b.nz;
+  plan.nfac = b.nfac;
+  for (int q = 0; q < b.nfac; ++q) plan.fac[q] = b.fac[q];
+  LAUNCH(b, "fourxz", nrows, 64, 2 * b.nz * sizeof(double2), fourxz_kernel, fou, phys, iopt, plan,
+         (const double2 *)b.tw, b.nz1, b.nz2);
+}
+
+// ---------------------------------------------------------------------------
+// Poisson x transforms: DCT-II / DCT-III over planes 2..nx as a tiled FP64
+// product out[q][l] = sum_n M[q][n] in[n][l] (M = dct2 or dct3, N x N).
+// ---------------------------------------------------------------------------
+__global__ void dct_gemm_kernel(const double *__restrict__ M, const double *__restrict__ in,
+                                double *__restrict__ out, int N, int L, size_t ps) {
+  __shared__ double As[16][64 + 1];
+  __shared__ double Bs[16][64];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int q0 = blockIdx.y * 64, l0 = blockIdx.x * 64;
+  double acc[4][4] = {{0}};
+  for (int n0 = 0; n0 < N; n0 += 16) {
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+      int qq = e / 16, nn = e % 16;
+      int q = q0 + qq, n = n0 + nn;
+      As[nn][qq] = (q < N && n < N) ? M[(size_t)q * N + n] : 0.0;
+      int ll = e % 64, n2 = e / 64;
+      int l = l0 + ll, nb = n0 + n2;
+      Bs[n2][ll] = (l < L && nb < N) ? in[(size_t)nb * ps + l] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int nn = 0; nn < 16; ++nn) {
+      double a[4], bb[4];
+      for (int r = 0; r < 4; ++r) a[r] = As[nn][ty * 4 + r];
+      for (int c = 0; c < 4; ++c) bb[c] = Bs[nn][tx + 16 * c];
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * bb[c];
+    }
+    __syncthreads();
+  }
+  for (int r = 0; r < 4; ++r) {
+    int q = q0 + ty * 4 + r;
+    if (q >= N) continue;
+    for (int c = 0; c < 4; ++c) {
+      int l = l0 + tx + 16 * c;
+      if (l < L) out[(size_t)q * ps + l] = acc[r][c];
+    }
+  }
+}
+
+static void dct_apply(Box &b, const double *M, const double *in, double *out) {
+  int N = b.nx - 1;
+  int L = (b.ny - 1) * b.nz1; // rows 1..ny-1 of each plane
+  dim3 grid(nblk(L, 64), nblk(N, 64));
+  LAUNCH(b, "dct", grid, 256, 0, dct_gemm_kernel, M, in + b.pv, out + b.pv, N, L, b.pv);
+}
+
+void k_cosftx(Box &b, const double *in, double *out, int iopt) {
+  dct_apply(b, iopt < 0 ? b.dct2 : b.dct3, in, out);
+}
+
+// ---------------------------------------------------------------------------
+// rhsp pieces
+// ---------------------------------------------------------------------------
+__global__ void reset_max_kernel(DevScalars *ds, unsigned long long *vmbits, int ny) {
+  for (int j = threadIdx.x; j < ny; j += blockDim.x) vmbits[j] = 0ull;
+  if (threadIdx.x == 0) {
+    ds->umax_bits = 0ull;
+    ds->wmax_bits = 0ull;
+  }
+}
+
+// energies / ministats (rhsp.F90:116,493-509, stotf.F90:395-440) at
+// planes 1 and nx/2, one block.
+__global__ void energies_kernel(const double *u, const double *v, const double *w, const double *hy,
+                                DevScalars *ds, int nx, int ny, int nz1, int nz2) {
+  __shared__ double red[TPB];
+  double uner[16];
+  for (int q = 0; q < 16; ++q) uner[q] = 0.0;
+  const double *fl[3] = {u, v, w};
+  int rows[3] = {ny + 1, ny, ny + 1};
+  int ih = nx / 2;
+  for (int pass = 0; pass < 2; ++pass) {
+    int i = pass == 0 ? 1 : ih;
+    for (int c = 0; c < 3; ++c) {
+      const double *a = fl[c] + (size_t)(i - 1) * rows[c] * nz1;
+      double r01 = a[0];
+      double part = 0.0;
+      for (int j = threadIdx.x + 1; j <= rows[c]; j += blockDim.x) {
+        double aux = 0.0;
+        int k0 = pass == 0 ? 0 : 1;
+        for (int k = k0; k <= nz2; ++k) {
+          double re = a[(size_t)(j - 1) * nz1 + 2 * k], im = a[(size_t)(j - 1) * nz1 + 2 * k + 1];
+          aux += re * re + im * im;
+        }
+        if (pass == 1) aux = 2.0 * aux + r01 * r01;
+        part += aux * hy[j - 1];
+      }
+      red[threadIdx.x] = part;
+      __syncthreads();
+      for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) uner[1 + c] = red[0];
+      __syncthreads();
+      if (pass == 1 && threadIdx.x == 0) {
+        double a1 = 0.0, a2 = 0.0;
+        for (int k = 1; k <= nz2; ++k) {
+          a1 += a[2 * k] * a[2 * k] + a[2 * k + 1] * a[2 * k + 1];
+          a2 += a[nz1 + 2 * k] * a[nz1 + 2 * k] + a[nz1 + 2 * k + 1] * a[nz1 + 2 * k + 1];
+        }
+        uner[4 + 4 * c] = a[0];
+        uner[5 + 4 * c] = a[nz1];
+        uner[6 + 4 * c] = 2.0 * a1;
+        uner[7 + 4 * c] = 2.0 * a2;
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    for (int q = 1; q <= 15; ++q) ds->ener[q] = uner[q];
+    ds->ener[13] = ds->ener[14] = ds->ener[15] = 0.0; // rhsp.F90:487
+  }
+}
+
+// rhs = d(uv)/dy + i kz (uw) on all planes (rhsp.F90:164-169); the
+// d/dy part is differzy from v rows to u rows.
+__global__ void rhsu_loop1_kernel(const double *__restrict__ uv, const double *__restrict__ uw,
+                                  double *__restrict__ rhsu, DevCoef c, int ny, int nz1, int nplanes) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * nplanes) return;
+  int kr = (int)(cidx % nz1);
+  size_t ip = cidx / nz1;
+  size_t ps = (size_t)nz1 * (ny + 1);
+  double *o = rhsu + ip * ps + kr;
+  differzy_col(uv + ip * (size_t)nz1 * ny + kr, o, nz1, 0, ny, c);
+  const double kz = c.kaz[kr / 2];
+  const double *w = uw + ip * ps + (kr ^ 1);
+  const double sg = (kr & 1) ? 1.0 : -1.0;
+  for (int j = 1; j <= ny + 1; ++j) o[(size_t)(j - 1) * nz1] += sg * w[(size_t)(j - 1) * nz1] * kz;
+}
+
+// Save the velocities (rhsp.F90:281-285) and apply the pressure-gradient
+// predictor (rhsp.F90:291-339) in one pass over u-type elements.
+__global__ void save_predict_kernel(double *u, double *v, double *w, const double *__restrict__ p,
+                                    double *resu, double *resv, double *resw, const double *idyy,
+                                    const double *kaz, const DevScalars *ds, double idxx, int nx,
+                                    int ny, int nz1) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t pu = (size_t)nz1 * (ny + 1), pv = (size_t)nz1 * ny;
+  if (e >= pu * nx) return;
+  int i = (int)(e / pu) + 1;
+  size_t r = e % pu;
+  int j = (int)(r / nz1) + 1;
+  int kr = (int)(r % nz1);
+  const double var1 = ds->var1;
+  size_t ev = (size_t)(i - 1) * pv + (size_t)(j - 1) * nz1 + kr;
+  double uu = u[e], ww = w[e];
+  resu[e] = uu;
+  resw[e] = ww;
+  if (i >= 2 && i <= nx - 1 && j >= 2 && j <= ny) {
+    u[e] = uu - var1 * idxx * (p[ev + pv] - p[ev]);
+  }
+  if (i >= 2 && j >= 2 && j <= ny) {
+    // w -= var1 * (i kz p): Re gets -(-kz Im p), Im gets -(kz Re p)
+    double kz = kaz[kr / 2];
+    double pp = p[ev ^ 1];
+    double t = (kr & 1) ? (kz * pp) : (-kz * pp);
+    w[e] = ww - var1 * t;
+  }
+  if (j <= ny) {
+    double vv = v[ev];
+    resv[ev] = vv;
+    if (i >= 2 && j >= 2 && j <= ny - 1) v[ev] = vv - var1 * idyy[j] * (p[ev + nz1] - p[ev]);
+  }
+}
+
+// dt from the CFL limits (rhsp.F90:214-262, coeft.f90:432-454) and the RK3
+// constants of substep m (rhsp.F90:264-267, 479).
+__global__ void dt_kernel(DevScalars *ds, const unsigned long long *vmbits, const double *dymin,
+                          int ny, int m, int every_step, double dxmin, double dtret, double rex,
+                          double rkc, double rkd, double rkcv, double rkdv) {
+  if (threadIdx.x != 0) return;
+  // setstep is set at every step start (bl_1.F90:91); a frozen dt is only
+  // computed when none is known yet
+  if (m == 1 && (every_step || !ds->dt_valid)) {
+    const double poco = 1e-7;
+    double um = __longlong_as_double((long long)ds->umax_bits);
+    double wm = __longlong_as_double((long long)ds->wmax_bits);
+    double dt1 = dxmin / fmax(poco, um);
+    double dt2 = dxmin / fmax(poco, wm);
+    double dt3 = 1e300;
+    for (int j = 1; j <= ny; ++j) {
+      double vm = __longlong_as_double((long long)vmbits[j - 1]);
+      dt3 = fmin(dt3, dymin[j] / fmax(poco, vm));
+    }
+    ds->dt = fmin(fmin(dt1, dt2), fmin(dt3, dtret));
+    ds->dt_valid = 1;
+  }
+  double dt = ds->dt;
+  ds->var1 = dt * (rkcv + rkdv);
+  ds->var2 = dt * rex * rkcv;
+  ds->var3 = dt * rkc;
+  ds->var4 = dt * rkd;
+  ds->rkk = rex * dt * rkdv;
+  ds->varstep = 0.5 / dt / rkdv;
+  (void)m;
+}
+
+// genflu (genflu.F90:3-167, NEWGENFLU branch), one workgroup.  Reads the
+// recycling plane xout from the saved velocities (res*, equal to u, v, w
+// before the predictor) and rewrites plane 1 of u, v, w.
+__device__ void genflu_profile(double *ut, double *wk, int ntop, const int *kl, const double *fsca,
+                               int n, int ny, int nz1, double ggg) {
+  const int s = nz1, nk = nz1 - 2;
+  // the copy of row ntop-3 below needs ntop >= 4 (genflu.F90:232-234 reads
+  // out of bounds otherwise); such a profile is left unchanged
+  if (ntop < 4) return;
+  if (n == ny + 1) {
+    for (int e = threadIdx.x; e < ntop * nk; e += blockDim.x) {
+      int i = e / nk + 1, k = e % nk + 2, kll = kl[i];
+      wk[(size_t)i * s + k] = ut[(size_t)(kll + 1) * s + k] * fsca[i] + ut[(size_t)kll * s + k] * (1 - fsca[i]);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x + 2; k < s; k += blockDim.x) wk[k] = -wk[(size_t)s + k];
+  } else {
+    for (int e = threadIdx.x; e < ntop * nk; e += blockDim.x) {
+      int i = e / nk + 1, k = e % nk + 2, kll = kl[i];
+      wk[(size_t)(i - 1) * s + k] = ut[(size_t)kll * s + k] * fsca[i] + ut[(size_t)(kll - 1) * s + k] * (1 - fsca[i]);
+    }
+  }
+  __syncthreads();
+  int nrow = n - (ntop - 2) + 1;
+  for (int e = threadIdx.x; e < nrow * nk; e += blockDim.x) {
+    int i = ntop - 2 + e / nk, k = e % nk + 2;
+    wk[(size_t)(i - 1) * s + k] = wk[(size_t)(ntop - 4) * s + k];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * nk; e += blockDim.x) {
+    int i = e / nk + 1, k = e % nk + 2;
+    ut[(size_t)(i - 1) * s + k] = wk[(size_t)(i - 1) * s + k] * ggg;
+  }
+  __syncthreads();
+}
+
+__global__ void genflu_kernel(double *u, double *v, double *w, const double *resu, const double *resv,
+                              const double *resw, double *um, const double *u0, const double *v0,
+                              double *wk, int *ibuf, double *dbuf, DevCoef c, DevScalars *ds,
+                              int ny, int nz1, int xout, double re) {
+  const int s = nz1, ru = ny + 1;
+  double *u1 = u, *v1 = v, *w1 = w; // plane 1
+  const double *ux = resu + (size_t)(xout - 1) * s * ru, *vx = resv + (size_t)(xout - 1) * s * ny,
+               *wx = resw + (size_t)(xout - 1) * s * ru;
+  const int jy = 119 < ny ? 119 : ny; // y(119) clamped to the grid
+  const double yb = c.y[jy];
+  for (size_t e = threadIdx.x; e < (size_t)s * ru; e += blockDim.x) {
+    int j = (int)(e / s) + 1, k = (int)(e % s);
+    double f = (k >= 2) ? 0.5 * (1 - tanh(c.y[j] - yb)) : 1.0;
+    u1[e] = ux[e] * f;
+    w1[e] = wx[e] * f;
+  }
+  for (size_t e = threadIdx.x; e < (size_t)s * ny; e += blockDim.x) v1[e] = vx[e];
+  __syncthreads();
+  __shared__ double sh_dtc;
+  if (threadIdx.x == 0) {
+    double tloc = ds->tiempo - ds->timeinit;
+    double timec = tloc < 20 ? 20.0 : (tloc < 100 ? 100.0 : tloc);
+    ds->timec = timec;
+    sh_dtc = ds->dt / timec;
+  }
+  __syncthreads();
+  double dtc = sh_dtc;
+  for (int j = threadIdx.x + 1; j <= ny + 1; j += blockDim.x)
+    um[j - 1] = dtc * u1[(size_t)(j - 1) * s] + um[j - 1] * (1.0 - dtc);
+  __syncthreads();
+  // scalar part: thread 0
+  int *kl = ibuf;                 // kl[2][ny+1]
+  int *ntop = ibuf + 2 * (ny + 1); // ntop[2]
+  double *fsca = dbuf;            // fsca[2][ny+1]
+  double *ysca = dbuf + 2 * (ny + 1), *wlwa = dbuf + 3 * (ny + 1);
+  if (threadIdx.x == 0) {
+    const double *y = c.y, *dy = c.dy;
+    double utau = sqrt(fabs((um[1] - um[0]) / (y[2] - y[1]) / re));
+    double uinf = 0.0;
+    for (int j = ny + 1 - 20; j <= ny + 1 - 5; ++j) uinf += um[j - 1];
+    uinf /= 16.0;
+    double u99 = (double)0.99f * uinf;
+    int jtop = 0;
+    for (int j = 1; j <= ny - 2; ++j) {
+      if (um[j - 1] >= u99) break;
+      jtop = j + 1;
+    }
+    double sum = 0.0;
+    for (int j = 2; j <= jtop + 3; ++j) sum += (uinf - um[j - 1]) * um[j - 1] * dy[j - 1];
+    double rthout = sum * re / uinf;
+    double g2 = ds->rthin / rthout, g1 = pow(g2, 0.125);
+    ds->utauout = utau;
+    ds->dout = y[jtop];
+    ds->rthout = rthout;
+    ds->gamma1 = g1;
+    ds->gamma2 = g2;
+    for (int kk = 0; kk < 2; ++kk) {
+      for (int j = 1; j <= ny; ++j)
+        ysca[j] = (kk == 0) ? 0.5 * (y[j] + y[j + 1]) / ds->din : y[j] / ds->din;
+      // dobleu (genflu.F90:173-199)
+      const double ag = 4.0, bg = 0.2;
+      double bg2 = 1.0 - 2.0 * bg, tag = 1.0 / tanh(ag), www1 = (double)0.7895f;
+      wlwa[1] = 0.0;
+      for (int q = 2; q <= ny; ++q) {
+        double xxx = 0.5 * (ysca[q] + ysca[q - 1]);
+        double yyy = (xxx >= 1.0) ? 1.0 : 0.5 * (1.0 + tag * tanh(ag * (xxx - bg) / (bg2 * xxx + bg)));
+        wlwa[q] = wlwa[q - 1] + yyy * (ysca[q] - ysca[q - 1]);
+        if (ysca[q] >= 1.0 && ysca[q - 1] < 1.0) www1 = wlwa[q - 1] + (1.0 - ysca[q - 1]) * yyy;
+      }
+      for (int q = 1; q <= ny; ++q) wlwa[q] = ysca[q] / g1 + (1.0 / g2 - 1.0 / g1) / www1 * wlwa[q];
+      int kll = 1, khh;
+      ntop[kk] = 0;
+      for (int q = 1; q <= ny; ++q) {
+        kl[kk * (ny + 1) + q] = 0;
+        fsca[kk * (ny + 1) + q] = 0.0;
+      }
+      for (int q = 1; q <= ny; ++q) {
+        if (wlwa[q] <= ysca[ny]) {
+          ntop[kk] = q;
+          khh = ny;
+          while (khh - kll > 1) {
+            int km = (khh + kll) / 2;
+            if (ysca[km] >= wlwa[q]) khh = km; else kll = km;
+          }
+          kl[kk * (ny + 1) + q] = kll;
+          fsca[kk * (ny + 1) + q] = (wlwa[q] - ysca[kll]) / (ysca[khh] - ysca[kll]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // the zero mode (genflu.F90:146-150)
+  for (int j = threadIdx.x + 1; j <= ny + 1; j += blockDim.x) {
+    u1[(size_t)(j - 1) * s] = u0[j - 1];
+    u1[(size_t)(j - 1) * s + 1] = 0.0;
+    w1[(size_t)(j - 1) * s] = 0.0;
+    w1[(size_t)(j - 1) * s + 1] = 0.0;
+    if (j <= ny) {
+      v1[(size_t)(j - 1) * s] = v0[j - 1];
+      v1[(size_t)(j - 1) * s + 1] = 0.0;
+    }
+  }
+  __syncthreads();
+  double ggg = 1.0 / ds->gamma1;
+  genflu_profile(u1, wk, ntop[0], kl, fsca, ny + 1, ny, nz1, ggg);
+  genflu_profile(w1, wk, ntop[0], kl, fsca, ny + 1, ny, nz1, ggg);
+  genflu_profile(v1, wk, ntop[1], kl + (ny + 1), fsca + (ny + 1), ny, ny, nz1, ggg);
+}
+
+// z-viscous terms and the rest of rhsv/rhsw (rhsp.F90:355-415) for planes
+// 2..nx-1.  Columns (kr, i).
+__global__ void rhs_loop2_kernel(double *rhsu, double *rhsv, double *rhsw, const double *resu,
+                                 const double *resv, const double *resw, const double *vw,
+                                 const double *vv, const double *ww, double *scr, DevCoef c, int ny,
+                                 int nz1, int nplanes) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * nplanes) return;
+  int kr = (int)(cidx % nz1);
+  size_t ip = cidx / nz1 + 1; // plane index 0-based (planes 2..nx-1)
+  const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
+  const double kv = c.kvis[kr / 2], kz = c.kaz[kr / 2];
+  const double sg = (kr & 1) ? 1.0 : -1.0;
+  const int kp = kr ^ 1;
+  double *ru = rhsu + ip * pu + kr, *rv = rhsv + ip * pv + kr, *rw = rhsw + ip * pu + kr;
+  const double *su = resu + ip * pu + kr, *sv = resv + ip * pv + kr, *sw = resw + ip * pu + kr;
+  for (int j = 2; j <= ny; ++j) {
+    size_t o = (size_t)(j - 1) * s;
+    ru[o] -= su[o] * kv;
+    rw[o] -= sw[o] * kv;
+    if (j <= ny - 1) rv[o] -= sv[o] * kv;
+  }
+  // rhsv += i kz vw (rows 1..ny), then += d(vv)/dy
+  const double *vwc = vw + ip * pu;
+  for (int j = 1; j <= ny; ++j) rv[(size_t)(j - 1) * s] += sg * vwc[(size_t)(j - 1) * s + kp] * kz;
+  double *t = scr + ip * pu + kr;
+  differzy_col(vv + ip * pu + kr, t, s, 1, ny, c);
+  for (int j = 1; j <= ny; ++j) rv[(size_t)(j - 1) * s] += t[(size_t)(j - 1) * s];
+  // rhsw += d(vw)/dy, then += i kz ww (rows 1..ny)
+  differzy_col(vw + ip * pu + kr, t, s, 0, ny, c);
+  for (int j = 1; j <= ny + 1; ++j) rw[(size_t)(j - 1) * s] += t[(size_t)(j - 1) * s];
+  const double *wwc = ww + ip * pu;
+  for (int j = 1; j <= ny; ++j) rw[(size_t)(j - 1) * s] += sg * wwc[(size_t)(j - 1) * s + kp] * kz;
+}
+
+// boun (boun.f90:7-56) and the right-hand-side boundary rows
+// (rhsp.F90:421-445) for every plane.  Columns (kr, i).
+__global__ void boun_kernel(double *u, double *v, double *w, double *rhsu, double *rhsv, double *rhsw,
+                            DevCoef c, int ny, int nz1, int nx) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * nx) return;
+  int kr = (int)(cidx % nz1);
+  size_t ip = cidx / nz1;
+  const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
+  double *vi = v + ip * pv + kr;
+  vi[0] = 0.0;
+  vi[(size_t)(ny - 1) * s] = (kr == 0) ? c.vmagic[ip + 1] : 0.0;
+  const double i21 = -1.0 / M4(c.inby, 2, 1), c11 = -1.0 / M2(c.ccon, 1, 1);
+  double *f[2] = {u + ip * pu + kr, w + ip * pu + kr};
+  for (int q = 0; q < 2; ++q) {
+    double *a = f[q];
+    a[0] = i21 * (M4(c.inby, 2, 2) * a[s] + M4(c.inby, 2, 3) * a[2 * s] + M4(c.inby, 2, 4) * a[3 * s]);
+    a[(size_t)ny * s] = c11 * (M2(c.ccon, 1, 2) * a[(size_t)(ny - 1) * s] + M2(c.ccon, 1, 3) * a[(size_t)(ny - 2) * s] +
+                               M2(c.ccon, 1, 4) * a[(size_t)(ny - 3) * s]);
+  }
+  // rhs: plane 1 zero, v rows 1 and ny zero, u/w row 1 extrapolated, row ny+1 zero
+  double *rvi = rhsv + ip * pv + kr;
+  double *rf[2] = {rhsu + ip * pu + kr, rhsw + ip * pu + kr};
+  if (ip == 0) {
+    for (int j = 0; j < ny; ++j) rvi[(size_t)j * s] = 0.0;
+    for (int q = 0; q < 2; ++q)
+      for (int j = 0; j <= ny; ++j) rf[q][(size_t)j * s] = 0.0;
+  }
+  rvi[0] = 0.0;
+  rvi[(size_t)(ny - 1) * s] = 0.0;
+  for (int q = 0; q < 2; ++q) {
+    double *a = rf[q];
+    a[0] = i21 * (M4(c.inby, 2, 2) * a[s] + M4(c.inby, 2, 3) * a[2 * s] + M4(c.inby, 2, 4) * a[3 * s]);
+    a[(size_t)ny * s] = 0.0;
+  }
+}
+
+// Explicit RK3 combination with vistyy (rhsp.F90:448-475) and the implicit
+// y-viscous solve (rhsp.F90:479-484) for one component, columns (kr, i) over
+// planes 1..nx (plane 1: rhs_prev = rhs only).  wk: scratch of field size.
+__global__ void rk_final_kernel(double *u, const double *res, const double *rhs, double *rhsp,
+                                double *wk, DevCoef c, const double *f, const DevScalars *ds, int jee,
+                                int ny, int nz1, int nx) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * nx) return;
+  int kr = (int)(cidx % nz1);
+  size_t ip = cidx / nz1;
+  const size_t s = nz1, ps = s * jee;
+  const size_t off = ip * ps + kr;
+  const double *rh = rhs + off;
+  double *rp = rhsp + off;
+  if (ip == 0) {
+    for (int j = 0; j < jee; ++j) rp[(size_t)j * s] = rh[(size_t)j * s];
+    return;
+  }
+  double *uu = u + off, *t = wk + off;
+  const double var2 = (ip == (size_t)(nx - 1)) ? 0.0 : ds->var2;
+  const double var3 = ds->var3, var4 = ds->var4;
+  vistyy_col(res + off, t, s, jee, ny, c);
+  for (int j = 0; j < jee; ++j) {
+    size_t o = (size_t)j * s;
+    double r = rh[o];
+    t[o] = uu[o] - (var3 * r + var2 * t[o] + var4 * rp[o]);
+    rp[o] = r;
+  }
+  implzy_col(uu, t, s, jee, jee == ny + 1 ? c.vyui : c.vyvi, f);
+}
+
+// ---------------------------------------------------------------------------
+// outflow_correction (boun.f90:63-93), FP64, one workgroup
+// ---------------------------------------------------------------------------
+__global__ void outflow_kernel(double *u, const double *v, double *rhsupa, DevCoef c, DevScalars *ds,
+                               int nx, int ny, int nz1, double dx1, double idx) {
+  __shared__ double red[3][TPB];
+  const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
+  double a = 0.0, b1 = 0.0, b2 = 0.0;
+  for (int i = threadIdx.x + 2; i <= nx; i += blockDim.x) a += v[(size_t)(i - 1) * pv + (size_t)(ny - 1) * s];
+  for (int j = threadIdx.x + 2; j <= ny; j += blockDim.x) {
+    b1 += u[(size_t)(j - 1) * s] * c.dy[j - 1];
+    b2 += u[(size_t)(nx - 1) * pu + (size_t)(j - 1) * s] * c.dy[j - 1];
+  }
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b1;
+  red[2][threadIdx.x] = b2;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+    __syncthreads();
+  }
+  double sumax = red[0][0] * dx1;
+  sumax = sumax - red[1][0];
+  sumax = sumax + red[2][0];
+  double alpha = -sumax / (c.y[ny] - c.y[1]);
+  if (threadIdx.x == 0) ds->sumax = sumax;
+  for (int j = threadIdx.x + 2; j <= ny; j += blockDim.x) {
+    size_t e = (size_t)(nx - 1) * pu + (size_t)(j - 1) * s;
+    u[e] += alpha;
+    rhsupa[e] += alpha * idx;
+  }
+}
+
+void k_outflow(Box &b) {
+  LAUNCH(b, "outflow", 1, TPB, 0, outflow_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_RHSUPA], b.dc,
+         b.ds, b.nx, b.ny, b.nz1, b.hc.dx[1], b.hc.idx);
+}
+
+// ---------------------------------------------------------------------------
+// pois (pois.f90:11-189)
+// ---------------------------------------------------------------------------
+// divergence (pois.f90:41-54) into rest, planes 2..nx, rows 1..ny-1
+__global__ void div_kernel(const double *u, const double *v, const double *w, double *rest, DevCoef c,
+                           double idx, int nx, int ny, int nz1) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
+  size_t per = (size_t)(ny - 1) * s;
+  if (e >= per * (nx - 1)) return;
+  int i = (int)(e / per) + 2;
+  size_t r = e % per;
+  int j = (int)(r / s) + 1;
+  int kr = (int)(r % s);
+  double kz = c.kaz[kr / 2];
+  size_t ow = (size_t)(i - 1) * pu + (size_t)j * s;       // row j+1
+  size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s; // row j
+  double wp = w[ow + (kr ^ 1)];
+  double t = (kr & 1) ? wp * kz : -wp * kz;
+  double d = t + (v[ov + s + kr] - v[ov + kr]) * c.idy[j];
+  d = d + idx * (u[ow + kr] - u[ow - pu + kr]);
+  rest[ov + kr] = d;
+}
+
+// per-mode y tridiagonal (pois.f90:102-117, soltriy pois.f90:211-239) on
+// the DCT coefficients in rr (planes 2..nx), columns (kr, i); a2inv: scratch
+__global__ void poisson_y_kernel(double *rr, double *a2s, DevCoef c, int nx, int ny, int nz1) {
+  size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= (size_t)nz1 * (nx - 1)) return;
+  int kr = (int)(cidx % nz1);
+  int i = (int)(cidx / nz1) + 2;
+  const size_t s = nz1, pv = s * ny;
+  double *col = rr + (size_t)(i - 1) * pv + kr;
+  double *d2 = a2s + (size_t)(i - 1) * pv + kr;
+  const int ny1 = ny - 1;
+  if (kr == 1) { // the zero mode is real (pois.f90:102)
+    for (int j = 0; j < ny1; ++j) col[(size_t)j * s] = 0.0;
+    return;
+  }
+  const double kz2 = c.kaz2[kr / 2], km = c.kmod[i - 1];
+  const bool pin = (i == 2 && kr == 0);
+  if (pin) col[(size_t)(ny1 - 1) * s] = 0.0;
+  // LU as soltriy does it, forward elimination fused
+  double a2prev = C3(c.ayp, 2, 1) - kz2 - km;
+  if (pin && ny1 == 1) a2prev = 2.0 * C3(c.ayp, 2, 1);
+  double a3prev = C3(c.ayp, 3, 1);
+  double y = col[0];
+  for (int j = 2; j <= ny1; ++j) {
+    double d = 1.0 / a2prev;
+    d2[(size_t)(j - 2) * s] = d;
+    double a1 = -C3(c.ayp, 1, j) * d;
+    double a2 = (pin && j == ny1) ? 2.0 * C3(c.ayp, 2, j) : C3(c.ayp, 2, j) - kz2 - km;
+    a2 = a2 + a1 * a3prev;
+    y = col[(size_t)(j - 1) * s] + a1 * y;
+    col[(size_t)(j - 1) * s] = y;
+    a2prev = a2;
+    a3prev = C3(c.ayp, 3, j);
+  }
+  double x = y * (1.0 / a2prev);
+  col[(size_t)(ny1 - 1) * s] = x;
+  for (int j = ny1 - 1; j >= 1; --j) {
+    x = (col[(size_t)(j - 1) * s] - C3(c.ayp, 3, j) * x) * d2[(size_t)(j - 1) * s];
+    col[(size_t)(j - 1) * s] = x;
+  }
+}
+
+// projection (pois.f90:144-162): p, w, v, u from phi = rest
+__global__ void project_kernel(double *u, double *v, double *w, double *p, const double *rest, DevCoef c,
+                               const DevScalars *ds, double vstep, int use_m, double idxx, int nx, int ny,
+                               int nz1) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
+  if (e >= pu * nx) return;
+  int i = (int)(e / pu) + 1;
+  size_t r = e % pu;
+  int j = (int)(r / s) + 1;
+  int kr = (int)(r % s);
+  if (i < 2) return;
+  const double varstep = use_m ? ds->varstep : vstep;
+  const double *ph = rest + (size_t)(i - 1) * pv;
+  if (j >= 2 && j <= ny) {
+    size_t o = (size_t)(j - 2) * s; // phi row j-1
+    size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s;
+    p[ov + kr] = varstep * ph[o + kr] + p[ov + kr];
+    double kz = c.kaz[kr / 2];
+    double pp = ph[o + (kr ^ 1)];
+    double t = (kr & 1) ? kz * pp : -kz * pp;
+    w[e] -= t;
+    if (j <= ny - 1) v[ov + kr] -= c.idyy[j] * (ph[(size_t)(j - 1) * s + kr] - ph[o + kr]);
+  }
+  if (i <= nx - 1 && j >= 2 && j <= ny) {
+    const double *pn = rest + (size_t)i * pv;
+    size_t o = (size_t)(j - 2) * s + kr; // phi row j-1 for u row j
+    u[e] -= idxx * (pn[o] - ph[o]);
+  }
+}
+
+void k_pois(Box &b, int m, double varstep, bool use_m) {
+  const int nx = b.nx, ny = b.ny, nz1 = b.nz1;
+  (void)m;
+  size_t ndiv = (size_t)(ny - 1) * nz1 * (nx - 1);
+  LAUNCH(b, "div", nblk(ndiv), TPB, 0, div_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest, b.dc,
+         b.hc.idx, nx, ny, nz1);
+  dct_apply(b, b.dct2, b.rest, b.rest2);
+  size_t ncol = (size_t)nz1 * (nx - 1);
+  LAUNCH(b, "poisson_y", nblk(ncol), TPB, 0, poisson_y_kernel, b.rest2, b.rest, b.dc, nx, ny, nz1);
+  dct_apply(b, b.dct3, b.rest2, b.rest);
+  LAUNCH(b, "project", nblk(b.nu), TPB, 0, project_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W],
+         b.f[OTBL_P], b.rest, b.dc, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1);
+}
+
+// ---------------------------------------------------------------------------
+// rhsp (rhsp.F90:22-489)
+// ---------------------------------------------------------------------------
+void k_substep_rhsp(Box &b, int m) {
+  const int nx = b.nx, ny = b.ny, nz1 = b.nz1;
+  const int ru = ny + 1;
+  double *u = b.f[OTBL_U], *v = b.f[OTBL_V], *w = b.f[OTBL_W], *p = b.f[OTBL_P];
+  FFTPlan plan;
+  plan.n = b.nz;
+  plan.nfac = b.nfac;
+  for (int q = 0; q < b.nfac; ++q) plan.fac[q] = b.fac[q];
+
+  LAUNCH(b, "reset_max", 1, TPB, 0, reset_max_kernel, b.ds, b.vmbits, ny);
+  // x interpolation (rhsp.F90:105-107)
+  k_interpxx(b, u, b.wki1, ru, 1);
+  k_interpxx(b, v, b.wki2, ny, 0);
+  k_interpxx(b, w, b.wki3, ru, 0);
+  if (m == 1)
+    LAUNCH(b, "energies", 1, TPB, 0, energies_kernel, u, v, w, b.dc.hy, b.ds, nx, ny, nz1, b.nz2);
+  // u interpolated to v rows in spectral space (interpyy is linear along y and
+  // commutes with the z transform of rhsp.F90:122,146)
+  k_interpyy(b, u, b.yc, 0, 1, nx, b.pu, b.pu);
+  // plane loop 1 products
+  size_t shm = 4 * (size_t)b.nz * sizeof(double2);
+  LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel, u, b.wki1, b.wki2, b.wki3, b.yc,
+         plan, (const double2 *)b.tw, ny, nz1, b.nz2, b.ds, b.vmbits);
+  LAUNCH(b, "rhsu_loop1", nblk((size_t)nz1 * nx), TPB, 0, rhsu_loop1_kernel, b.wki2, b.wki3, b.rhsu, b.dc,
+         ny, nz1, nx);
+  // x derivatives (rhsp.F90:181-204)
+  k_difvisxx(b, b.wki1, u, b.rhsu, ru, 1, 1);
+  k_difvisxx(b, b.wki2, v, b.rhsv, ny, 0, 0);
+  k_difvisxx(b, b.wki3, w, b.rhsw, ru, 0, 0);
+  // time step and RK constants
+  LAUNCH(b, "dt", 1, 64, 0, dt_kernel, b.ds, b.vmbits, b.dc.dymin, ny, m,
+         b.dt_mode == OTBL_DT_EVERY_STEP ? 1 : 0, b.hc.dxmin, b.hc.dtret, b.hc.rex, b.hc.rkc[m],
+         b.hc.rkd[m], b.hc.rkcv[m], b.hc.rkdv[m]);
+  // save velocities, pressure-gradient predictor
+  LAUNCH(b, "save_predict", nblk(b.nu), TPB, 0, save_predict_kernel, u, v, w, p, b.resu, b.resv, b.resw,
+         b.dc.idyy, b.dc.kaz, b.ds, b.hc.idxx, nx, ny, nz1);
+  // recycling inflow (rhsp.F90:287)
+  LAUNCH(b, "genflu", 1, TPB, 0, genflu_kernel, u, v, w, b.resu, b.resv, b.resw, b.um, b.u0, b.v0, b.gwork,
+         (int *)(b.gwork + (size_t)nz1 * (ny + 1)), b.gwork + (size_t)nz1 * (ny + 1) + 4 * (ny + 2), b.dc, b.ds,
+         ny, nz1, b.xout, b.re);
+  // plane loop 2 (planes 2..nx-1)
+  if (nx > 2) {
+    k_interpyy(b, b.resw, b.ya, 0, 2, nx - 2, b.pu, b.pu);
+    k_interpyy(b, b.resv, b.yb, 1, 2, nx - 2, b.pv, b.pu);
+    LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, b.resv, b.resw, b.ya, b.yb, b.yc,
+           plan, (const double2 *)b.tw, ny, nz1, b.nz2);
+    LAUNCH(b, "rhs_loop2", nblk((size_t)nz1 * (nx - 2)), TPB, 0, rhs_loop2_kernel, b.rhsu, b.rhsv, b.rhsw,
+           b.resu, b.resv, b.resw, b.ya, b.yb, b.yc, b.wki1, b.dc, ny, nz1, nx - 2);
+  }
+  // boundary conditions (rhsp.F90:421-445)
+  LAUNCH(b, "boun", nblk((size_t)nz1 * nx), TPB, 0, boun_kernel, u, v, w, b.rhsu, b.rhsv, b.rhsw, b.dc, ny,
+         nz1, nx);
+  // implicit factors for this dt, then the final update per component
+  LAUNCH(b, "impl_factor", 1, 64, 0, impl_factor_kernel, b.implu, b.implv, b.dc, ny, b.ds, 0.0, 0);
+  size_t ncol = (size_t)nz1 * nx;
+  LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, v, b.resv, b.rhsv, b.f[OTBL_RHSVPA], b.wki2, b.dc,
+         b.implv, b.ds, ny, ny, nz1, nx);
+  LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, u, b.resu, b.rhsu, b.f[OTBL_RHSUPA], b.wki1, b.dc,
+         b.implu, b.ds, ny + 1, ny, nz1, nx);
+  LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, w, b.resw, b.rhsw, b.f[OTBL_RHSWPA], b.wki3, b.dc,
+         b.implu, b.ds, ny + 1, ny, nz1, nx);
+}
+
+void k_setup_attrs(Box &b) {
+  size_t shm = 4 * (size_t)b.nz * sizeof(double2);
+  OTBL_HIP(hipFuncSetAttribute((const void *)products1_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  OTBL_HIP(hipFuncSetAttribute((const void *)products2_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+}
+
+__global__ void end_step_kernel(DevScalars *ds) {
+  if (threadIdx.x == 0) ds->tiempo += ds->dt;
+}
+
+void k_end_step(Box &b) { LAUNCH(b, "end_step", 1, 64, 0, end_step_kernel, b.ds); }
+
+} // namespace otbl

@@ -1,0 +1,246 @@
+"""ctypes binding of libopentbl_amd.so (the C ABI in include/opentbl.h) and a
+small host-side mirror of the reference's driver interface.
+
+The library is loaded from this package directory; if it has not been built
+(python -m opentbl_amd.build) every entry point raises, there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libopentbl_amd.so")
+_LIB = None
+
+D, I, SZ = C.c_double, C.c_int, C.c_size_t
+PD = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+OTBL_FP64, OTBL_MIXED = 0, 1
+OTBL_DT_EVERY_STEP, OTBL_DT_FROZEN = 0, 1
+FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "rhsupa": 4, "rhsvpa": 5, "rhswpa": 6}
+
+
+class Params(C.Structure):
+    _fields_ = [("nx", I), ("ny", I), ("nz", I), ("xout", I), ("re", D), ("ax", D),
+                ("az", D), ("cfl", D), ("precision", I), ("dt_mode", I)]
+
+
+# exported entry points: name -> (restype, argtypes)
+SIGNATURES = {
+    "otbl_create": (I, [C.POINTER(Params), PD, C.POINTER(VP)]),
+    "otbl_destroy": (I, [VP]),
+    "otbl_last_error": (C.c_char_p, []),
+    "otbl_set_field": (I, [VP, I, PD]),
+    "otbl_get_field": (I, [VP, I, PD]),
+    "otbl_field_len": (SZ, [VP, I]),
+    "otbl_set_inflow": (I, [VP, PD, PD, PD]),
+    "otbl_get_um": (I, [VP, PD]),
+    "otbl_get_genflu": (I, [VP, PD]),
+    "otbl_get_ener": (I, [VP, PD]),
+    "otbl_set_time": (I, [VP, D, D, D]),
+    "otbl_get_time": (I, [VP, PD, PD]),
+    "otbl_get_coef": (I, [VP, C.c_char_p, PD, C.POINTER(I), C.POINTER(I)]),
+    "otbl_rhsp": (I, [VP, I]),
+    "otbl_outflow_correction": (I, [VP]),
+    "otbl_pois": (I, [VP, D]),
+    "otbl_pois_m": (I, [VP, I]),
+    "otbl_substep": (I, [VP, I]),
+    "otbl_step": (I, [VP, I]),
+    "otbl_use_graphs": (I, [VP, I]),
+    "otbl_sync": (I, [VP]),
+    "otbl_interpxx": (I, [VP, VP, VP, I, I]),
+    "otbl_difvisxx": (I, [VP, VP, VP, VP, I, I]),
+    "otbl_interpyy": (I, [VP, VP, VP, I, I]),
+    "otbl_differzy": (I, [VP, VP, VP, I, I]),
+    "otbl_vistyy": (I, [VP, VP, VP, I, I]),
+    "otbl_implzy": (I, [VP, VP, VP, I, D, I]),
+    "otbl_fourxz": (I, [VP, VP, VP, I, I]),
+    "otbl_cosftx": (I, [VP, VP, VP, I]),
+    "otbl_dev_alloc": (I, [SZ, C.POINTER(VP)]),
+    "otbl_dev_free": (I, [VP]),
+    "otbl_h2d": (I, [VP, VP, SZ]),
+    "otbl_d2h": (I, [VP, VP, SZ]),
+    "otbl_kprof": (I, [VP, I]),
+    "otbl_kprof_report": (I, [VP, C.c_char_p, SZ]),
+}
+
+
+class OtblError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise OtblError(f"{LIB_PATH} is not built: run python -m opentbl_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().otbl_last_error().decode()
+        raise OtblError(f"{what}: status {rc}: {msg}")
+
+
+def _pd(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(PD)
+
+
+class DevArray:
+    """A float64 device buffer managed through the C ABI."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = VP()
+        check(lib().otbl_dev_alloc(self.n * 8, C.byref(p)), "dev_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a).view(np.float64).ravel()
+        d = cls(a.size)
+        check(lib().otbl_h2d(d.ptr, a.ctypes.data_as(VP), a.size * 8), "h2d")
+        return d
+
+    def to_host(self):
+        out = np.empty(self.n, np.float64)
+        check(lib().otbl_d2h(out.ctypes.data_as(VP), self.ptr, self.n * 8), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().otbl_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class GpuBox:
+    """One box of the MI355X substep, set up from an ic.make_state() dict."""
+
+    def __init__(self, cfg, state, precision=OTBL_FP64, dt_mode=OTBL_DT_EVERY_STEP, graphs=False):
+        L = lib()
+        self.cfg = cfg
+        prm = Params(cfg.nx, cfg.ny, cfg.nz, cfg.xout, state["re"], cfg.ax, cfg.az, cfg.cfl,
+                     precision, dt_mode)
+        y = np.ascontiguousarray(state["y"], np.float64)
+        h = VP()
+        check(L.otbl_create(C.byref(prm), _pd(y), C.byref(h)), "otbl_create")
+        self.h = h
+        for name in ("u", "v", "w", "p"):
+            self.set(name, state[name])
+        for name in ("rhsupa", "rhsvpa", "rhswpa"):
+            if name in state:
+                self.set(name, state[name])
+        u0 = np.ascontiguousarray(state["u0"], np.float64)
+        v0 = np.ascontiguousarray(state["v0"], np.float64)
+        um = np.ascontiguousarray(state["um"], np.float64)
+        check(L.otbl_set_inflow(self.h, _pd(u0), _pd(v0), _pd(um)), "set_inflow")
+        check(L.otbl_set_time(self.h, state.get("tiempo", 0.0), state.get("timeinit", 0.0),
+                              state.get("dt", 0.0)), "set_time")
+        check(L.otbl_use_graphs(self.h, int(graphs)), "use_graphs")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().otbl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, name):
+        c = self.cfg
+        R = c.ny if name in ("v", "p", "rhsvpa") else c.ny + 1
+        return (c.nx, R, c.mz)
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, np.complex128).reshape(self._shape(name))
+        check(lib().otbl_set_field(self.h, FIELDS[name], _pd(a.view(np.float64))), "set_field")
+
+    def get(self, name):
+        out = np.empty(self._shape(name), np.complex128)
+        check(lib().otbl_get_field(self.h, FIELDS[name], _pd(out.view(np.float64))), "get_field")
+        return out
+
+    def rhsp(self, m):
+        check(lib().otbl_rhsp(self.h, m), "rhsp")
+
+    def outflow_correction(self):
+        check(lib().otbl_outflow_correction(self.h), "outflow_correction")
+
+    def pois(self, varstep):
+        check(lib().otbl_pois(self.h, varstep), "pois")
+
+    def pois_m(self, m):
+        check(lib().otbl_pois_m(self.h, m), "pois_m")
+
+    def substep(self, m):
+        check(lib().otbl_substep(self.h, m), "substep")
+
+    def step(self, n=1):
+        check(lib().otbl_step(self.h, n), "step")
+
+    def use_graphs(self, on):
+        check(lib().otbl_use_graphs(self.h, int(on)), "use_graphs")
+
+    def sync(self):
+        check(lib().otbl_sync(self.h), "sync")
+
+    def time(self):
+        t, d = D(), D()
+        check(lib().otbl_get_time(self.h, C.byref(t), C.byref(d)), "get_time")
+        return t.value, d.value
+
+    def um(self):
+        o = np.zeros(self.cfg.ny + 1)
+        check(lib().otbl_get_um(self.h, _pd(o)), "get_um")
+        return o
+
+    def genflu_info(self):
+        o = np.zeros(6)
+        check(lib().otbl_get_genflu(self.h, _pd(o)), "get_genflu")
+        return dict(zip(("rthin", "din", "rthout", "dout", "gamma1", "gamma2"), o))
+
+    def ener(self):
+        o = np.zeros(15)
+        check(lib().otbl_get_ener(self.h, _pd(o)), "get_ener")
+        return o
+
+    def coef(self, name):
+        r, c = I(), I()
+        check(lib().otbl_get_coef(self.h, name.encode(), None, C.byref(r), C.byref(c)), "get_coef")
+        out = np.zeros(r.value * c.value)
+        check(lib().otbl_get_coef(self.h, name.encode(), _pd(out), C.byref(r), C.byref(c)), "get_coef")
+        return out.reshape(c.value, r.value).T if r.value > 1 else out
+
+    def kprof(self, on):
+        check(lib().otbl_kprof(self.h, int(on)), "kprof")
+
+    def kprof_report(self):
+        buf = C.create_string_buffer(1 << 16)
+        check(lib().otbl_kprof_report(self.h, buf, len(buf)), "kprof_report")
+        rows = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            rows[name] = (int(n), float(ms))
+        return rows
+
+    def op(self, name, *args):
+        check(getattr(lib(), "otbl_" + name)(self.h, *args), name)
