@@ -1,0 +1,169 @@
+"""Throughput of the RK3 substep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
+
+A step is one RK3 step (3 substeps: rhsp, outflow_correction, pois) of the
+whole box, with every field resident in HBM; each substep runs as one
+captured hipGraph.  The metric is grid-point substeps per second,
+nx*ny*nz*3*K / wall time / 1e6 (Mpts*substep/s).  One JSON line is printed;
+per-kernel timings (HIP events on the library's stream) go to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from opentbl_amd import config, ic  # noqa: E402
+from opentbl_amd.lib import GpuBox, OTBL_DT_EVERY_STEP, OTBL_FP64  # noqa: E402
+
+METRIC = "Mpts*RK3-substep/s, mixed vs FP64 on 1 GPU at <=1e-5 rel err; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_model(cfg):
+    """Algorithmic bytes (each distinct array read once / written once) and
+    flops per launch, FP64, for the kernels of one substep.  Returned as
+    name -> (bytes per launch, flops per launch, launches per substep)."""
+    X, Ru, Rv, z, s = cfg.nx, cfg.ny + 1, cfg.ny, cfg.nz1, 8
+    nz = cfg.nz
+    f = X * z * s  # bytes of one row-plane column set (per row of a field)
+    m = {}
+    m["interpxx"] = ((2 * (Ru + Rv + Ru) * f) / 3, 0.0, 3)
+    m["difvisxx"] = (((4 * Ru + 3 * Rv + 3 * Ru) * f) / 3, 0.0, 3)
+    m["products1"] = (X * (Rv * 8 + 5) * z * s, X * (Rv * 5 + 3) * 5 * nz * math.log2(nz) * 2, 1)
+    m["products2"] = ((X - 2) * Rv * 7 * z * s, (X - 2) * Rv * 4 * 5 * nz * math.log2(nz) * 2, 1)
+    m["interpyy"] = ((2 * Ru * X + (Ru + Rv) * (X - 2) + (Rv + Ru) * (X - 2)) * z * s / 3, 0.0, 3)
+    m["rhsu_loop1"] = ((Rv + 2 * Ru) * f, 0.0, 1)
+    m["rhs_loop2"] = ((X - 2) * z * s * (2 * Ru + 2 * Rv + 2 * Ru + Ru + Rv + Ru + Ru + Rv), 0.0, 1)
+    m["rk_final"] = ((6 * Ru + 6 * Rv + 6 * Ru) * f / 3, 0.0, 3)
+    m["save_predict"] = ((3 * 3 * Ru + Rv) * f / 1, 0.0, 1)  # u,v,w read, res write, u,v,w write, p
+    N, L = cfg.nx - 1, (cfg.ny - 1) * z
+    m["dct"] = (2 * N * L * s, 2.0 * N * N * L, 2)
+    m["div"] = ((Ru + Rv + Ru + Rv) * X * z * s, 0.0, 1)
+    m["poisson_y"] = (2 * (X - 1) * (cfg.ny - 1) * z * s, 0.0, 1)
+    m["project"] = ((Rv + 2 * (Ru + Rv + Ru + Rv)) * X * z * s, 0.0, 1)
+    m["boun"] = (2 * 4 * (Ru + Rv) * X * z * s / (cfg.ny), 0.0, 1)
+    return m
+
+
+def cpu_baseline(cfg_full):
+    """The CPU FP64 oracle on a bounded sample of the same workload: one
+    substep of an nx=32 slab with the box's ny, nz (single thread)."""
+    from dataclasses import replace
+    from oracle.oracle import OracleBox, build
+    build()
+    slab = replace(cfg_full, nx=32, name=cfg_full.name + "-slab32",
+                   kzmax=min(cfg_full.kzmax, 8)).check()
+    st = ic.make_state(slab)
+    ob = OracleBox(slab, st)
+    t0 = time.perf_counter()
+    ob.substep(1)
+    dt = time.perf_counter() - t0
+    return {"value": slab.points / dt / 1e6, "unit": "Mpts*substep/s", "cores": 1, "kind": "port",
+            "sample": f"one RK3 substep of a {slab.nx}x{slab.ny}x{slab.nz} slab of the "
+                      f"{cfg_full.nx}x{cfg_full.ny}x{cfg_full.nz} box, CPU FP64 oracle, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=sorted(config.CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one box on one GPU; --gpus {args.gpus} is not "
+              "supported (the x-slab multi-GPU layout is design only)", file=sys.stderr)
+        sys.exit(2)
+    cfg = config.CONFIGS[args.workload]
+    log(f"[bench] workload {cfg.name} {cfg.nx}x{cfg.ny}x{cfg.nz}: generating the state")
+    t0 = time.perf_counter()
+    st = ic.make_state(cfg)
+    log(f"[bench] state ready in {time.perf_counter() - t0:.1f} s; uploading")
+    box = GpuBox(cfg, st, precision=OTBL_FP64, dt_mode=OTBL_DT_EVERY_STEP,
+                 graphs=not args.no_graphs)
+    del st
+    for w in range(args.warmup):
+        box.step(1)
+        box.sync()
+        log(f"[bench] warmup step {w + 1}/{args.warmup} done")
+    box.sync()
+    t0 = time.perf_counter()
+    box.step(args.steps)
+    box.sync()
+    wall = time.perf_counter() - t0
+    log(f"[bench] timed {args.steps} steps in {wall:.3f} s")
+    u = box.get("u")
+    if not np.isfinite(u).all():
+        raise SystemExit("bench.py: non-finite velocity after the timed steps")
+    del u
+    # per-kernel durations: one more step, kernels launched one by one and
+    # bracketed by HIP events on the library's stream
+    box.kprof(True)
+    box.step(1)
+    box.sync()
+    rep = box.kprof_report()
+    box.kprof(False)
+    model = kernel_model(cfg)
+    rows = []
+    for name, (n, ms) in sorted(rep.items(), key=lambda kv: -kv[1][1]):
+        avg = ms / n
+        by, fl, _ = model.get(name, (None, None, None))
+        gbs = by / (avg * 1e-3) / 1e9 if by else None
+        tfs = fl / (avg * 1e-3) / 1e12 if fl else None
+        rows.append((name, n, ms, avg, gbs, tfs))
+        log(f"[kprof] {name:14s} launches {n:3d} total {ms:9.3f} ms avg {avg:9.4f} ms"
+            + (f"  {gbs:8.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:5.1f}% HBM)" if gbs else "")
+            + (f"  {tfs:6.2f} TFLOP/s" if tfs else ""))
+    dom = next(r for r in rows if r[0] in model)
+    name, n, ms, avg, gbs, tfs = dom
+    by, fl, _ = model[name]
+    intensity = fl / by if fl else 0.0
+    if intensity > FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9):
+        roof = {"bound": "mfma", "achieved": tfs, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tfs / FP64_PEAK_TFLOPS, "traffic": None}
+    else:
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS, "traffic": None}
+    roof["kernel"] = name
+    roof["avg_ms"] = avg
+    roof["algorithmic_bytes"] = by
+    steps = args.steps
+    value = cfg.points * 3 * steps / wall / 1e6
+    out = {
+        "metric": METRIC, "value": value, "unit": "Mpts*substep/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{cfg.name}: {cfg.nx}x{cfg.ny}x{cfg.nz} ZPG boundary layer, "
+                               f"Re_theta_in={cfg.re_theta:g}, FP64 path, one hipGraph per substep",
+                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": "fp64",
+                   "graphs": not args.no_graphs, "dt": "CFL every step"},
+        "roofline": roof,
+    }
+    box.close()
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle on a slab)")
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
