@@ -59,5 +59,38 @@ def build(verbose=False):
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "otbl_driver")
+
+
+def flang():
+    for c in (os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "amdflang"), "amdflang", "flang"):
+        if os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran():
+    """The ISO_C_BINDING layer and the bl_1-style driver, linked against the
+    library (amdflang).  Returns the driver path, or None without a Fortran
+    compiler."""
+    fc = flang()
+    if fc is None:
+        return None
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    mod_src = os.path.join(FORTRAN, "otbl_gpu.f90")
+    obj = os.path.join(bdir, "otbl_gpu.o")
+    cmds = [[fc, "-O2", "-c", mod_src, "-o", obj, "-J", bdir],
+            [fc, "-O2", os.path.join(FORTRAN, "main.F90"), obj, "-I", bdir, "-J", bdir,
+             "-L", HERE, "-lopentbl_amd", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER]]
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran())

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -150,6 +151,76 @@ static void upload_coef(Box &b) {
   OTBL_HIP(hipMemcpy(b.coefbuf, host.data(), total * sizeof(double), hipMemcpyHostToDevice));
 }
 
+static int good_length(int m) {
+  for (;; ++m) {
+    int x = m;
+    for (int p : {2, 3, 5, 7})
+      while (x % p == 0) x /= p;
+    if (x == 1) return m;
+  }
+}
+
+static FFTPlan make_plan(int n) {
+  FFTPlan p{};
+  p.n = n;
+  p.nfac = small_factors(n, p.fac);
+  if (p.nfac < 0 || p.nfac > 14) throw std::invalid_argument("transform length must factor into 2, 3, 5, 7");
+  return p;
+}
+
+// Bluestein tables for the length-N real-to-real x transforms
+// (ffts.f90:124-181): chirp exp(-i pi n^2/N), the transform of the
+// conjugate chirp wrapped to length M (divided by M), and exp(-i pi k/2N).
+static void build_dct_fft(Box &b) {
+  const int N = b.nx - 1;
+  const int M = good_length(2 * N - 1);
+  const double pi = b.hc.pi;
+  b.dct_M = M;
+  b.dct_plan = make_plan(M);
+  std::vector<double> chirp(2 * (size_t)N), bm(2 * (size_t)M, 0.0), bf(2 * (size_t)M), tw4(2 * (size_t)N);
+  for (int n = 0; n < N; ++n) {
+    long q = ((long)n * n) % (2L * N);
+    chirp[2 * n] = std::cos(pi * q / N);
+    chirp[2 * n + 1] = -std::sin(pi * q / N);
+  }
+  for (int t = 0; t < N; ++t) {
+    bm[2 * t] = chirp[2 * t];
+    bm[2 * t + 1] = -chirp[2 * t + 1];
+    if (t > 0) {
+      bm[2 * (M - t)] = chirp[2 * t];
+      bm[2 * (M - t) + 1] = -chirp[2 * t + 1];
+    }
+  }
+  std::vector<double> ct(M), st(M);
+  for (int t = 0; t < M; ++t) {
+    ct[t] = std::cos(2.0 * pi * t / M);
+    st[t] = std::sin(2.0 * pi * t / M);
+  }
+  for (int m = 0; m < M; ++m) {
+    double sr = 0.0, si = 0.0;
+    for (int t = 0; t < M; ++t) {
+      double br = bm[2 * t], bi = bm[2 * t + 1];
+      if (br == 0.0 && bi == 0.0) continue;
+      int a = (int)(((long)m * t) % M);
+      // exp(-2 pi i m t / M)
+      sr += br * ct[a] + bi * st[a];
+      si += bi * ct[a] - br * st[a];
+    }
+    bf[2 * m] = sr / M;
+    bf[2 * m + 1] = si / M;
+  }
+  for (int k = 0; k < N; ++k) {
+    tw4[2 * k] = std::cos(pi * k / (2.0 * N));
+    tw4[2 * k + 1] = -std::sin(pi * k / (2.0 * N));
+  }
+  dalloc(b.dct_chirp, chirp.size());
+  dalloc(b.dct_bf, bf.size());
+  dalloc(b.dct_tw4, tw4.size());
+  OTBL_HIP(hipMemcpy(b.dct_chirp, chirp.data(), chirp.size() * sizeof(double), hipMemcpyHostToDevice));
+  OTBL_HIP(hipMemcpy(b.dct_bf, bf.data(), bf.size() * sizeof(double), hipMemcpyHostToDevice));
+  OTBL_HIP(hipMemcpy(b.dct_tw4, tw4.data(), tw4.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
 static void build_dct(Box &b) {
   const int N = b.nx - 1;
   std::vector<double> c2((size_t)N * N), c3((size_t)N * N);
@@ -175,7 +246,7 @@ static void free_box(Box *b) {
     if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
   double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
-                    b->implv, b->um, b->u0, b->v0, b->gwork, b->tw};
+                    b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
@@ -189,8 +260,6 @@ static void free_box(Box *b) {
   if (b->s) (void)hipStreamDestroy(b->s);
   delete b;
 }
-
-void k_setup_attrs(Box &b);
 
 static void enqueue_substep(Box &b, int m) {
   k_substep_rhsp(b, m);
@@ -279,9 +348,12 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   b->precision = prm->precision;
   b->dt_mode = prm->dt_mode;
   b->re = prm->re; b->ax = prm->ax; b->az = prm->az; b->cfl = prm->cfl;
-  b->nfac = small_factors(b->nz, b->fac);
-  need(b->nfac > 0, "nz must factor into 2, 3, 5 and 7");
-  need((size_t)4 * b->nz * 16 <= 160 * 1024, "nz too large for the FP64 LDS transform");
+  {
+    int fac[32];
+    need(small_factors(b->nz, fac) > 0, "nz must factor into 2, 3, 5 and 7");
+  }
+  b->zplan = make_plan(b->nz);
+  need(k_zfft_shm_bytes(b->nz, 4) <= 160 * 1024, "nz too large for the FP64 LDS transform");
   b->pu = (size_t)b->nz1 * (b->ny + 1);
   b->pv = (size_t)b->nz1 * b->ny;
   b->nu = b->pu * b->nx;
@@ -304,15 +376,13 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   OTBL_HIP(hipMemset(b->ds, 0, sizeof(DevScalars)));
   dalloc(b->vmbits, b->ny + 1);
   {
-    std::vector<double> tw(2 * (size_t)b->nz);
-    for (int t = 0; t < b->nz; ++t) {
-      tw[2 * t] = std::cos(2.0 * b->hc.pi * t / b->nz);
-      tw[2 * t + 1] = std::sin(2.0 * b->hc.pi * t / b->nz);
-    }
-    dalloc(b->tw, tw.size());
-    OTBL_HIP(hipMemcpy(b->tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+    const int M = good_length(2 * (b->nx - 1) - 1);
+    b->dct_fft = k_dct_shm_bytes(M) <= 160 * 1024 && std::getenv("OTBL_DCT_GEMM") == nullptr;
   }
-  build_dct(*b);
+  if (b->dct_fft)
+    build_dct_fft(*b);
+  else
+    build_dct(*b);
   k_setup_attrs(*b);
   k_factor_coef(*b, b->coefbuf);
   OTBL_HIP(hipStreamSynchronize(b->s));

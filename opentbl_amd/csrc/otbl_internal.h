@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "opentbl.h"
+#include "otbl_fft.h"
 
 namespace otbl {
 
@@ -83,9 +84,15 @@ struct Box {
   double *um = nullptr, *u0 = nullptr, *v0 = nullptr, *gwork = nullptr;
   DevScalars *ds = nullptr;
   unsigned long long *vmbits = nullptr; // per-row max |v_x|
-  // FFT twiddles e^{2 pi i t / nz}
-  double *tw = nullptr;
-  int nfac = 0, fac[16] = {0};
+  // z transform plan (length nz)
+  FFTPlan zplan{};
+  // Poisson x transforms: Bluestein length M and tables (complex as
+  // double pairs), or the dense matrices when FP64 buffers of length M do
+  // not fit in LDS
+  bool dct_fft = false;
+  int dct_M = 0;
+  FFTPlan dct_plan{};
+  double *dct_chirp = nullptr, *dct_bf = nullptr, *dct_tw4 = nullptr;
   // graphs
   bool use_graphs = false;
   hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -113,7 +120,9 @@ void k_substep_rhsp(Box &b, int m);
 void k_outflow(Box &b);
 void k_pois(Box &b, int m, double varstep, bool use_m);
 void k_end_step(Box &b);
-void k_set_inflow_dev(Box &b);
+void k_setup_attrs(Box &b);
+size_t k_dct_shm_bytes(int M);
+size_t k_zfft_shm_bytes(int nz, int nbuf);
 
 // launch bookkeeping (otbl_api.cpp)
 void kbegin(Box &b, const char *name);

@@ -13,6 +13,7 @@
 // The z transforms work on one (j, i) row per workgroup in LDS (mixed-radix
 // Stockham, radices 8/4/2/3/5/7), two real signals per complex transform.
 #include "otbl_internal.h"
+#include "otbl_fft.h"
 
 #include <cmath>
 #include <cstdio>
@@ -418,96 +419,33 @@ void k_implzy_op(Box &b, double *u, const double *res, int R, double rkk, int np
 }
 
 // ---------------------------------------------------------------------------
-// z transforms in LDS
+// z transforms in LDS (FFT core in otbl_fft.h)
 // ---------------------------------------------------------------------------
-struct FFTPlan {
-  int n, nfac;
-  int fac[12];
-};
+// LDS layout of the transform kernels: lo[64], hi[128] twiddle tables, then
+// the signal buffers.
+static constexpr int TW_SLOTS = 64 + 128;
 
-__device__ inline double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// One Stockham stage of radix R (decimation in time, self-sorting).
-template <int R>
-__device__ inline void fft_stage(const double2 *__restrict__ a, double2 *__restrict__ b, int n,
-                                 int Ns, const double2 *__restrict__ tw, double sgn) {
-  const int nb = n / R;
-  const int tstep = n / (Ns * R);
-  const int rstep = n / R;
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
-    int k = j % Ns;
-    double2 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      double2 x = a[j + r * nb];
-      double2 w = tw[(k * r * tstep) % n];
-      w.y *= sgn;
-      v[r] = (r == 0) ? x : cmul(x, w);
+// Hermitian-extended spectrum of two real signals packed as z = x + i y,
+// x = c2r(X), y = c2r(Y).  X, Y rows hold modes 0..nz2 (re, im pairs); Im of
+// mode 0 does not enter a real inverse transform.  Every z entry is written.
+__device__ inline void pack2(double2 *z, const double *X, const double *Y, int n, int nz2) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    int kk = k <= nz2 ? k : (k >= n - nz2 ? n - k : -1);
+    double2 v = make_double2(0.0, 0.0);
+    if (kk >= 0) {
+      double xr = X ? X[2 * kk] : 0.0, xi = X ? X[2 * kk + 1] : 0.0;
+      double yr = Y ? Y[2 * kk] : 0.0, yi = Y ? Y[2 * kk + 1] : 0.0;
+      if (k == 0) v = make_double2(xr, yr);
+      else if (k == kk) v = make_double2(xr - yi, xi + yr);
+      else v = make_double2(xr + yi, -xi + yr);
     }
-    int idxD = (j / Ns) * Ns * R + k;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      double2 acc = v[0];
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        double2 w = tw[((r * q) % R) * rstep];
-        w.y *= sgn;
-        double2 t = cmul(v[r], w);
-        acc.x += t.x;
-        acc.y += t.y;
-      }
-      b[idxD + q * Ns] = acc;
-    }
+    z[k] = v;
   }
-}
-
-// Complex transform of length plan.n in LDS; sgn = -1 forward (FFTW sign),
-// +1 backward (unnormalised).  Returns the buffer holding the result.
-__device__ double2 *lds_fft(double2 *a, double2 *t, const FFTPlan &p, const double2 *tw, double sgn) {
-  int Ns = 1;
-  for (int s = 0; s < p.nfac; ++s) {
-    int R = p.fac[s];
-    switch (R) {
-    case 8: fft_stage<8>(a, t, p.n, Ns, tw, sgn); break;
-    case 4: fft_stage<4>(a, t, p.n, Ns, tw, sgn); break;
-    case 2: fft_stage<2>(a, t, p.n, Ns, tw, sgn); break;
-    case 3: fft_stage<3>(a, t, p.n, Ns, tw, sgn); break;
-    case 5: fft_stage<5>(a, t, p.n, Ns, tw, sgn); break;
-    case 7: fft_stage<7>(a, t, p.n, Ns, tw, sgn); break;
-    }
-    __syncthreads();
-    double2 *tmp = a;
-    a = t;
-    t = tmp;
-    Ns *= R;
-  }
-  return a;
-}
-
-// Hermitian-extended spectrum of two real signals x = c2r(X), y = c2r(Y)
-// packed as z = x + i y.  X, Y rows hold modes 0..nz2 (re, im pairs);
-// Im of mode 0 is ignored as in a real inverse transform.
-__device__ inline void pack_pair(double2 *z, const double *X, const double *Y, int n, int nz2) {
-  for (int k = threadIdx.x; k < n; k += blockDim.x) z[k] = make_double2(0.0, 0.0);
-  __syncthreads();
-  for (int k = threadIdx.x; k <= nz2; k += blockDim.x) {
-    double xr = X ? X[2 * k] : 0.0, xi = X ? X[2 * k + 1] : 0.0;
-    double yr = Y ? Y[2 * k] : 0.0, yi = Y ? Y[2 * k + 1] : 0.0;
-    if (k == 0) {
-      z[0] = make_double2(xr, yr);
-    } else {
-      z[k] = make_double2(xr - yi, xi + yr);
-      z[n - k] = make_double2(xr + yi, -xi + yr);
-    }
-  }
-  __syncthreads();
 }
 
 // Split the transform Z of z = a + i b into the spectra of a and b, modes
 // 0..nz2, scaled by 1/n (the r2c scaling of ffts.f90:51).
-__device__ inline void unpack_pair(const double2 *Z, double *A, double *B, int n, int nz2) {
+__device__ inline void unpack2(const double2 *Z, double *A, double *B, int n, int nz2) {
   const double sc = 1.0 / n;
   for (int k = threadIdx.x; k <= nz2; k += blockDim.x) {
     double2 z = Z[k];
@@ -538,136 +476,221 @@ __device__ inline void block_atomic_max(double v, unsigned long long *dst) {
   __syncthreads();
 }
 
-// The buffers of bufs[4] other than x and y (first two found).
-__device__ inline void pick_free(double2 *const bufs[4], const double2 *x, const double2 *y,
-                                 double2 *out[2]) {
-  int c = 0;
-  for (int q = 0; q < 4 && c < 2; ++q)
-    if (bufs[q] != x && bufs[q] != y) out[c++] = bufs[q];
-}
-
 // Plane loop 1 of rhsp (rhsp.F90:120-153) for one (j, i) row: with u, u_x,
 // w_x, v_x and u interpolated to v rows (uy) in spectral space, form
 // r2c(u_x^2) -> wki1, r2c(u w_x) -> wki3, r2c(v_x uy) -> wki2 (rows <= ny),
 // and the maxima |u|, |w_x|, |v_x|(j) for the CFL time step.
-__global__ void products1_kernel(const double *__restrict__ u, double *wki1, double *wki2,
-                                 double *wki3, const double *__restrict__ uy, FFTPlan plan,
-                                 const double2 *__restrict__ tw, int ny, int nz1, int nz2,
-                                 DevScalars *ds, unsigned long long *vmbits) {
+__global__ void __launch_bounds__(TPB)
+products1_kernel(const double *__restrict__ u, double *wki1, double *wki2, double *wki3,
+                 const double *__restrict__ uy, FFTPlan plan, int ny, int nz1, int nz2,
+                 DevScalars *ds, unsigned long long *vmbits) {
   extern __shared__ double2 sh[];
   const int n = plan.n;
-  double2 *A = sh, *B = sh + n, *Cc = sh + 2 * n, *T = sh + 3 * n;
+  double2 *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n;
   const int j = blockIdx.x + 1, i = blockIdx.y + 1;
   const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
   const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
   const bool vrow = j <= ny;
-  double2 *bufs[4] = {A, B, Cc, T};
-  pack_pair(A, u + ou, wki1 + ou, n, nz2);
-  double2 *rA = lds_fft(A, T, plan, tw, 1.0);
-  pack_pair(B, wki3 + ou, vrow ? wki2 + ov : nullptr, n, nz2);
-  double2 *fr[2];
-  pick_free(bufs, rA, B, fr);
-  double2 *rB = lds_fft(B, fr[0], plan, tw, 1.0);
-  double2 *rC = nullptr;
+  fill_twiddles(lo, hi, n);
+  pack2(Q0, u + ou, wki1 + ou, n, nz2);
+  pack2(Q0 + n, wki3 + ou, vrow ? wki2 + ov : nullptr, n, nz2);
+  __syncthreads();
+  const bool inA = fft_multi<+1>(Q0, Q1, n, 2, plan, lo, hi);
+  double2 *R = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
+  const double2 *UY = nullptr;
   if (vrow) {
-    pick_free(bufs, rA, rB, fr);
-    pack_pair(fr[0], uy + ou, nullptr, n, nz2);
-    rC = lds_fft(fr[0], fr[1], plan, tw, 1.0);
+    pack2(F, uy + ou, nullptr, n, nz2);
+    __syncthreads();
+    UY = fft_multi<+1>(F, F + n, n, 1, plan, lo, hi) ? F : F + n;
   }
   double um = 0.0, wm = 0.0, vm = 0.0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    double2 a = rA[q], b2 = rB[q];
+    double2 a = R[q], b2 = R[n + q];
     double uu = a.x, ux = a.y, wx = b2.x, vx = b2.y;
     um = fmax(um, fabs(uu));
     wm = fmax(wm, fabs(wx));
-    rA[q] = make_double2(ux * ux, uu * wx);
+    R[q] = make_double2(ux * ux, uu * wx);
     if (vrow) {
       vm = fmax(vm, fabs(vx));
-      rB[q] = make_double2(vx * rC[q].x, 0.0);
+      R[n + q] = make_double2(vx * UY[q].x, 0.0);
     }
   }
   __syncthreads();
   block_atomic_max(um, &ds->umax_bits);
   block_atomic_max(wm, &ds->wmax_bits);
   if (vrow) block_atomic_max(vm, &vmbits[j - 1]);
-  // forward transforms; buffers other than rA, rB are free now
-  pick_free(bufs, rA, rB, fr);
-  double2 *FA = lds_fft(rA, fr[0], plan, tw, -1.0);
-  unpack_pair(FA, wki1 + ou, wki3 + ou, n, nz2);
-  if (vrow) {
-    __syncthreads();
-    pick_free(bufs, rB, rB, fr);
-    double2 *FB = lds_fft(rB, fr[0], plan, tw, -1.0);
-    unpack_pair(FB, wki2 + ov, nullptr, n, nz2);
-  }
+  const double2 *O = fft_multi<-1>(R, F, n, vrow ? 2 : 1, plan, lo, hi) ? R : F;
+  unpack2(O, wki1 + ou, wki3 + ou, n, nz2);
+  if (vrow) unpack2(O + n, wki2 + ov, nullptr, n, nz2);
 }
 
 // Plane loop 2 of rhsp (rhsp.F90:367-415) for one (j, i) row, j <= ny:
 // vw = r2c(v * w_v) -> ya (w interpolated to v rows is read from ya),
 // vv = r2c(v_u^2) -> yb (v interpolated to u rows is read from yb),
 // ww = r2c(w^2) -> yc.
-__global__ void products2_kernel(const double *__restrict__ resv, const double *__restrict__ resw,
-                                 double *ya, double *yb, double *yc, FFTPlan plan,
-                                 const double2 *__restrict__ tw, int ny, int nz1, int nz2) {
+__global__ void __launch_bounds__(TPB)
+products2_kernel(const double *__restrict__ resv, const double *__restrict__ resw, double *ya,
+                 double *yb, double *yc, FFTPlan plan, int ny, int nz1, int nz2) {
   extern __shared__ double2 sh[];
   const int n = plan.n;
-  double2 *A = sh, *B = sh + n, *T = sh + 2 * n, *T2 = sh + 3 * n;
+  double2 *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n;
   const int j = blockIdx.x + 1, i = blockIdx.y + 2;
   const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
   const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
-  pack_pair(A, resv + ov, ya + ou, n, nz2);
-  double2 *rA = lds_fft(A, T, plan, tw, 1.0);
-  double2 *fB = (rA == A) ? T : A;
-  pack_pair(B, yb + ou, resw + ou, n, nz2);
-  double2 *rB = lds_fft(B, T2, plan, tw, 1.0);
+  fill_twiddles(lo, hi, n);
+  pack2(Q0, resv + ov, ya + ou, n, nz2);
+  pack2(Q0 + n, yb + ou, resw + ou, n, nz2);
+  __syncthreads();
+  const bool inA = fft_multi<+1>(Q0, Q1, n, 2, plan, lo, hi);
+  double2 *R = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    double2 a = rA[q], b2 = rB[q];
-    rA[q] = make_double2(a.x * a.y, b2.x * b2.x);
-    rB[q] = make_double2(b2.y * b2.y, 0.0);
+    double2 a = R[q], b2 = R[n + q];
+    R[q] = make_double2(a.x * a.y, b2.x * b2.x);
+    R[n + q] = make_double2(b2.y * b2.y, 0.0);
   }
   __syncthreads();
-  double2 *FA = lds_fft(rA, fB, plan, tw, -1.0);
-  unpack_pair(FA, ya + ou, yb + ou, n, nz2);
-  double2 *spareB = (rB == B) ? T2 : B;
-  __syncthreads();
-  double2 *FB = lds_fft(rB, spareB, plan, tw, -1.0);
-  unpack_pair(FB, yc + ou, nullptr, n, nz2);
+  const double2 *O = fft_multi<-1>(R, F, n, 2, plan, lo, hi) ? R : F;
+  unpack2(O, ya + ou, yb + ou, n, nz2);
+  unpack2(O + n, yc + ou, nullptr, n, nz2);
 }
 
 // fourxz (ffts.f90:27-67) on independent rows, for the operator-level API.
-__global__ void fourxz_kernel(double *fou, double *phys, int iopt, FFTPlan plan,
-                              const double2 *__restrict__ tw, int nz1, int nz2) {
+__global__ void __launch_bounds__(TPB)
+fourxz_kernel(double *fou, double *phys, int iopt, FFTPlan plan, int nz1, int nz2) {
   extern __shared__ double2 sh[];
   const int n = plan.n;
-  double2 *A = sh, *T = sh + n;
+  double2 *lo = sh, *hi = sh + 64, *A = sh + TW_SLOTS, *T = A + n;
   size_t r = blockIdx.x;
   double *F = fou + r * nz1;
   double *Ph = phys + r * n;
+  fill_twiddles(lo, hi, n);
   if (iopt >= 0) {
-    pack_pair(A, F, nullptr, n, nz2);
-    double2 *R = lds_fft(A, T, plan, tw, 1.0);
+    pack2(A, F, nullptr, n, nz2);
+    __syncthreads();
+    const double2 *R = fft_multi<+1>(A, T, n, 1, plan, lo, hi) ? A : T;
     for (int q = threadIdx.x; q < n; q += blockDim.x) Ph[q] = R[q].x;
   } else {
     for (int q = threadIdx.x; q < n; q += blockDim.x) A[q] = make_double2(Ph[q], 0.0);
     __syncthreads();
-    double2 *R = lds_fft(A, T, plan, tw, -1.0);
-    unpack_pair(R, F, nullptr, n, nz2);
+    const double2 *R = fft_multi<-1>(A, T, n, 1, plan, lo, hi) ? A : T;
+    unpack2(R, F, nullptr, n, nz2);
   }
 }
 
+static size_t zfft_shm(const Box &b, int nbuf) { return (TW_SLOTS + (size_t)nbuf * b.nz) * sizeof(double2); }
+
 void k_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows) {
-  FFTPlan plan;
-  plan.n = b.nz;
-  plan.nfac = b.nfac;
-  for (int q = 0; q < b.nfac; ++q) plan.fac[q] = b.fac[q];
-  LAUNCH(b, "fourxz", nrows, 64, 2 * b.nz * sizeof(double2), fourxz_kernel, fou, phys, iopt, plan,
-         (const double2 *)b.tw, b.nz1, b.nz2);
+  LAUNCH(b, "fourxz", nrows, TPB, zfft_shm(b, 2), fourxz_kernel, fou, phys, iopt, b.zplan, b.nz1, b.nz2);
 }
 
 // ---------------------------------------------------------------------------
-// Poisson x transforms: DCT-II / DCT-III over planes 2..nx as a tiled FP64
-// product out[q][l] = sum_n M[q][n] in[n][l] (M = dct2 or dct3, N x N).
+// Poisson x transforms.  FFTW's REDFT10 / REDFT01 of length N = nx-1
+// (ffts.f90:124-181) by Makhoul's reordering into one complex DFT of length
+// N per pair of real lines, evaluated with Bluestein's chirp-z algorithm as
+// a circular convolution of length M >= 2N-1 (M with factors 2, 3, 5, 7).
+// Lines are x-contiguous pencils: pencil l (l = (j-1)*nz1 + kr, rows
+// j = 1..ny-1) holds planes 2..nx at P[l*N + x], x = i-2.
 // ---------------------------------------------------------------------------
+struct DctArgs {
+  int N, M;
+  FFTPlan fp;
+  const double2 *chirp; // exp(-i pi n^2 / N), n < N
+  const double2 *bf;    // FFT_M of the conjugate chirp, / M
+  const double2 *tw4;   // exp(-i pi k / (2N)), k < N
+};
+
+// out of place allowed; in place when src == dst.  iopt < 0: DCT-II x 1/N,
+// iopt > 0: DCT-III x 0.5.
+__global__ void __launch_bounds__(TPB) dct_rows_kernel(const double *src, double *dst, int Lp, int iopt, DctArgs d) {
+  extern __shared__ double2 sh[];
+  const int N = d.N, M = d.M;
+  double2 *lo = sh, *hi = sh + 64, *A = sh + TW_SLOTS, *B = A + M;
+  fill_twiddles(lo, hi, M);
+  const int l0 = 2 * blockIdx.x, l1 = l0 + 1;
+  const bool has1 = l1 < Lp;
+  const double *x0 = src + (size_t)l0 * N, *x1 = src + (size_t)l1 * N;
+  const int half = (N + 1) / 2;
+  for (int k = threadIdx.x; k < M; k += blockDim.x) {
+    double2 a = make_double2(0.0, 0.0);
+    if (k < N) {
+      double2 z;
+      if (iopt < 0) { // Makhoul reordering of the two real lines
+        int s = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
+        z = make_double2(x0[s], has1 ? x1[s] : 0.0);
+      } else { // V = (X_k - i X_{N-k}) exp(+i pi k / 2N), Z = V0 + i V1, use conj(Z)
+        double2 t = d.tw4[k];
+        t.y = -t.y;
+        double a0 = x0[k], b0 = k ? x0[N - k] : 0.0;
+        double a1 = has1 ? x1[k] : 0.0, b1 = (has1 && k) ? x1[N - k] : 0.0;
+        double2 v0 = cmulc(make_double2(a0, -b0), t), v1 = cmulc(make_double2(a1, -b1), t);
+        z = make_double2(v0.x - v1.y, -(v0.y + v1.x));
+      }
+      a = cmulc(z, d.chirp[k]);
+    }
+    A[k] = a;
+  }
+  __syncthreads();
+  double2 *Y = fft_multi<-1>(A, B, M, 1, d.fp, lo, hi) ? A : B;
+  double2 *F = (Y == A) ? B : A;
+  for (int k = threadIdx.x; k < M; k += blockDim.x) Y[k] = cmulc(Y[k], d.bf[k]);
+  __syncthreads();
+  Y = fft_multi<+1>(Y, F, M, 1, d.fp, lo, hi) ? Y : F;
+  F = (Y == A) ? B : A;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) F[k] = cmulc(d.chirp[k], Y[k]);
+  __syncthreads();
+  double *y0 = dst + (size_t)l0 * N, *y1 = dst + (size_t)l1 * N;
+  if (iopt < 0) {
+    const double sc = 2.0 / N;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+      double2 z = F[k], w = F[k ? N - k : 0];
+      double2 v0 = make_double2(0.5 * (z.x + w.x), 0.5 * (z.y - w.y));
+      double2 v1 = make_double2(0.5 * (z.y + w.y), -0.5 * (z.x - w.x));
+      double2 t = d.tw4[k];
+      y0[k] = sc * (t.x * v0.x - t.y * v0.y);
+      if (has1) y1[k] = sc * (t.x * v1.x - t.y * v1.y);
+    }
+  } else {
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+      double2 zc = F[k]; // conj of sum_k Z_k exp(+2 pi i k n / N)
+      int o = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
+      y0[o] = 0.5 * zc.x;
+      if (has1) y1[o] = -0.5 * zc.y;
+    }
+  }
+}
+
+// Tiled transposes between the planes layout (planes 2..nx, lines
+// 0..Lp-1, plane stride ps) and x-contiguous pencils P[l*N + x].
+__global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F, double *P, int Lp, int N, size_t ps) {
+  __shared__ double tile[64][65];
+  const int l0 = blockIdx.x * 64, x0 = blockIdx.y * 64, t = threadIdx.x;
+  for (int xx = t / 64; xx < 64; xx += TPB / 64) {
+    int l = l0 + (t & 63), x = x0 + xx;
+    if (l < Lp && x < N) tile[xx][t & 63] = F[(size_t)(x + 1) * ps + l];
+  }
+  __syncthreads();
+  for (int ll = t / 64; ll < 64; ll += TPB / 64) {
+    int l = l0 + ll, x = x0 + (t & 63);
+    if (l < Lp && x < N) P[(size_t)l * N + x] = tile[t & 63][ll];
+  }
+}
+
+__global__ void __launch_bounds__(TPB) pencils_to_planes_kernel(const double *P, double *F, int Lp, int N, size_t ps) {
+  __shared__ double tile[64][65];
+  const int l0 = blockIdx.x * 64, x0 = blockIdx.y * 64, t = threadIdx.x;
+  for (int ll = t / 64; ll < 64; ll += TPB / 64) {
+    int l = l0 + ll, x = x0 + (t & 63);
+    if (l < Lp && x < N) tile[ll][t & 63] = P[(size_t)l * N + x];
+  }
+  __syncthreads();
+  for (int xx = t / 64; xx < 64; xx += TPB / 64) {
+    int l = l0 + (t & 63), x = x0 + xx;
+    if (l < Lp && x < N) F[(size_t)(x + 1) * ps + l] = tile[t & 63][xx];
+  }
+}
+
+// Dense fallback for Bluestein lengths whose FP64 buffers exceed LDS:
+// out[q][l] = sum_n M[q][n] in[n][l] over planes 2..nx.
 __global__ void dct_gemm_kernel(const double *__restrict__ M, const double *__restrict__ in,
                                 double *__restrict__ out, int N, int L, size_t ps) {
   __shared__ double As[16][64 + 1];
@@ -705,15 +728,43 @@ __global__ void dct_gemm_kernel(const double *__restrict__ M, const double *__re
   }
 }
 
-static void dct_apply(Box &b, const double *M, const double *in, double *out) {
-  int N = b.nx - 1;
-  int L = (b.ny - 1) * b.nz1; // rows 1..ny-1 of each plane
-  dim3 grid(nblk(L, 64), nblk(N, 64));
-  LAUNCH(b, "dct", grid, 256, 0, dct_gemm_kernel, M, in + b.pv, out + b.pv, N, L, b.pv);
+static DctArgs dct_args(const Box &b) {
+  DctArgs d;
+  d.N = b.nx - 1;
+  d.M = b.dct_M;
+  d.fp = b.dct_plan;
+  d.chirp = (const double2 *)b.dct_chirp;
+  d.bf = (const double2 *)b.dct_bf;
+  d.tw4 = (const double2 *)b.dct_tw4;
+  return d;
 }
 
+static size_t dct_shm(const Box &b) { return (TW_SLOTS + 2 * (size_t)b.dct_M) * sizeof(double2); }
+
+// Row transforms on pencils (in place).
+static void dct_pencils(Box &b, double *P, int iopt) {
+  const int Lp = (b.ny - 1) * b.nz1;
+  LAUNCH(b, iopt < 0 ? "dct2_rows" : "dct3_rows", (Lp + 1) / 2, TPB, dct_shm(b), dct_rows_kernel, P, P, Lp,
+         iopt, dct_args(b));
+}
+
+static dim3 tgrid(const Box &b) {
+  const int Lp = (b.ny - 1) * b.nz1, N = b.nx - 1;
+  return dim3(nblk(Lp, 64), nblk(N, 64));
+}
+
+// cosftx on a planes-layout v-type field (rows 1..ny-1 of planes 2..nx).
 void k_cosftx(Box &b, const double *in, double *out, int iopt) {
-  dct_apply(b, iopt < 0 ? b.dct2 : b.dct3, in, out);
+  const int Lp = (b.ny - 1) * b.nz1, N = b.nx - 1;
+  if (b.dct_fft) {
+    LAUNCH(b, "to_pencils", tgrid(b), TPB, 0, planes_to_pencils_kernel, in, b.rest2, Lp, N, b.pv);
+    dct_pencils(b, b.rest2, iopt);
+    LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel, b.rest2, out, Lp, N, b.pv);
+  } else {
+    dim3 grid(nblk(Lp, 64), nblk(N, 64));
+    LAUNCH(b, "dct_gemm", grid, 256, 0, dct_gemm_kernel, iopt < 0 ? b.dct2 : b.dct3, in + b.pv, out + b.pv, N,
+           Lp, b.pv);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1217,6 +1268,76 @@ __global__ void poisson_y_kernel(double *rr, double *a2s, DevCoef c, int nx, int
   }
 }
 
+// divergence (pois.f90:41-54) written straight into x-contiguous pencils
+// through an LDS tile (64 lines x 64 planes)
+__global__ void __launch_bounds__(TPB)
+div_pencils_kernel(const double *u, const double *v, const double *w, double *P, DevCoef c, double idx,
+                   int nx, int ny, int nz1) {
+  __shared__ double tile[64][65];
+  const int Lp = (ny - 1) * nz1, N = nx - 1;
+  const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
+  const int l0 = blockIdx.x * 64, x0 = blockIdx.y * 64, t = threadIdx.x;
+  for (int xx = t / 64; xx < 64; xx += TPB / 64) {
+    int l = l0 + (t & 63), x = x0 + xx;
+    if (l < Lp && x < N) {
+      int i = x + 2, j = l / nz1 + 1, kr = l % nz1;
+      double kz = c.kaz[kr / 2];
+      size_t ow = (size_t)(i - 1) * pu + (size_t)j * s;       // row j+1
+      size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s; // row j
+      double wp = w[ow + (kr ^ 1)];
+      double tz = (kr & 1) ? wp * kz : -wp * kz;
+      double d = tz + (v[ov + s + kr] - v[ov + kr]) * c.idy[j];
+      tile[xx][t & 63] = d + idx * (u[ow + kr] - u[ow - pu + kr]);
+    }
+  }
+  __syncthreads();
+  for (int ll = t / 64; ll < 64; ll += TPB / 64) {
+    int l = l0 + ll, x = x0 + (t & 63);
+    if (l < Lp && x < N) P[(size_t)l * N + x] = tile[t & 63][ll];
+  }
+}
+
+// per-mode y tridiagonal on pencils (pois.f90:102-117, soltriy
+// pois.f90:211-239): one thread per (kr, kx-mode x), rows at stride nz1*N.
+__global__ void __launch_bounds__(TPB)
+poisson_y_pencil_kernel(double *P, double *S, DevCoef c, int nx, int ny, int nz1) {
+  const int N = nx - 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nz1 * N) return;
+  const int x = (int)(t % N), kr = (int)(t / N), i = x + 2;
+  const size_t rs = (size_t)nz1 * N;
+  double *col = P + (size_t)kr * N + x;
+  double *d2 = S + (size_t)kr * N + x;
+  const int ny1 = ny - 1;
+  if (kr == 1) { // the zero mode is real (pois.f90:102)
+    for (int j = 0; j < ny1; ++j) col[(size_t)j * rs] = 0.0;
+    return;
+  }
+  const double kz2 = c.kaz2[kr / 2], km = c.kmod[i - 1];
+  const bool pin = (i == 2 && kr == 0);
+  if (pin) col[(size_t)(ny1 - 1) * rs] = 0.0;
+  double a2prev = C3(c.ayp, 2, 1) - kz2 - km;
+  double a3prev = C3(c.ayp, 3, 1);
+  double y = col[0];
+  for (int j = 2; j <= ny1; ++j) {
+    double d = 1.0 / a2prev;
+    d2[(size_t)(j - 2) * rs] = d;
+    double a1 = -C3(c.ayp, 1, j) * d;
+    double a2 = (pin && j == ny1) ? 2.0 * C3(c.ayp, 2, j) : C3(c.ayp, 2, j) - kz2 - km;
+    a2 = a2 + a1 * a3prev;
+    y = col[(size_t)(j - 1) * rs] + a1 * y;
+    col[(size_t)(j - 1) * rs] = y;
+    a2prev = a2;
+    a3prev = C3(c.ayp, 3, j);
+  }
+  double x_ = y * (1.0 / a2prev);
+  col[(size_t)(ny1 - 1) * rs] = x_;
+  for (int j = ny1 - 1; j >= 1; --j) {
+    x_ = (col[(size_t)(j - 1) * rs] - C3(c.ayp, 3, j) * x_) * d2[(size_t)(j - 1) * rs];
+    col[(size_t)(j - 1) * rs] = x_;
+  }
+}
+
 // projection (pois.f90:144-162): p, w, v, u from phi = rest
 __global__ void project_kernel(double *u, double *v, double *w, double *p, const double *rest, DevCoef c,
                                const DevScalars *ds, double vstep, int use_m, double idxx, int nx, int ny,
@@ -1250,14 +1371,25 @@ __global__ void project_kernel(double *u, double *v, double *w, double *p, const
 
 void k_pois(Box &b, int m, double varstep, bool use_m) {
   const int nx = b.nx, ny = b.ny, nz1 = b.nz1;
+  const int Lp = (ny - 1) * nz1, N = nx - 1;
   (void)m;
-  size_t ndiv = (size_t)(ny - 1) * nz1 * (nx - 1);
-  LAUNCH(b, "div", nblk(ndiv), TPB, 0, div_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest, b.dc,
-         b.hc.idx, nx, ny, nz1);
-  dct_apply(b, b.dct2, b.rest, b.rest2);
-  size_t ncol = (size_t)nz1 * (nx - 1);
-  LAUNCH(b, "poisson_y", nblk(ncol), TPB, 0, poisson_y_kernel, b.rest2, b.rest, b.dc, nx, ny, nz1);
-  dct_apply(b, b.dct3, b.rest2, b.rest);
+  if (b.dct_fft) {
+    LAUNCH(b, "div", tgrid(b), TPB, 0, div_pencils_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest2, b.dc,
+           b.hc.idx, nx, ny, nz1);
+    dct_pencils(b, b.rest2, -1);
+    LAUNCH(b, "poisson_y", nblk((size_t)nz1 * N), TPB, 0, poisson_y_pencil_kernel, b.rest2, b.rest, b.dc, nx, ny,
+           nz1);
+    dct_pencils(b, b.rest2, 1);
+    LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel, b.rest2, b.rest, Lp, N, b.pv);
+  } else {
+    size_t ndiv = (size_t)(ny - 1) * nz1 * (nx - 1);
+    LAUNCH(b, "div", nblk(ndiv), TPB, 0, div_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest, b.dc,
+           b.hc.idx, nx, ny, nz1);
+    k_cosftx(b, b.rest, b.rest2, -1);
+    size_t ncol = (size_t)nz1 * (nx - 1);
+    LAUNCH(b, "poisson_y", nblk(ncol), TPB, 0, poisson_y_kernel, b.rest2, b.rest, b.dc, nx, ny, nz1);
+    k_cosftx(b, b.rest2, b.rest, 1);
+  }
   LAUNCH(b, "project", nblk(b.nu), TPB, 0, project_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W],
          b.f[OTBL_P], b.rest, b.dc, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1);
 }
@@ -1269,10 +1401,7 @@ void k_substep_rhsp(Box &b, int m) {
   const int nx = b.nx, ny = b.ny, nz1 = b.nz1;
   const int ru = ny + 1;
   double *u = b.f[OTBL_U], *v = b.f[OTBL_V], *w = b.f[OTBL_W], *p = b.f[OTBL_P];
-  FFTPlan plan;
-  plan.n = b.nz;
-  plan.nfac = b.nfac;
-  for (int q = 0; q < b.nfac; ++q) plan.fac[q] = b.fac[q];
+  const FFTPlan &plan = b.zplan;
 
   LAUNCH(b, "reset_max", 1, TPB, 0, reset_max_kernel, b.ds, b.vmbits, ny);
   // x interpolation (rhsp.F90:105-107)
@@ -1285,9 +1414,9 @@ void k_substep_rhsp(Box &b, int m) {
   // commutes with the z transform of rhsp.F90:122,146)
   k_interpyy(b, u, b.yc, 0, 1, nx, b.pu, b.pu);
   // plane loop 1 products
-  size_t shm = 4 * (size_t)b.nz * sizeof(double2);
+  const size_t shm = zfft_shm(b, 4);
   LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel, u, b.wki1, b.wki2, b.wki3, b.yc,
-         plan, (const double2 *)b.tw, ny, nz1, b.nz2, b.ds, b.vmbits);
+         plan, ny, nz1, b.nz2, b.ds, b.vmbits);
   LAUNCH(b, "rhsu_loop1", nblk((size_t)nz1 * nx), TPB, 0, rhsu_loop1_kernel, b.wki2, b.wki3, b.rhsu, b.dc,
          ny, nz1, nx);
   // x derivatives (rhsp.F90:181-204)
@@ -1310,7 +1439,7 @@ void k_substep_rhsp(Box &b, int m) {
     k_interpyy(b, b.resw, b.ya, 0, 2, nx - 2, b.pu, b.pu);
     k_interpyy(b, b.resv, b.yb, 1, 2, nx - 2, b.pv, b.pu);
     LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, b.resv, b.resw, b.ya, b.yb, b.yc,
-           plan, (const double2 *)b.tw, ny, nz1, b.nz2);
+           plan, ny, nz1, b.nz2);
     LAUNCH(b, "rhs_loop2", nblk((size_t)nz1 * (nx - 2)), TPB, 0, rhs_loop2_kernel, b.rhsu, b.rhsv, b.rhsw,
            b.resu, b.resv, b.resw, b.ya, b.yb, b.yc, b.wki1, b.dc, ny, nz1, nx - 2);
   }
@@ -1329,12 +1458,18 @@ void k_substep_rhsp(Box &b, int m) {
 }
 
 void k_setup_attrs(Box &b) {
-  size_t shm = 4 * (size_t)b.nz * sizeof(double2);
-  OTBL_HIP(hipFuncSetAttribute((const void *)products1_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  OTBL_HIP(hipFuncSetAttribute((const void *)products2_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  const int shm = (int)zfft_shm(b, 4);
+  OTBL_HIP(hipFuncSetAttribute((const void *)products1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  OTBL_HIP(hipFuncSetAttribute((const void *)products2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  OTBL_HIP(hipFuncSetAttribute((const void *)fourxz_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)zfft_shm(b, 2)));
+  if (b.dct_fft)
+    OTBL_HIP(hipFuncSetAttribute((const void *)dct_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)dct_shm(b)));
 }
+
+size_t k_dct_shm_bytes(int M) { return (TW_SLOTS + 2 * (size_t)M) * sizeof(double2); }
+size_t k_zfft_shm_bytes(int nz, int nbuf) { return (TW_SLOTS + (size_t)nbuf * nz) * sizeof(double2); }
 
 __global__ void end_step_kernel(DevScalars *ds) {
   if (threadIdx.x == 0) ds->tiempo += ds->dt;

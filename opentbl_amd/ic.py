@@ -32,7 +32,10 @@ def ygrid(cfg: BoxConfig) -> np.ndarray:
     ny = cfg.ny
     eta = np.arange(ny, dtype=np.float64) / (ny - 1)
     g = cfg.stretch
-    yv = cfg.Ly * (1.0 + np.tanh(g * (eta - 1.0)) / math.tanh(g))
+    if g > 0:
+        yv = cfg.Ly * (1.0 + np.tanh(g * (eta - 1.0)) / math.tanh(g))
+    else:                      # stretch 0: uniform grid
+        yv = cfg.Ly * eta
     yv[0] = 0.0
     y = np.empty(ny + 2)
     y[1:ny + 1] = yv

@@ -1107,6 +1107,10 @@ static void profile(const oc_box *b, double *ut, double *wk, int ntop, const int
                     const double *fsca, int n) {
   int s = b->nz1;
   double ggg = 1.0 / b->gam[0];
+  /* the copy of row ntop-3 below (genflu.F90:232-234) needs ntop >= 4; the
+   * reference indexes out of bounds otherwise.  Such a profile is left
+   * unchanged. */
+  if (ntop < 4) return;
   if (n == b->ny + 1) {
     for (int i = 1; i <= ntop; i++) {
       int kll = kl[i];

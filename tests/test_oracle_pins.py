@@ -1,0 +1,462 @@
+"""Pins of the CPU FP64 oracle against mathematics that fixes each part
+(there are no golden vectors in the reference: it has no tests and no data
+beyond its parameter files).  Every check below would fail for a plausible
+transcription mistake: a wrong sign, index or term in a stencil, a dropped
+boundary row, a transposed operand.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.fft as sfft
+
+from opentbl_amd import config, ic
+
+
+@pytest.fixture(scope="module")
+def box(oracle_lib):
+    cfg = config.small(nx=48, ny=40, nz=24)
+    st = ic.make_state(cfg)
+    return cfg, st, oracle_lib.OracleBox(cfg, st)
+
+
+def fac_tridiag(c):
+    """Rebuild the unfactored tridiagonal (l, d, u) from coefinv's LU
+    storage (l = -a/d_prev, 1/d, c) -- coeft.f90:1338-1353."""
+    n = c.shape[1]
+    A = np.zeros((n, n))
+    d = 1.0 / c[1]
+    for i in range(n):
+        A[i, i] = d[i] if i == 0 else d[i] + (-c[0, i]) * c[2, i - 1]
+        if i > 0:
+            A[i, i - 1] = -c[0, i] * d[i - 1]
+        if i < n - 1:
+            A[i, i + 1] = c[2, i]
+    return A
+
+
+# ---------------------------------------------------------------- grids
+def x_grid(cfg):
+    h0 = cfg.ax * math.pi / (cfg.nx + 1)
+    return np.arange(cfg.nx + 2) * h0
+
+
+def test_lele_interior_values(box):
+    """Uniform-x interior rows reduce to the textbook 6th/4th-order Lele
+    schemes: first derivative alpha = 1/22 ... is not used here; the
+    staggered midpoint interpolation has alpha = 3/10 (coeft.f90:209-216 for
+    the collocated version); on the staggered uniform grid the reference's
+    non-uniform formula gives alpha = 1/6, a/2 = ... -- check the closed form
+    of the implicit rows."""
+    cfg, st, ob = box
+    inxv = ob.coef("inxv")
+    # unfactored implicit row i of inxv on a uniform grid: a^2/((b+a)(a+2b)) = 1/6
+    A = fac_tridiag(inxv)
+    i = cfg.nx // 2
+    assert A[i, i - 1] == pytest.approx(1.0 / 6.0, rel=1e-12)
+    assert A[i, i + 1] == pytest.approx(1.0 / 6.0, rel=1e-12)
+    # explicit part (cofivx): 4a^2/((b+a)(2a+b)) = 2/3 on a uniform grid
+    c = ob.coef("cofivx")
+    assert c[0, i] == pytest.approx(2.0 / 3.0, rel=1e-12)
+    assert c[1, i] == pytest.approx(2.0 / 3.0, rel=1e-12)
+    # staggered first derivative on a uniform grid: alpha = 1/22, a = 12/11 h^-1
+    A = fac_tridiag(ob.coef("dcxu"))
+    assert A[i, i - 1] == pytest.approx(1.0 / 22.0, rel=1e-12)
+    h = x_grid(cfg)[1]
+    cc = ob.coef("cofcxu")
+    assert cc[1, i] * h == pytest.approx(24.0 / 22.0, rel=1e-12)
+    # second derivative: alpha = 1/10, explicit 6/5 h^-2
+    A = fac_tridiag(ob.coef("vixu"))
+    assert A[i, i - 1] == pytest.approx(1.0 / 10.0, rel=1e-12)
+    ev = ob.coef("cofvxu")
+    assert ev[0, i] * h * h == pytest.approx(12.0 / 10.0, rel=1e-12)
+    assert ev[1, i] * h * h == pytest.approx(-12.0 / 5.0, rel=1e-12)
+
+
+def _apply_interp_x(ob, cfg, f_u, which_u):
+    """interpxx on one line through the operator entry point."""
+    from oracle.oracle import ptr
+    R = cfg.ny + 1
+    g = np.zeros((cfg.nx, R, cfg.nz1))
+    g[:, 0, 0] = f_u
+    out = np.zeros_like(g)
+    ob.op("interpxx", ptr(g), ptr(out), R, which_u)
+    return out[:, 0, 0]
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_interpxx_polynomial_exactness(box, deg):
+    """u -> midpoint interpolation (interpxx with the u coefficients) is
+    exact on quadratics everywhere and on cubics away from the outflow end,
+    whose 3-point closure (inbx row 2, coefb flag 0) is second order; its
+    error decays geometrically away from the last row."""
+    cfg, st, ob = box
+    x = x_grid(cfg)
+    xu = x[1:cfg.nx + 1]                    # u at x(i)
+    xm = 0.5 * (x[1:cfg.nx + 1] + x[2:cfg.nx + 2])  # midpoints
+    f = lambda t: (t / x[-1]) ** deg
+    got = _apply_interp_x(ob, cfg, f(xu), 1)
+    err = np.abs(got - f(xm))
+    if deg <= 2:
+        assert err.max() < 1e-11
+    else:
+        assert err[: cfg.nx // 2].max() < 1e-11
+        assert err[-1] > 1e-7
+
+
+def _y_apply(ob, cfg, name, col, extra=None):
+    from oracle.oracle import ptr
+    R = cfg.ny + 1
+    g = np.zeros((cfg.nx, R, cfg.nz1))
+    g[0, :len(col), 0] = col
+    out = np.zeros_like(g)
+    if name == "interpyy":
+        ob.op("interpyy", ptr(g if extra == 0 else np.ascontiguousarray(g[:, :cfg.ny])),
+              ptr(out), extra, cfg.nx)
+    elif name == "differzy":
+        ob.op("differzy", ptr(g), ptr(out), extra, cfg.nx)
+    return out[0, :, 0]
+
+
+@pytest.fixture(scope="module")
+def ubox(oracle_lib):
+    """A box on a uniform y grid, where the reference's non-uniform compact
+    formulas reduce to the textbook staggered schemes."""
+    cfg = config.small(nx=32, ny=40, nz=24, stretch=0.0)
+    st = ic.make_state(cfg, with_perturbation=False)
+    return cfg, st, oracle_lib.OracleBox(cfg, st)
+
+
+def _ycols(st, cfg):
+    y = st["y"]
+    return 0.5 * (y[:-1] + y[1:]), y[1:cfg.ny + 1]
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3])
+def test_interpyy_u_to_v_exact_uniform(ubox, deg):
+    """interpyy u rows -> v rows (lip.f90:376-392): exact for cubics that
+    vanish at the wall (row 1 is set to zero for no-slip)."""
+    cfg, st, ob = ubox
+    yu, yv = _ycols(st, cfg)
+    f = lambda t: (t / 4.0) ** deg
+    got = _y_apply(ob, cfg, "interpyy", f(yu), 0)[:cfg.ny]
+    assert np.abs(got - f(yv)).max() < 1e-12
+    assert got[0] == 0.0
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_interpyy_v_to_u_exact_uniform(ubox, deg):
+    """interpyy v rows -> u rows 1..ny (lip.f90:393-410) with its one-sided
+    end rows inby(3,:), inby(4,:)."""
+    cfg, st, ob = ubox
+    yu, yv = _ycols(st, cfg)
+    f = lambda t: ((t + 0.5) / 4.0) ** deg
+    got = _y_apply(ob, cfg, "interpyy", f(yv), 1)[:cfg.ny]
+    tol = 1e-12 if deg <= 2 else 1e-4   # the 3-point closure inby(3,:) is second order
+    assert np.abs(got - f(yu[:cfg.ny])).max() < tol
+    assert np.abs(got[cfg.ny // 2:] - f(yu[cfg.ny // 2:cfg.ny])).max() < 1e-11
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3])
+def test_differzy_exact_uniform(ubox, deg):
+    """d/dy v rows -> u rows (dcyu) and u rows -> v rows (dcyv): the
+    staggered compact first derivative and its 4-point closures are exact
+    for cubics on a uniform grid."""
+    cfg, st, ob = ubox
+    yu, yv = _ycols(st, cfg)
+    f = lambda t: ((t + 0.5) / 4.0) ** deg
+    df = lambda t: deg * ((t + 0.5) / 4.0) ** (deg - 1) / 4.0
+    col = np.zeros(cfg.ny + 1)
+    col[:cfg.ny] = f(yv)
+    got = _y_apply(ob, cfg, "differzy", col, 0)[:cfg.ny + 1]
+    assert np.abs(got - df(yu)).max() < 1e-11
+    col = f(yu).copy()
+    got = _y_apply(ob, cfg, "differzy", col, 1)[:cfg.ny]
+    assert np.abs(got - df(yv)).max() < 1e-11
+
+
+def test_vistyy_exact_uniform(ubox):
+    """vistyy: the compact second derivative (interior alpha = 1/10) and its
+    closures (coefb flag 1) are exact for polynomials up to degree 3."""
+    from oracle.oracle import ptr
+    cfg, st, ob = ubox
+    yu, yv = _ycols(st, cfg)
+    for R, yy in ((cfg.ny + 1, yu), (cfg.ny, yv)):
+        for deg in (0, 1, 2, 3):
+            f = ((yy + 0.5) / 4.0) ** deg
+            d2 = deg * (deg - 1) * ((yy + 0.5) / 4.0) ** max(deg - 2, 0) / 16.0
+            g = np.zeros((cfg.nx, R, cfg.nz1))
+            g[0, :, 0] = f
+            out = np.zeros_like(g)
+            ob.op("vistyy", ptr(g), ptr(out), R, cfg.nx)
+            assert np.abs(out[0, :, 0] - d2).max() < 1e-9, (R, deg)
+
+
+def _stretched_err(oracle_lib, ny, which):
+    """Max error of an operator on a smooth function on the stretched grid."""
+    from oracle.oracle import ptr
+    cfg = config.small(nx=16, ny=ny, nz=12, kzmax=2)
+    st = ic.make_state(cfg, with_perturbation=False)
+    ob = oracle_lib.OracleBox(cfg, st)
+    yu, yv = _ycols(st, cfg)
+    f = lambda t: np.sin(0.9 * t) * t
+    df = lambda t: 0.9 * np.cos(0.9 * t) * t + np.sin(0.9 * t)
+    d2f = lambda t: -0.81 * np.sin(0.9 * t) * t + 1.8 * np.cos(0.9 * t)
+    R = cfg.ny + 1
+    g = np.zeros((cfg.nx, R, cfg.nz1))
+    out = np.zeros_like(g)
+    if which == "differzy":
+        g[0, :cfg.ny, 0] = f(yv)
+        ob.op("differzy", ptr(g), ptr(out), 0, cfg.nx)
+        return np.abs(out[0, :, 0] - df(yu)).max()
+    if which == "interpyy":
+        g[0, :, 0] = f(yu)
+        ob.op("interpyy", ptr(g), ptr(out), 0, cfg.nx)
+        return np.abs(out[0, :cfg.ny, 0] - f(yv)).max()
+    g[0, :, 0] = f(yu)
+    ob.op("vistyy", ptr(g), ptr(out), R, cfg.nx)
+    return np.abs(out[0, 1:-1, 0] - d2f(yu[1:-1])).max()
+
+
+@pytest.mark.parametrize("which", ["differzy", "interpyy", "vistyy"])
+def test_stretched_grid_convergence(oracle_lib, which):
+    """On the tanh-stretched grid the non-uniform schemes are consistent:
+    doubling ny cuts the error of a smooth function by at least 4x."""
+    e1 = _stretched_err(oracle_lib, 40, which)
+    e2 = _stretched_err(oracle_lib, 80, which)
+    assert e1 < 1e-1 and e2 < e1 / 4.0, (e1, e2)
+
+
+def test_tridiagonal_sweeps_match_dense_solve(box):
+    """The LU sweep of interpxx equals a dense solve of the assembled
+    tridiagonal system with the explicit right-hand side."""
+    cfg, st, ob = box
+    inxu = ob.coef("inxu")
+    A = fac_tridiag(inxu)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(cfg.nx)
+    e = ob.coef("cofiux")
+    inbx = ob.coef("inbx")
+    b = np.zeros(cfg.nx)
+    b[1:-1] = e[0, 1:-1] * f[1:-1] + e[1, 1:-1] * f[2:]
+    b[0] = inbx[2, :] @ f[:4]                 # chi2 = chi+1 = 3 for u
+    b[-1] = inbx[1, :] @ f[::-1][:4]          # chi = 2
+    ref = np.linalg.solve(A, b)
+    got = _apply_interp_x(ob, cfg, f, 1)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_kmod_closed_form(box):
+    """kmod(i) are the eigenvalues of the Neumann second difference
+    (coeft.f90:1181-1183): check against a dense eigen-decomposition."""
+    cfg, st, ob = box
+    km = ob.coef("kmod")
+    n = cfg.nx - 1
+    h = x_grid(cfg)[2] - x_grid(cfg)[1]
+    L = (np.diag(-2.0 * np.ones(n)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1))
+    L[0, 0] = L[-1, -1] = -1.0
+    ev = np.sort(-np.linalg.eigvalsh(L)) / h ** 2
+    assert np.abs(np.sort(km) - ev).max() < 1e-9 * ev.max()
+
+
+def test_mtr_is_div_of_grad(box):
+    """ayp (mtr, coeft.f90:1244-1332) equals the y part of div(grad) as pois
+    uses them: div with idy(j) on v differences, grad with idyy(j)."""
+    cfg, st, ob = box
+    ayp = ob.coef("ayp")
+    ny = cfg.ny
+    dy = np.diff(st["y"])
+    idy = 1.0 / dy
+    idyy = 2.0 / (dy[:-1] + dy[1:])      # idyy(1:ny)
+    # phi at p rows j = 1..ny-1 (rest rows); v_j = (phi_j - phi_{j-1}) idyy(j) at v rows 2..ny-1
+    n = ny - 1
+    G = np.zeros((ny, n))
+    for j in range(2, ny):               # v rows 2..ny-1 (1-based)
+        G[j - 1, j - 1] = idyy[j - 1]
+        G[j - 1, j - 2] = -idyy[j - 1]
+    D = np.zeros((n, ny))
+    for j in range(1, ny):               # div rows 1..ny-1: (v(j+1)-v(j)) idy(j)
+        D[j - 1, j] = idy[j]
+        D[j - 1, j - 1] = -idy[j]
+    L = D @ G
+    A = np.zeros((n, n))
+    for j in range(n):
+        A[j, j] = ayp[1, j]
+        if j > 0:
+            A[j, j - 1] = ayp[0, j]
+        if j < n - 1:
+            A[j, j + 1] = ayp[2, j]
+    assert np.abs(A - L).max() < 1e-9 * np.abs(L).max()
+
+
+def test_fourxz_matches_numpy(box, oracle_lib):
+    """c2r / r2c with the reference scaling (ffts.f90:48-58): unnormalised
+    inverse with modes above nz2 zeroed, forward times 1/nz."""
+    from oracle.oracle import ptr
+    cfg, st, ob = box
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal(cfg.nz1)
+    phys = np.zeros(cfg.nz)
+    oracle_lib.lib().oc_c2r(ob.h, ptr(X), ptr(phys))
+    spec = np.zeros(cfg.nz // 2 + 1, complex)
+    spec[:cfg.mz] = X[0::2] + 1j * X[1::2]
+    spec[0] = spec[0].real
+    ref = np.fft.irfft(spec, cfg.nz) * cfg.nz
+    assert np.abs(phys - ref).max() < 1e-13 * np.abs(ref).max()
+    back = np.zeros(cfg.nz1)
+    oracle_lib.lib().oc_r2c(ob.h, ptr(ref.copy()), ptr(back))
+    ref2 = np.fft.rfft(ref)[:cfg.mz] / cfg.nz
+    got = back[0::2] + 1j * back[1::2]
+    assert np.abs(got - ref2).max() < 1e-13
+    assert back[1] == 0.0
+
+
+@pytest.mark.parametrize("n", [10, 31, 255])
+def test_cosft_matches_scipy(oracle_lib, n):
+    """cosftx (ffts.f90:147-181): FFTW REDFT10 / n and REDFT01 * 0.5 are
+    scipy's unnormalised DCT-II / n and DCT-III * 0.5; round trip = 1."""
+    from oracle.oracle import ptr
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((3, n))
+    a = x.copy()
+    oracle_lib.lib().oc_cosft(n, 3, ptr(a), -1)
+    assert np.abs(a - sfft.dct(x, 2, axis=1) / n).max() < 1e-13
+    b = a.copy()
+    oracle_lib.lib().oc_cosft(n, 3, ptr(b), 1)
+    assert np.abs(b - x).max() < 1e-12
+
+
+def _div(cfg, st_or_box, u, v, w):
+    kaz = ic.kaz_values(cfg)
+    dy = np.diff(st_or_box["y"])
+    h0 = cfg.ax * np.pi / (cfg.nx + 1)
+    idx = 1.0 / (2.0 * h0 - 1.0 * h0)
+    ny = cfg.ny
+    return (w[1:, 1:ny, :] * 1j * kaz[None, None, :]
+            + (v[1:, 1:ny, :] - v[1:, 0:ny - 1, :]) / dy[1:ny][None, :, None]
+            + idx * (u[1:, 1:ny, :] - u[:-1, 1:ny, :]))
+
+
+def test_pois_projects_to_zero_divergence(oracle_lib):
+    """After outflow_correction and pois (pois.f90:11-189) the discrete
+    divergence is at round-off on planes 2..nx, for a state with O(1e-2)
+    divergence."""
+    cfg = config.small()
+    st = dict(ic.make_state(cfg))
+    rng = np.random.default_rng(9)
+    for name in ("u", "v", "w"):
+        a = st[name].copy()
+        shp = a[1:, 1:-1, :3].shape
+        a[1:, 1:-1, :3] += 1e-2 * (rng.standard_normal(shp) + 1j * rng.standard_normal(shp))
+        a[:, :, 0] = a[:, :, 0].real
+        st[name] = a
+    ob = oracle_lib.OracleBox(cfg, st)
+    d0 = np.abs(_div(cfg, st, ob.get("u"), ob.get("v"), ob.get("w"))).max()
+    ob.outflow_correction()
+    ob.pois(1.0)
+    d1 = np.abs(_div(cfg, st, ob.get("u"), ob.get("v"), ob.get("w"))).max()
+    assert d0 > 1e-2 and d1 < 1e-12 * d0 * 1e3
+
+
+def test_outflow_correction_zeroes_net_flux(oracle_lib):
+    """The mass-flux correction (boun.f90:63-93) makes the net flux of the
+    mean mode through inflow, outflow and top exactly zero (to round-off)."""
+    cfg = config.small()
+    st = dict(ic.make_state(cfg))
+    st["u"] = st["u"].copy()
+    st["u"][-1, 1:cfg.ny, 0] *= 1.01
+    ob = oracle_lib.OracleBox(cfg, st)
+    ob.outflow_correction()
+    u, v = ob.get("u"), ob.get("v")
+    dy = np.diff(st["y"])
+    h0 = cfg.ax * np.pi / (cfg.nx + 1)
+    dx1 = 2.0 * h0 - 1.0 * h0
+    net = (v[1:, cfg.ny - 1, 0].real.sum() * dx1
+           - (u[0, 1:cfg.ny, 0].real * dy[1:cfg.ny]).sum()
+           + (u[-1, 1:cfg.ny, 0].real * dy[1:cfg.ny]).sum())
+    assert abs(net) < 1e-13
+
+
+def test_boundary_rows_satisfy_their_identities(oracle_lib):
+    """After rhsp (boun, boun.f90:7-56, and the right-hand-side rows,
+    rhsp.F90:423-445): v = 0 at the wall, the top mean v equals vmagic
+    (implzy keeps rows 1 and ny), the saved rhs is zero on plane 1, on v rows
+    1 and ny and on u, w row ny+1, and its row 1 satisfies the wall
+    extrapolation inby(2,:)."""
+    cfg = config.small()
+    st = ic.make_state(cfg)
+    ob = oracle_lib.OracleBox(cfg, st)
+    ob.rhsp(1)
+    v = ob.get("v")
+    inby, vm = ob.coef("inby"), ob.coef("vmagic")
+    assert np.abs(v[:, 0]).max() == 0.0
+    assert np.abs(v[:, -1, 0].real - vm).max() == 0.0
+    assert np.abs(v[:, -1, 1:]).max() == 0.0
+    ru, rv, rw = ob.get("rhsupa"), ob.get("rhsvpa"), ob.get("rhswpa")
+    assert np.abs(ru[0]).max() == 0 and np.abs(rv[0]).max() == 0 and np.abs(rw[0]).max() == 0
+    assert np.abs(rv[:, 0]).max() == 0 and np.abs(rv[:, -1]).max() == 0
+    for a in (ru, rw):
+        assert np.abs(a[:, -1]).max() == 0
+        r1 = inby[1, 0] * a[:, 0] + inby[1, 1] * a[:, 1] + inby[1, 2] * a[:, 2] + inby[1, 3] * a[:, 3]
+        assert np.abs(r1).max() < 1e-13 * max(1.0, np.abs(a).max())
+
+
+def test_genflu_identity_rescaling(oracle_lib):
+    """genflu with Re_theta,out = Re_theta,in (gamma = 1): the rescaled
+    inflow modes >= 1 equal the blended recycled plane (the interpolation
+    weights reduce to the identity), the mode-0 plane is u0/v0."""
+    cfg = config.small()
+    st = ic.make_state(cfg)
+    ob = oracle_lib.OracleBox(cfg, st)
+    u_before = ob.get("u")
+    ob.rhsp(1)
+    info = ob.genflu_info()
+    assert info["gamma2"] == pytest.approx(1.0, abs=1e-3)
+    u = ob.get("u")
+    # rows 2..ny of the mean mode are u0 (boun then rewrites the ghost rows)
+    assert np.abs(u[0, 1:cfg.ny, 0].real - st["u0"][1:cfg.ny]).max() == 0.0
+    # with gamma ~ 1 the rescaled plane stays close to the blended copy of plane xout
+    y = st["y"]
+    jy = min(119, cfg.ny)
+    fb = 0.5 * (1 - np.tanh(y[1:cfg.ny + 2] - y[jy]))
+    ref = u_before[cfg.xout - 1, :, 1:] * fb[:, None]
+    assert np.abs(u[0, 3:-3, 1:] - ref[3:-3]).max() < 5e-2 * np.abs(ref).max()
+
+
+def test_viscous_decay_of_a_spanwise_mode(oracle_lib):
+    """A single spanwise mode on a uniform-in-x, y-independent interior field
+    decays by the explicit z-viscous term of the RK3 step: after one
+    substep the mode changes by ~ -gamma dt nu kz^2 u (rhsp.F90:355-365 and
+    the combination rhsp.F90:448-475)."""
+    cfg = config.small(kzmax=0)
+    st = dict(ic.make_state(cfg, with_perturbation=False))
+    st["u"] = np.zeros_like(st["u"])
+    st["v"] = np.zeros_like(st["v"])
+    st["w"] = np.zeros_like(st["w"])
+    eps = 1e-6
+    st["u"][:, :, 1] = eps
+    ob = oracle_lib.OracleBox(cfg, st)
+    ob.set("u", st["u"])
+    ob.rhsp(1)
+    dt = ob.time()[1]
+    kz = ic.kaz_values(cfg)[1]
+    u = ob.get("u")
+    i, j = cfg.nx // 2, cfg.ny // 2
+    expect = eps * (1.0 - (8.0 / 15.0) * dt * kz ** 2 / st["re"])
+    # the implicit y step leaves a y-constant interior unchanged
+    assert u[i, j, 1].real == pytest.approx(expect, rel=1e-8)
+
+
+def test_fp32_transposes_round_at_the_transposes(oracle_lib):
+    """The reference rounds through FP32 at every plane<->pencil transpose
+    (change.f90:565-686).  The oracle's emulation of that differs from the
+    FP64-transpose variant at the FP32 level, not more."""
+    cfg = config.small()
+    st = ic.make_state(cfg)
+    a = oracle_lib.OracleBox(cfg, st)
+    b = oracle_lib.OracleBox(cfg, st, fp32_transposes=True)
+    a.step()
+    b.step()
+    ua, ub = a.get("u"), b.get("u")
+    rel = np.abs(ua - ub).max() / np.abs(ua).max()
+    assert 1e-9 < rel < 1e-5
