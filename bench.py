@@ -58,6 +58,26 @@ def kernel_model(cfg):
     return m
 
 
+# kernel-event names -> HIP kernel function names in rocprofv3 output
+TRACE_NAMES = {"dct2": "dct_rows_kernel", "dct3": "dct_rows_kernel", "div": "div_pencils_kernel",
+               "poisson_y": "poisson_y_pencil_kernel", "to_planes": "pencils_to_planes_kernel"}
+
+
+def profiled_traffic(workload, precision, kname):
+    """HBM bytes per launch of one kernel from the committed rocprofv3 summary
+    of this workload (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by
+    what the counter reads for a plain 1 GiB device copy); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"substep_kernels_{workload}_{precision}.md")
+    if not os.path.exists(path):
+        return None
+    fn = TRACE_NAMES.get(kname, kname + "_kernel")
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 7 and cells[0] == fn and cells[5] and cells[6]:
+            return (float(cells[5]) + float(cells[6])) * 1e9
+    return None
+
+
 def cpu_baseline(cfg_full):
     """The CPU FP64 oracle on a bounded sample of the same workload: one
     substep of an nx=32 slab with the box's ny, nz (single thread)."""
@@ -98,6 +118,13 @@ def main():
     box = GpuBox(cfg, st, precision=OTBL_FP64, dt_mode=OTBL_DT_EVERY_STEP,
                  graphs=not args.no_graphs)
     del st
+    # a plain device copy of 1 GiB: the known-size reference for hardware
+    # byte counters (FETCH_SIZE / WRITE_SIZE) read by rocprofv3 on this command
+    from opentbl_amd.lib import DevArray, lib as _lib, check as _check
+    _a, _b = DevArray(1 << 27), DevArray(1 << 27)
+    _check(_lib().otbl_d2d(_b.ptr, _a.ptr, 1 << 30), "d2d")
+    _a.free()
+    _b.free()
     for w in range(args.warmup):
         box.step(1)
         box.sync()
@@ -140,6 +167,7 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": gbs / HBM_PEAK_GBS, "traffic": None}
+    roof["traffic"] = profiled_traffic(cfg.name, "fp64", name)
     roof["kernel"] = name
     roof["avg_ms"] = avg
     roof["algorithmic_bytes"] = by

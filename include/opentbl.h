@@ -120,6 +120,9 @@ int otbl_dev_alloc(size_t bytes, void **ptr);
 int otbl_dev_free(void *ptr);
 int otbl_h2d(void *dst, const void *src, size_t bytes);
 int otbl_d2h(void *dst, const void *src, size_t bytes);
+/* device-to-device copy on the device's default stream (also the plain copy
+ * of known size that calibrates hardware byte counters) */
+int otbl_d2d(void *dst, const void *src, size_t bytes);
 
 /* ---- measurement ----
  * With kprof on, every kernel launch is bracketed by HIP events on the

@@ -686,6 +686,13 @@ int otbl_d2h(void *dst, const void *src, size_t bytes) {
   API_END
 }
 
+int otbl_d2d(void *dst, const void *src, size_t bytes) {
+  API_BEGIN
+  OTBL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  OTBL_HIP(hipDeviceSynchronize());
+  API_END
+}
+
 int otbl_kprof(otbl_ctx *ctx, int on) {
   API_BEGIN
   Box &b = box_of(ctx);

@@ -62,6 +62,7 @@ SIGNATURES = {
     "otbl_dev_free": (I, [VP]),
     "otbl_h2d": (I, [VP, VP, SZ]),
     "otbl_d2h": (I, [VP, VP, SZ]),
+    "otbl_d2d": (I, [VP, VP, SZ]),
     "otbl_kprof": (I, [VP, I]),
     "otbl_kprof_report": (I, [VP, C.c_char_p, SZ]),
 }
