@@ -9,6 +9,7 @@
 #include <string>
 
 #include "otbl_internal.h"
+#include "otbl_ycol.h"
 
 namespace otbl {
 
@@ -246,7 +247,8 @@ static void free_box(Box *b) {
     if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
   double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
-                    b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4};
+                    b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
+                    b->ytab};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
@@ -385,6 +387,13 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     build_dct(*b);
   k_setup_attrs(*b);
   k_factor_coef(*b, b->coefbuf);
+  // chunked y-line solver unless the columns are longer than its
+  // instantiations (or OTBL_Y_MARCH asks for one thread per column)
+  b->yL = std::getenv("OTBL_Y_MARCH") ? 0 : ytab_rows(b->ny);
+  if (b->yL) {
+    dalloc(b->ytab, YT_COUNT * ytab_size(b->yL));
+    k_ytab_fixed(*b);
+  }
   OTBL_HIP(hipStreamSynchronize(b->s));
   *out = new otbl_ctx{b};
   b = nullptr;

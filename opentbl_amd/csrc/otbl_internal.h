@@ -80,6 +80,10 @@ struct Box {
   double *dct2 = nullptr, *dct3 = nullptr;
   // implicit y LU (dt dependent), u-type and v-type: a1, a2, a3 rows
   double *implu = nullptr, *implv = nullptr;
+  // chunked y-line solver: rows per chunk (0: one thread marches each
+  // column) and the tables of its eight systems
+  int yL = 0;
+  double *ytab = nullptr;
   // genflu / inflow
   double *um = nullptr, *u0 = nullptr, *v0 = nullptr, *gwork = nullptr;
   DevScalars *ds = nullptr;

@@ -12,30 +12,15 @@
 //    (kr, i); the thread marches over rows, again reading consecutive kr.
 // The z transforms work on one (j, i) row per workgroup in LDS (mixed-radix
 // Stockham, radices 8/4/2/3/5/7), two real signals per complex transform.
-#include "otbl_internal.h"
+#include "otbl_kcommon.h"
 #include "otbl_fft.h"
+#include "otbl_ycol.h"
 
 #include <cmath>
 #include <cstdio>
 
 namespace otbl {
 
-#define C3(a, r, i) (a)[3 * (size_t)(i) + (r)-1]
-#define C2(a, r, i) (a)[2 * (size_t)(i) + (r)-1]
-#define M4(a, r, c) (a)[4 * ((c)-1) + (r)-1]
-#define M5(a, r, c) (a)[5 * ((c)-1) + (r)-1]
-#define M2(a, r, c) (a)[2 * ((c)-1) + (r)-1]
-
-static constexpr int TPB = 256;
-
-static inline unsigned nblk(size_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
-
-#define LAUNCH(b, name, grid, block, shm, kern, ...)                          \
-  do {                                                                         \
-    kbegin(b, name);                                                           \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shm, (b).s, __VA_ARGS__); \
-    kend(b);                                                                   \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // coefficient factorisation (coefinv, coeft.f90:1338-1353) in FP64
@@ -252,6 +237,10 @@ __global__ void interpyy_kernel(const double *__restrict__ gen, double *__restri
 
 void k_interpyy(Box &b, const double *gen, double *geni, int loguv, int i0, int nplanes,
                 size_t ips, size_t ops) {
+  if (b.yL) {
+    ky_interpyy(b, gen, geni, loguv, i0, nplanes, ips, ops);
+    return;
+  }
   size_t n = (size_t)b.nz1 * nplanes;
   LAUNCH(b, "interpyy", nblk(n), TPB, 0, interpyy_kernel, gen, geni,
          loguv == 0 ? b.dc.inyu : b.dc.inyv, loguv == 0 ? b.dc.cofiuy : b.dc.cofivy, b.dc.inby,
@@ -301,6 +290,10 @@ __global__ void differzy_kernel(const double *__restrict__ gen, double *__restri
 
 void k_differzy(Box &b, const double *geni, double *res, int which, int nplanes, size_t ips,
                 size_t ops) {
+  if (b.yL) {
+    ky_differzy(b, geni, res, which, nplanes, ips, ops);
+    return;
+  }
   size_t n = (size_t)b.nz1 * nplanes;
   LAUNCH(b, "differzy", nblk(n), TPB, 0, differzy_kernel, geni, res, b.dc, which, b.ny, b.nz1,
          nplanes, ips, ops);
@@ -334,6 +327,10 @@ __global__ void vistyy_kernel(const double *__restrict__ u, double *__restrict__
 }
 
 void k_vistyy(Box &b, const double *u, double *res, int R, int nplanes) {
+  if (b.yL) {
+    ky_vistyy(b, u, res, R, nplanes);
+    return;
+  }
   size_t n = (size_t)b.nz1 * nplanes;
   LAUNCH(b, "vistyy", nblk(n), TPB, 0, vistyy_kernel, u, res, b.dc, R, b.ny, b.nz1, nplanes);
 }
@@ -364,10 +361,23 @@ __device__ void impl_factor(double *f, const double *icon, const double *econ, i
 }
 
 __global__ void impl_factor_kernel(double *fu, double *fv, DevCoef c, int ny, const DevScalars *ds,
-                                   double rkk_explicit, int use_explicit) {
+                                   double rkk_explicit, int use_explicit, double *tu, double *tv, int L) {
   double rkk = use_explicit ? rkk_explicit : ds->rkk;
-  if (threadIdx.x == 0) impl_factor(fu, c.vyui, c.cofvyu, ny + 1, rkk);
-  if (threadIdx.x == 1) impl_factor(fv, c.vyvi, c.cofvyv, ny, rkk);
+  if (threadIdx.x == 0) {
+    impl_factor(fu, c.vyui, c.cofvyu, ny + 1, rkk);
+    if (L) ytab_impl(tu, fu, c.vyui, ny + 1, L);
+  }
+  if (threadIdx.x == 1) {
+    impl_factor(fv, c.vyvi, c.cofvyv, ny, rkk);
+    if (L) ytab_impl(tv, fv, c.vyvi, ny, L);
+  }
+}
+
+static void launch_impl_factor(Box &b, double rkk, int use_explicit) {
+  double *tu = b.yL ? b.ytab + YT_IMPU * ytab_size(b.yL) : nullptr;
+  double *tv = b.yL ? b.ytab + YT_IMPV * ytab_size(b.yL) : nullptr;
+  LAUNCH(b, "impl_factor", 1, 64, 0, impl_factor_kernel, b.implu, b.implv, b.dc, b.ny, b.ds, rkk, use_explicit,
+         tu, tv, b.yL);
 }
 
 // implzy (lip.f90:528-597): interior rows 2..jee-1 of u from res; rows 1 and
@@ -412,7 +422,11 @@ __global__ void implzy_kernel(double *u, const double *res, DevCoef c, const dou
 }
 
 void k_implzy_op(Box &b, double *u, const double *res, int R, double rkk, int nplanes) {
-  LAUNCH(b, "impl_factor", 1, 64, 0, impl_factor_kernel, b.implu, b.implv, b.dc, b.ny, b.ds, rkk, 1);
+  launch_impl_factor(b, rkk, 1);
+  if (b.yL) {
+    ky_implzy(b, u, res, R, nplanes);
+    return;
+  }
   size_t n = (size_t)b.nz1 * nplanes;
   LAUNCH(b, "implzy", nblk(n), TPB, 0, implzy_kernel, u, res, b.dc, R == b.ny + 1 ? b.implu : b.implv,
          R, b.ny, b.nz1, nplanes);
@@ -1417,8 +1431,11 @@ void k_substep_rhsp(Box &b, int m) {
   const size_t shm = zfft_shm(b, 4);
   LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel, u, b.wki1, b.wki2, b.wki3, b.yc,
          plan, ny, nz1, b.nz2, b.ds, b.vmbits);
-  LAUNCH(b, "rhsu_loop1", nblk((size_t)nz1 * nx), TPB, 0, rhsu_loop1_kernel, b.wki2, b.wki3, b.rhsu, b.dc,
-         ny, nz1, nx);
+  if (b.yL)
+    ky_rhsu_loop1(b, b.wki2, b.wki3, b.rhsu);
+  else
+    LAUNCH(b, "rhsu_loop1", nblk((size_t)nz1 * nx), TPB, 0, rhsu_loop1_kernel, b.wki2, b.wki3, b.rhsu, b.dc,
+           ny, nz1, nx);
   // x derivatives (rhsp.F90:181-204)
   k_difvisxx(b, b.wki1, u, b.rhsu, ru, 1, 1);
   k_difvisxx(b, b.wki2, v, b.rhsv, ny, 0, 0);
@@ -1440,14 +1457,23 @@ void k_substep_rhsp(Box &b, int m) {
     k_interpyy(b, b.resv, b.yb, 1, 2, nx - 2, b.pv, b.pu);
     LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, b.resv, b.resw, b.ya, b.yb, b.yc,
            plan, ny, nz1, b.nz2);
-    LAUNCH(b, "rhs_loop2", nblk((size_t)nz1 * (nx - 2)), TPB, 0, rhs_loop2_kernel, b.rhsu, b.rhsv, b.rhsw,
-           b.resu, b.resv, b.resw, b.ya, b.yb, b.yc, b.wki1, b.dc, ny, nz1, nx - 2);
+    if (b.yL)
+      ky_rhs_loop2(b, b.ya, b.yb, b.yc);
+    else
+      LAUNCH(b, "rhs_loop2", nblk((size_t)nz1 * (nx - 2)), TPB, 0, rhs_loop2_kernel, b.rhsu, b.rhsv, b.rhsw,
+             b.resu, b.resv, b.resw, b.ya, b.yb, b.yc, b.wki1, b.dc, ny, nz1, nx - 2);
   }
   // boundary conditions (rhsp.F90:421-445)
   LAUNCH(b, "boun", nblk((size_t)nz1 * nx), TPB, 0, boun_kernel, u, v, w, b.rhsu, b.rhsv, b.rhsw, b.dc, ny,
          nz1, nx);
   // implicit factors for this dt, then the final update per component
-  LAUNCH(b, "impl_factor", 1, 64, 0, impl_factor_kernel, b.implu, b.implv, b.dc, ny, b.ds, 0.0, 0);
+  launch_impl_factor(b, 0.0, 0);
+  if (b.yL) {
+    ky_rk_final(b, v, b.resv, b.rhsv, b.f[OTBL_RHSVPA], ny);
+    ky_rk_final(b, u, b.resu, b.rhsu, b.f[OTBL_RHSUPA], ny + 1);
+    ky_rk_final(b, w, b.resw, b.rhsw, b.f[OTBL_RHSWPA], ny + 1);
+    return;
+  }
   size_t ncol = (size_t)nz1 * nx;
   LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, v, b.resv, b.rhsv, b.f[OTBL_RHSVPA], b.wki2, b.dc,
          b.implv, b.ds, ny, ny, nz1, nx);

@@ -1,0 +1,114 @@
+// Chunked y-line solver: constants, the table builder shared by the kernel
+// translation units, and the launchers of otbl_ycol.hip.
+#pragma once
+#include "otbl_kcommon.h"
+
+namespace otbl {
+
+static constexpr int YNCH = 16;           // chunks (waves) per column
+static constexpr int YTPB = 64 * YNCH;    // threads per workgroup
+
+// Table of one chunked y system, YNCH*L rows (NL) in chunk order:
+//   M, A2, B   factored tridiagonal: y_j = r_j + M_j y_{j-1},
+//              x_j = A2_j y_j + B_j x_{j+1}
+//   G, S       responses of a chunk to unit forward / backward carries
+//   E1, E2, E3 the explicit 3-point stencil whose result the system solves
+//              for: rhs_j = E1 g(j-1) + E2 g(j) + E3 g(j+1) (interior rows)
+//   PE, G0, S0 per chunk: carry gain (product of M), G and S at the start
+__host__ __device__ constexpr size_t ytab_size(int L) { return (size_t)8 * YNCH * L + 3 * YNCH; }
+
+// rows per chunk for a box with ny (u columns have ny+1 rows); 0 when the
+// columns are too long for the instantiated chunk lengths
+inline int ytab_rows(int ny) {
+  const int need = (ny + 1 + YNCH - 1) / YNCH;
+  for (int L : {4, 8, 12, 16, 20, 24})
+    if (L >= need) return L;
+  return 0;
+}
+
+// tables in Box::ytab, in this order
+enum { YT_INYU = 0, YT_INYV, YT_DCYU, YT_DCYV, YT_VIYU, YT_VIYV, YT_IMPU, YT_IMPV, YT_COUNT };
+
+struct TriRow {
+  double m, a2, c;
+};
+struct Sten3 {
+  double e1, e2, e3;
+};
+
+// Builds the table of a system of n rows from 1-based accessors: row(j) the
+// factored tridiagonal row, sten(j) the stencil of interior row j
+// (2 <= j <= n-1; zero elsewhere).  Rows beyond n are zero, so a partly
+// empty last chunk needs no special case.
+template <class F, class E>
+__device__ inline void ytab_build(double *tab, int n, int L, F row, E sten) {
+  const int NL = YNCH * L;
+  double *M = tab, *A2 = tab + NL, *B = tab + 2 * NL, *G = tab + 3 * NL, *S = tab + 4 * NL;
+  double *E1 = tab + 5 * NL, *E2 = tab + 6 * NL, *E3 = tab + 7 * NL;
+  double *PE = tab + 8 * NL, *G0 = PE + YNCH, *S0 = G0 + YNCH;
+  for (int j = 0; j < NL; ++j) {
+    if (j < n) {
+      const TriRow t = row(j + 1);
+      M[j] = j == 0 ? 0.0 : t.m;
+      A2[j] = t.a2;
+      B[j] = j == n - 1 ? 0.0 : -t.c * t.a2;
+    } else {
+      M[j] = A2[j] = B[j] = 0.0;
+    }
+    const Sten3 e = (j >= 1 && j <= n - 2) ? sten(j + 1) : Sten3{0.0, 0.0, 0.0};
+    E1[j] = e.e1;
+    E2[j] = e.e2;
+    E3[j] = e.e3;
+  }
+  for (int q = 0; q < YNCH; ++q) {
+    double P = 1.0;
+    for (int r = 0; r < L; ++r) {
+      P *= M[q * L + r];
+      G[q * L + r] = P; // forward response to a unit carry-in
+    }
+    PE[q] = P;
+    double g = 0.0, sa = 1.0;
+    for (int r = L - 1; r >= 0; --r) {
+      const int j = q * L + r;
+      g = A2[j] * G[j] + B[j] * g;
+      G[j] = g;
+      sa = B[j] * sa;
+      S[j] = sa;
+    }
+    G0[q] = G[q * L];
+    S0[q] = S[q * L];
+  }
+}
+
+// matrices factored by coefinv (coeft.f90:1338-1353): a(1,j) multiplier,
+// a(2,j) inverse pivot, a(3,j) super-diagonal
+__device__ inline TriRow coefinv_row(const double *a, int j) {
+  return TriRow{C3(a, 1, j), C3(a, 2, j), C3(a, 3, j)};
+}
+
+// Table of the implicit y-viscous system (implzy, lip.f90:528-597) factored
+// by impl_factor: rows 2..jee-1 as factored, rows 1 and jee identity rows
+// carrying the boundary values; stencil: the icon rows (vyui / vyvi).
+__device__ inline void ytab_impl(double *tab, const double *f, const double *icon, int jee, int L) {
+  const double *a1 = f, *a2 = f + (jee + 1), *a3 = f + 2 * (jee + 1);
+  ytab_build(
+      tab, jee, L,
+      [=](int j) {
+        if (j == 1 || j == jee) return TriRow{0.0, 1.0, 0.0};
+        return TriRow{a1[j], a2[j], a3[j]};
+      },
+      [=](int j) { return Sten3{C3(icon, 1, j), C3(icon, 2, j), C3(icon, 3, j)}; });
+}
+
+// launchers (otbl_ycol.hip); valid when Box::yL > 0
+void k_ytab_fixed(Box &b);
+void ky_interpyy(Box &b, const double *gen, double *geni, int loguv, int i0, int nplanes, size_t ips,
+                 size_t ops);
+void ky_differzy(Box &b, const double *geni, double *res, int which, int nplanes, size_t ips, size_t ops);
+void ky_vistyy(Box &b, const double *u, double *res, int R, int nplanes);
+void ky_implzy(Box &b, double *u, const double *res, int R, int nplanes);
+void ky_rhsu_loop1(Box &b, const double *uv, const double *uw, double *rhsu);
+void ky_rhs_loop2(Box &b, const double *vw, const double *vv, const double *ww);
+void ky_rk_final(Box &b, double *u, const double *res, const double *rhs, double *rhsp, int jee);
+
+} // namespace otbl
