@@ -336,48 +336,64 @@ void k_vistyy(Box &b, const double *u, double *res, int R, int nplanes) {
 }
 
 // Implicit y-viscous factors for the current dt (implzy's a1, a2, a3 and
-// their LU, lip.f90:545-568), computed once per substep in FP64.
+// their LU, lip.f90:545-568) in FP64, one workgroup per set (blockIdx.x 0:
+// u-type, jee = ny+1; 1: v-type, jee = ny).  The rows are formed by all
+// threads, the (sequential) factorisation runs in LDS, then the chunked
+// solver's table of the factored system is built.
 // layout per set (n = jee): f[0..n] a1 factored, f[n+1..2n+1] a2 (inverse
 // diagonal), f[2n+2..3n+2] a3, f[3n+3] = a1(2) and f[3n+4] = a3(jee-1)
-// before they are zeroed.
-__device__ void impl_factor(double *f, const double *icon, const double *econ, int jee, double v1) {
-  double *a1 = f, *a2 = f + (jee + 1), *a3 = f + 2 * (jee + 1);
-  for (int j = 2; j <= jee - 1; ++j) {
+// before they are zeroed.  Dynamic LDS: 3*(ny+2) + 4*YNCH*L doubles.
+__global__ void impl_factor_kernel(double *fu, double *fv, DevCoef c, int ny, const DevScalars *ds,
+                                   double rkk_explicit, int use_explicit, double *tu, double *tv, int L) {
+  extern __shared__ double shf[];
+  const int jee = blockIdx.x == 0 ? ny + 1 : ny;
+  const double *icon = blockIdx.x == 0 ? c.vyui : c.vyvi;
+  const double *econ = blockIdx.x == 0 ? c.cofvyu : c.cofvyv;
+  double *f = blockIdx.x == 0 ? fu : fv;
+  double *tab = blockIdx.x == 0 ? tu : tv;
+  const double v1 = use_explicit ? rkk_explicit : ds->rkk;
+  double *a1 = shf, *a2 = shf + (ny + 2), *a3 = shf + 2 * (ny + 2);
+  for (int j = threadIdx.x + 2; j <= jee - 1; j += blockDim.x) {
     a1[j] = C3(icon, 1, j) + v1 * C3(econ, 3, j);
     a2[j] = C3(icon, 2, j) + v1 * C3(econ, 2, j);
     a3[j] = C3(icon, 3, j) + v1 * C3(econ, 1, j);
   }
-  f[3 * (jee + 1)] = a1[2];
-  f[3 * (jee + 1) + 1] = a3[jee - 1];
-  a1[2] = 0.0;
-  a3[jee - 1] = 0.0;
-  for (int j = 3; j <= jee - 1; ++j) {
-    double d = 1.0 / a2[j - 1];
-    a2[j - 1] = d;
-    a1[j] = -a1[j] * d;
-    a2[j] = a2[j] + a1[j] * a3[j - 1];
-  }
-  a2[jee - 1] = 1.0 / a2[jee - 1];
-}
-
-__global__ void impl_factor_kernel(double *fu, double *fv, DevCoef c, int ny, const DevScalars *ds,
-                                   double rkk_explicit, int use_explicit, double *tu, double *tv, int L) {
-  double rkk = use_explicit ? rkk_explicit : ds->rkk;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    impl_factor(fu, c.vyui, c.cofvyu, ny + 1, rkk);
-    if (L) ytab_impl(tu, fu, c.vyui, ny + 1, L);
+    f[3 * (jee + 1)] = a1[2];
+    f[3 * (jee + 1) + 1] = a3[jee - 1];
+    a1[2] = 0.0;
+    a3[jee - 1] = 0.0;
+    for (int j = 3; j <= jee - 1; ++j) {
+      double d = 1.0 / a2[j - 1];
+      a2[j - 1] = d;
+      a1[j] = -a1[j] * d;
+      a2[j] = a2[j] + a1[j] * a3[j - 1];
+    }
+    a2[jee - 1] = 1.0 / a2[jee - 1];
   }
-  if (threadIdx.x == 1) {
-    impl_factor(fv, c.vyvi, c.cofvyv, ny, rkk);
-    if (L) ytab_impl(tv, fv, c.vyvi, ny, L);
+  __syncthreads();
+  for (int j = threadIdx.x + 2; j <= jee - 1; j += blockDim.x) {
+    f[j] = a1[j];
+    f[(jee + 1) + j] = a2[j];
+    f[2 * (jee + 1) + j] = a3[j];
   }
+  if (L)
+    ytab_build_block(
+        tab, jee, L,
+        [=](int j) {
+          if (j == 1 || j == jee) return TriRow{0.0, 1.0, 0.0};
+          return TriRow{a1[j], a2[j], a3[j]};
+        },
+        [=](int j) { return Sten3{C3(icon, 1, j), C3(icon, 2, j), C3(icon, 3, j)}; }, shf + 3 * (ny + 2));
 }
 
 static void launch_impl_factor(Box &b, double rkk, int use_explicit) {
   double *tu = b.yL ? b.ytab + YT_IMPU * ytab_size(b.yL) : nullptr;
   double *tv = b.yL ? b.ytab + YT_IMPV * ytab_size(b.yL) : nullptr;
-  LAUNCH(b, "impl_factor", 1, 64, 0, impl_factor_kernel, b.implu, b.implv, b.dc, b.ny, b.ds, rkk, use_explicit,
-         tu, tv, b.yL);
+  const size_t shm = (3 * (size_t)(b.ny + 2) + 4 * (size_t)YNCH * b.yL) * sizeof(double);
+  LAUNCH(b, "impl_factor", 2, TPB, shm, impl_factor_kernel, b.implu, b.implv, b.dc, b.ny, b.ds, rkk,
+         use_explicit, tu, tv, b.yL);
 }
 
 // implzy (lip.f90:528-597): interior rows 2..jee-1 of u from res; rows 1 and
@@ -792,56 +808,45 @@ __global__ void reset_max_kernel(DevScalars *ds, unsigned long long *vmbits, int
   }
 }
 
-// energies / ministats (rhsp.F90:116,493-509, stotf.F90:395-440) at
-// planes 1 and nx/2, one block.
-__global__ void energies_kernel(const double *u, const double *v, const double *w, const double *hy,
-                                DevScalars *ds, int nx, int ny, int nz1, int nz2) {
-  __shared__ double red[TPB];
-  double uner[16];
-  for (int q = 0; q < 16; ++q) uner[q] = 0.0;
-  const double *fl[3] = {u, v, w};
-  int rows[3] = {ny + 1, ny, ny + 1};
-  int ih = nx / 2;
-  for (int pass = 0; pass < 2; ++pass) {
-    int i = pass == 0 ? 1 : ih;
-    for (int c = 0; c < 3; ++c) {
-      const double *a = fl[c] + (size_t)(i - 1) * rows[c] * nz1;
-      double r01 = a[0];
-      double part = 0.0;
-      for (int j = threadIdx.x + 1; j <= rows[c]; j += blockDim.x) {
-        double aux = 0.0;
-        int k0 = pass == 0 ? 0 : 1;
-        for (int k = k0; k <= nz2; ++k) {
-          double re = a[(size_t)(j - 1) * nz1 + 2 * k], im = a[(size_t)(j - 1) * nz1 + 2 * k + 1];
-          aux += re * re + im * im;
-        }
-        if (pass == 1) aux = 2.0 * aux + r01 * r01;
-        part += aux * hy[j - 1];
-      }
-      red[threadIdx.x] = part;
-      __syncthreads();
-      for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) uner[1 + c] = red[0];
-      __syncthreads();
-      if (pass == 1 && threadIdx.x == 0) {
-        double a1 = 0.0, a2 = 0.0;
-        for (int k = 1; k <= nz2; ++k) {
-          a1 += a[2 * k] * a[2 * k] + a[2 * k + 1] * a[2 * k + 1];
-          a2 += a[nz1 + 2 * k] * a[nz1 + 2 * k] + a[nz1 + 2 * k + 1] * a[nz1 + 2 * k + 1];
-        }
-        uner[4 + 4 * c] = a[0];
-        uner[5 + 4 * c] = a[nz1];
-        uner[6 + 4 * c] = 2.0 * a1;
-        uner[7 + 4 * c] = 2.0 * a2;
-      }
-    }
+// energies / ministats (rhsp.F90:116,493-509, stotf.F90:395-440).  The
+// reference's plane-1 sums land in the slots the plane nx/2 sums overwrite,
+// so only plane nx/2 is evaluated: block c sums hy_j (2 sum_{k>=1} |a_jk|^2
+// + r01^2) over the rows of component c (r01 = Re of mode 0, row 1) and
+// records rows 1-2 of mode 0 and their mode sums; slots 13-15 are cleared
+// afterwards (rhsp.F90:487).
+__global__ void __launch_bounds__(1024)
+energies_kernel(const double *u, const double *v, const double *w, const double *hy, DevScalars *ds, int nx,
+                int ny, int nz1, int nz2) {
+  __shared__ double red[16];
+  const int c = blockIdx.x;
+  const double *fl = c == 0 ? u : (c == 1 ? v : w);
+  const int rows = c == 1 ? ny : ny + 1;
+  const double *a = fl + (size_t)(nx / 2 - 1) * rows * nz1;
+  const double r01 = a[0];
+  double part = 0.0;
+  for (size_t e = threadIdx.x; e < (size_t)rows * nz1; e += blockDim.x) {
+    const int j = (int)(e / nz1), kr = (int)(e % nz1);
+    const double x = a[e];
+    if (kr >= 2 && kr <= 2 * nz2 + 1) part += 2.0 * x * x * hy[j];
   }
+  for (int j = threadIdx.x; j < rows; j += blockDim.x) part += r01 * r01 * hy[j];
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = part;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    for (int q = 1; q <= 15; ++q) ds->ener[q] = uner[q];
-    ds->ener[13] = ds->ener[14] = ds->ener[15] = 0.0; // rhsp.F90:487
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x / 64); ++q) t += red[q];
+    double a1 = 0.0, a2 = 0.0;
+    for (int k = 1; k <= nz2; ++k) {
+      a1 += a[2 * k] * a[2 * k] + a[2 * k + 1] * a[2 * k + 1];
+      a2 += a[nz1 + 2 * k] * a[nz1 + 2 * k] + a[nz1 + 2 * k + 1] * a[nz1 + 2 * k + 1];
+    }
+    ds->ener[1 + c] = t;
+    ds->ener[4 + 4 * c] = a[0];
+    ds->ener[5 + 4 * c] = a[nz1];
+    ds->ener[6 + 4 * c] = 2.0 * a1;
+    ds->ener[7 + 4 * c] = 2.0 * a2;
+    if (c == 2) ds->ener[13] = ds->ener[14] = ds->ener[15] = 0.0;
   }
 }
 
@@ -929,60 +934,20 @@ __global__ void dt_kernel(DevScalars *ds, const unsigned long long *vmbits, cons
   (void)m;
 }
 
-// genflu (genflu.F90:3-167, NEWGENFLU branch), one workgroup.  Reads the
-// recycling plane xout from the saved velocities (res*, equal to u, v, w
-// before the predictor) and rewrites plane 1 of u, v, w.
-__device__ void genflu_profile(double *ut, double *wk, int ntop, const int *kl, const double *fsca,
-                               int n, int ny, int nz1, double ggg) {
-  const int s = nz1, nk = nz1 - 2;
-  // the copy of row ntop-3 below needs ntop >= 4 (genflu.F90:232-234 reads
-  // out of bounds otherwise); such a profile is left unchanged
-  if (ntop < 4) return;
-  if (n == ny + 1) {
-    for (int e = threadIdx.x; e < ntop * nk; e += blockDim.x) {
-      int i = e / nk + 1, k = e % nk + 2, kll = kl[i];
-      wk[(size_t)i * s + k] = ut[(size_t)(kll + 1) * s + k] * fsca[i] + ut[(size_t)kll * s + k] * (1 - fsca[i]);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x + 2; k < s; k += blockDim.x) wk[k] = -wk[(size_t)s + k];
-  } else {
-    for (int e = threadIdx.x; e < ntop * nk; e += blockDim.x) {
-      int i = e / nk + 1, k = e % nk + 2, kll = kl[i];
-      wk[(size_t)(i - 1) * s + k] = ut[(size_t)kll * s + k] * fsca[i] + ut[(size_t)(kll - 1) * s + k] * (1 - fsca[i]);
-    }
-  }
-  __syncthreads();
-  int nrow = n - (ntop - 2) + 1;
-  for (int e = threadIdx.x; e < nrow * nk; e += blockDim.x) {
-    int i = ntop - 2 + e / nk, k = e % nk + 2;
-    wk[(size_t)(i - 1) * s + k] = wk[(size_t)(ntop - 4) * s + k];
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < n * nk; e += blockDim.x) {
-    int i = e / nk + 1, k = e % nk + 2;
-    ut[(size_t)(i - 1) * s + k] = wk[(size_t)(i - 1) * s + k] * ggg;
-  }
-  __syncthreads();
-}
-
-__global__ void genflu_kernel(double *u, double *v, double *w, const double *resu, const double *resv,
-                              const double *resw, double *um, const double *u0, const double *v0,
-                              double *wk, int *ibuf, double *dbuf, DevCoef c, DevScalars *ds,
-                              int ny, int nz1, int xout, double re) {
-  const int s = nz1, ru = ny + 1;
-  double *u1 = u, *v1 = v, *w1 = w; // plane 1
-  const double *ux = resu + (size_t)(xout - 1) * s * ru, *vx = resv + (size_t)(xout - 1) * s * ny,
-               *wx = resw + (size_t)(xout - 1) * s * ru;
-  const int jy = 119 < ny ? 119 : ny; // y(119) clamped to the grid
-  const double yb = c.y[jy];
-  for (size_t e = threadIdx.x; e < (size_t)s * ru; e += blockDim.x) {
-    int j = (int)(e / s) + 1, k = (int)(e % s);
-    double f = (k >= 2) ? 0.5 * (1 - tanh(c.y[j] - yb)) : 1.0;
-    u1[e] = ux[e] * f;
-    w1[e] = wx[e] * f;
-  }
-  for (size_t e = threadIdx.x; e < (size_t)s * ny; e += blockDim.x) v1[e] = vx[e];
-  __syncthreads();
+// genflu (genflu.F90:3-167, NEWGENFLU branch) in two kernels.  Every value
+// genflu writes into plane 1 is a function of the saved velocities (res*,
+// equal to u, v, w before the predictor) on the recycling plane xout, so:
+//  * genflu_stats (one workgroup): running mean um from plane xout's mode 0,
+//    utau, u_inf, the outlet momentum thickness, the rescaling gammas and
+//    the interpolation tables kl / fsca / ntop of both profiles (dobleu);
+//  * genflu_plane (a grid over plane 1): the copy with the outer damping
+//    factor, the zero modes from u0 / v0, the profile remapping, the copy of
+//    row ntop-3 above ntop-2 and the 1/gamma1 scaling, evaluated per element.
+// ibuf: kl[2][ny+1], ntop[2]; dbuf: fsca[2][ny+1], ysca[ny+1], wlwa[ny+1].
+__global__ void genflu_stats_kernel(const double *resu, double *um, int *ibuf, double *dbuf, DevCoef c,
+                                    DevScalars *ds, int ny, int nz1, int xout, double re) {
+  const int s = nz1;
+  const double *ux = resu + (size_t)(xout - 1) * s * (ny + 1);
   __shared__ double sh_dtc;
   if (threadIdx.x == 0) {
     double tloc = ds->tiempo - ds->timeinit;
@@ -991,87 +956,127 @@ __global__ void genflu_kernel(double *u, double *v, double *w, const double *res
     sh_dtc = ds->dt / timec;
   }
   __syncthreads();
-  double dtc = sh_dtc;
+  const double dtc = sh_dtc;
+  // u1 mode 0 after the copy is ux mode 0 (the damping factor is 1 there)
   for (int j = threadIdx.x + 1; j <= ny + 1; j += blockDim.x)
-    um[j - 1] = dtc * u1[(size_t)(j - 1) * s] + um[j - 1] * (1.0 - dtc);
+    um[j - 1] = dtc * ux[(size_t)(j - 1) * s] + um[j - 1] * (1.0 - dtc);
   __syncthreads();
-  // scalar part: thread 0
-  int *kl = ibuf;                 // kl[2][ny+1]
+  int *kl = ibuf;                  // kl[2][ny+1]
   int *ntop = ibuf + 2 * (ny + 1); // ntop[2]
-  double *fsca = dbuf;            // fsca[2][ny+1]
-  double *ysca = dbuf + 2 * (ny + 1), *wlwa = dbuf + 3 * (ny + 1);
-  if (threadIdx.x == 0) {
-    const double *y = c.y, *dy = c.dy;
-    double utau = sqrt(fabs((um[1] - um[0]) / (y[2] - y[1]) / re));
-    double uinf = 0.0;
-    for (int j = ny + 1 - 20; j <= ny + 1 - 5; ++j) uinf += um[j - 1];
-    uinf /= 16.0;
-    double u99 = (double)0.99f * uinf;
-    int jtop = 0;
-    for (int j = 1; j <= ny - 2; ++j) {
-      if (um[j - 1] >= u99) break;
-      jtop = j + 1;
+  double *fsca = dbuf;             // fsca[2][ny+1]
+  // the sequential profile work below runs in LDS: y, um, ysca, wlwa
+  extern __shared__ double gsh[];
+  double *y = gsh, *ums = gsh + (ny + 2), *ysca = gsh + 2 * (ny + 2), *wlwa = gsh + 3 * (ny + 2);
+  for (int j = threadIdx.x; j < ny + 2; j += blockDim.x) {
+    y[j] = c.y[j];
+    ums[j] = j <= ny ? um[j] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double *dy = c.dy;
+  double utau = sqrt(fabs((ums[1] - ums[0]) / (y[2] - y[1]) / re));
+  double uinf = 0.0;
+  for (int j = ny + 1 - 20; j <= ny + 1 - 5; ++j) uinf += ums[j - 1];
+  uinf /= 16.0;
+  double u99 = (double)0.99f * uinf;
+  int jtop = 0;
+  for (int j = 1; j <= ny - 2; ++j) {
+    if (ums[j - 1] >= u99) break;
+    jtop = j + 1;
+  }
+  double sum = 0.0;
+  for (int j = 2; j <= jtop + 3; ++j) sum += (uinf - ums[j - 1]) * ums[j - 1] * dy[j - 1];
+  double rthout = sum * re / uinf;
+  double g2 = ds->rthin / rthout, g1 = pow(g2, 0.125);
+  ds->utauout = utau;
+  ds->dout = y[jtop];
+  ds->rthout = rthout;
+  ds->gamma1 = g1;
+  ds->gamma2 = g2;
+  for (int kk = 0; kk < 2; ++kk) {
+    for (int j = 1; j <= ny; ++j) ysca[j] = (kk == 0) ? 0.5 * (y[j] + y[j + 1]) / ds->din : y[j] / ds->din;
+    // dobleu (genflu.F90:173-199)
+    const double ag = 4.0, bg = 0.2;
+    double bg2 = 1.0 - 2.0 * bg, tag = 1.0 / tanh(ag), www1 = (double)0.7895f;
+    wlwa[1] = 0.0;
+    for (int q = 2; q <= ny; ++q) {
+      double xxx = 0.5 * (ysca[q] + ysca[q - 1]);
+      double yyy = (xxx >= 1.0) ? 1.0 : 0.5 * (1.0 + tag * tanh(ag * (xxx - bg) / (bg2 * xxx + bg)));
+      wlwa[q] = wlwa[q - 1] + yyy * (ysca[q] - ysca[q - 1]);
+      if (ysca[q] >= 1.0 && ysca[q - 1] < 1.0) www1 = wlwa[q - 1] + (1.0 - ysca[q - 1]) * yyy;
     }
-    double sum = 0.0;
-    for (int j = 2; j <= jtop + 3; ++j) sum += (uinf - um[j - 1]) * um[j - 1] * dy[j - 1];
-    double rthout = sum * re / uinf;
-    double g2 = ds->rthin / rthout, g1 = pow(g2, 0.125);
-    ds->utauout = utau;
-    ds->dout = y[jtop];
-    ds->rthout = rthout;
-    ds->gamma1 = g1;
-    ds->gamma2 = g2;
-    for (int kk = 0; kk < 2; ++kk) {
-      for (int j = 1; j <= ny; ++j)
-        ysca[j] = (kk == 0) ? 0.5 * (y[j] + y[j + 1]) / ds->din : y[j] / ds->din;
-      // dobleu (genflu.F90:173-199)
-      const double ag = 4.0, bg = 0.2;
-      double bg2 = 1.0 - 2.0 * bg, tag = 1.0 / tanh(ag), www1 = (double)0.7895f;
-      wlwa[1] = 0.0;
-      for (int q = 2; q <= ny; ++q) {
-        double xxx = 0.5 * (ysca[q] + ysca[q - 1]);
-        double yyy = (xxx >= 1.0) ? 1.0 : 0.5 * (1.0 + tag * tanh(ag * (xxx - bg) / (bg2 * xxx + bg)));
-        wlwa[q] = wlwa[q - 1] + yyy * (ysca[q] - ysca[q - 1]);
-        if (ysca[q] >= 1.0 && ysca[q - 1] < 1.0) www1 = wlwa[q - 1] + (1.0 - ysca[q - 1]) * yyy;
-      }
-      for (int q = 1; q <= ny; ++q) wlwa[q] = ysca[q] / g1 + (1.0 / g2 - 1.0 / g1) / www1 * wlwa[q];
-      int kll = 1, khh;
-      ntop[kk] = 0;
-      for (int q = 1; q <= ny; ++q) {
-        kl[kk * (ny + 1) + q] = 0;
-        fsca[kk * (ny + 1) + q] = 0.0;
-      }
-      for (int q = 1; q <= ny; ++q) {
-        if (wlwa[q] <= ysca[ny]) {
-          ntop[kk] = q;
-          khh = ny;
-          while (khh - kll > 1) {
-            int km = (khh + kll) / 2;
-            if (ysca[km] >= wlwa[q]) khh = km; else kll = km;
-          }
-          kl[kk * (ny + 1) + q] = kll;
-          fsca[kk * (ny + 1) + q] = (wlwa[q] - ysca[kll]) / (ysca[khh] - ysca[kll]);
+    for (int q = 1; q <= ny; ++q) wlwa[q] = ysca[q] / g1 + (1.0 / g2 - 1.0 / g1) / www1 * wlwa[q];
+    int kll = 1, khh;
+    ntop[kk] = 0;
+    for (int q = 1; q <= ny; ++q) {
+      kl[kk * (ny + 1) + q] = 0;
+      fsca[kk * (ny + 1) + q] = 0.0;
+    }
+    for (int q = 1; q <= ny; ++q) {
+      if (wlwa[q] <= ysca[ny]) {
+        ntop[kk] = q;
+        khh = ny;
+        while (khh - kll > 1) {
+          int km = (khh + kll) / 2;
+          if (ysca[km] >= wlwa[q]) khh = km; else kll = km;
         }
+        kl[kk * (ny + 1) + q] = kll;
+        fsca[kk * (ny + 1) + q] = (wlwa[q] - ysca[kll]) / (ysca[khh] - ysca[kll]);
       }
     }
   }
-  __syncthreads();
-  // the zero mode (genflu.F90:146-150)
-  for (int j = threadIdx.x + 1; j <= ny + 1; j += blockDim.x) {
-    u1[(size_t)(j - 1) * s] = u0[j - 1];
-    u1[(size_t)(j - 1) * s + 1] = 0.0;
-    w1[(size_t)(j - 1) * s] = 0.0;
-    w1[(size_t)(j - 1) * s + 1] = 0.0;
-    if (j <= ny) {
-      v1[(size_t)(j - 1) * s] = v0[j - 1];
-      v1[(size_t)(j - 1) * s + 1] = 0.0;
-    }
+}
+
+// Plane 1 of u (blockIdx.y 0), w (1) and v (2).  For modes k >= 1 the value
+// of row i is the remapped profile (genflu.F90:203-240, u/w rows shifted by
+// one against v rows, row 1 of u/w = -row 2) with rows >= ntop-2 copied from
+// row ntop-3, times 1/gamma1; with ntop < 4 the copied plane stays (the
+// reference's copy of row ntop-3 would read out of bounds).
+__global__ void genflu_plane_kernel(double *u, double *v, double *w, const double *resu, const double *resv,
+                                    const double *resw, const double *u0, const double *v0, const int *ibuf,
+                                    const double *dbuf, DevCoef c, const DevScalars *ds, int ny, int nz1,
+                                    int xout) {
+  const int f = blockIdx.y;
+  const int n = f == 2 ? ny : ny + 1, s = nz1;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)s * n) return;
+  const int i = (int)(e / s) + 1, k = (int)(e % s);
+  double *out = f == 0 ? u : (f == 1 ? w : v);
+  if (k < 2) {
+    double z = 0.0;
+    if (k == 0 && f == 0) z = u0[i - 1];
+    if (k == 0 && f == 2) z = v0[i - 1];
+    out[e] = z;
+    return;
   }
-  __syncthreads();
-  double ggg = 1.0 / ds->gamma1;
-  genflu_profile(u1, wk, ntop[0], kl, fsca, ny + 1, ny, nz1, ggg);
-  genflu_profile(w1, wk, ntop[0], kl, fsca, ny + 1, ny, nz1, ggg);
-  genflu_profile(v1, wk, ntop[1], kl + (ny + 1), fsca + (ny + 1), ny, ny, nz1, ggg);
+  const double *rx = (f == 0 ? resu : (f == 1 ? resw : resv)) + (size_t)(xout - 1) * s * n;
+  const double yb = c.y[119 < ny ? 119 : ny]; // y(119) clamped to the grid
+  auto cval = [&](int row) { // plane 1 after the copy, 1-based row
+    double r = rx[(size_t)(row - 1) * s + k];
+    if (f != 2) r *= 0.5 * (1 - tanh(c.y[row] - yb));
+    return r;
+  };
+  const int *kl = ibuf + (f == 2 ? ny + 1 : 0);
+  const double *fsca = dbuf + (f == 2 ? ny + 1 : 0);
+  const int nt = ibuf[2 * (ny + 1) + (f == 2 ? 1 : 0)];
+  if (nt < 4) {
+    out[e] = cval(i);
+    return;
+  }
+  const int isrc = i >= nt - 2 ? nt - 3 : i;
+  double W;
+  if (f != 2) {
+    const int ii = isrc == 1 ? 2 : isrc;
+    const int kll = kl[ii - 1];
+    const double fs = fsca[ii - 1];
+    W = cval(kll + 2) * fs + cval(kll + 1) * (1 - fs);
+    if (isrc == 1) W = -W;
+  } else {
+    const int kll = kl[isrc];
+    const double fs = fsca[isrc];
+    W = cval(kll + 1) * fs + cval(kll) * (1 - fs);
+  }
+  out[e] = W * (1.0 / ds->gamma1);
 }
 
 // z-viscous terms and the rest of rhsv/rhsw (rhsp.F90:355-415) for planes
@@ -1423,7 +1428,7 @@ void k_substep_rhsp(Box &b, int m) {
   k_interpxx(b, v, b.wki2, ny, 0);
   k_interpxx(b, w, b.wki3, ru, 0);
   if (m == 1)
-    LAUNCH(b, "energies", 1, TPB, 0, energies_kernel, u, v, w, b.dc.hy, b.ds, nx, ny, nz1, b.nz2);
+    LAUNCH(b, "energies", 3, 1024, 0, energies_kernel, u, v, w, b.dc.hy, b.ds, nx, ny, nz1, b.nz2);
   // u interpolated to v rows in spectral space (interpyy is linear along y and
   // commutes with the z transform of rhsp.F90:122,146)
   k_interpyy(b, u, b.yc, 0, 1, nx, b.pu, b.pu);
@@ -1448,9 +1453,14 @@ void k_substep_rhsp(Box &b, int m) {
   LAUNCH(b, "save_predict", nblk(b.nu), TPB, 0, save_predict_kernel, u, v, w, p, b.resu, b.resv, b.resw,
          b.dc.idyy, b.dc.kaz, b.ds, b.hc.idxx, nx, ny, nz1);
   // recycling inflow (rhsp.F90:287)
-  LAUNCH(b, "genflu", 1, TPB, 0, genflu_kernel, u, v, w, b.resu, b.resv, b.resw, b.um, b.u0, b.v0, b.gwork,
-         (int *)(b.gwork + (size_t)nz1 * (ny + 1)), b.gwork + (size_t)nz1 * (ny + 1) + 4 * (ny + 2), b.dc, b.ds,
-         ny, nz1, b.xout, b.re);
+  {
+    int *ibuf = (int *)(b.gwork + (size_t)nz1 * (ny + 1));
+    double *dbuf = b.gwork + (size_t)nz1 * (ny + 1) + 4 * (ny + 2);
+    LAUNCH(b, "genflu", 1, TPB, 4 * (size_t)(ny + 2) * sizeof(double), genflu_stats_kernel, b.resu, b.um, ibuf,
+           dbuf, b.dc, b.ds, ny, nz1, b.xout, b.re);
+    LAUNCH(b, "genflu", dim3(nblk((size_t)nz1 * (ny + 1)), 3), TPB, 0, genflu_plane_kernel, u, v, w, b.resu,
+           b.resv, b.resw, b.u0, b.v0, ibuf, dbuf, b.dc, b.ds, ny, nz1, b.xout);
+  }
   // plane loop 2 (planes 2..nx-1)
   if (nx > 2) {
     k_interpyy(b, b.resw, b.ya, 0, 2, nx - 2, b.pu, b.pu);

@@ -80,6 +80,55 @@ __device__ inline void ytab_build(double *tab, int n, int L, F row, E sten) {
   }
 }
 
+// The same table built by a whole workgroup: rows by all threads, the
+// chunk recurrences by one thread per chunk.  sh: LDS scratch of 4*YNCH*L
+// doubles.  Ends with the workgroup synchronised.
+template <class F, class E>
+__device__ inline void ytab_build_block(double *tab, int n, int L, F row, E sten, double *sh) {
+  const int NL = YNCH * L;
+  double *M = sh, *A2 = sh + NL, *B = sh + 2 * NL, *P = sh + 3 * NL;
+  for (int j = threadIdx.x; j < NL; j += blockDim.x) {
+    double m = 0.0, a2 = 0.0, b = 0.0;
+    if (j < n) {
+      const TriRow t = row(j + 1);
+      m = j == 0 ? 0.0 : t.m;
+      a2 = t.a2;
+      b = j == n - 1 ? 0.0 : -t.c * t.a2;
+    }
+    const Sten3 e = (j >= 1 && j <= n - 2) ? sten(j + 1) : Sten3{0.0, 0.0, 0.0};
+    M[j] = tab[j] = m;
+    A2[j] = tab[NL + j] = a2;
+    B[j] = tab[2 * NL + j] = b;
+    tab[5 * NL + j] = e.e1;
+    tab[6 * NL + j] = e.e2;
+    tab[7 * NL + j] = e.e3;
+  }
+  __syncthreads();
+  if (threadIdx.x < YNCH) {
+    const int q = threadIdx.x;
+    double *G = tab + 3 * NL, *S = tab + 4 * NL, *PE = tab + 8 * NL, *G0 = PE + YNCH, *S0 = G0 + YNCH;
+    double pr = 1.0;
+    for (int r = 0; r < L; ++r) {
+      pr *= M[q * L + r];
+      P[q * L + r] = pr;
+    }
+    PE[q] = pr;
+    double g = 0.0, sa = 1.0;
+    for (int r = L - 1; r >= 0; --r) {
+      const int j = q * L + r;
+      g = A2[j] * P[j] + B[j] * g;
+      sa = B[j] * sa;
+      G[j] = g;
+      S[j] = sa;
+      if (r == 0) {
+        G0[q] = g;
+        S0[q] = sa;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // matrices factored by coefinv (coeft.f90:1338-1353): a(1,j) multiplier,
 // a(2,j) inverse pivot, a(3,j) super-diagonal
 __device__ inline TriRow coefinv_row(const double *a, int j) {
