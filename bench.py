@@ -50,7 +50,11 @@ def kernel_model(cfg):
     m["rk_final"] = ((6 * Ru + 6 * Rv + 6 * Ru) * f / 3, 0.0, 3)
     m["save_predict"] = ((3 * 3 * Ru + Rv) * f / 1, 0.0, 1)  # u,v,w read, res write, u,v,w write, p
     N, L = cfg.nx - 1, (cfg.ny - 1) * z
-    m["dct"] = (2 * N * L * s, 2.0 * N * N * L, 2)
+    M = 1 << (2 * N - 2).bit_length()  # power-of-two Bluestein length >= 2N-1
+    # Poisson x transforms: one pencil read and written; Bluestein FFTs of
+    # length M, two lines per complex transform pair (forward + inverse)
+    for nm in ("dct2_rows", "dct3_rows"):
+        m[nm] = (2 * N * L * s, (L / 2) * 2 * 5 * M * math.log2(M), 1)
     m["div"] = ((Ru + Rv + Ru + Rv) * X * z * s, 0.0, 1)
     m["poisson_y"] = (2 * (X - 1) * (cfg.ny - 1) * z * s, 0.0, 1)
     m["project"] = ((Rv + 2 * (Ru + Rv + Ru + Rv)) * X * z * s, 0.0, 1)
@@ -59,8 +63,11 @@ def kernel_model(cfg):
 
 
 # kernel-event names -> HIP kernel function names in rocprofv3 output
-TRACE_NAMES = {"dct2": "dct_rows_kernel", "dct3": "dct_rows_kernel", "div": "div_pencils_kernel",
-               "poisson_y": "poisson_y_pencil_kernel", "to_planes": "pencils_to_planes_kernel"}
+TRACE_NAMES = {"dct2_rows": "dct_rows_z", "dct3_rows": "dct_rows_z", "div": "div_pencils_kernel",
+               "poisson_y": "poisson_y_pencil_kernel", "to_planes": "pencils_to_planes_kernel",
+               "products1": "products1_z", "products2": "products2_z", "interpxx": "interpxx_w",
+               "difvisxx": "difvisxx_w", "interpyy": "interpyy_y", "rhsu_loop1": "rhsu_loop1_y",
+               "rhs_loop2": "rhs_loop2_y", "rk_final": "rk_final_y"}
 
 
 def profiled_traffic(workload, precision, kname):
@@ -118,6 +125,8 @@ def main():
     box = GpuBox(cfg, st, precision=OTBL_FP64, dt_mode=OTBL_DT_EVERY_STEP,
                  graphs=not args.no_graphs)
     del st
+    paths = box.paths()
+    log(f"[bench] kernel paths {paths}")
     # a plain device copy of 1 GiB: the known-size reference for hardware
     # byte counters (FETCH_SIZE / WRITE_SIZE) read by rocprofv3 on this command
     from opentbl_amd.lib import DevArray, lib as _lib, check as _check
@@ -181,7 +190,7 @@ def main():
         "config": {"workload": f"{cfg.name}: {cfg.nx}x{cfg.ny}x{cfg.nz} ZPG boundary layer, "
                                f"Re_theta_in={cfg.re_theta:g}, FP64 path, one hipGraph per substep",
                    "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": "fp64",
-                   "graphs": not args.no_graphs, "dt": "CFL every step"},
+                   "graphs": not args.no_graphs, "dt": "CFL every step", "paths": paths},
         "roofline": roof,
     }
     box.close()

@@ -124,6 +124,11 @@ int otbl_d2h(void *dst, const void *src, size_t bytes);
  * of known size that calibrates hardware byte counters) */
 int otbl_d2d(void *dst, const void *src, size_t bytes);
 
+/* Kernel paths the context chose at setup, as text: "y:chunk<L>|march0
+ * z:engine<nz>|stockham<nz> dct:engine<M>|stockham<M>|gemm<M> x:window|march"
+ * (no counterpart in the reference; for tests and benchmark records). */
+int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len);
+
 /* ---- measurement ----
  * With kprof on, every kernel launch is bracketed by HIP events on the
  * context's stream; the report lists name, launches, total ms.  Kernels run

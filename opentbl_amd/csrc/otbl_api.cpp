@@ -1,5 +1,6 @@
 // C ABI of libopentbl_amd: box setup, state transfer, the RK3 substep (eager
 // or as captured hipGraphs), operator entry points and kernel timing.
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -10,6 +11,7 @@
 
 #include "otbl_internal.h"
 #include "otbl_ycol.h"
+#include "otbl_zfft.h"
 
 namespace otbl {
 
@@ -174,7 +176,11 @@ static FFTPlan make_plan(int n) {
 // conjugate chirp wrapped to length M (divided by M), and exp(-i pi k/2N).
 static void build_dct_fft(Box &b) {
   const int N = b.nx - 1;
-  const int M = good_length(2 * N - 1);
+  // a power-of-two convolution length runs on the register engine
+  int M2 = 1;
+  while (M2 < 2 * N - 1) M2 *= 2;
+  b.dct_zfast = zdct_supported(M2) && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
+  const int M = b.dct_zfast ? M2 : good_length(2 * N - 1);
   const double pi = b.hc.pi;
   b.dct_M = M;
   b.dct_plan = make_plan(M);
@@ -241,6 +247,28 @@ static void build_dct(Box &b) {
   OTBL_HIP(hipMemcpy(b.dct3, c3.data(), c3.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
+// Largest product of |c_j a2_j| over XLINE_WINDOW + 1 consecutive rows of
+// a (3, n) tridiagonal system after the coefinv factorisation
+// (coeft.f90:1338-1353, as factor_kernel does it on the GPU): the factor by
+// which the windowed backward sweep damps its zero start.
+static double window_decay(std::vector<double> a, int n) {
+  auto A = [&](int r, int j) -> double & { return a[3 * (size_t)j + r - 1]; };
+  for (int j = 2; j <= n; ++j) {
+    double d = 1.0 / A(2, j - 1);
+    A(2, j - 1) = d;
+    A(1, j) = -A(1, j) * d;
+    A(2, j) = A(2, j) + A(1, j) * A(3, j - 1);
+  }
+  A(2, n) = 1.0 / A(2, n);
+  double worst = 0.0;
+  for (int i = 1; i + XLINE_WINDOW <= n - 1; ++i) {
+    double p = 1.0;
+    for (int k = i; k <= i + XLINE_WINDOW; ++k) p *= std::fabs(A(3, k) * A(2, k));
+    worst = std::max(worst, p);
+  }
+  return worst;
+}
+
 static void free_box(Box *b) {
   if (!b) return;
   for (int q = 0; q < 4; ++q)
@@ -248,7 +276,7 @@ static void free_box(Box *b) {
   double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
                     b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
-                    b->ytab};
+                    b->ytab, b->ztw, b->zrow, b->dct_ztw};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
@@ -378,15 +406,44 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   OTBL_HIP(hipMemset(b->ds, 0, sizeof(DevScalars)));
   dalloc(b->vmbits, b->ny + 1);
   {
-    const int M = good_length(2 * (b->nx - 1) - 1);
-    b->dct_fft = k_dct_shm_bytes(M) <= 160 * 1024 && std::getenv("OTBL_DCT_GEMM") == nullptr;
+    // FFT-based Poisson x transforms: on the register engine when the
+    // power-of-two convolution length is instantiated, else on the LDS
+    // Stockham kernel when its buffers fit; dense matrices otherwise
+    const int N = b->nx - 1;
+    int M2 = 1;
+    while (M2 < 2 * N - 1) M2 *= 2;
+    const bool zf = zdct_supported(M2) && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
+    const int M = good_length(2 * N - 1);
+    b->dct_fft = std::getenv("OTBL_DCT_GEMM") == nullptr && (zf || k_dct_shm_bytes(M) <= 160 * 1024);
   }
   if (b->dct_fft)
     build_dct_fft(*b);
   else
     build_dct(*b);
   k_setup_attrs(*b);
+  // register-resident z transforms for the lengths they are instantiated
+  // for (OTBL_Z_STOCKHAM keeps the LDS Stockham kernels)
+  b->zfast = zfft_supported(b->nz) && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
+  if (b->zfast) {
+    dalloc(b->ztw, 2 * (size_t)b->nz);
+    dalloc(b->zrow, (size_t)b->nz1);
+    OTBL_HIP(hipMemset(b->zrow, 0, (size_t)b->nz1 * sizeof(double)));
+    kz_twiddles(*b, b->ztw, b->nz);
+  }
+  if (b->dct_zfast) {
+    dalloc(b->dct_ztw, 2 * (size_t)b->dct_M);
+    kz_twiddles(*b, b->dct_ztw, b->dct_M);
+  }
   k_factor_coef(*b, b->coefbuf);
+  // windowed x lines when the lines are long enough and every x system
+  // damps the window's zero start below round-off
+  {
+    const HostCoef &h = b->hc;
+    double worst = 0.0;
+    for (const std::vector<double> *a : {&h.inxu, &h.inxv, &h.dcxu, &h.dcxv, &h.vixu, &h.vixv})
+      worst = std::max(worst, window_decay(*a, b->nx));
+    b->xwin = b->nx > 2 * XLINE_WINDOW && worst < 1e-17 && std::getenv("OTBL_X_MARCH") == nullptr;
+  }
   // chunked y-line solver unless the columns are longer than its
   // instantiations (or OTBL_Y_MARCH asks for one thread per column)
   b->yL = std::getenv("OTBL_Y_MARCH") ? 0 : ytab_rows(b->ny);
@@ -699,6 +756,18 @@ int otbl_d2d(void *dst, const void *src, size_t bytes) {
   API_BEGIN
   OTBL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
   OTBL_HIP(hipDeviceSynchronize());
+  API_END
+}
+
+int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(buf && len > 0, "otbl_get_paths: null buffer");
+  char tmp[256];
+  std::snprintf(tmp, sizeof tmp, "y:%s%d z:%s%d dct:%s%d x:%s", b.yL ? "chunk" : "march", b.yL,
+                b.zfast ? "engine" : "stockham", b.nz, b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"),
+                b.dct_M, b.xwin ? "window" : "march");
+  std::snprintf(buf, len, "%s", tmp);
   API_END
 }
 

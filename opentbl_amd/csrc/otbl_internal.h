@@ -80,6 +80,9 @@ struct Box {
   double *dct2 = nullptr, *dct3 = nullptr;
   // implicit y LU (dt dependent), u-type and v-type: a1, a2, a3 rows
   double *implu = nullptr, *implv = nullptr;
+  // x lines with the windowed backward sweep (enabled when every window
+  // product of the factored x matrices is below round-off)
+  bool xwin = false;
   // chunked y-line solver: rows per chunk (0: one thread marches each
   // column) and the tables of its eight systems
   int yL = 0;
@@ -88,12 +91,18 @@ struct Box {
   double *um = nullptr, *u0 = nullptr, *v0 = nullptr, *gwork = nullptr;
   DevScalars *ds = nullptr;
   unsigned long long *vmbits = nullptr; // per-row max |v_x|
-  // z transform plan (length nz)
+  // z transform plan (length nz); zfast: register-resident engine with the
+  // twiddle table ztw (nz complex), else the LDS Stockham kernels
   FFTPlan zplan{};
+  bool zfast = false;
+  double *ztw = nullptr;
+  double *zrow = nullptr; // nz1 zeros: the absent partner of a packed transform
   // Poisson x transforms: Bluestein length M and tables (complex as
   // double pairs), or the dense matrices when FP64 buffers of length M do
   // not fit in LDS
   bool dct_fft = false;
+  bool dct_zfast = false; // M a power of two run on the register engine
+  double *dct_ztw = nullptr; // exp(2 pi i m / M)
   int dct_M = 0;
   FFTPlan dct_plan{};
   double *dct_chirp = nullptr, *dct_bf = nullptr, *dct_tw4 = nullptr;
@@ -126,6 +135,19 @@ void k_pois(Box &b, int m, double varstep, bool use_m);
 void k_end_step(Box &b);
 void k_setup_attrs(Box &b);
 size_t k_dct_shm_bytes(int M);
+// register-resident z transforms (otbl_zfft.hip), valid when Box::zfast
+void kz_twiddles(Box &b, double *tw, int n);
+// windowed x lines (otbl_xline.hip), valid when Box::xwin
+constexpr int XLINE_WINDOW = 24; // planes of the backward window
+void kx_interpxx(Box &b, const double *gen, double *geni, const double *icon, const double *econ, int chi,
+                 int chi2, size_t P);
+void kx_difvisxx(Box &b, const double *prod, const double *vel, double *out, const double *icon,
+                 const double *iconv, const double *econ, const double *econv, int chv, int ic, size_t P,
+                 int mode);
+void kz_dct_pencils(Box &b, double *P, int iopt);
+void kz_products1(Box &b);
+void kz_products2(Box &b);
+void kz_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows);
 size_t k_zfft_shm_bytes(int nz, int nbuf);
 
 // launch bookkeeping (otbl_api.cpp)

@@ -104,6 +104,10 @@ void k_interpxx(Box &b, const double *gen, double *geni, int R, int which_u) {
   const double *econ = which_u ? b.dc.cofiux : b.dc.cofivx;
   int chi = which_u ? 2 : 1;
   int chi2 = which_u ? chi + 1 : chi + 3;
+  if (b.xwin) {
+    kx_interpxx(b, gen, geni, icon, econ, chi, chi2, P);
+    return;
+  }
   LAUNCH(b, "interpxx", nblk(P), TPB, 0, interpxx_kernel, gen, geni, icon, econ, b.dc.inbx, chi,
          chi2, b.nx, P);
 }
@@ -178,6 +182,12 @@ void k_difvisxx(Box &b, const double *prod, const double *vel, double *out, int 
   size_t P = (size_t)b.nz1 * R;
   const DevCoef &c = b.dc;
   int vel_u = which_u; // viscous coefficient set: u (ic=3) or v/w (ic=1)
+  if (b.xwin) {
+    kx_difvisxx(b, prod, vel, out, which_u ? c.dcxu : c.dcxv, which_u ? c.vixu : c.vixv,
+                which_u ? c.cofcxu : c.cofcxv, which_u ? c.cofvxu : c.cofvxv, which_u ? 1 : 2, vel_u ? 3 : 1, P,
+                mode);
+    return;
+  }
   LAUNCH(b, "difvisxx", nblk(P), TPB, 0, difvisxx_kernel, prod, vel, out, b.ya, b.yb,
          which_u ? c.dcxu : c.dcxv, which_u ? c.vixu : c.vixv, which_u ? c.cofcxu : c.cofcxv,
          which_u ? c.cofvxu : c.cofvxv, c.dcbx, c.cofvbx, which_u ? 1 : 2, vel_u ? 3 : 1, b.nx,
@@ -610,6 +620,10 @@ fourxz_kernel(double *fou, double *phys, int iopt, FFTPlan plan, int nz1, int nz
 static size_t zfft_shm(const Box &b, int nbuf) { return (TW_SLOTS + (size_t)nbuf * b.nz) * sizeof(double2); }
 
 void k_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows) {
+  if (b.zfast) {
+    kz_fourxz(b, fou, phys, iopt, nrows);
+    return;
+  }
   LAUNCH(b, "fourxz", nrows, TPB, zfft_shm(b, 2), fourxz_kernel, fou, phys, iopt, b.zplan, b.nz1, b.nz2);
 }
 
@@ -773,6 +787,10 @@ static size_t dct_shm(const Box &b) { return (TW_SLOTS + 2 * (size_t)b.dct_M) * 
 
 // Row transforms on pencils (in place).
 static void dct_pencils(Box &b, double *P, int iopt) {
+  if (b.dct_zfast) {
+    kz_dct_pencils(b, P, iopt);
+    return;
+  }
   const int Lp = (b.ny - 1) * b.nz1;
   LAUNCH(b, iopt < 0 ? "dct2_rows" : "dct3_rows", (Lp + 1) / 2, TPB, dct_shm(b), dct_rows_kernel, P, P, Lp,
          iopt, dct_args(b));
@@ -1434,8 +1452,11 @@ void k_substep_rhsp(Box &b, int m) {
   k_interpyy(b, u, b.yc, 0, 1, nx, b.pu, b.pu);
   // plane loop 1 products
   const size_t shm = zfft_shm(b, 4);
-  LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel, u, b.wki1, b.wki2, b.wki3, b.yc,
-         plan, ny, nz1, b.nz2, b.ds, b.vmbits);
+  if (b.zfast)
+    kz_products1(b);
+  else
+    LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel, u, b.wki1, b.wki2, b.wki3, b.yc,
+           plan, ny, nz1, b.nz2, b.ds, b.vmbits);
   if (b.yL)
     ky_rhsu_loop1(b, b.wki2, b.wki3, b.rhsu);
   else
@@ -1465,8 +1486,11 @@ void k_substep_rhsp(Box &b, int m) {
   if (nx > 2) {
     k_interpyy(b, b.resw, b.ya, 0, 2, nx - 2, b.pu, b.pu);
     k_interpyy(b, b.resv, b.yb, 1, 2, nx - 2, b.pv, b.pu);
-    LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, b.resv, b.resw, b.ya, b.yb, b.yc,
-           plan, ny, nz1, b.nz2);
+    if (b.zfast)
+      kz_products2(b);
+    else
+      LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, b.resv, b.resw, b.ya, b.yb, b.yc,
+             plan, ny, nz1, b.nz2);
     if (b.yL)
       ky_rhs_loop2(b, b.ya, b.yb, b.yc);
     else
@@ -1499,7 +1523,7 @@ void k_setup_attrs(Box &b) {
   OTBL_HIP(hipFuncSetAttribute((const void *)products2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
   OTBL_HIP(hipFuncSetAttribute((const void *)fourxz_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)zfft_shm(b, 2)));
-  if (b.dct_fft)
+  if (b.dct_fft && !b.dct_zfast)
     OTBL_HIP(hipFuncSetAttribute((const void *)dct_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)dct_shm(b)));
 }

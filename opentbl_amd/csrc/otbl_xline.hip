@@ -1,0 +1,222 @@
+// x-line compact operators (interpxx, difvisxx) with a windowed backward
+// sweep.  A thread owns one line (kr, j) and marches over the planes once:
+// the forward elimination y_i = r_i + m_i y_{i-1} is exact; the backward
+// substitution x_i = a2_i (y_i - c_i x_{i+1}) of a block of XB planes is run
+// from XW = (XK-1) XB planes further on, starting there from x = 0.  The
+// x schemes live on the uniform x grid, where the backward recurrence damps
+// a start error by a factor |c a2| < 0.2 per plane, so after XW planes the
+// error is below round-off (the host checks every window product of the
+// factored matrices before this path is enabled).  The last XK blocks of the
+// line are solved exactly from plane nx.  The y values of the last XK blocks
+// stay in registers (a ring of XK x XB), so every input is read once and
+// every output written once, with no line scratch in HBM.
+#include "otbl_kcommon.h"
+
+namespace otbl {
+
+static constexpr int XB = 8, XK = 4; // block of planes, ring length in blocks
+static_assert((XK - 1) * XB == XLINE_WINDOW, "window length");
+
+// interpxx (lip.f90:248-300): midpoint interpolation along x
+__global__ void __launch_bounds__(TPB)
+interpxx_w(const double *__restrict__ gen, double *__restrict__ geni, const double *__restrict__ icon,
+           const double *__restrict__ econ, const double *__restrict__ inbx, int chi, int chi2, int nx, size_t P) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= P) return;
+  const double *g = gen + l;
+  double *o = geni + l;
+  const int nb = (nx + XB - 1) / XB;
+  double ring[XK][XB];
+  double y = 0.0;
+  double gi = g[P]; // g(i) of the interior plane i being processed, from i = 2
+  for (int it = 0; it < nb; it += XK) {
+#pragma unroll
+    for (int kk = 0; kk < XK; ++kk) {
+      const int qq = it + kk;
+      if (qq >= nb) break;
+      // forward elimination over block qq
+#pragma unroll
+      for (int b = 0; b < XB; ++b) {
+        const int i = qq * XB + b + 1;
+        if (i <= nx) {
+          double rhs;
+          if (i == 1) {
+            rhs = M4(inbx, chi2, 1) * g[0] + M4(inbx, chi2, 2) * g[P] + M4(inbx, chi2, 3) * g[2 * P] +
+                  M4(inbx, chi2, 4) * g[3 * P];
+            y = rhs;
+          } else if (i < nx) {
+            const double gn = g[(size_t)i * P]; // g(i+1)
+            rhs = C2(econ, 1, i) * gi + C2(econ, 2, i) * gn;
+            gi = gn;
+            y = rhs + C3(icon, 1, i) * y;
+          } else {
+            rhs = M4(inbx, chi, 1) * g[(size_t)(nx - 1) * P] + M4(inbx, chi, 2) * g[(size_t)(nx - 2) * P] +
+                  M4(inbx, chi, 3) * g[(size_t)(nx - 3) * P] + M4(inbx, chi, 4) * g[(size_t)(nx - 4) * P];
+            y = rhs + C3(icon, 1, nx) * y;
+          }
+          ring[kk][b] = y;
+        }
+      }
+      if (qq < nb - 1) {
+        if (qq >= XK - 1) {
+          // windowed backward sweep: blocks qq .. qq-XK+1, emit the oldest
+          double x = 0.0;
+#pragma unroll
+          for (int d = 0; d < XK; ++d) {
+            const int slot = (kk - d + XK) % XK;
+#pragma unroll
+            for (int b = XB - 1; b >= 0; --b) {
+              const int i = (qq - d) * XB + b + 1;
+              x = (ring[slot][b] - C3(icon, 3, i) * x) * C3(icon, 2, i);
+              if (d == XK - 1) o[(size_t)(i - 1) * P] = x;
+            }
+          }
+        }
+      } else {
+        // last block: exact backward sweep from plane nx over the blocks
+        // not emitted yet
+        double x = 0.0;
+        bool started = false;
+#pragma unroll
+        for (int d = 0; d < XK; ++d) {
+          const int slot = (kk - d + XK) % XK;
+          if (qq - d >= 0) {
+#pragma unroll
+            for (int b = XB - 1; b >= 0; --b) {
+              const int i = (qq - d) * XB + b + 1;
+              if (i <= nx) {
+                const double yy = ring[slot][b];
+                x = started ? (yy - C3(icon, 3, i) * x) * C3(icon, 2, i) : yy * C3(icon, 2, nx);
+                started = true;
+                o[(size_t)(i - 1) * P] = x;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// difvisxx (lip.f90:52-142): convective d/dx of the product plus rex times
+// the compact d2/dx2 of the velocity; mode 1 adds to out except on plane nx,
+// which gets the outflow derivative U_inf du/dx (lip.f90:105,132).
+__global__ void __launch_bounds__(TPB)
+difvisxx_w(const double *__restrict__ gp, const double *__restrict__ q, double *out,
+           const double *__restrict__ icon, const double *__restrict__ iconv, const double *__restrict__ econ,
+           const double *__restrict__ econv, const double *__restrict__ dcbx, const double *__restrict__ cofvbx,
+           int chv, int ic, int nx, size_t P, double rex, double idx, int mode) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= P) return;
+  const double *g = gp + l, *v = q + l;
+  double *o = out + l;
+  const int chv1 = chv + 2;
+  const int nb = (nx + XB - 1) / XB;
+  double rc_[XK][XB], rv_[XK][XB];
+  double yc = 0.0, yv = 0.0;
+  double gim1 = g[0], qim1 = v[0], qi = v[P];
+  for (int it = 0; it < nb; it += XK) {
+#pragma unroll
+    for (int kk = 0; kk < XK; ++kk) {
+      const int qq = it + kk;
+      if (qq >= nb) break;
+#pragma unroll
+      for (int b = 0; b < XB; ++b) {
+        const int i = qq * XB + b + 1;
+        if (i <= nx) {
+          if (i == 1) {
+            yc = M5(dcbx, chv, 1) * g[0] + M5(dcbx, chv, 2) * g[P] + M5(dcbx, chv, 3) * g[2 * P] +
+                 M5(dcbx, chv, 4) * g[3 * P];
+            yv = M4(cofvbx, ic, 1) * v[0] + M4(cofvbx, ic, 2) * v[P] + M4(cofvbx, ic, 3) * v[2 * P] +
+                 M4(cofvbx, ic, 4) * v[3 * P];
+          } else if (i < nx) {
+            const double gi = g[(size_t)(i - 1) * P];
+            const double qn = v[(size_t)i * P];
+            const double rc = C2(econ, 1, i) * gim1 + C2(econ, 2, i) * gi;
+            const double rv = C3(econv, 1, i) * qn + C3(econv, 2, i) * qi + C3(econv, 3, i) * qim1;
+            yc = rc + C3(icon, 1, i) * yc;
+            yv = rv + C3(iconv, 1, i) * yv;
+            gim1 = gi;
+            qim1 = qi;
+            qi = qn;
+          } else {
+            const double gn = g[(size_t)(nx - 1) * P], gn1 = g[(size_t)(nx - 2) * P],
+                         gn2 = g[(size_t)(nx - 3) * P], gn3 = g[(size_t)(nx - 4) * P];
+            const double qn2 = v[(size_t)(nx - 3) * P], qn3 = v[(size_t)(nx - 4) * P];
+            const double rc = M5(dcbx, chv1, 1) * gn + M5(dcbx, chv1, 2) * gn1 + M5(dcbx, chv1, 3) * gn2 +
+                              M5(dcbx, chv1, 4) * gn3;
+            const double rv = M4(cofvbx, ic + 1, 1) * qi + M4(cofvbx, ic + 1, 2) * qim1 +
+                              M4(cofvbx, ic + 1, 3) * qn2 + M4(cofvbx, ic + 1, 4) * qn3;
+            yc = rc + C3(icon, 1, nx) * yc;
+            yv = rv + C3(iconv, 1, nx) * yv;
+          }
+          rc_[kk][b] = yc;
+          rv_[kk][b] = yv;
+        }
+      }
+      if (qq < nb - 1) {
+        if (qq >= XK - 1) {
+          double xc = 0.0, xv = 0.0;
+#pragma unroll
+          for (int d = 0; d < XK; ++d) {
+            const int slot = (kk - d + XK) % XK;
+#pragma unroll
+            for (int b = XB - 1; b >= 0; --b) {
+              const int i = (qq - d) * XB + b + 1;
+              xc = (rc_[slot][b] - C3(icon, 3, i) * xc) * C3(icon, 2, i);
+              xv = (rv_[slot][b] - C3(iconv, 3, i) * xv) * C3(iconv, 2, i);
+              if (d == XK - 1) {
+                double r = xc + rex * xv;
+                if (mode == 1) r = o[(size_t)(i - 1) * P] + r;
+                o[(size_t)(i - 1) * P] = r;
+              }
+            }
+          }
+        }
+      } else {
+        double xc = 0.0, xv = 0.0;
+        bool started = false;
+#pragma unroll
+        for (int d = 0; d < XK; ++d) {
+          const int slot = (kk - d + XK) % XK;
+          if (qq - d >= 0) {
+#pragma unroll
+            for (int b = XB - 1; b >= 0; --b) {
+              const int i = (qq - d) * XB + b + 1;
+              if (i <= nx) {
+                const size_t oi = (size_t)(i - 1) * P;
+                if (!started) {
+                  xc = rc_[slot][b] * C3(icon, 2, nx);
+                  xv = rv_[slot][b] * C3(iconv, 2, nx);
+                  // outflow row: the velocity difference qi - qim1 = q(nx) - q(nx-1)
+                  o[oi] = 1.0 * (qi - qim1) * idx + rex * 0.0;
+                  started = true;
+                } else {
+                  xc = (rc_[slot][b] - C3(icon, 3, i) * xc) * C3(icon, 2, i);
+                  xv = (rv_[slot][b] - C3(iconv, 3, i) * xv) * C3(iconv, 2, i);
+                  double r = xc + rex * xv;
+                  if (mode == 1) r = o[oi] + r;
+                  o[oi] = r;
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+void kx_interpxx(Box &b, const double *gen, double *geni, const double *icon, const double *econ, int chi,
+                 int chi2, size_t P) {
+  LAUNCH(b, "interpxx", nblk(P), TPB, 0, interpxx_w, gen, geni, icon, econ, b.dc.inbx, chi, chi2, b.nx, P);
+}
+
+void kx_difvisxx(Box &b, const double *prod, const double *vel, double *out, const double *icon,
+                 const double *iconv, const double *econ, const double *econv, int chv, int ic, size_t P,
+                 int mode) {
+  LAUNCH(b, "difvisxx", nblk(P), TPB, 0, difvisxx_w, prod, vel, out, icon, iconv, econ, econv, b.dc.dcbx,
+         b.dc.cofvbx, chv, ic, b.nx, P, b.hc.rex, b.hc.idx, mode);
+}
+
+} // namespace otbl

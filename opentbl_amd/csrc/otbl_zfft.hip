@@ -1,0 +1,314 @@
+// z-direction product kernels of rhsp on the register-resident FFT engine
+// (otbl_zfft.h): one (j, i) row per team of W wavefronts, every transform of
+// the row held in registers, products formed in physical space without data
+// movement, one LDS exchange buffer per team.
+#include "otbl_kcommon.h"
+#include "otbl_zfft.h"
+
+namespace otbl {
+
+// exp(2 pi i m / n), m = 0..n-1
+__global__ void ztw_kernel(double2 *tw, int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double s, c;
+  sincospi(2.0 * m / n, &s, &c);
+  tw[m] = make_double2(c, s);
+}
+
+void kz_twiddles(Box &b, double *tw, int n) { LAUNCH(b, "ztw", nblk(n), TPB, 0, ztw_kernel, (double2 *)tw, n); }
+
+template <int N> struct ZTeam {
+  static constexpr int W = ZPlan<N>::W, G = 64 * W, E = N / G;
+  static constexpr int TEAMS = W == 1 ? 4 : 1; // teams per workgroup
+  static constexpr int TPB_ = TEAMS * G;
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Plane loop 1 of rhsp (rhsp.F90:120-153) for one (j, i) row: from u, u_x
+// (wki1), w_x (wki3), v_x (wki2, rows <= ny) and u interpolated to v rows
+// (uy) in spectral space form r2c(u_x^2) -> wki1, r2c(u w_x) -> wki3,
+// r2c(v_x uy) -> wki2 (rows <= ny), and the maxima |u|, |w_x|, |v_x|(j)
+// of the CFL time step.  Rows are numbered (i-1)(ny+1) + (j-1).
+template <int N>
+__global__ void __launch_bounds__(ZTeam<N>::TPB_)
+products1_z(const double *__restrict__ u, double *wki1, double *wki2, double *wki3, const double *__restrict__ uy,
+            const double2 *__restrict__ tw, const double *__restrict__ zrow, int ny, int nz1, int nz2, int nrows,
+            DevScalars *ds, unsigned long long *vmbits) {
+  using Z = ZTeam<N>;
+  __shared__ double2 lds_all[Z::TEAMS * zlds_elems<N>()];
+  // the team (and so the row and every row pointer) is uniform per wavefront
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
+  double2 *lds = lds_all + team * zlds_elems<N>();
+  const int row = blockIdx.x * Z::TEAMS + team;
+  if (row >= nrows) return; // a whole team (one wavefront when TEAMS > 1)
+  const int j = row % (ny + 1) + 1, i = row / (ny + 1) + 1;
+  const size_t ou = (size_t)row * nz1, ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
+  const bool vrow = j <= ny;
+  // Pairs chosen so that almost nothing is held across a transform:
+  // (v_x, u_y) gives v_x u_y, inverted at once; (u, w_x) gives u w_x, kept
+  // through the transform of u_x, which gives u_x^2.
+  double um = 0.0, wm = 0.0, vm = 0.0;
+  if (vrow) {
+    double2 C[Z::E];
+    zload_spectral<N>(C, t, wki2 + ov, uy + ou);
+    zrun<N, +1, false>(C, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) {
+      vm = fmax(vm, fabs(C[e].x));
+      C[e] = make_double2(C[e].x * C[e].y, 0.0);
+    }
+    zrun<N, -1, true>(C, t, tw, lds);
+    zstore_spectral<N>(C, t, lds, wki2 + ov, nullptr, nz2);
+  }
+  double s1[Z::E];
+  {
+    double2 A[Z::E];
+    zload_spectral<N>(A, t, u + ou, wki3 + ou);
+    zrun<N, +1, false>(A, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) {
+      um = fmax(um, fabs(A[e].x));
+      wm = fmax(wm, fabs(A[e].y));
+      s1[e] = A[e].x * A[e].y;
+    }
+  }
+  {
+    double2 B[Z::E];
+    zload_spectral<N>(B, t, wki1 + ou, zrow);
+    zrun<N, +1, false>(B, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) B[e] = make_double2(B[e].x * B[e].x, s1[e]);
+    zrun<N, -1, true>(B, t, tw, lds);
+    zstore_spectral<N>(B, t, lds, wki1 + ou, wki3 + ou, nz2);
+  }
+  um = wave_max(um);
+  wm = wave_max(wm);
+  vm = wave_max(vm);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&ds->umax_bits, (unsigned long long)__double_as_longlong(um));
+    atomicMax(&ds->wmax_bits, (unsigned long long)__double_as_longlong(wm));
+    if (vrow) atomicMax(&vmbits[j - 1], (unsigned long long)__double_as_longlong(vm));
+  }
+}
+
+// Plane loop 2 of rhsp (rhsp.F90:367-415) for one (j, i) row, j <= ny,
+// planes 2..nx-1 (rows numbered (i-2) ny + (j-1)): vw = r2c(v w_v) -> ya
+// (w interpolated to v rows is read from ya), vv = r2c(v_u^2) -> yb (v
+// interpolated to u rows is read from yb), ww = r2c(w^2) -> yc.
+template <int N>
+__global__ void __launch_bounds__(ZTeam<N>::TPB_)
+products2_z(const double *__restrict__ resv, const double *__restrict__ resw, double *ya, double *yb, double *yc,
+            const double2 *__restrict__ tw, int ny, int nz1, int nz2, int nrows) {
+  using Z = ZTeam<N>;
+  __shared__ double2 lds_all[Z::TEAMS * zlds_elems<N>()];
+  // the team (and so the row and every row pointer) is uniform per wavefront
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
+  double2 *lds = lds_all + team * zlds_elems<N>();
+  const int row = blockIdx.x * Z::TEAMS + team;
+  if (row >= nrows) return;
+  const int j = row % ny + 1, i = row / ny + 2;
+  const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
+  const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
+  // (v_u, w) gives (v_u^2, w^2), inverted at once; (v, w_v) gives v w_v
+  {
+    double2 B[Z::E];
+    zload_spectral<N>(B, t, yb + ou, resw + ou);
+    zrun<N, +1, false>(B, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) B[e] = make_double2(B[e].x * B[e].x, B[e].y * B[e].y);
+    zrun<N, -1, true>(B, t, tw, lds);
+    zstore_spectral<N>(B, t, lds, yb + ou, yc + ou, nz2);
+  }
+  {
+    double2 A[Z::E];
+    zload_spectral<N>(A, t, resv + ov, ya + ou);
+    zrun<N, +1, false>(A, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) A[e] = make_double2(A[e].x * A[e].y, 0.0);
+    zrun<N, -1, true>(A, t, tw, lds);
+    zstore_spectral<N>(A, t, lds, ya + ou, nullptr, nz2);
+  }
+}
+
+// fourxz (ffts.f90:27-67) on independent rows: iopt >= 0 c2r (spectral ->
+// physical, nz doubles per row), iopt < 0 r2c.
+template <int N>
+__global__ void __launch_bounds__(ZTeam<N>::TPB_)
+fourxz_z(double *fou, double *phys, int iopt, const double2 *__restrict__ tw, const double *__restrict__ zrow,
+         int nz1, int nz2, int nrows) {
+  using Z = ZTeam<N>;
+  __shared__ double2 lds_all[Z::TEAMS * zlds_elems<N>()];
+  // the team (and so the row and every row pointer) is uniform per wavefront
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
+  double2 *lds = lds_all + team * zlds_elems<N>();
+  const int row = blockIdx.x * Z::TEAMS + team;
+  if (row >= nrows) return;
+  double *F = fou + (size_t)row * nz1;
+  double *Ph = phys + (size_t)row * N;
+  double2 A[Z::E];
+  if (iopt >= 0) {
+    zload_spectral<N>(A, t, F, zrow);
+    zrun<N, +1, false>(A, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) Ph[zphys_index<N>(t, e)] = A[e].x;
+  } else {
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) A[e] = make_double2(Ph[zphys_index<N>(t, e)], 0.0);
+    zrun<N, -1, true>(A, t, tw, lds);
+    zstore_spectral<N>(A, t, lds, F, nullptr, nz2);
+  }
+}
+
+// Poisson x transforms (ffts.f90:124-181: FFTW REDFT10 / REDFT01 of length
+// N = nx-1) on x-contiguous pencils, two real lines per complex transform:
+// Makhoul's reordering turns each DCT into a complex DFT of length N, and
+// Bluestein's chirp-z evaluates that DFT as a circular convolution of
+// length M (a power of two >= 2N-1) on the register engine: forward
+// transform (sign -1), product with the transformed conjugate chirp bf at
+// each register's physical index, inverse transform (sign +1), then the
+// chirp and the Makhoul untwisting through the team's LDS buffer.
+// iopt < 0: DCT-II x 1/N, iopt > 0: DCT-III x 0.5; in place (src == dst)
+// allowed, each team reads its two lines before writing them.
+template <int M>
+__global__ void __launch_bounds__(ZTeam<M>::TPB_)
+dct_rows_z(const double *src, double *dst, int Lp, int N, int iopt, const double2 *__restrict__ chirp,
+           const double2 *__restrict__ bf, const double2 *__restrict__ tw4, const double2 *__restrict__ tw) {
+  using Z = ZTeam<M>;
+  __shared__ double2 lds_all[Z::TEAMS * zlds_elems<M>()];
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
+  double2 *lds = lds_all + team * zlds_elems<M>();
+  const int pair = blockIdx.x * Z::TEAMS + team;
+  const int l0 = 2 * pair, l1 = l0 + 1;
+  if (l0 >= Lp) return; // a whole team
+  const bool has1 = l1 < Lp;
+  const double *x0 = src + (size_t)l0 * N, *x1 = src + (size_t)(has1 ? l1 : l0) * N;
+  const int half = (N + 1) / 2;
+  constexpr int R0 = ZPlan<M>::rad(0), NB0 = M / (R0 * Z::G);
+  double2 v[Z::E];
+#pragma unroll
+  for (int m = 0; m < NB0; ++m)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int k = t + Z::G * m + r * (M / R0);
+      double2 a = make_double2(0.0, 0.0);
+      if (k < N) {
+        double2 z;
+        if (iopt < 0) { // Makhoul reordering of the two real lines
+          const int sidx = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
+          z = make_double2(x0[sidx], has1 ? x1[sidx] : 0.0);
+        } else { // V = (X_k - i X_{N-k}) exp(+i pi k / 2N), Z = V0 + i V1, conj(Z)
+          double2 tt = tw4[k];
+          tt.y = -tt.y;
+          const double a0 = x0[k], b0 = k ? x0[N - k] : 0.0;
+          const double a1 = has1 ? x1[k] : 0.0, b1 = (has1 && k) ? x1[N - k] : 0.0;
+          const double2 v0 = cmulc(make_double2(a0, -b0), tt), v1 = cmulc(make_double2(a1, -b1), tt);
+          z = make_double2(v0.x - v1.y, -(v0.y + v1.x));
+        }
+        a = cmulc(z, chirp[k]);
+      }
+      v[m * R0 + r] = a;
+    }
+  zrun<M, -1, false>(v, t, tw, lds);
+#pragma unroll
+  for (int e = 0; e < Z::E; ++e) v[e] = cmulc(v[e], bf[zphys_index<M>(t, e)]);
+  zrun<M, +1, true>(v, t, tw, lds);
+  // natural-order outputs t + G m + q M / R0; keep k < N times the chirp
+#pragma unroll
+  for (int m = 0; m < NB0; ++m)
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      const int k = t + Z::G * m + q * (M / R0);
+      if (k < N) lds[zpad(k)] = cmulc(chirp[k], v[m * R0 + q]);
+    }
+  zsync<Z::W>();
+  double *y0 = dst + (size_t)l0 * N, *y1 = dst + (size_t)l1 * N;
+  if (iopt < 0) {
+    const double sc = 2.0 / N;
+    for (int k = t; k < N; k += Z::G) {
+      const double2 z = lds[zpad(k)], w = lds[zpad(k ? N - k : 0)];
+      const double2 v0 = make_double2(0.5 * (z.x + w.x), 0.5 * (z.y - w.y));
+      const double2 v1 = make_double2(0.5 * (z.y + w.y), -0.5 * (z.x - w.x));
+      const double2 tt = tw4[k];
+      y0[k] = sc * (tt.x * v0.x - tt.y * v0.y);
+      if (has1) y1[k] = sc * (tt.x * v1.x - tt.y * v1.y);
+    }
+  } else {
+    for (int k = t; k < N; k += Z::G) {
+      const double2 zc = lds[zpad(k)]; // conj of sum_k Z_k exp(+2 pi i k n / N)
+      const int o = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
+      y0[o] = 0.5 * zc.x;
+      if (has1) y1[o] = -0.5 * zc.y;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define ZDISPATCH(n, CALL)                                                     \
+  switch (n) {                                                                 \
+  case 256: { constexpr int N = 256; CALL; } break;                            \
+  case 512: { constexpr int N = 512; CALL; } break;                            \
+  case 1024: { constexpr int N = 1024; CALL; } break;                          \
+  case 2048: { constexpr int N = 2048; CALL; } break;                          \
+  case 4096: { constexpr int N = 4096; CALL; } break;                          \
+  default: throw std::invalid_argument("z transform length not instantiated"); \
+  }
+
+template <int N> static unsigned zgrid(int nrows) {
+  return (unsigned)((nrows + ZTeam<N>::TEAMS - 1) / ZTeam<N>::TEAMS);
+}
+
+// the engine fixes the dealiased mode count by the length
+static void zcheck(const Box &b) {
+  if (b.nz2 != b.nz / 3 - 1 || b.nz1 != 2 * (b.nz / 3))
+    throw std::invalid_argument("z transform: nz1/nz2 do not follow nz");
+}
+
+void kz_products1(Box &b) {
+  zcheck(b);
+  const int nrows = (b.ny + 1) * b.nx;
+  ZDISPATCH(b.nz, LAUNCH(b, "products1", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, products1_z<N>, b.f[OTBL_U], b.wki1,
+                         b.wki2, b.wki3, b.yc, (const double2 *)b.ztw, b.zrow, b.ny, b.nz1, b.nz2, nrows, b.ds,
+                         b.vmbits));
+}
+
+void kz_products2(Box &b) {
+  zcheck(b);
+  const int nrows = b.ny * (b.nx - 2);
+  ZDISPATCH(b.nz, LAUNCH(b, "products2", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, products2_z<N>, b.resv, b.resw, b.ya,
+                         b.yb, b.yc, (const double2 *)b.ztw, b.ny, b.nz1, b.nz2, nrows));
+}
+
+void kz_dct_pencils(Box &b, double *P, int iopt) {
+  const int Lp = (b.ny - 1) * b.nz1, N = b.nx - 1, pairs = (Lp + 1) / 2;
+  const char *name = iopt < 0 ? "dct2_rows" : "dct3_rows";
+  switch (b.dct_M) {
+#define ZDCT(MM)                                                                                              \
+  case MM:                                                                                                     \
+    LAUNCH(b, name, zgrid<MM>(pairs), ZTeam<MM>::TPB_, 0, dct_rows_z<MM>, P, P, Lp, N, iopt,                     \
+           (const double2 *)b.dct_chirp, (const double2 *)b.dct_bf, (const double2 *)b.dct_tw4,                 \
+           (const double2 *)b.dct_ztw);                                                                        \
+    break;
+    ZDCT(512)
+    ZDCT(1024)
+    ZDCT(2048)
+    ZDCT(4096)
+    ZDCT(8192)
+#undef ZDCT
+  default: throw std::invalid_argument("Bluestein length not instantiated");
+  }
+}
+
+void kz_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows) {
+  zcheck(b);
+  ZDISPATCH(b.nz, LAUNCH(b, "fourxz", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, fourxz_z<N>, fou, phys, iopt,
+                         (const double2 *)b.ztw, b.zrow, b.nz1, b.nz2, nrows));
+}
+
+} // namespace otbl

@@ -63,6 +63,7 @@ SIGNATURES = {
     "otbl_h2d": (I, [VP, VP, SZ]),
     "otbl_d2h": (I, [VP, VP, SZ]),
     "otbl_d2d": (I, [VP, VP, SZ]),
+    "otbl_get_paths": (I, [VP, C.c_char_p, SZ]),
     "otbl_kprof": (I, [VP, I]),
     "otbl_kprof_report": (I, [VP, C.c_char_p, SZ]),
 }
@@ -230,6 +231,12 @@ class GpuBox:
         out = np.zeros(r.value * c.value)
         check(lib().otbl_get_coef(self.h, name.encode(), _pd(out), C.byref(r), C.byref(c)), "get_coef")
         return out.reshape(c.value, r.value).T if r.value > 1 else out
+
+    def paths(self):
+        """The kernel paths chosen at setup, e.g. {"y": "chunk20", "z": "engine1024", ...}."""
+        buf = C.create_string_buffer(256)
+        check(lib().otbl_get_paths(self.h, buf, len(buf)), "get_paths")
+        return dict(kv.split(":") for kv in buf.value.decode().split())
 
     def kprof(self, on):
         check(lib().otbl_kprof(self.h, int(on)), "kprof")

@@ -9,6 +9,7 @@ import pytest
 
 from opentbl_amd import config, ic
 from opentbl_amd.lib import DevArray, GpuBox, OTBL_DT_FROZEN
+from parity_util import assert_state, state_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -185,11 +186,8 @@ def test_cosftx(boxes, tiny, oracle_lib, iopt):
 
 
 def _compare_state(ob, gb, tol, names=("u", "v", "w", "p")):
-    errs = {}
-    for n in names:
-        errs[n] = rel(gb.get(n), ob.get(n))
-    bad = {k: v for k, v in errs.items() if not v < tol}
-    assert not bad, errs
+    errs = state_errors(gb, ob, names)
+    assert_state(errs, tol)
     return errs
 
 

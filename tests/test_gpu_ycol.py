@@ -12,6 +12,7 @@ import pytest
 
 from opentbl_amd import config, ic
 from opentbl_amd.lib import DevArray, GpuBox
+from parity_util import assert_state, state_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -92,10 +93,9 @@ def test_step_tall(tall, oracle_lib):
     gb = GpuBox(cfg, st)
     ob.step()
     gb.step(1)
-    for n in ("u", "v", "w", "p"):
-        e = rel(gb.get(n), ob.get(n))
-        assert e < STEP_TOL, (n, e)
+    errs = state_errors(gb, ob)
     gb.close()
+    assert_state(errs)
 
 
 def test_chunked_matches_marching(tall):
@@ -106,6 +106,7 @@ def test_chunked_matches_marching(tall):
         gm = GpuBox(cfg, st)
     finally:
         del os.environ["OTBL_Y_MARCH"]
+    assert gc.paths()["y"].startswith("chunk") and gm.paths()["y"] == "march0"
     gc.step(2)
     gm.step(2)
     for n in ("u", "v", "w", "p"):
