@@ -276,7 +276,7 @@ static void free_box(Box *b) {
   double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
                     b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
-                    b->ytab, b->ztw, b->zrow, b->dct_ztw};
+                    b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
@@ -427,6 +427,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   if (b->zfast) {
     dalloc(b->ztw, 2 * (size_t)b->nz);
     dalloc(b->zrow, (size_t)b->nz1);
+    dalloc(b->rowmax, 3 * (size_t)(b->ny + 1) * b->nx);
     OTBL_HIP(hipMemset(b->zrow, 0, (size_t)b->nz1 * sizeof(double)));
     kz_twiddles(*b, b->ztw, b->nz);
   }

@@ -97,6 +97,7 @@ struct Box {
   bool zfast = false;
   double *ztw = nullptr;
   double *zrow = nullptr; // nz1 zeros: the absent partner of a packed transform
+  double *rowmax = nullptr; // per-row CFL maxima of products1, 3 (ny+1) nx
   // Poisson x transforms: Bluestein length M and tables (complex as
   // double pairs), or the dense matrices when FP64 buffers of length M do
   // not fit in LDS

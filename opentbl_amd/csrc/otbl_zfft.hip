@@ -38,7 +38,7 @@ template <int N>
 __global__ void __launch_bounds__(ZTeam<N>::TPB_)
 products1_z(const double *__restrict__ u, double *wki1, double *wki2, double *wki3, const double *__restrict__ uy,
             const double2 *__restrict__ tw, const double *__restrict__ zrow, int ny, int nz1, int nz2, int nrows,
-            DevScalars *ds, unsigned long long *vmbits) {
+            double *rowmax) {
   using Z = ZTeam<N>;
   __shared__ double2 lds_all[Z::TEAMS * zlds_elems<N>()];
   // the team (and so the row and every row pointer) is uniform per wavefront
@@ -86,13 +86,59 @@ products1_z(const double *__restrict__ u, double *wki1, double *wki2, double *wk
     zrun<N, -1, true>(B, t, tw, lds);
     zstore_spectral<N>(B, t, lds, wki1 + ou, wki3 + ou, nz2);
   }
+  // per-row maxima; rowmax_kernel reduces them (no atomics on shared
+  // addresses from every row)
+  um = wave_max(um);
+  wm = wave_max(wm);
+  vm = wave_max(vm);
+  if constexpr (Z::W > 1) { // combine the team's wavefronts through its LDS buffer
+    if ((t & 63) == 0) lds[t / 64] = make_double2(um, wm), lds[Z::W + t / 64] = make_double2(vm, 0.0);
+    zsync<Z::W>();
+    if (t == 0)
+      for (int q = 1; q < Z::W; ++q) {
+        um = fmax(um, lds[q].x);
+        wm = fmax(wm, lds[q].y);
+        vm = fmax(vm, lds[Z::W + q].x);
+      }
+  }
+  if (t == 0) {
+    rowmax[row] = um;
+    rowmax[nrows + row] = wm;
+    rowmax[2 * (size_t)nrows + row] = vrow ? vm : 0.0;
+  }
+}
+
+// max |u|, max |w_x| over the box and max |v_x| per row j from the per-row
+// maxima of products1_z: block j-1 reduces rows (i, j) over the planes.
+__global__ void __launch_bounds__(256)
+rowmax_kernel(const double *__restrict__ rowmax, DevScalars *ds, unsigned long long *vmbits, int ny, int nx) {
+  const int j = blockIdx.x + 1, nrows = (ny + 1) * nx;
+  double um = 0.0, wm = 0.0, vm = 0.0;
+  for (int i = threadIdx.x + 1; i <= nx; i += blockDim.x) {
+    const int row = (i - 1) * (ny + 1) + (j - 1);
+    um = fmax(um, rowmax[row]);
+    wm = fmax(wm, rowmax[nrows + row]);
+    vm = fmax(vm, rowmax[2 * (size_t)nrows + row]);
+  }
+  __shared__ double red[3][4];
   um = wave_max(um);
   wm = wave_max(wm);
   vm = wave_max(vm);
   if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x / 64] = um;
+    red[1][threadIdx.x / 64] = wm;
+    red[2][threadIdx.x / 64] = vm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; ++q) {
+      um = fmax(um, red[0][q]);
+      wm = fmax(wm, red[1][q]);
+      vm = fmax(vm, red[2][q]);
+    }
     atomicMax(&ds->umax_bits, (unsigned long long)__double_as_longlong(um));
     atomicMax(&ds->wmax_bits, (unsigned long long)__double_as_longlong(wm));
-    if (vrow) atomicMax(&vmbits[j - 1], (unsigned long long)__double_as_longlong(vm));
+    if (j <= ny) atomicMax(&vmbits[j - 1], (unsigned long long)__double_as_longlong(vm));
   }
 }
 
@@ -274,8 +320,9 @@ void kz_products1(Box &b) {
   zcheck(b);
   const int nrows = (b.ny + 1) * b.nx;
   ZDISPATCH(b.nz, LAUNCH(b, "products1", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, products1_z<N>, b.f[OTBL_U], b.wki1,
-                         b.wki2, b.wki3, b.yc, (const double2 *)b.ztw, b.zrow, b.ny, b.nz1, b.nz2, nrows, b.ds,
-                         b.vmbits));
+                         b.wki2, b.wki3, b.yc, (const double2 *)b.ztw, b.zrow, b.ny, b.nz1, b.nz2, nrows,
+                         b.rowmax));
+  LAUNCH(b, "rowmax", b.ny + 1, 256, 0, rowmax_kernel, b.rowmax, b.ds, b.vmbits, b.ny, b.nx);
 }
 
 void kz_products2(Box &b) {
