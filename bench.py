@@ -1,12 +1,17 @@
 """Throughput of the RK3 substep on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
+                    [--precision both|mixed|fp64]
 
 A step is one RK3 step (3 substeps: rhsp, outflow_correction, pois) of the
 whole box, with every field resident in HBM; each substep runs as one
 captured hipGraph.  The metric is grid-point substeps per second,
-nx*ny*nz*3*K / wall time / 1e6 (Mpts*substep/s).  One JSON line is printed;
-per-kernel timings (HIP events on the library's stream) go to stderr.
+nx*ny*nz*3*K / wall time / 1e6 (Mpts*substep/s).  By default the FP64
+substep and the mixed-precision substep are run from the same state for the
+same steps; the line reports the mixed rate as `value`, the FP64 rate beside
+it and the largest relative difference of u, v, w between the two.  One JSON
+line is printed; per-kernel timings (HIP events on the library's stream) go
+to stderr.
 """
 import argparse
 import json
@@ -21,22 +26,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from opentbl_amd import config, ic  # noqa: E402
-from opentbl_amd.lib import GpuBox, OTBL_DT_EVERY_STEP, OTBL_FP64  # noqa: E402
+from opentbl_amd.lib import GpuBox, OTBL_DT_EVERY_STEP, OTBL_FP64, OTBL_MIXED  # noqa: E402
 
 METRIC = "Mpts*RK3-substep/s, mixed vs FP64 on 1 GPU at <=1e-5 rel err; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_model(cfg):
+def kernel_model(cfg, s=8):
     """Algorithmic bytes (each distinct array read once / written once) and
-    flops per launch, FP64, for the kernels of one substep.  Returned as
+    flops per launch for field elements of s bytes (8: FP64, 4: the mixed
+    substep's FP32 fields), for the kernels of one substep.  Returned as
     name -> (bytes per launch, flops per launch, launches per substep)."""
-    X, Ru, Rv, z, s = cfg.nx, cfg.ny + 1, cfg.ny, cfg.nz1, 8
+    X, Ru, Rv, z = cfg.nx, cfg.ny + 1, cfg.ny, cfg.nz1
     nz = cfg.nz
     f = X * z * s  # bytes of one row-plane column set (per row of a field)
     m = {}
@@ -104,12 +111,67 @@ def cpu_baseline(cfg_full):
                       f"{dt:.1f} s"}
 
 
+def run_box(cfg, st, precision, args):
+    """Warm up, time args.steps RK3 steps, then one profiled step with the
+    kernels bracketed by HIP events.  Returns wall seconds of the timed
+    steps, the per-kernel report, the chosen paths and the final box."""
+    name = "mixed" if precision == OTBL_MIXED else "fp64"
+    box = GpuBox(cfg, st, precision=precision, dt_mode=OTBL_DT_EVERY_STEP, graphs=not args.no_graphs)
+    paths = box.paths()
+    log(f"[bench] {name}: kernel paths {paths}")
+    for w in range(args.warmup):
+        box.step(1)
+        box.sync()
+        log(f"[bench] {name}: warmup step {w + 1}/{args.warmup} done")
+    box.sync()
+    t0 = time.perf_counter()
+    box.step(args.steps)
+    box.sync()
+    wall = time.perf_counter() - t0
+    log(f"[bench] {name}: timed {args.steps} steps in {wall:.3f} s")
+    box.kprof(True)
+    box.step(1)
+    box.sync()
+    rep = box.kprof_report()
+    box.kprof(False)
+    return wall, rep, paths, box
+
+
+def report(name, rep, model):
+    rows = []
+    for kname, (n, ms) in sorted(rep.items(), key=lambda kv: -kv[1][1]):
+        avg = ms / n
+        by, fl, _ = model.get(kname, (None, None, None))
+        gbs = by / (avg * 1e-3) / 1e9 if by else None
+        tfs = fl / (avg * 1e-3) / 1e12 if fl else None
+        rows.append((kname, n, ms, avg, gbs, tfs))
+        log(f"[kprof {name}] {kname:14s} launches {n:3d} total {ms:9.3f} ms avg {avg:9.4f} ms"
+            + (f"  {gbs:8.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:5.1f}% HBM)" if gbs else "")
+            + (f"  {tfs:6.2f} TFLOP/s" if tfs else ""))
+    return rows
+
+
+def roofline(rows, model, workload, precision):
+    name, n, ms, avg, gbs, tfs = next(r for r in rows if r[0] in model)
+    by, fl, _ = model[name]
+    peak_tf = FP32_PEAK_TFLOPS if precision == "mixed" else FP64_PEAK_TFLOPS
+    intensity = fl / by if fl else 0.0
+    if intensity > peak_tf * 1e12 / (HBM_PEAK_GBS * 1e9):
+        roof = {"bound": "mfma", "achieved": tfs, "peak": peak_tf, "unit": "TFLOP/s", "frac": tfs / peak_tf}
+    else:
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    roof["traffic"] = profiled_traffic(workload, precision, name)
+    roof.update({"kernel": name, "avg_ms": avg, "algorithmic_bytes": by})
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=sorted(config.CONFIGS))
+    ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
     args = ap.parse_args()
@@ -121,12 +183,7 @@ def main():
     log(f"[bench] workload {cfg.name} {cfg.nx}x{cfg.ny}x{cfg.nz}: generating the state")
     t0 = time.perf_counter()
     st = ic.make_state(cfg)
-    log(f"[bench] state ready in {time.perf_counter() - t0:.1f} s; uploading")
-    box = GpuBox(cfg, st, precision=OTBL_FP64, dt_mode=OTBL_DT_EVERY_STEP,
-                 graphs=not args.no_graphs)
-    del st
-    paths = box.paths()
-    log(f"[bench] kernel paths {paths}")
+    log(f"[bench] state ready in {time.perf_counter() - t0:.1f} s")
     # a plain device copy of 1 GiB: the known-size reference for hardware
     # byte counters (FETCH_SIZE / WRITE_SIZE) read by rocprofv3 on this command
     from opentbl_amd.lib import DevArray, lib as _lib, check as _check
@@ -134,66 +191,53 @@ def main():
     _check(_lib().otbl_d2d(_b.ptr, _a.ptr, 1 << 30), "d2d")
     _a.free()
     _b.free()
-    for w in range(args.warmup):
-        box.step(1)
-        box.sync()
-        log(f"[bench] warmup step {w + 1}/{args.warmup} done")
-    box.sync()
-    t0 = time.perf_counter()
-    box.step(args.steps)
-    box.sync()
-    wall = time.perf_counter() - t0
-    log(f"[bench] timed {args.steps} steps in {wall:.3f} s")
-    u = box.get("u")
-    if not np.isfinite(u).all():
-        raise SystemExit("bench.py: non-finite velocity after the timed steps")
-    del u
-    # per-kernel durations: one more step, kernels launched one by one and
-    # bracketed by HIP events on the library's stream
-    box.kprof(True)
-    box.step(1)
-    box.sync()
-    rep = box.kprof_report()
-    box.kprof(False)
-    model = kernel_model(cfg)
-    rows = []
-    for name, (n, ms) in sorted(rep.items(), key=lambda kv: -kv[1][1]):
-        avg = ms / n
-        by, fl, _ = model.get(name, (None, None, None))
-        gbs = by / (avg * 1e-3) / 1e9 if by else None
-        tfs = fl / (avg * 1e-3) / 1e12 if fl else None
-        rows.append((name, n, ms, avg, gbs, tfs))
-        log(f"[kprof] {name:14s} launches {n:3d} total {ms:9.3f} ms avg {avg:9.4f} ms"
-            + (f"  {gbs:8.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:5.1f}% HBM)" if gbs else "")
-            + (f"  {tfs:6.2f} TFLOP/s" if tfs else ""))
-    dom = next(r for r in rows if r[0] in model)
-    name, n, ms, avg, gbs, tfs = dom
-    by, fl, _ = model[name]
-    intensity = fl / by if fl else 0.0
-    if intensity > FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9):
-        roof = {"bound": "mfma", "achieved": tfs, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": tfs / FP64_PEAK_TFLOPS, "traffic": None}
-    else:
-        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": gbs / HBM_PEAK_GBS, "traffic": None}
-    roof["traffic"] = profiled_traffic(cfg.name, "fp64", name)
-    roof["kernel"] = name
-    roof["avg_ms"] = avg
-    roof["algorithmic_bytes"] = by
+    runs = {}
+    ref = None
+    order = {"both": [OTBL_FP64, OTBL_MIXED], "mixed": [OTBL_MIXED], "fp64": [OTBL_FP64]}[args.precision]
+    for prec in order:
+        name = "mixed" if prec == OTBL_MIXED else "fp64"
+        wall, rep, paths, box = run_box(cfg, st, prec, args)
+        vel = {n: box.get(n) for n in ("u", "v", "w", "p")}
+        if not all(np.isfinite(a).all() for a in vel.values()):
+            raise SystemExit(f"bench.py: non-finite velocity after the {name} steps")
+        box.close()
+        model = kernel_model(cfg, 4 if prec == OTBL_MIXED else 8)
+        rows = report(name, rep, model)
+        runs[name] = {"wall": wall, "value": cfg.points * 3 * args.steps / wall / 1e6, "paths": paths,
+                      "roofline": roofline(rows, model, cfg.name, name)}
+        if prec == OTBL_FP64:
+            ref = vel
+        elif ref is not None:
+            # velocity error relative to the largest velocity component (the
+            # free stream); each field also relative to its own maximum
+            scale = max(np.abs(ref[n]).max() for n in ("u", "v", "w"))
+            err = {"vel": max(float(np.abs(vel[n] - ref[n]).max()) for n in ("u", "v", "w")) / scale}
+            err.update({n: float(np.abs(vel[n] - ref[n]).max() / np.abs(ref[n]).max()) for n in vel})
+            runs[name]["rel_err_vs_fp64"] = err
+        del vel
+    del st, ref
+    head = "mixed" if "mixed" in runs else "fp64"
+    h = runs[head]
     steps = args.steps
-    value = cfg.points * 3 * steps / wall / 1e6
+    cfgd = {"workload": f"{cfg.name}: {cfg.nx}x{cfg.ny}x{cfg.nz} ZPG boundary layer, "
+                        f"Re_theta_in={cfg.re_theta:g}, {head} substep, one hipGraph per substep",
+            "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": head, "graphs": not args.no_graphs,
+            "dt": "CFL every step", "paths": h["paths"]}
+    for name, r in runs.items():
+        cfgd[f"{name}_value"] = r["value"]
+        cfgd[f"{name}_ms_per_step"] = 1e3 * r["wall"] / steps
+    if "mixed" in runs and "fp64" in runs:
+        cfgd["mixed_over_fp64"] = runs["mixed"]["value"] / runs["fp64"]["value"]
+        cfgd["mixed_rel_err_vs_fp64"] = runs["mixed"]["rel_err_vs_fp64"]
+        cfgd["fp64_roofline"] = runs["fp64"]["roofline"]
     out = {
-        "metric": METRIC, "value": value, "unit": "Mpts*substep/s", "n_gpus": 1,
-        "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{cfg.name}: {cfg.nx}x{cfg.ny}x{cfg.nz} ZPG boundary layer, "
-                               f"Re_theta_in={cfg.re_theta:g}, FP64 path, one hipGraph per substep",
-                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": "fp64",
-                   "graphs": not args.no_graphs, "dt": "CFL every step", "paths": paths},
-        "roofline": roof,
+        "metric": METRIC, "value": h["value"], "unit": "Mpts*substep/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * h["wall"] / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64", "data": "synthetic",
+        "config": cfgd,
+        "roofline": h["roofline"],
     }
-    box.close()
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle on a slab)")
         out["cpu_baseline"] = cpu_baseline(cfg)

@@ -37,7 +37,17 @@ typedef struct otbl_params {
   int nx, ny, nz;            /* grid: mod.F90:22 (parameter nx, ny, nz) */
   int xout;                  /* recycling plane: mod.F90:23 (xout) */
   double re, ax, az, cfl;    /* hrem_1.dat values read at bl_1.F90:239-244 */
-  int precision;             /* OTBL_FP64 or OTBL_MIXED */
+  int precision;             /* OTBL_FP64, or OTBL_MIXED: FP32 fields and
+                                FP32 derivative / transform / product
+                                arithmetic with FP64 statistics, CFL, mass
+                                flux, coefficient factorisation, mean-flow
+                                (kz = 0) pressure solve and Poisson residuals;
+                                needs the register z engine (nz 256..4096),
+                                a power-of-two Bluestein length (nx-1 >= 129)
+                                and the chunked y solver (ny <= 384).  Fields
+                                cross otbl_set_field / otbl_get_field as FP64
+                                either way; the operator entry points
+                                (otbl_interpxx ...) stay FP64. */
   int dt_mode;               /* OTBL_DT_EVERY_STEP or OTBL_DT_FROZEN */
 } otbl_params;
 

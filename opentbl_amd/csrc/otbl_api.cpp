@@ -65,15 +65,25 @@ static int small_factors(int n, int *fac) {
 template <class T> static void dalloc(T *&p, size_t n) {
   OTBL_HIP(hipMalloc((void **)&p, n * sizeof(T) + 64));
   OTBL_HIP(hipMemset(p, 0, n * sizeof(T) + 64));
+  // the clear runs on the null stream, which the box's non-blocking stream
+  // does not wait for
+  OTBL_HIP(hipDeviceSynchronize());
 }
 
-static void put_ds(Box &b, size_t off, const void *src, size_t bytes) {
-  OTBL_HIP(hipMemcpy((char *)b.ds + off, src, bytes, hipMemcpyHostToDevice));
-}
-static void get_ds(Box &b, size_t off, void *dst, size_t bytes) {
+// Host <-> device copies ordered on the box's stream, complete on return.
+// (The stream is non-blocking: a copy on the null stream is not ordered
+// before kernels enqueued on it, and a pageable host-to-device hipMemcpy may
+// return before its transfer lands.)
+static void h2d(Box &b, void *dst, const void *src, size_t bytes) {
+  OTBL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b.s));
   OTBL_HIP(hipStreamSynchronize(b.s));
-  OTBL_HIP(hipMemcpy(dst, (char *)b.ds + off, bytes, hipMemcpyDeviceToHost));
 }
+static void d2h(Box &b, void *dst, const void *src, size_t bytes) {
+  OTBL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, b.s));
+  OTBL_HIP(hipStreamSynchronize(b.s));
+}
+static void put_ds(Box &b, size_t off, const void *src, size_t bytes) { h2d(b, (char *)b.ds + off, src, bytes); }
+static void get_ds(Box &b, size_t off, void *dst, size_t bytes) { d2h(b, dst, (char *)b.ds + off, bytes); }
 
 // Coefficient arrays, in upload order, with their device slot.  cols: the
 // Fortran extent ('X' nx, 'Y' ny, 'U' ny+1, 'P' ny-1, 'M' nx-1, 'K' nz2+1,
@@ -151,7 +161,7 @@ static void upload_coef(Box &b) {
     b.dc.*(f.dev) = b.coefbuf + off;
     off += 24;
   }
-  OTBL_HIP(hipMemcpy(b.coefbuf, host.data(), total * sizeof(double), hipMemcpyHostToDevice));
+  h2d(b, b.coefbuf, host.data(), total * sizeof(double));
 }
 
 static int good_length(int m) {
@@ -223,9 +233,9 @@ static void build_dct_fft(Box &b) {
   dalloc(b.dct_chirp, chirp.size());
   dalloc(b.dct_bf, bf.size());
   dalloc(b.dct_tw4, tw4.size());
-  OTBL_HIP(hipMemcpy(b.dct_chirp, chirp.data(), chirp.size() * sizeof(double), hipMemcpyHostToDevice));
-  OTBL_HIP(hipMemcpy(b.dct_bf, bf.data(), bf.size() * sizeof(double), hipMemcpyHostToDevice));
-  OTBL_HIP(hipMemcpy(b.dct_tw4, tw4.data(), tw4.size() * sizeof(double), hipMemcpyHostToDevice));
+  h2d(b, b.dct_chirp, chirp.data(), chirp.size() * sizeof(double));
+  h2d(b, b.dct_bf, bf.data(), bf.size() * sizeof(double));
+  h2d(b, b.dct_tw4, tw4.data(), tw4.size() * sizeof(double));
 }
 
 static void build_dct(Box &b) {
@@ -243,8 +253,8 @@ static void build_dct(Box &b) {
     }
   dalloc(b.dct2, (size_t)N * N);
   dalloc(b.dct3, (size_t)N * N);
-  OTBL_HIP(hipMemcpy(b.dct2, c2.data(), c2.size() * sizeof(double), hipMemcpyHostToDevice));
-  OTBL_HIP(hipMemcpy(b.dct3, c3.data(), c3.size() * sizeof(double), hipMemcpyHostToDevice));
+  h2d(b, b.dct2, c2.data(), c2.size() * sizeof(double));
+  h2d(b, b.dct3, c3.data(), c3.size() * sizeof(double));
 }
 
 // Largest product of |c_j a2_j| over XLINE_WINDOW + 1 consecutive rows of
@@ -276,8 +286,12 @@ static void free_box(Box *b) {
   double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
                     b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
-                    b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax};
+                    b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax, b->P0, b->S0, b->p0, b->um64, b->rm64, b->tm64};
   for (double *p : bufs)
+    if (p) (void)hipFree(p);
+  float *fbufs[] = {b->coefbuf_f, b->ytab_f, b->ztw_f, b->dct_chirp_f, b->dct_bf_f, b->dct_tw4_f, b->dct_ztw_f,
+                    b->pdl, b->pdu};
+  for (float *p : fbufs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
     if (b->f[f]) (void)hipFree(b->f[f]);
@@ -353,6 +367,48 @@ static void need(bool ok, const char *msg) {
   if (!ok) throw std::invalid_argument(msg);
 }
 
+// FP32 copies of the factored FP64 coefficients and tables, the Poisson
+// pivots and the FP64 mean-flow pencils of the mixed-precision substep.
+static void setup_mixed(Box &b) {
+  need(b.yL && b.zfast && b.dct_zfast,
+       "OTBL_MIXED needs the chunked y solver (ny <= 24*16), a z length of 256..4096 and a power-of-two "
+       "Bluestein length of 512..8192 (nx-1)");
+  need(b.xwin, "OTBL_MIXED needs x lines that the windowed sweep solves (nx <= 32 or decaying x systems)");
+  size_t total = 0;
+  for (const auto &sl : kSlots) total += (b.hc.*(sl.vec)).size() + 8;
+  total += 7 * 24;
+  dalloc(b.coefbuf_f, total);
+  k_d2f(b, b.coefbuf, b.coefbuf_f, total);
+  // the same offsets as the FP64 set
+  const double *const *src = reinterpret_cast<const double *const *>(&b.dc);
+  const float **dst = reinterpret_cast<const float **>(&b.dcf);
+  static_assert(sizeof(DevCoefT<double>) == sizeof(DevCoefT<float>), "coefficient pointer sets");
+  for (size_t q = 0; q < sizeof(DevCoef) / sizeof(void *); ++q) dst[q] = b.coefbuf_f + (src[q] - b.coefbuf);
+  dalloc(b.ytab_f, YT_COUNT * ytab_size(b.yL));
+  k_d2f(b, b.ytab, b.ytab_f, YT_COUNT * ytab_size(b.yL));
+  dalloc(b.ztw_f, 2 * (size_t)b.nz);
+  k_d2f(b, b.ztw, b.ztw_f, 2 * (size_t)b.nz);
+  const size_t N = b.nx - 1, M = b.dct_M;
+  dalloc(b.dct_chirp_f, 2 * N);
+  dalloc(b.dct_tw4_f, 2 * N);
+  dalloc(b.dct_bf_f, 2 * M);
+  dalloc(b.dct_ztw_f, 2 * M);
+  k_d2f(b, b.dct_chirp, b.dct_chirp_f, 2 * N);
+  k_d2f(b, b.dct_tw4, b.dct_tw4_f, 2 * N);
+  k_d2f(b, b.dct_bf, b.dct_bf_f, 2 * M);
+  k_d2f(b, b.dct_ztw, b.dct_ztw_f, 2 * M);
+  const size_t npiv = (size_t)(b.ny - 1) * (b.nz1 / 2) * N;
+  dalloc(b.pdl, npiv);
+  dalloc(b.pdu, npiv);
+  k_poisson_pivots(b);
+  dalloc(b.P0, 2 * (size_t)(b.ny - 1) * N);
+  dalloc(b.S0, 2 * (size_t)(b.ny - 1) * N);
+  dalloc(b.p0, (size_t)b.nx * b.ny);
+  dalloc(b.um64, (size_t)b.nx * (b.ny + 1));
+  dalloc(b.rm64, (size_t)b.nx * (b.ny + 1));
+  dalloc(b.tm64, (size_t)b.nx * (b.ny + 1));
+}
+
 extern "C" {
 
 const char *otbl_last_error(void) { return g_err.c_str(); }
@@ -367,7 +423,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   need(prm->nz >= 6 && prm->nz % 2 == 0, "nz must be even and >= 6");
   need(prm->xout >= 2 && prm->xout <= prm->nx - 1, "xout must lie in 2..nx-1 (bl_1.F90:287-297)");
   need(prm->re > 0 && prm->ax > 0 && prm->az > 0 && prm->cfl > 0, "re, ax, az, cfl must be > 0");
-  need(prm->precision == OTBL_FP64, "only OTBL_FP64 is available in this build");
+  need(prm->precision == OTBL_FP64 || prm->precision == OTBL_MIXED, "precision must be OTBL_FP64 or OTBL_MIXED");
   need(prm->dt_mode == OTBL_DT_EVERY_STEP || prm->dt_mode == OTBL_DT_FROZEN, "bad dt_mode");
   b = new Box();
   b->nx = prm->nx; b->ny = prm->ny; b->nz = prm->nz;
@@ -376,6 +432,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   b->mz = b->nz2 + 1;
   b->xout = prm->xout;
   b->precision = prm->precision;
+  b->esz = prm->precision == OTBL_MIXED ? 4 : 8;
   b->dt_mode = prm->dt_mode;
   b->re = prm->re; b->ax = prm->ax; b->az = prm->az; b->cfl = prm->cfl;
   {
@@ -404,6 +461,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   dalloc(b->gwork, (size_t)b->nz1 * (b->ny + 1) + 16 * (size_t)(b->ny + 2));
   OTBL_HIP(hipMalloc((void **)&b->ds, sizeof(DevScalars)));
   OTBL_HIP(hipMemset(b->ds, 0, sizeof(DevScalars)));
+  OTBL_HIP(hipDeviceSynchronize());
   dalloc(b->vmbits, b->ny + 1);
   {
     // FFT-based Poisson x transforms: on the register engine when the
@@ -429,6 +487,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     dalloc(b->zrow, (size_t)b->nz1);
     dalloc(b->rowmax, 3 * (size_t)(b->ny + 1) * b->nx);
     OTBL_HIP(hipMemset(b->zrow, 0, (size_t)b->nz1 * sizeof(double)));
+    OTBL_HIP(hipDeviceSynchronize());
     kz_twiddles(*b, b->ztw, b->nz);
   }
   if (b->dct_zfast) {
@@ -444,6 +503,9 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     for (const std::vector<double> *a : {&h.inxu, &h.inxv, &h.dcxu, &h.dcxv, &h.vixu, &h.vixv})
       worst = std::max(worst, window_decay(*a, b->nx));
     b->xwin = b->nx > 2 * XLINE_WINDOW && worst < 1e-17 && std::getenv("OTBL_X_MARCH") == nullptr;
+    // the mixed substep has no marching fallback: short lines, which the
+    // windowed kernel solves exactly, or any decaying x systems
+    if (b->precision == OTBL_MIXED) b->xwin = b->nx <= XLINE_EXACT || worst < 1e-17;
   }
   // chunked y-line solver unless the columns are longer than its
   // instantiations (or OTBL_Y_MARCH asks for one thread per column)
@@ -452,6 +514,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     dalloc(b->ytab, YT_COUNT * ytab_size(b->yL));
     k_ytab_fixed(*b);
   }
+  if (b->precision == OTBL_MIXED) setup_mixed(*b);
   OTBL_HIP(hipStreamSynchronize(b->s));
   *out = new otbl_ctx{b};
   b = nullptr;
@@ -490,7 +553,22 @@ int otbl_set_field(otbl_ctx *ctx, int id, const double *host) {
   Box &b = box_of(ctx);
   need(id >= 0 && id < OTBL_NFIELDS && host, "otbl_set_field: bad id or null pointer");
   OTBL_HIP(hipStreamSynchronize(b.s));
-  OTBL_HIP(hipMemcpy(b.f[id], host, otbl_field_len(ctx, id) * sizeof(double), hipMemcpyHostToDevice));
+  const size_t n = otbl_field_len(ctx, id);
+  if (b.esz == 8) {
+    h2d(b, b.f[id], host, n * sizeof(double));
+  } else {
+    // staged through a work field (sized for FP64 elements), rounded to FP32
+    h2d(b, b.ya, host, n * sizeof(double));
+    k_d2f(b, b.ya, fld<float>(b.f[id]), n);
+    OTBL_HIP(hipStreamSynchronize(b.s));
+    double *mean = id == OTBL_P ? b.p0 : (id == OTBL_U ? b.um64 : nullptr);
+    if (mean) {
+      // the mean-flow pressure and u (real part of mode 0) are kept in FP64
+      std::vector<double> m(n / b.nz1);
+      for (size_t r = 0; r < m.size(); ++r) m[r] = host[r * b.nz1];
+      h2d(b, mean, m.data(), m.size() * sizeof(double));
+    }
+  }
   API_END
 }
 
@@ -499,7 +577,20 @@ int otbl_get_field(otbl_ctx *ctx, int id, double *host) {
   Box &b = box_of(ctx);
   need(id >= 0 && id < OTBL_NFIELDS && host, "otbl_get_field: bad id or null pointer");
   OTBL_HIP(hipStreamSynchronize(b.s));
-  OTBL_HIP(hipMemcpy(host, b.f[id], otbl_field_len(ctx, id) * sizeof(double), hipMemcpyDeviceToHost));
+  const size_t n = otbl_field_len(ctx, id);
+  if (b.esz == 8) {
+    d2h(b, host, b.f[id], n * sizeof(double));
+  } else {
+    k_f2d(b, fld<float>(b.f[id]), b.ya, n);
+    OTBL_HIP(hipStreamSynchronize(b.s));
+    d2h(b, host, b.ya, n * sizeof(double));
+    const double *mean = id == OTBL_P ? b.p0 : (id == OTBL_U ? b.um64 : nullptr);
+    if (mean) {
+      std::vector<double> m(n / b.nz1);
+      d2h(b, m.data(), mean, m.size() * sizeof(double));
+      for (size_t r = 0; r < m.size(); ++r) host[r * b.nz1] = m[r];
+    }
+  }
   API_END
 }
 
@@ -510,11 +601,14 @@ int otbl_set_inflow(otbl_ctx *ctx, const double *u0, const double *v0, const dou
   OTBL_HIP(hipStreamSynchronize(b.s));
   double rthin = 0, din = 0;
   inlet_setup(b.hc, u0, &rthin, &din);
-  OTBL_HIP(hipMemcpy(b.u0, u0, (b.ny + 1) * sizeof(double), hipMemcpyHostToDevice));
-  OTBL_HIP(hipMemcpy(b.v0, v0, b.ny * sizeof(double), hipMemcpyHostToDevice));
-  OTBL_HIP(hipMemcpy(b.um, um, (b.ny + 1) * sizeof(double), hipMemcpyHostToDevice));
-  OTBL_HIP(hipMemcpy((void *)b.dc.vmagic, b.hc.vmagic.data(), b.hc.vmagic.size() * sizeof(double),
-                     hipMemcpyHostToDevice));
+  h2d(b, b.u0, u0, (b.ny + 1) * sizeof(double));
+  h2d(b, b.v0, v0, b.ny * sizeof(double));
+  h2d(b, b.um, um, (b.ny + 1) * sizeof(double));
+  h2d(b, (void *)b.dc.vmagic, b.hc.vmagic.data(), b.hc.vmagic.size() * sizeof(double));
+  if (b.dcf.vmagic) {
+    k_d2f(b, b.dc.vmagic, (float *)b.dcf.vmagic, b.hc.vmagic.size());
+    OTBL_HIP(hipStreamSynchronize(b.s));
+  }
   put_ds(b, offsetof(DevScalars, rthin), &rthin, sizeof(double));
   put_ds(b, offsetof(DevScalars, din), &din, sizeof(double));
   API_END
@@ -524,7 +618,7 @@ int otbl_get_um(otbl_ctx *ctx, double *um) {
   API_BEGIN
   Box &b = box_of(ctx);
   OTBL_HIP(hipStreamSynchronize(b.s));
-  OTBL_HIP(hipMemcpy(um, b.um, (b.ny + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  d2h(b, um, b.um, (b.ny + 1) * sizeof(double));
   API_END
 }
 
@@ -577,7 +671,7 @@ int otbl_get_coef(otbl_ctx *ctx, const char *name, double *out, int *rows, int *
     if (strcmp(sl.name, name)) continue;
     const auto &hv = b.hc.*(sl.vec);
     std::vector<double> dv(hv.size());
-    OTBL_HIP(hipMemcpy(dv.data(), b.dc.*(sl.dev), dv.size() * sizeof(double), hipMemcpyDeviceToHost));
+    d2h(b, dv.data(), b.dc.*(sl.dev), dv.size() * sizeof(double));
     const int r = sl.rows, first = sl.first, n = slot_cols(b, sl.cols);
     *rows = r;
     *cols = n;
@@ -598,7 +692,7 @@ int otbl_get_coef(otbl_ctx *ctx, const char *name, double *out, int *rows, int *
   need(src != nullptr, "otbl_get_coef: unknown coefficient name");
   *rows = r;
   *cols = c;
-  if (out) OTBL_HIP(hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (out) d2h(b, out, src, n * sizeof(double));
   API_END
 }
 
@@ -743,6 +837,7 @@ int otbl_dev_free(void *ptr) {
 int otbl_h2d(void *dst, const void *src, size_t bytes) {
   API_BEGIN
   OTBL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  OTBL_HIP(hipDeviceSynchronize());
   API_END
 }
 

@@ -38,15 +38,17 @@ void build_coef(HostCoef &hc, int nx, int ny, int nz, double re, double ax, doub
 // inlet_retheta (bl_1.F90:372-396) and magicnumber (boun.f90:102-143)
 void inlet_setup(HostCoef &hc, const double *u0, double *rthin, double *din);
 
-// Device coefficient pointers (all FP64, same (r, n+1) layout as HostCoef).
-struct DevCoef {
-  const double *inxu, *inxv, *inyu, *inyv, *dcxu, *dcxv, *dcyu, *dcyv;
-  const double *vixu, *vixv, *viyu, *viyv, *vyui, *vyvi;
-  const double *cofiux, *cofivx, *cofiuy, *cofivy, *cofcxu, *cofcxv, *cofcyu, *cofcyv;
-  const double *cofvxu, *cofvxv, *cofvyu, *cofvyv, *ayp;
-  const double *inbx, *inby, *dcbx, *dcby, *ccon, *cofvbx, *cofvby;
-  const double *y, *dy, *idy, *idyy, *dymin, *hy, *kaz, *kaz2, *kvis, *kmod, *vmagic;
+// Device coefficient pointers (same (r, n+1) layout as HostCoef), FP64 or
+// an FP32 copy of the factored FP64 set.
+template <typename T> struct DevCoefT {
+  const T *inxu, *inxv, *inyu, *inyv, *dcxu, *dcxv, *dcyu, *dcyv;
+  const T *vixu, *vixv, *viyu, *viyv, *vyui, *vyvi;
+  const T *cofiux, *cofivx, *cofiuy, *cofivy, *cofcxu, *cofcxv, *cofcyu, *cofcyv;
+  const T *cofvxu, *cofvxv, *cofvyu, *cofvyv, *ayp;
+  const T *inbx, *inby, *dcbx, *dcby, *ccon, *cofvbx, *cofvby;
+  const T *y, *dy, *idy, *idyy, *dymin, *hy, *kaz, *kaz2, *kvis, *kmod, *vmagic;
 };
+using DevCoef = DevCoefT<double>;
 
 // Scalars that live on the device so a substep never waits for the host.
 struct DevScalars {
@@ -61,6 +63,10 @@ struct DevScalars {
 
 struct Box {
   int nx, ny, nz, nz1, nz2, mz, xout, precision, dt_mode;
+  // bytes per field element: 8 (OTBL_FP64) or 4 (OTBL_MIXED, FP32 fields);
+  // buffers are sized for 8-byte elements either way, so the FP64 operator
+  // entry points stay valid on a mixed box
+  int esz = 8;
   double re, ax, az, cfl;
   size_t nu, nv;     // doubles per u-type / v-type field
   size_t pu, pv;     // doubles per u-type / v-type plane
@@ -68,7 +74,26 @@ struct Box {
   HostCoef hc;
   DevCoef dc{};
   double *coefbuf = nullptr;
-  // persistent state
+  // FP32 copies for the mixed-precision substep (made after the FP64
+  // factorisation): coefficients, y-line tables, twiddles, Bluestein tables
+  DevCoefT<float> dcf{};
+  float *coefbuf_f = nullptr, *ytab_f = nullptr, *ztw_f = nullptr;
+  float *dct_chirp_f = nullptr, *dct_bf_f = nullptr, *dct_tw4_f = nullptr, *dct_ztw_f = nullptr;
+  // mixed Poisson: FP32 pivots of every (kz, kx) y operator (top-down and
+  // bottom-up eliminations, [ny-1][nz1/2][nx-1]) and the FP64 pencils of the
+  // mean-flow (kz = 0: kr = 0, 1) lines with their scratch (2(ny-1)(nx-1)
+  // each)
+  float *pdl = nullptr, *pdu = nullptr;
+  double *P0 = nullptr, *S0 = nullptr;
+  // mixed: the mean-flow pressure (real part of mode 0 of p) in FP64,
+  // [nx][ny]; its gradients feed the predictor and its phi the projection
+  double *p0 = nullptr;
+  // mixed: the mean-flow u (real part of mode 0 of u) in FP64, [nx][ny+1],
+  // updated in FP64 wherever the substep updates u (the FP32 field holds its
+  // rounding); its saved copy and a scratch of the same size
+  double *um64 = nullptr, *rm64 = nullptr, *tm64 = nullptr;
+  // persistent state; every field and work-field pointer below holds
+  // elements of esz bytes (read through fld<T>)
   double *f[OTBL_NFIELDS] = {nullptr};
   // work fields
   double *resu = nullptr, *resv = nullptr, *resw = nullptr;
@@ -117,6 +142,12 @@ struct Box {
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
 };
 
+// a field buffer as elements of the precision the substep runs in
+template <typename T> inline T *fld(double *p) { return reinterpret_cast<T *>(p); }
+template <typename T> inline const DevCoefT<T> &coefs(const Box &b);
+template <> inline const DevCoefT<double> &coefs<double>(const Box &b) { return b.dc; }
+template <> inline const DevCoefT<float> &coefs<float>(const Box &b) { return b.dcf; }
+
 // kernel launchers (otbl_kernels.hip); each enqueues on b.s
 void k_factor_coef(Box &b, double *coefbuf_rw);
 void k_interpxx(Box &b, const double *gen, double *geni, int R, int which_u);
@@ -134,20 +165,29 @@ void k_substep_rhsp(Box &b, int m);
 void k_outflow(Box &b);
 void k_pois(Box &b, int m, double varstep, bool use_m);
 void k_end_step(Box &b);
+void k_poisson_pivots(Box &b);
+// element conversions between FP64 and FP32 buffers on b.s
+void k_d2f(Box &b, const double *a, float *o, size_t n);
+void k_f2d(Box &b, const float *a, double *o, size_t n);
 void k_setup_attrs(Box &b);
 size_t k_dct_shm_bytes(int M);
 // register-resident z transforms (otbl_zfft.hip), valid when Box::zfast
 void kz_twiddles(Box &b, double *tw, int n);
 // windowed x lines (otbl_xline.hip), valid when Box::xwin
 constexpr int XLINE_WINDOW = 24; // planes of the backward window
-void kx_interpxx(Box &b, const double *gen, double *geni, const double *icon, const double *econ, int chi,
-                 int chi2, size_t P);
-void kx_difvisxx(Box &b, const double *prod, const double *vel, double *out, const double *icon,
-                 const double *iconv, const double *econ, const double *econv, int chv, int ic, size_t P,
-                 int mode);
-void kz_dct_pencils(Box &b, double *P, int iopt);
-void kz_products1(Box &b);
-void kz_products2(Box &b);
+constexpr int XLINE_EXACT = 32;  // lines up to this length are solved exactly
+template <typename T>
+void kx_interpxx(Box &b, const T *gen, T *geni, const T *icon, const T *econ, int chi, int chi2, size_t P);
+template <typename T>
+void kx_difvisxx(Box &b, const T *prod, const T *vel, T *out, const T *icon, const T *iconv, const T *econ,
+                 const T *econv, int chv, int ic, size_t P, int mode);
+// register-resident z transforms (otbl_zfft.hip), valid when Box::zfast /
+// Box::dct_zfast
+// Lp lines of the Poisson x transforms on pencils (all (ny-1)*nz1 lines,
+// or the 2(ny-1) mean-flow lines of the mixed path)
+template <typename R> void kz_dct_pencils(Box &b, R *P, int iopt, int Lp);
+template <typename R> void kz_products1(Box &b);
+template <typename R> void kz_products2(Box &b);
 void kz_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows);
 size_t k_zfft_shm_bytes(int nz, int nbuf);
 

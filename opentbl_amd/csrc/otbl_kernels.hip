@@ -16,8 +16,10 @@
 #include "otbl_fft.h"
 #include "otbl_ycol.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 
 namespace otbl {
 
@@ -354,7 +356,8 @@ void k_vistyy(Box &b, const double *u, double *res, int R, int nplanes) {
 // diagonal), f[2n+2..3n+2] a3, f[3n+3] = a1(2) and f[3n+4] = a3(jee-1)
 // before they are zeroed.  Dynamic LDS: 3*(ny+2) + 4*YNCH*L doubles.
 __global__ void impl_factor_kernel(double *fu, double *fv, DevCoef c, int ny, const DevScalars *ds,
-                                   double rkk_explicit, int use_explicit, double *tu, double *tv, int L) {
+                                   double rkk_explicit, int use_explicit, double *tu, double *tv, int L, float *tuf,
+                                   float *tvf) {
   extern __shared__ double shf[];
   const int jee = blockIdx.x == 0 ? ny + 1 : ny;
   const double *icon = blockIdx.x == 0 ? c.vyui : c.vyvi;
@@ -396,14 +399,22 @@ __global__ void impl_factor_kernel(double *fu, double *fv, DevCoef c, int ny, co
           return TriRow{a1[j], a2[j], a3[j]};
         },
         [=](int j) { return Sten3{C3(icon, 1, j), C3(icon, 2, j), C3(icon, 3, j)}; }, shf + 3 * (ny + 2));
+  // FP32 copy of the table for the mixed-precision substep
+  float *tabf = blockIdx.x == 0 ? tuf : tvf;
+  if (L && tabf) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < ytab_size(L); q += blockDim.x) tabf[q] = (float)tab[q];
+  }
 }
 
 static void launch_impl_factor(Box &b, double rkk, int use_explicit) {
   double *tu = b.yL ? b.ytab + YT_IMPU * ytab_size(b.yL) : nullptr;
   double *tv = b.yL ? b.ytab + YT_IMPV * ytab_size(b.yL) : nullptr;
+  float *tuf = b.ytab_f ? b.ytab_f + YT_IMPU * ytab_size(b.yL) : nullptr;
+  float *tvf = b.ytab_f ? b.ytab_f + YT_IMPV * ytab_size(b.yL) : nullptr;
   const size_t shm = (3 * (size_t)(b.ny + 2) + 4 * (size_t)YNCH * b.yL) * sizeof(double);
   LAUNCH(b, "impl_factor", 2, TPB, shm, impl_factor_kernel, b.implu, b.implv, b.dc, b.ny, b.ds, rkk,
-         use_explicit, tu, tv, b.yL);
+         use_explicit, tu, tv, b.yL, tuf, tvf);
 }
 
 // implzy (lip.f90:528-597): interior rows 2..jee-1 of u from res; rows 1 and
@@ -719,12 +730,18 @@ __global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F,
   }
 }
 
-__global__ void __launch_bounds__(TPB) pencils_to_planes_kernel(const double *P, double *F, int Lp, int N, size_t ps) {
-  __shared__ double tile[64][65];
+// With P0, the lines of mode 0 (kr = 0, 1) are taken from the FP64 pencils
+// P0 (line (j-1, kr) at P0[(2 (j-1) + kr) N]) instead of P.
+template <typename T>
+__global__ void __launch_bounds__(TPB)
+pencils_to_planes_kernel(const T *P, T *F, int Lp, int N, size_t ps, const double *P0, int nz1) {
+  __shared__ T tile[64][65];
   const int l0 = blockIdx.x * 64, x0 = blockIdx.y * 64, t = threadIdx.x;
   for (int ll = t / 64; ll < 64; ll += TPB / 64) {
     int l = l0 + ll, x = x0 + (t & 63);
-    if (l < Lp && x < N) tile[ll][t & 63] = P[(size_t)l * N + x];
+    if (l < Lp && x < N)
+      tile[ll][t & 63] = (P0 && l % nz1 < 2) ? (T)P0[(size_t)(2 * (l / nz1) + l % nz1) * N + x]
+                                             : P[(size_t)l * N + x];
   }
   __syncthreads();
   for (int xx = t / 64; xx < 64; xx += TPB / 64) {
@@ -788,7 +805,7 @@ static size_t dct_shm(const Box &b) { return (TW_SLOTS + 2 * (size_t)b.dct_M) * 
 // Row transforms on pencils (in place).
 static void dct_pencils(Box &b, double *P, int iopt) {
   if (b.dct_zfast) {
-    kz_dct_pencils(b, P, iopt);
+    kz_dct_pencils<double>(b, P, iopt, (b.ny - 1) * b.nz1);
     return;
   }
   const int Lp = (b.ny - 1) * b.nz1;
@@ -807,7 +824,8 @@ void k_cosftx(Box &b, const double *in, double *out, int iopt) {
   if (b.dct_fft) {
     LAUNCH(b, "to_pencils", tgrid(b), TPB, 0, planes_to_pencils_kernel, in, b.rest2, Lp, N, b.pv);
     dct_pencils(b, b.rest2, iopt);
-    LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel, b.rest2, out, Lp, N, b.pv);
+    LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<double>, b.rest2, out, Lp, N, b.pv,
+           (const double *)nullptr, b.nz1);
   } else {
     dim3 grid(nblk(Lp, 64), nblk(N, 64));
     LAUNCH(b, "dct_gemm", grid, 256, 0, dct_gemm_kernel, iopt < 0 ? b.dct2 : b.dct3, in + b.pv, out + b.pv, N,
@@ -832,14 +850,15 @@ __global__ void reset_max_kernel(DevScalars *ds, unsigned long long *vmbits, int
 // + r01^2) over the rows of component c (r01 = Re of mode 0, row 1) and
 // records rows 1-2 of mode 0 and their mode sums; slots 13-15 are cleared
 // afterwards (rhsp.F90:487).
+template <typename T>
 __global__ void __launch_bounds__(1024)
-energies_kernel(const double *u, const double *v, const double *w, const double *hy, DevScalars *ds, int nx,
-                int ny, int nz1, int nz2) {
+energies_kernel(const T *u, const T *v, const T *w, const double *hy, DevScalars *ds, int nx, int ny, int nz1,
+                int nz2) {
   __shared__ double red[16];
   const int c = blockIdx.x;
-  const double *fl = c == 0 ? u : (c == 1 ? v : w);
+  const T *fl = c == 0 ? u : (c == 1 ? v : w);
   const int rows = c == 1 ? ny : ny + 1;
-  const double *a = fl + (size_t)(nx / 2 - 1) * rows * nz1;
+  const T *a = fl + (size_t)(nx / 2 - 1) * rows * nz1;
   const double r01 = a[0];
   double part = 0.0;
   for (size_t e = threadIdx.x; e < (size_t)rows * nz1; e += blockDim.x) {
@@ -856,8 +875,9 @@ energies_kernel(const double *u, const double *v, const double *w, const double 
     for (int q = 0; q < (int)(blockDim.x / 64); ++q) t += red[q];
     double a1 = 0.0, a2 = 0.0;
     for (int k = 1; k <= nz2; ++k) {
-      a1 += a[2 * k] * a[2 * k] + a[2 * k + 1] * a[2 * k + 1];
-      a2 += a[nz1 + 2 * k] * a[nz1 + 2 * k] + a[nz1 + 2 * k + 1] * a[nz1 + 2 * k + 1];
+      const double x0 = a[2 * k], x1 = a[2 * k + 1], y0 = a[nz1 + 2 * k], y1 = a[nz1 + 2 * k + 1];
+      a1 += x0 * x0 + x1 * x1;
+      a2 += y0 * y0 + y1 * y1;
     }
     ds->ener[1 + c] = t;
     ds->ener[4 + 4 * c] = a[0];
@@ -887,10 +907,15 @@ __global__ void rhsu_loop1_kernel(const double *__restrict__ uv, const double *_
 
 // Save the velocities (rhsp.F90:281-285) and apply the pressure-gradient
 // predictor (rhsp.F90:291-339) in one pass over u-type elements.
-__global__ void save_predict_kernel(double *u, double *v, double *w, const double *__restrict__ p,
-                                    double *resu, double *resv, double *resw, const double *idyy,
-                                    const double *kaz, const DevScalars *ds, double idxx, int nx,
-                                    int ny, int nz1) {
+// With FP32 fields: p0, the FP64 mean-flow pressure [nx][ny], whose
+// gradients replace those of p's mode-0 real part; um, the FP64 mean-flow u
+// [nx][ny+1] (mode-0 real part), updated in FP64 and saved to rm (else
+// all null).
+template <typename T>
+__global__ void save_predict_kernel(T *u, T *v, T *w, const T *__restrict__ p, T *resu, T *resv, T *resw,
+                                    const T *idyy, const T *kaz, const DevScalars *ds, double idxx_d, int nx,
+                                    int ny, int nz1, const double *__restrict__ p0, const double *idyy_d,
+                                    double *um, double *rm) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t pu = (size_t)nz1 * (ny + 1), pv = (size_t)nz1 * ny;
   if (e >= pu * nx) return;
@@ -898,25 +923,45 @@ __global__ void save_predict_kernel(double *u, double *v, double *w, const doubl
   size_t r = e % pu;
   int j = (int)(r / nz1) + 1;
   int kr = (int)(r % nz1);
-  const double var1 = ds->var1;
+  const T var1 = (T)ds->var1, idxx = (T)idxx_d;
   size_t ev = (size_t)(i - 1) * pv + (size_t)(j - 1) * nz1 + kr;
-  double uu = u[e], ww = w[e];
+  T uu = u[e], ww = w[e];
   resu[e] = uu;
   resw[e] = ww;
-  if (i >= 2 && i <= nx - 1 && j >= 2 && j <= ny) {
-    u[e] = uu - var1 * idxx * (p[ev + pv] - p[ev]);
+  const bool mean = p0 && kr == 0;
+  const double *pm = mean ? p0 + (size_t)(i - 1) * ny + (j - 1) : nullptr; // mean pressure at (i, j)
+  const bool upred = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny;
+  if (um && kr == 0) {
+    const size_t q = (size_t)(i - 1) * (ny + 1) + (j - 1);
+    double U = um[q];
+    rm[q] = U;
+    if (upred) {
+      U -= ds->var1 * idxx_d * (pm[ny] - pm[0]);
+      um[q] = U;
+      u[e] = (T)U;
+    }
+  } else if (upred) {
+    if (mean)
+      u[e] = (T)((double)uu - ds->var1 * idxx_d * (pm[ny] - pm[0]));
+    else
+      u[e] = uu - var1 * idxx * (p[ev + pv] - p[ev]);
   }
   if (i >= 2 && j >= 2 && j <= ny) {
     // w -= var1 * (i kz p): Re gets -(-kz Im p), Im gets -(kz Re p)
-    double kz = kaz[kr / 2];
-    double pp = p[ev ^ 1];
-    double t = (kr & 1) ? (kz * pp) : (-kz * pp);
+    T kz = kaz[kr / 2];
+    T pp = p[ev ^ 1];
+    T t = (kr & 1) ? (kz * pp) : (-kz * pp);
     w[e] = ww - var1 * t;
   }
   if (j <= ny) {
-    double vv = v[ev];
+    T vv = v[ev];
     resv[ev] = vv;
-    if (i >= 2 && j >= 2 && j <= ny - 1) v[ev] = vv - var1 * idyy[j] * (p[ev + nz1] - p[ev]);
+    if (i >= 2 && j >= 2 && j <= ny - 1) {
+      if (mean)
+        v[ev] = (T)((double)vv - ds->var1 * idyy_d[j] * (pm[1] - pm[0]));
+      else
+        v[ev] = vv - var1 * idyy[j] * (p[ev + nz1] - p[ev]);
+    }
   }
 }
 
@@ -962,10 +1007,11 @@ __global__ void dt_kernel(DevScalars *ds, const unsigned long long *vmbits, cons
 //    factor, the zero modes from u0 / v0, the profile remapping, the copy of
 //    row ntop-3 above ntop-2 and the 1/gamma1 scaling, evaluated per element.
 // ibuf: kl[2][ny+1], ntop[2]; dbuf: fsca[2][ny+1], ysca[ny+1], wlwa[ny+1].
-__global__ void genflu_stats_kernel(const double *resu, double *um, int *ibuf, double *dbuf, DevCoef c,
-                                    DevScalars *ds, int ny, int nz1, int xout, double re) {
+template <typename T>
+__global__ void genflu_stats_kernel(const T *resu, double *um, int *ibuf, double *dbuf, DevCoef c, DevScalars *ds,
+                                    int ny, int nz1, int xout, double re) {
   const int s = nz1;
-  const double *ux = resu + (size_t)(xout - 1) * s * (ny + 1);
+  const T *ux = resu + (size_t)(xout - 1) * s * (ny + 1);
   __shared__ double sh_dtc;
   if (threadIdx.x == 0) {
     double tloc = ds->tiempo - ds->timeinit;
@@ -977,7 +1023,7 @@ __global__ void genflu_stats_kernel(const double *resu, double *um, int *ibuf, d
   const double dtc = sh_dtc;
   // u1 mode 0 after the copy is ux mode 0 (the damping factor is 1 there)
   for (int j = threadIdx.x + 1; j <= ny + 1; j += blockDim.x)
-    um[j - 1] = dtc * ux[(size_t)(j - 1) * s] + um[j - 1] * (1.0 - dtc);
+    um[j - 1] = dtc * (double)ux[(size_t)(j - 1) * s] + um[j - 1] * (1.0 - dtc);
   __syncthreads();
   int *kl = ibuf;                  // kl[2][ny+1]
   int *ntop = ibuf + 2 * (ny + 1); // ntop[2]
@@ -1050,27 +1096,30 @@ __global__ void genflu_stats_kernel(const double *resu, double *um, int *ibuf, d
 // one against v rows, row 1 of u/w = -row 2) with rows >= ntop-2 copied from
 // row ntop-3, times 1/gamma1; with ntop < 4 the copied plane stays (the
 // reference's copy of row ntop-3 would read out of bounds).
-__global__ void genflu_plane_kernel(double *u, double *v, double *w, const double *resu, const double *resv,
-                                    const double *resw, const double *u0, const double *v0, const int *ibuf,
-                                    const double *dbuf, DevCoef c, const DevScalars *ds, int ny, int nz1,
-                                    int xout) {
+template <typename T>
+__global__ void genflu_plane_kernel(T *u, T *v, T *w, const T *resu, const T *resv, const T *resw, const double *u0,
+                                    const double *v0, const int *ibuf, const double *dbuf, DevCoef c,
+                                    const DevScalars *ds, int ny, int nz1, int xout, double *um) {
   const int f = blockIdx.y;
   const int n = f == 2 ? ny : ny + 1, s = nz1;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)s * n) return;
   const int i = (int)(e / s) + 1, k = (int)(e % s);
-  double *out = f == 0 ? u : (f == 1 ? w : v);
+  T *out = f == 0 ? u : (f == 1 ? w : v);
   if (k < 2) {
     double z = 0.0;
-    if (k == 0 && f == 0) z = u0[i - 1];
+    if (k == 0 && f == 0) {
+      z = u0[i - 1];
+      if (um) um[i - 1] = z; // plane 1 of the FP64 mean-flow u
+    }
     if (k == 0 && f == 2) z = v0[i - 1];
-    out[e] = z;
+    out[e] = (T)z;
     return;
   }
-  const double *rx = (f == 0 ? resu : (f == 1 ? resw : resv)) + (size_t)(xout - 1) * s * n;
+  const T *rx = (f == 0 ? resu : (f == 1 ? resw : resv)) + (size_t)(xout - 1) * s * n;
   const double yb = c.y[119 < ny ? 119 : ny]; // y(119) clamped to the grid
   auto cval = [&](int row) { // plane 1 after the copy, 1-based row
-    double r = rx[(size_t)(row - 1) * s + k];
+    double r = (double)rx[(size_t)(row - 1) * s + k];
     if (f != 2) r *= 0.5 * (1 - tanh(c.y[row] - yb));
     return r;
   };
@@ -1078,7 +1127,7 @@ __global__ void genflu_plane_kernel(double *u, double *v, double *w, const doubl
   const double *fsca = dbuf + (f == 2 ? ny + 1 : 0);
   const int nt = ibuf[2 * (ny + 1) + (f == 2 ? 1 : 0)];
   if (nt < 4) {
-    out[e] = cval(i);
+    out[e] = (T)cval(i);
     return;
   }
   const int isrc = i >= nt - 2 ? nt - 3 : i;
@@ -1094,7 +1143,7 @@ __global__ void genflu_plane_kernel(double *u, double *v, double *w, const doubl
     const double fs = fsca[isrc];
     W = cval(kll + 1) * fs + cval(kll) * (1 - fs);
   }
-  out[e] = W * (1.0 / ds->gamma1);
+  out[e] = (T)(W * (1.0 / ds->gamma1));
 }
 
 // z-viscous terms and the rest of rhsv/rhsw (rhsp.F90:355-415) for planes
@@ -1134,38 +1183,50 @@ __global__ void rhs_loop2_kernel(double *rhsu, double *rhsv, double *rhsw, const
 
 // boun (boun.f90:7-56) and the right-hand-side boundary rows
 // (rhsp.F90:421-445) for every plane.  Columns (kr, i).
-__global__ void boun_kernel(double *u, double *v, double *w, double *rhsu, double *rhsv, double *rhsw,
-                            DevCoef c, int ny, int nz1, int nx) {
+// um: with FP32 fields, the FP64 mean-flow u [nx][ny+1], whose boundary
+// rows are extrapolated in FP64 from its own interior (else null).
+template <typename T>
+__global__ void boun_kernel(T *u, T *v, T *w, T *rhsu, T *rhsv, T *rhsw, DevCoefT<T> c, int ny, int nz1, int nx,
+                            double *um, DevCoef cd) {
   size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cidx >= (size_t)nz1 * nx) return;
   int kr = (int)(cidx % nz1);
   size_t ip = cidx / nz1;
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
-  double *vi = v + ip * pv + kr;
-  vi[0] = 0.0;
-  vi[(size_t)(ny - 1) * s] = (kr == 0) ? c.vmagic[ip + 1] : 0.0;
-  const double i21 = -1.0 / M4(c.inby, 2, 1), c11 = -1.0 / M2(c.ccon, 1, 1);
-  double *f[2] = {u + ip * pu + kr, w + ip * pu + kr};
+  T *vi = v + ip * pv + kr;
+  vi[0] = T(0);
+  vi[(size_t)(ny - 1) * s] = (kr == 0) ? c.vmagic[ip + 1] : T(0);
+  const T i21 = T(-1) / M4(c.inby, 2, 1), c11 = T(-1) / M2(c.ccon, 1, 1);
+  T *f[2] = {u + ip * pu + kr, w + ip * pu + kr};
   for (int q = 0; q < 2; ++q) {
-    double *a = f[q];
+    T *a = f[q];
+    if (q == 0 && um && kr == 0) {
+      double *U = um + ip * (ny + 1);
+      U[0] = -1.0 / M4(cd.inby, 2, 1) * (M4(cd.inby, 2, 2) * U[1] + M4(cd.inby, 2, 3) * U[2] + M4(cd.inby, 2, 4) * U[3]);
+      U[ny] = -1.0 / M2(cd.ccon, 1, 1) *
+              (M2(cd.ccon, 1, 2) * U[ny - 1] + M2(cd.ccon, 1, 3) * U[ny - 2] + M2(cd.ccon, 1, 4) * U[ny - 3]);
+      a[0] = (T)U[0];
+      a[(size_t)ny * s] = (T)U[ny];
+      continue;
+    }
     a[0] = i21 * (M4(c.inby, 2, 2) * a[s] + M4(c.inby, 2, 3) * a[2 * s] + M4(c.inby, 2, 4) * a[3 * s]);
     a[(size_t)ny * s] = c11 * (M2(c.ccon, 1, 2) * a[(size_t)(ny - 1) * s] + M2(c.ccon, 1, 3) * a[(size_t)(ny - 2) * s] +
                                M2(c.ccon, 1, 4) * a[(size_t)(ny - 3) * s]);
   }
   // rhs: plane 1 zero, v rows 1 and ny zero, u/w row 1 extrapolated, row ny+1 zero
-  double *rvi = rhsv + ip * pv + kr;
-  double *rf[2] = {rhsu + ip * pu + kr, rhsw + ip * pu + kr};
+  T *rvi = rhsv + ip * pv + kr;
+  T *rf[2] = {rhsu + ip * pu + kr, rhsw + ip * pu + kr};
   if (ip == 0) {
-    for (int j = 0; j < ny; ++j) rvi[(size_t)j * s] = 0.0;
+    for (int j = 0; j < ny; ++j) rvi[(size_t)j * s] = T(0);
     for (int q = 0; q < 2; ++q)
-      for (int j = 0; j <= ny; ++j) rf[q][(size_t)j * s] = 0.0;
+      for (int j = 0; j <= ny; ++j) rf[q][(size_t)j * s] = T(0);
   }
-  rvi[0] = 0.0;
-  rvi[(size_t)(ny - 1) * s] = 0.0;
+  rvi[0] = T(0);
+  rvi[(size_t)(ny - 1) * s] = T(0);
   for (int q = 0; q < 2; ++q) {
-    double *a = rf[q];
+    T *a = rf[q];
     a[0] = i21 * (M4(c.inby, 2, 2) * a[s] + M4(c.inby, 2, 3) * a[2 * s] + M4(c.inby, 2, 4) * a[3 * s]);
-    a[(size_t)ny * s] = 0.0;
+    a[(size_t)ny * s] = T(0);
   }
 }
 
@@ -1200,18 +1261,59 @@ __global__ void rk_final_kernel(double *u, const double *res, const double *rhs,
   implzy_col(uu, t, s, jee, jee == ny + 1 ? c.vyui : c.vyvi, f);
 }
 
+// FP64 mean-flow u of the mixed substep (fields FP32): the explicit RK3
+// combination and implicit y solve of rk_final_kernel for the kz = 0 column
+// of u of every plane, from the FP64 state um and its saved copy rm, with
+// the FP32 rhs and previous rhs (mode-0 real parts).  Runs before the FP32
+// rk_final of u, which then overwrites rhsp; mean_to_field_kernel copies the
+// result into the FP32 field afterwards.  One thread per plane; tm: scratch
+// [nx][ny+1].
+template <typename T>
+__global__ void mean_rk_final_kernel(double *um, const double *rm, const T *rhs, const T *rhsp, double *tm,
+                                     DevCoef c, const double *f, const DevScalars *ds, int ny, int nz1, int nx) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= nx || ip == 0) return; // plane 1 keeps u
+  const int jee = ny + 1;
+  const size_t pu = (size_t)nz1 * jee;
+  double *U = um + (size_t)ip * jee, *t = tm + (size_t)ip * jee;
+  const T *rh = rhs + ip * pu, *rp = rhsp + ip * pu;
+  const double var2 = (ip == nx - 1) ? 0.0 : ds->var2, var3 = ds->var3, var4 = ds->var4;
+  vistyy_col(rm + (size_t)ip * jee, t, 1, jee, ny, c);
+  for (int j = 0; j < jee; ++j)
+    t[j] = U[j] - (var3 * (double)rh[(size_t)j * nz1] + var2 * t[j] + var4 * (double)rp[(size_t)j * nz1]);
+  implzy_col(U, t, 1, jee, c.vyui, f);
+}
+
+template <typename T>
+__global__ void mean_to_field_kernel(T *u, const double *um, int nx, int jee, int nz1) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)nx * jee) return;
+  u[q * nz1] = (T)um[q];
+}
+
 // ---------------------------------------------------------------------------
 // outflow_correction (boun.f90:63-93), FP64, one workgroup
 // ---------------------------------------------------------------------------
-__global__ void outflow_kernel(double *u, const double *v, double *rhsupa, DevCoef c, DevScalars *ds,
-                               int nx, int ny, int nz1, double dx1, double idx) {
+template <typename T>
+__global__ void outflow_kernel(T *u, const T *v, T *rhsupa, DevCoef c, DevScalars *ds, int nx, int ny, int nz1,
+                               double dx1, double idx, double *um) {
   __shared__ double red[3][TPB];
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   double a = 0.0, b1 = 0.0, b2 = 0.0;
-  for (int i = threadIdx.x + 2; i <= nx; i += blockDim.x) a += v[(size_t)(i - 1) * pv + (size_t)(ny - 1) * s];
+  // With FP32 fields: u from the FP64 mean-flow u um [nx][ny+1]; a top-row
+  // v that is the FP32 image of its boundary value (vmagic, set by boun)
+  // enters as that FP64 value (the rounding of this smooth profile is
+  // correlated along the boundary and would bias the flux).
+  constexpr bool F32 = sizeof(T) == 4;
+  for (int i = threadIdx.x + 2; i <= nx; i += blockDim.x) {
+    const T vt = v[(size_t)(i - 1) * pv + (size_t)(ny - 1) * s];
+    a += (F32 && vt == (T)c.vmagic[i]) ? c.vmagic[i] : (double)vt;
+  }
   for (int j = threadIdx.x + 2; j <= ny; j += blockDim.x) {
-    b1 += u[(size_t)(j - 1) * s] * c.dy[j - 1];
-    b2 += u[(size_t)(nx - 1) * pu + (size_t)(j - 1) * s] * c.dy[j - 1];
+    const double u1 = um ? um[j - 1] : (double)u[(size_t)(j - 1) * s];
+    const double un = um ? um[(size_t)(nx - 1) * (ny + 1) + (j - 1)] : (double)u[(size_t)(nx - 1) * pu + (size_t)(j - 1) * s];
+    b1 += u1 * c.dy[j - 1];
+    b2 += un * c.dy[j - 1];
   }
   red[0][threadIdx.x] = a;
   red[1][threadIdx.x] = b1;
@@ -1229,14 +1331,28 @@ __global__ void outflow_kernel(double *u, const double *v, double *rhsupa, DevCo
   if (threadIdx.x == 0) ds->sumax = sumax;
   for (int j = threadIdx.x + 2; j <= ny; j += blockDim.x) {
     size_t e = (size_t)(nx - 1) * pu + (size_t)(j - 1) * s;
-    u[e] += alpha;
-    rhsupa[e] += alpha * idx;
+    if (um) {
+      double &U = um[(size_t)(nx - 1) * (ny + 1) + (j - 1)];
+      U += alpha;
+      u[e] = (T)U;
+    } else {
+      u[e] = (T)((double)u[e] + alpha);
+    }
+    rhsupa[e] = (T)((double)rhsupa[e] + alpha * idx);
   }
 }
 
+template <typename T> static void outflow_t(Box &b) {
+  auto kf = &outflow_kernel<T>;
+  LAUNCH(b, "outflow", 1, TPB, 0, kf, fld<T>(b.f[OTBL_U]), fld<T>(b.f[OTBL_V]), fld<T>(b.f[OTBL_RHSUPA]), b.dc,
+         b.ds, b.nx, b.ny, b.nz1, b.hc.dx[1], b.hc.idx, std::is_same<T, double>::value ? nullptr : b.um64);
+}
+
 void k_outflow(Box &b) {
-  LAUNCH(b, "outflow", 1, TPB, 0, outflow_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_RHSUPA], b.dc,
-         b.ds, b.nx, b.ny, b.nz1, b.hc.dx[1], b.hc.idx);
+  if (b.precision == OTBL_MIXED)
+    outflow_t<float>(b);
+  else
+    outflow_t<double>(b);
 }
 
 // ---------------------------------------------------------------------------
@@ -1307,10 +1423,12 @@ __global__ void poisson_y_kernel(double *rr, double *a2s, DevCoef c, int nx, int
 
 // divergence (pois.f90:41-54) written straight into x-contiguous pencils
 // through an LDS tile (64 lines x 64 planes)
+template <typename T>
 __global__ void __launch_bounds__(TPB)
-div_pencils_kernel(const double *u, const double *v, const double *w, double *P, DevCoef c, double idx,
-                   int nx, int ny, int nz1) {
-  __shared__ double tile[64][65];
+div_pencils_kernel(const T *u, const T *v, const T *w, T *P, DevCoefT<T> c, double idx_d, int nx, int ny,
+                   int nz1, double *P0, DevCoef cd, const double *um) {
+  __shared__ T tile[64][65];
+  const T idx = (T)idx_d;
   const int Lp = (ny - 1) * nz1, N = nx - 1;
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   const int l0 = blockIdx.x * 64, x0 = blockIdx.y * 64, t = threadIdx.x;
@@ -1318,12 +1436,21 @@ div_pencils_kernel(const double *u, const double *v, const double *w, double *P,
     int l = l0 + (t & 63), x = x0 + xx;
     if (l < Lp && x < N) {
       int i = x + 2, j = l / nz1 + 1, kr = l % nz1;
-      double kz = c.kaz[kr / 2];
       size_t ow = (size_t)(i - 1) * pu + (size_t)j * s;       // row j+1
       size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s; // row j
-      double wp = w[ow + (kr ^ 1)];
-      double tz = (kr & 1) ? wp * kz : -wp * kz;
-      double d = tz + (v[ov + s + kr] - v[ov + kr]) * c.idy[j];
+      if (P0 && kr < 2) {
+        // the mean-flow (kz = 0) lines of FP32 fields, formed in FP64 (with
+        // the FP64 mean-flow u um); line (j-1, kr) at P0[(2 (j-1) + kr) N]
+        const double wp = w[ow + (kr ^ 1)], kz = cd.kaz[0];
+        const double d = (kr ? wp * kz : -wp * kz) + ((double)v[ov + s + kr] - (double)v[ov + kr]) * cd.idy[j];
+        const double du = (um && kr == 0) ? um[(size_t)(i - 1) * (ny + 1) + j] - um[(size_t)(i - 2) * (ny + 1) + j]
+                                          : (double)u[ow + kr] - (double)u[ow - pu + kr];
+        P0[(size_t)(2 * (j - 1) + kr) * N + x] = d + idx_d * du;
+      }
+      T kz = c.kaz[kr / 2];
+      T wp = w[ow + (kr ^ 1)];
+      T tz = (kr & 1) ? wp * kz : -wp * kz;
+      T d = tz + (v[ov + s + kr] - v[ov + kr]) * c.idy[j];
       tile[xx][t & 63] = d + idx * (u[ow + kr] - u[ow - pu + kr]);
     }
   }
@@ -1375,10 +1502,135 @@ poisson_y_pencil_kernel(double *P, double *S, DevCoef c, int nx, int ny, int nz1
   }
 }
 
+// ---------------------------------------------------------------------------
+// mixed-precision Poisson y solves
+// ---------------------------------------------------------------------------
+// The y operator of mode (kz, kx) (pois.f90:102-117) depends on the grid
+// only, so its pivots are factored once, in FP64, and kept as FP32: dl[j] =
+// 1/a2_j of the elimination from row 1 down (soltriy's order, pois.f90:
+// 211-239) and du[j] = 1/c_j of the elimination from row ny-1 up.  Tables
+// [j-1][kz][x] for kz = 0..nz1/2-1, x = kx mode 0..nx-2.
+__global__ void __launch_bounds__(TPB)
+poisson_pivots_kernel(float *dl, float *du, DevCoef c, int nx, int ny, int nzh) {
+  const int N = nx - 1, ny1 = ny - 1;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nzh * N) return;
+  const int x = (int)(t % N), kz = (int)(t / N), i = x + 2;
+  const size_t rs = (size_t)nzh * N;
+  const double sh = c.kaz2[kz] + c.kmod[i - 1];
+  double a2 = C3(c.ayp, 2, 1) - sh;
+  dl[t] = (float)(1.0 / a2);
+  for (int j = 2; j <= ny1; ++j) {
+    const double d = 1.0 / a2;
+    a2 = (C3(c.ayp, 2, j) - sh) - C3(c.ayp, 1, j) * d * C3(c.ayp, 3, j - 1);
+    dl[(size_t)(j - 1) * rs + t] = (float)(1.0 / a2);
+  }
+  double cc = C3(c.ayp, 2, ny1) - sh;
+  du[(size_t)(ny1 - 1) * rs + t] = (float)(1.0 / cc);
+  for (int j = ny1 - 1; j >= 1; --j) {
+    const double d = 1.0 / cc;
+    cc = (C3(c.ayp, 2, j) - sh) - C3(c.ayp, 3, j) * C3(c.ayp, 1, j + 1) * d;
+    du[(size_t)(j - 1) * rs + t] = (float)(1.0 / cc);
+  }
+}
+
+// y solve of the modes kz >= 1 on FP32 pencils P (line (j-1) nz1 + kr, x
+// contiguous): one thread per (kz, kx) solves the Re and Im columns with the
+// FP32 pivots, forms the residual b - A x of that solution in FP64 and adds
+// the FP32 solution of A d = r through the bottom-up factors: one step of
+// iterative refinement.  Rows: a(1,j) x(j-1) + (a(2,j) - kz^2 - kmod) x(j) +
+// a(3,j) x(j+1) = b(j), a = ayp.  S: FP32 scratch of the pencils' size.
+// Threads of kz = 0 clear the Im column (the zero mode is real,
+// pois.f90:102); the Re column of kz = 0 is solved in FP64 on its own pencils.
+__global__ void __launch_bounds__(TPB)
+poisson_y_mixed_kernel(float *P, float *S, const float *__restrict__ dl, const float *__restrict__ du,
+                       DevCoefT<float> cf, DevCoef cd, int nx, int ny, int nz1) {
+  const int N = nx - 1, nzh = nz1 / 2, ny1 = ny - 1;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nzh * N) return;
+  const int x = (int)(t % N), kz = (int)(t / N), i = x + 2;
+  const size_t rs = (size_t)nz1 * N, prs = (size_t)nzh * N;
+  float *pr = P + (size_t)(2 * kz) * N + x, *pim = pr + N;
+  float *sr = S + (size_t)(2 * kz) * N + x, *sim = sr + N;
+  const float *dlp = dl + t, *dup = du + t;
+  if (kz == 0) {
+    for (int j = 0; j < ny1; ++j) pim[(size_t)j * rs] = 0.f;
+    return;
+  }
+  const double shd = cd.kaz2[kz] + cd.kmod[i - 1];
+  // forward elimination, rows ascending: y -> S, b stays in P
+  float yr = pr[0], yi = pim[0];
+  sr[0] = yr;
+  sim[0] = yi;
+  for (int j = 2; j <= ny1; ++j) {
+    const size_t o = (size_t)(j - 1) * rs;
+    const float a1 = -C3(cf.ayp, 1, j) * dlp[(size_t)(j - 2) * prs];
+    yr = pr[o] + a1 * yr;
+    yi = pim[o] + a1 * yi;
+    sr[o] = yr;
+    sim[o] = yi;
+  }
+  // back substitution, rows descending: x -> S; with x(j) known, the FP64
+  // residual of row j+1 and its bottom-up elimination z -> P
+  float xr1 = 0.f, xi1 = 0.f, xr2 = 0.f, xi2 = 0.f; // x(j+1), x(j+2)
+  float br1 = 0.f, bi1 = 0.f;                       // b(j+1)
+  float zr = 0.f, zi = 0.f;                         // z(j+2), then z(j+1)
+  for (int j = ny1; j >= 1; --j) {
+    const size_t o = (size_t)(j - 1) * rs;
+    const float d = dlp[(size_t)(j - 1) * prs];
+    const float a3 = j < ny1 ? C3(cf.ayp, 3, j) : 0.f;
+    const float br = pr[o], bi = pim[o];
+    const float xr = (sr[o] - a3 * xr1) * d, xi = (sim[o] - a3 * xi1) * d;
+    sr[o] = xr;
+    sim[o] = xi;
+    if (j < ny1) {
+      const int q = j + 1;
+      const double a1q = C3(cd.ayp, 1, q), dq = C3(cd.ayp, 2, q) - shd, a3q = q < ny1 ? C3(cd.ayp, 3, q) : 0.0;
+      const double rr = (double)br1 - (a1q * (double)xr + dq * (double)xr1 + a3q * (double)xr2);
+      const double ri = (double)bi1 - (a1q * (double)xi + dq * (double)xi1 + a3q * (double)xi2);
+      const float f = q < ny1 ? C3(cf.ayp, 3, q) * dup[(size_t)q * prs] : 0.f;
+      zr = (float)rr - f * zr;
+      zi = (float)ri - f * zi;
+      pr[o + rs] = zr;
+      pim[o + rs] = zi;
+    }
+    xr2 = xr1;
+    xi2 = xi1;
+    xr1 = xr;
+    xi1 = xi;
+    br1 = br;
+    bi1 = bi;
+  }
+  {
+    const double dq = C3(cd.ayp, 2, 1) - shd, a3q = ny1 > 1 ? C3(cd.ayp, 3, 1) : 0.0;
+    const double rr = (double)br1 - (dq * (double)xr1 + a3q * (double)xr2);
+    const double ri = (double)bi1 - (dq * (double)xi1 + a3q * (double)xi2);
+    const float f = ny1 > 1 ? C3(cf.ayp, 3, 1) * dup[prs] : 0.f;
+    pr[0] = (float)rr - f * zr;
+    pim[0] = (float)ri - f * zi;
+  }
+  // the correction, rows ascending, added to x -> P
+  float dr = 0.f, di = 0.f;
+  for (int j = 1; j <= ny1; ++j) {
+    const size_t o = (size_t)(j - 1) * rs;
+    const float u = dup[(size_t)(j - 1) * prs];
+    const float a1 = j > 1 ? C3(cf.ayp, 1, j) : 0.f;
+    dr = (pr[o] - a1 * dr) * u;
+    di = (pim[o] - a1 * di) * u;
+    pr[o] = sr[o] + dr;
+    pim[o] = sim[o] + di;
+  }
+}
+
 // projection (pois.f90:144-162): p, w, v, u from phi = rest
-__global__ void project_kernel(double *u, double *v, double *w, double *p, const double *rest, DevCoef c,
-                               const DevScalars *ds, double vstep, int use_m, double idxx, int nx, int ny,
-                               int nz1) {
+// With FP32 fields, P0 holds phi of the mean-flow lines in FP64 (line
+// (j-1, kr) at P0[(2 (j-1) + kr)(nx-1)], planes 2..nx), p0 the FP64
+// mean-flow pressure and um the FP64 mean-flow u: their differences are
+// taken in FP64 (else all null).
+template <typename T>
+__global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T> c, const DevScalars *ds,
+                               double vstep, int use_m, double idxx_d, int nx, int ny, int nz1,
+                               const double *__restrict__ P0, double *p0, const double *idyy_d, double *um) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   if (e >= pu * nx) return;
@@ -1387,37 +1639,89 @@ __global__ void project_kernel(double *u, double *v, double *w, double *p, const
   int j = (int)(r / s) + 1;
   int kr = (int)(r % s);
   if (i < 2) return;
-  const double varstep = use_m ? ds->varstep : vstep;
-  const double *ph = rest + (size_t)(i - 1) * pv;
+  const double vs = use_m ? ds->varstep : vstep;
+  if (P0 && kr == 0) {
+    // phi(row r, plane q) = P0[2 (r-1) N + q-2]; kaz(0) = 0 leaves w
+    const int N = nx - 1;
+    const size_t rN = 2 * (size_t)N;
+    if (j >= 2 && j <= ny) {
+      const double phm = P0[(size_t)(j - 2) * rN + (i - 2)];
+      const size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s;
+      double &pm = p0[(size_t)(i - 1) * ny + (j - 1)];
+      pm = vs * phm + pm;
+      p[ov] = (T)pm;
+      if (j <= ny - 1) v[ov] = (T)((double)v[ov] - idyy_d[j] * (P0[(size_t)(j - 1) * rN + (i - 2)] - phm));
+      if (i <= nx - 1) {
+        const double du = idxx_d * (P0[(size_t)(j - 2) * rN + (i - 1)] - phm);
+        if (um) {
+          double &U = um[(size_t)(i - 1) * (ny + 1) + (j - 1)];
+          U -= du;
+          u[e] = (T)U;
+        } else {
+          u[e] = (T)((double)u[e] - du);
+        }
+      }
+    }
+    return;
+  }
+  const T varstep = (T)vs, idxx = (T)idxx_d;
+  const T *ph = rest + (size_t)(i - 1) * pv;
   if (j >= 2 && j <= ny) {
     size_t o = (size_t)(j - 2) * s; // phi row j-1
     size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s;
     p[ov + kr] = varstep * ph[o + kr] + p[ov + kr];
-    double kz = c.kaz[kr / 2];
-    double pp = ph[o + (kr ^ 1)];
-    double t = (kr & 1) ? kz * pp : -kz * pp;
+    T kz = c.kaz[kr / 2];
+    T pp = ph[o + (kr ^ 1)];
+    T t = (kr & 1) ? kz * pp : -kz * pp;
     w[e] -= t;
     if (j <= ny - 1) v[ov + kr] -= c.idyy[j] * (ph[(size_t)(j - 1) * s + kr] - ph[o + kr]);
   }
   if (i <= nx - 1 && j >= 2 && j <= ny) {
-    const double *pn = rest + (size_t)i * pv;
+    const T *pn = rest + (size_t)i * pv;
     size_t o = (size_t)(j - 2) * s + kr; // phi row j-1 for u row j
     u[e] -= idxx * (pn[o] - ph[o]);
   }
+}
+
+// FP32 pressure solve of the mixed path: the kz = 0 lines run in FP64 on
+// their own pencils (divergence, both x transforms and the pinned y solve),
+// the other modes in FP32 with one FP64-residual refinement of the y solve.
+static void pois_mixed(Box &b, double varstep, bool use_m) {
+  const int nx = b.nx, ny = b.ny, nz1 = b.nz1;
+  const int Lp = (ny - 1) * nz1, N = nx - 1;
+  float *P = fld<float>(b.rest2), *F = fld<float>(b.rest);
+  LAUNCH(b, "div", tgrid(b), TPB, 0, div_pencils_kernel<float>, fld<float>(b.f[OTBL_U]), fld<float>(b.f[OTBL_V]),
+         fld<float>(b.f[OTBL_W]), P, b.dcf, b.hc.idx, nx, ny, nz1, b.P0, b.dc, (const double *)b.um64);
+  kz_dct_pencils<float>(b, P, -1, Lp);
+  kz_dct_pencils<double>(b, b.P0, -1, 2 * (ny - 1));
+  LAUNCH(b, "poisson_y", nblk((size_t)(nz1 / 2) * N), TPB, 0, poisson_y_mixed_kernel, P, F, b.pdl, b.pdu, b.dcf,
+         b.dc, nx, ny, nz1);
+  LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.P0, b.S0, b.dc, nx, ny, 2);
+  kz_dct_pencils<float>(b, P, 1, Lp);
+  kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1));
+  LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pv, b.P0, nz1);
+  LAUNCH(b, "project", nblk(b.nu), TPB, 0, project_kernel<float>, fld<float>(b.f[OTBL_U]), fld<float>(b.f[OTBL_V]),
+         fld<float>(b.f[OTBL_W]), fld<float>(b.f[OTBL_P]), F, b.dcf, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx,
+         ny, nz1, (const double *)b.P0, b.p0, b.dc.idyy, b.um64);
 }
 
 void k_pois(Box &b, int m, double varstep, bool use_m) {
   const int nx = b.nx, ny = b.ny, nz1 = b.nz1;
   const int Lp = (ny - 1) * nz1, N = nx - 1;
   (void)m;
+  if (b.precision == OTBL_MIXED) {
+    pois_mixed(b, varstep, use_m);
+    return;
+  }
   if (b.dct_fft) {
-    LAUNCH(b, "div", tgrid(b), TPB, 0, div_pencils_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest2, b.dc,
-           b.hc.idx, nx, ny, nz1);
+    LAUNCH(b, "div", tgrid(b), TPB, 0, div_pencils_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest2,
+           b.dc, b.hc.idx, nx, ny, nz1, (double *)nullptr, b.dc, (const double *)nullptr);
     dct_pencils(b, b.rest2, -1);
     LAUNCH(b, "poisson_y", nblk((size_t)nz1 * N), TPB, 0, poisson_y_pencil_kernel, b.rest2, b.rest, b.dc, nx, ny,
            nz1);
     dct_pencils(b, b.rest2, 1);
-    LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel, b.rest2, b.rest, Lp, N, b.pv);
+    LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<double>, b.rest2, b.rest, Lp, N, b.pv,
+           (const double *)nullptr, nz1);
   } else {
     size_t ndiv = (size_t)(ny - 1) * nz1 * (nx - 1);
     LAUNCH(b, "div", nblk(ndiv), TPB, 0, div_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest, b.dc,
@@ -1427,94 +1731,144 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
     LAUNCH(b, "poisson_y", nblk(ncol), TPB, 0, poisson_y_kernel, b.rest2, b.rest, b.dc, nx, ny, nz1);
     k_cosftx(b, b.rest2, b.rest, 1);
   }
-  LAUNCH(b, "project", nblk(b.nu), TPB, 0, project_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W],
-         b.f[OTBL_P], b.rest, b.dc, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1);
+  LAUNCH(b, "project", nblk(b.nu), TPB, 0, project_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W],
+         b.f[OTBL_P], b.rest, b.dc, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1, (const double *)nullptr,
+         (double *)nullptr, b.dc.idyy, (double *)nullptr);
+}
+
+void k_poisson_pivots(Box &b) {
+  const int nzh = b.nz1 / 2;
+  LAUNCH(b, "poisson_pivots", nblk((size_t)nzh * (b.nx - 1)), TPB, 0, poisson_pivots_kernel, b.pdl, b.pdu, b.dc,
+         b.nx, b.ny, nzh);
 }
 
 // ---------------------------------------------------------------------------
 // rhsp (rhsp.F90:22-489)
 // ---------------------------------------------------------------------------
-void k_substep_rhsp(Box &b, int m) {
+// The substep in the element type T of the fields: double (OTBL_FP64, with
+// the one-thread-per-line fallbacks) or float (OTBL_MIXED, fast paths only;
+// statistics, CFL maxima and the recycling profiles stay FP64).
+template <typename T> static void rhsp_t(Box &b, int m) {
+  constexpr bool F64 = std::is_same<T, double>::value;
   const int nx = b.nx, ny = b.ny, nz1 = b.nz1;
   const int ru = ny + 1;
-  double *u = b.f[OTBL_U], *v = b.f[OTBL_V], *w = b.f[OTBL_W], *p = b.f[OTBL_P];
+  T *u = fld<T>(b.f[OTBL_U]), *v = fld<T>(b.f[OTBL_V]), *w = fld<T>(b.f[OTBL_W]), *p = fld<T>(b.f[OTBL_P]);
+  T *resu = fld<T>(b.resu), *resv = fld<T>(b.resv), *resw = fld<T>(b.resw);
+  T *rhsu = fld<T>(b.rhsu), *rhsv = fld<T>(b.rhsv), *rhsw = fld<T>(b.rhsw);
+  T *wki1 = fld<T>(b.wki1), *wki2 = fld<T>(b.wki2), *wki3 = fld<T>(b.wki3);
+  T *ya = fld<T>(b.ya), *yb = fld<T>(b.yb), *yc = fld<T>(b.yc);
+  const DevCoefT<T> &c = coefs<T>(b);
   const FFTPlan &plan = b.zplan;
+  const size_t shm = zfft_shm(b, 4);
 
   LAUNCH(b, "reset_max", 1, TPB, 0, reset_max_kernel, b.ds, b.vmbits, ny);
   // x interpolation (rhsp.F90:105-107)
-  k_interpxx(b, u, b.wki1, ru, 1);
-  k_interpxx(b, v, b.wki2, ny, 0);
-  k_interpxx(b, w, b.wki3, ru, 0);
+  if constexpr (F64) {
+    k_interpxx(b, u, wki1, ru, 1);
+    k_interpxx(b, v, wki2, ny, 0);
+    k_interpxx(b, w, wki3, ru, 0);
+  } else {
+    kx_interpxx<T>(b, u, wki1, c.inxu, c.cofiux, 2, 3, b.pu);
+    kx_interpxx<T>(b, v, wki2, c.inxv, c.cofivx, 1, 4, b.pv);
+    kx_interpxx<T>(b, w, wki3, c.inxv, c.cofivx, 1, 4, b.pu);
+  }
   if (m == 1)
-    LAUNCH(b, "energies", 3, 1024, 0, energies_kernel, u, v, w, b.dc.hy, b.ds, nx, ny, nz1, b.nz2);
+    LAUNCH(b, "energies", 3, 1024, 0, energies_kernel<T>, u, v, w, b.dc.hy, b.ds, nx, ny, nz1, b.nz2);
   // u interpolated to v rows in spectral space (interpyy is linear along y and
   // commutes with the z transform of rhsp.F90:122,146)
-  k_interpyy(b, u, b.yc, 0, 1, nx, b.pu, b.pu);
+  if constexpr (F64)
+    k_interpyy(b, u, yc, 0, 1, nx, b.pu, b.pu);
+  else
+    ky_interpyy<T>(b, u, yc, 0, 1, nx, b.pu, b.pu);
   // plane loop 1 products
-  const size_t shm = zfft_shm(b, 4);
   if (b.zfast)
-    kz_products1(b);
-  else
-    LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel, u, b.wki1, b.wki2, b.wki3, b.yc,
-           plan, ny, nz1, b.nz2, b.ds, b.vmbits);
+    kz_products1<T>(b);
+  else if constexpr (F64)
+    LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel, u, wki1, wki2, wki3, yc, plan, ny, nz1,
+           b.nz2, b.ds, b.vmbits);
   if (b.yL)
-    ky_rhsu_loop1(b, b.wki2, b.wki3, b.rhsu);
-  else
-    LAUNCH(b, "rhsu_loop1", nblk((size_t)nz1 * nx), TPB, 0, rhsu_loop1_kernel, b.wki2, b.wki3, b.rhsu, b.dc,
-           ny, nz1, nx);
+    ky_rhsu_loop1<T>(b, wki2, wki3, rhsu);
+  else if constexpr (F64)
+    LAUNCH(b, "rhsu_loop1", nblk((size_t)nz1 * nx), TPB, 0, rhsu_loop1_kernel, wki2, wki3, rhsu, b.dc, ny, nz1,
+           nx);
   // x derivatives (rhsp.F90:181-204)
-  k_difvisxx(b, b.wki1, u, b.rhsu, ru, 1, 1);
-  k_difvisxx(b, b.wki2, v, b.rhsv, ny, 0, 0);
-  k_difvisxx(b, b.wki3, w, b.rhsw, ru, 0, 0);
+  if constexpr (F64) {
+    k_difvisxx(b, wki1, u, rhsu, ru, 1, 1);
+    k_difvisxx(b, wki2, v, rhsv, ny, 0, 0);
+    k_difvisxx(b, wki3, w, rhsw, ru, 0, 0);
+  } else {
+    kx_difvisxx<T>(b, wki1, u, rhsu, c.dcxu, c.vixu, c.cofcxu, c.cofvxu, 1, 3, b.pu, 1);
+    kx_difvisxx<T>(b, wki2, v, rhsv, c.dcxv, c.vixv, c.cofcxv, c.cofvxv, 2, 1, b.pv, 0);
+    kx_difvisxx<T>(b, wki3, w, rhsw, c.dcxv, c.vixv, c.cofcxv, c.cofvxv, 2, 1, b.pu, 0);
+  }
   // time step and RK constants
-  LAUNCH(b, "dt", 1, 64, 0, dt_kernel, b.ds, b.vmbits, b.dc.dymin, ny, m,
-         b.dt_mode == OTBL_DT_EVERY_STEP ? 1 : 0, b.hc.dxmin, b.hc.dtret, b.hc.rex, b.hc.rkc[m],
-         b.hc.rkd[m], b.hc.rkcv[m], b.hc.rkdv[m]);
+  LAUNCH(b, "dt", 1, 64, 0, dt_kernel, b.ds, b.vmbits, b.dc.dymin, ny, m, b.dt_mode == OTBL_DT_EVERY_STEP ? 1 : 0,
+         b.hc.dxmin, b.hc.dtret, b.hc.rex, b.hc.rkc[m], b.hc.rkd[m], b.hc.rkcv[m], b.hc.rkdv[m]);
   // save velocities, pressure-gradient predictor
-  LAUNCH(b, "save_predict", nblk(b.nu), TPB, 0, save_predict_kernel, u, v, w, p, b.resu, b.resv, b.resw,
-         b.dc.idyy, b.dc.kaz, b.ds, b.hc.idxx, nx, ny, nz1);
+  double *um = F64 ? nullptr : b.um64; // the FP64 mean-flow u of the mixed substep
+  LAUNCH(b, "save_predict", nblk(b.nu), TPB, 0, save_predict_kernel<T>, u, v, w, p, resu, resv, resw, c.idyy, c.kaz,
+         b.ds, b.hc.idxx, nx, ny, nz1, F64 ? nullptr : (const double *)b.p0, b.dc.idyy, um, b.rm64);
   // recycling inflow (rhsp.F90:287)
   {
     int *ibuf = (int *)(b.gwork + (size_t)nz1 * (ny + 1));
     double *dbuf = b.gwork + (size_t)nz1 * (ny + 1) + 4 * (ny + 2);
-    LAUNCH(b, "genflu", 1, TPB, 4 * (size_t)(ny + 2) * sizeof(double), genflu_stats_kernel, b.resu, b.um, ibuf,
+    LAUNCH(b, "genflu", 1, TPB, 4 * (size_t)(ny + 2) * sizeof(double), genflu_stats_kernel<T>, resu, b.um, ibuf,
            dbuf, b.dc, b.ds, ny, nz1, b.xout, b.re);
-    LAUNCH(b, "genflu", dim3(nblk((size_t)nz1 * (ny + 1)), 3), TPB, 0, genflu_plane_kernel, u, v, w, b.resu,
-           b.resv, b.resw, b.u0, b.v0, ibuf, dbuf, b.dc, b.ds, ny, nz1, b.xout);
+    LAUNCH(b, "genflu", dim3(nblk((size_t)nz1 * (ny + 1)), 3), TPB, 0, genflu_plane_kernel<T>, u, v, w, resu, resv,
+           resw, b.u0, b.v0, ibuf, dbuf, b.dc, b.ds, ny, nz1, b.xout, um);
   }
   // plane loop 2 (planes 2..nx-1)
   if (nx > 2) {
-    k_interpyy(b, b.resw, b.ya, 0, 2, nx - 2, b.pu, b.pu);
-    k_interpyy(b, b.resv, b.yb, 1, 2, nx - 2, b.pv, b.pu);
+    if constexpr (F64) {
+      k_interpyy(b, resw, ya, 0, 2, nx - 2, b.pu, b.pu);
+      k_interpyy(b, resv, yb, 1, 2, nx - 2, b.pv, b.pu);
+    } else {
+      ky_interpyy<T>(b, resw, ya, 0, 2, nx - 2, b.pu, b.pu);
+      ky_interpyy<T>(b, resv, yb, 1, 2, nx - 2, b.pv, b.pu);
+    }
     if (b.zfast)
-      kz_products2(b);
-    else
-      LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, b.resv, b.resw, b.ya, b.yb, b.yc,
-             plan, ny, nz1, b.nz2);
+      kz_products2<T>(b);
+    else if constexpr (F64)
+      LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, resv, resw, ya, yb, yc, plan, ny, nz1,
+             b.nz2);
     if (b.yL)
-      ky_rhs_loop2(b, b.ya, b.yb, b.yc);
-    else
-      LAUNCH(b, "rhs_loop2", nblk((size_t)nz1 * (nx - 2)), TPB, 0, rhs_loop2_kernel, b.rhsu, b.rhsv, b.rhsw,
-             b.resu, b.resv, b.resw, b.ya, b.yb, b.yc, b.wki1, b.dc, ny, nz1, nx - 2);
+      ky_rhs_loop2<T>(b, ya, yb, yc);
+    else if constexpr (F64)
+      LAUNCH(b, "rhs_loop2", nblk((size_t)nz1 * (nx - 2)), TPB, 0, rhs_loop2_kernel, rhsu, rhsv, rhsw, resu, resv,
+             resw, ya, yb, yc, wki1, b.dc, ny, nz1, nx - 2);
   }
   // boundary conditions (rhsp.F90:421-445)
-  LAUNCH(b, "boun", nblk((size_t)nz1 * nx), TPB, 0, boun_kernel, u, v, w, b.rhsu, b.rhsv, b.rhsw, b.dc, ny,
-         nz1, nx);
+  LAUNCH(b, "boun", nblk((size_t)nz1 * nx), TPB, 0, boun_kernel<T>, u, v, w, rhsu, rhsv, rhsw, c, ny, nz1, nx, um,
+         b.dc);
   // implicit factors for this dt, then the final update per component
   launch_impl_factor(b, 0.0, 0);
+  T *rp[3] = {fld<T>(b.f[OTBL_RHSUPA]), fld<T>(b.f[OTBL_RHSVPA]), fld<T>(b.f[OTBL_RHSWPA])};
   if (b.yL) {
-    ky_rk_final(b, v, b.resv, b.rhsv, b.f[OTBL_RHSVPA], ny);
-    ky_rk_final(b, u, b.resu, b.rhsu, b.f[OTBL_RHSUPA], ny + 1);
-    ky_rk_final(b, w, b.resw, b.rhsw, b.f[OTBL_RHSWPA], ny + 1);
-    return;
+    ky_rk_final<T>(b, v, resv, rhsv, rp[1], ny);
+    if (um) // the FP64 mean-flow u first: it reads the previous rhs
+      LAUNCH(b, "rk_mean", nblk(nx, 64), 64, 0, mean_rk_final_kernel<T>, um, (const double *)b.rm64, rhsu, rp[0],
+             b.tm64, b.dc, b.implu, b.ds, ny, nz1, nx);
+    ky_rk_final<T>(b, u, resu, rhsu, rp[0], ny + 1);
+    if (um)
+      LAUNCH(b, "rk_mean", nblk((size_t)nx * (ny + 1)), TPB, 0, mean_to_field_kernel<T>, u, (const double *)um, nx,
+             ny + 1, nz1);
+    ky_rk_final<T>(b, w, resw, rhsw, rp[2], ny + 1);
+  } else if constexpr (F64) {
+    size_t ncol = (size_t)nz1 * nx;
+    LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, v, resv, rhsv, rp[1], wki2, b.dc, b.implv, b.ds, ny,
+           ny, nz1, nx);
+    LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, u, resu, rhsu, rp[0], wki1, b.dc, b.implu, b.ds,
+           ny + 1, ny, nz1, nx);
+    LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, w, resw, rhsw, rp[2], wki3, b.dc, b.implu, b.ds,
+           ny + 1, ny, nz1, nx);
   }
-  size_t ncol = (size_t)nz1 * nx;
-  LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, v, b.resv, b.rhsv, b.f[OTBL_RHSVPA], b.wki2, b.dc,
-         b.implv, b.ds, ny, ny, nz1, nx);
-  LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, u, b.resu, b.rhsu, b.f[OTBL_RHSUPA], b.wki1, b.dc,
-         b.implu, b.ds, ny + 1, ny, nz1, nx);
-  LAUNCH(b, "rk_final", nblk(ncol), TPB, 0, rk_final_kernel, w, b.resw, b.rhsw, b.f[OTBL_RHSWPA], b.wki3, b.dc,
-         b.implu, b.ds, ny + 1, ny, nz1, nx);
+}
+
+void k_substep_rhsp(Box &b, int m) {
+  if (b.precision == OTBL_MIXED)
+    rhsp_t<float>(b, m);
+  else
+    rhsp_t<double>(b, m);
 }
 
 void k_setup_attrs(Box &b) {
@@ -1530,6 +1884,24 @@ void k_setup_attrs(Box &b) {
 
 size_t k_dct_shm_bytes(int M) { return (TW_SLOTS + 2 * (size_t)M) * sizeof(double2); }
 size_t k_zfft_shm_bytes(int nz, int nbuf) { return (TW_SLOTS + (size_t)nbuf * nz) * sizeof(double2); }
+
+__global__ void d2f_kernel(const double *a, float *o, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    o[e] = (float)a[e];
+}
+
+__global__ void f2d_kernel(const float *a, double *o, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    o[e] = (double)a[e];
+}
+
+static unsigned cgrid(size_t n) { return (unsigned)std::min<size_t>(nblk(n), 8192); }
+void k_d2f(Box &b, const double *a, float *o, size_t n) {
+  if (n) LAUNCH(b, "d2f", cgrid(n), TPB, 0, d2f_kernel, a, o, n);
+}
+void k_f2d(Box &b, const float *a, double *o, size_t n) {
+  if (n) LAUNCH(b, "f2d", cgrid(n), TPB, 0, f2d_kernel, a, o, n);
+}
 
 __global__ void end_step_kernel(DevScalars *ds) {
   if (threadIdx.x == 0) ds->tiempo += ds->dt;

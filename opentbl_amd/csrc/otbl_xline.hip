@@ -16,19 +16,21 @@ namespace otbl {
 
 static constexpr int XB = 8, XK = 4; // block of planes, ring length in blocks
 static_assert((XK - 1) * XB == XLINE_WINDOW, "window length");
+static_assert(XK * XB == XLINE_EXACT, "lines of at most XK blocks take the exact sweep alone");
 
 // interpxx (lip.f90:248-300): midpoint interpolation along x
+template <typename T>
 __global__ void __launch_bounds__(TPB)
-interpxx_w(const double *__restrict__ gen, double *__restrict__ geni, const double *__restrict__ icon,
-           const double *__restrict__ econ, const double *__restrict__ inbx, int chi, int chi2, int nx, size_t P) {
+interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict__ icon,
+           const T *__restrict__ econ, const T *__restrict__ inbx, int chi, int chi2, int nx, size_t P) {
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= P) return;
-  const double *g = gen + l;
-  double *o = geni + l;
+  const T *g = gen + l;
+  T *o = geni + l;
   const int nb = (nx + XB - 1) / XB;
-  double ring[XK][XB];
-  double y = 0.0;
-  double gi = g[P]; // g(i) of the interior plane i being processed, from i = 2
+  T ring[XK][XB];
+  T y = 0;
+  T gi = g[P]; // g(i) of the interior plane i being processed, from i = 2
   for (int it = 0; it < nb; it += XK) {
 #pragma unroll
     for (int kk = 0; kk < XK; ++kk) {
@@ -39,13 +41,13 @@ interpxx_w(const double *__restrict__ gen, double *__restrict__ geni, const doub
       for (int b = 0; b < XB; ++b) {
         const int i = qq * XB + b + 1;
         if (i <= nx) {
-          double rhs;
+          T rhs;
           if (i == 1) {
             rhs = M4(inbx, chi2, 1) * g[0] + M4(inbx, chi2, 2) * g[P] + M4(inbx, chi2, 3) * g[2 * P] +
                   M4(inbx, chi2, 4) * g[3 * P];
             y = rhs;
           } else if (i < nx) {
-            const double gn = g[(size_t)i * P]; // g(i+1)
+            const T gn = g[(size_t)i * P]; // g(i+1)
             rhs = C2(econ, 1, i) * gi + C2(econ, 2, i) * gn;
             gi = gn;
             y = rhs + C3(icon, 1, i) * y;
@@ -60,7 +62,7 @@ interpxx_w(const double *__restrict__ gen, double *__restrict__ geni, const doub
       if (qq < nb - 1) {
         if (qq >= XK - 1) {
           // windowed backward sweep: blocks qq .. qq-XK+1, emit the oldest
-          double x = 0.0;
+          T x = 0;
 #pragma unroll
           for (int d = 0; d < XK; ++d) {
             const int slot = (kk - d + XK) % XK;
@@ -75,7 +77,7 @@ interpxx_w(const double *__restrict__ gen, double *__restrict__ geni, const doub
       } else {
         // last block: exact backward sweep from plane nx over the blocks
         // not emitted yet
-        double x = 0.0;
+        T x = 0;
         bool started = false;
 #pragma unroll
         for (int d = 0; d < XK; ++d) {
@@ -85,7 +87,7 @@ interpxx_w(const double *__restrict__ gen, double *__restrict__ geni, const doub
             for (int b = XB - 1; b >= 0; --b) {
               const int i = (qq - d) * XB + b + 1;
               if (i <= nx) {
-                const double yy = ring[slot][b];
+                const T yy = ring[slot][b];
                 x = started ? (yy - C3(icon, 3, i) * x) * C3(icon, 2, i) : yy * C3(icon, 2, nx);
                 started = true;
                 o[(size_t)(i - 1) * P] = x;
@@ -101,20 +103,22 @@ interpxx_w(const double *__restrict__ gen, double *__restrict__ geni, const doub
 // difvisxx (lip.f90:52-142): convective d/dx of the product plus rex times
 // the compact d2/dx2 of the velocity; mode 1 adds to out except on plane nx,
 // which gets the outflow derivative U_inf du/dx (lip.f90:105,132).
+template <typename T>
 __global__ void __launch_bounds__(TPB)
-difvisxx_w(const double *__restrict__ gp, const double *__restrict__ q, double *out,
-           const double *__restrict__ icon, const double *__restrict__ iconv, const double *__restrict__ econ,
-           const double *__restrict__ econv, const double *__restrict__ dcbx, const double *__restrict__ cofvbx,
-           int chv, int ic, int nx, size_t P, double rex, double idx, int mode) {
+difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
+           const T *__restrict__ icon, const T *__restrict__ iconv, const T *__restrict__ econ,
+           const T *__restrict__ econv, const T *__restrict__ dcbx, const T *__restrict__ cofvbx,
+           int chv, int ic, int nx, size_t P, double rex_d, double idx_d, int mode) {
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= P) return;
-  const double *g = gp + l, *v = q + l;
-  double *o = out + l;
+  const T *g = gp + l, *v = q + l;
+  T *o = out + l;
   const int chv1 = chv + 2;
+  const T rex = (T)rex_d, idx = (T)idx_d;
   const int nb = (nx + XB - 1) / XB;
-  double rc_[XK][XB], rv_[XK][XB];
-  double yc = 0.0, yv = 0.0;
-  double gim1 = g[0], qim1 = v[0], qi = v[P];
+  T rc_[XK][XB], rv_[XK][XB];
+  T yc = 0, yv = 0;
+  T gim1 = g[0], qim1 = v[0], qi = v[P];
   for (int it = 0; it < nb; it += XK) {
 #pragma unroll
     for (int kk = 0; kk < XK; ++kk) {
@@ -130,22 +134,22 @@ difvisxx_w(const double *__restrict__ gp, const double *__restrict__ q, double *
             yv = M4(cofvbx, ic, 1) * v[0] + M4(cofvbx, ic, 2) * v[P] + M4(cofvbx, ic, 3) * v[2 * P] +
                  M4(cofvbx, ic, 4) * v[3 * P];
           } else if (i < nx) {
-            const double gi = g[(size_t)(i - 1) * P];
-            const double qn = v[(size_t)i * P];
-            const double rc = C2(econ, 1, i) * gim1 + C2(econ, 2, i) * gi;
-            const double rv = C3(econv, 1, i) * qn + C3(econv, 2, i) * qi + C3(econv, 3, i) * qim1;
+            const T gi = g[(size_t)(i - 1) * P];
+            const T qn = v[(size_t)i * P];
+            const T rc = C2(econ, 1, i) * gim1 + C2(econ, 2, i) * gi;
+            const T rv = C3(econv, 1, i) * qn + C3(econv, 2, i) * qi + C3(econv, 3, i) * qim1;
             yc = rc + C3(icon, 1, i) * yc;
             yv = rv + C3(iconv, 1, i) * yv;
             gim1 = gi;
             qim1 = qi;
             qi = qn;
           } else {
-            const double gn = g[(size_t)(nx - 1) * P], gn1 = g[(size_t)(nx - 2) * P],
+            const T gn = g[(size_t)(nx - 1) * P], gn1 = g[(size_t)(nx - 2) * P],
                          gn2 = g[(size_t)(nx - 3) * P], gn3 = g[(size_t)(nx - 4) * P];
-            const double qn2 = v[(size_t)(nx - 3) * P], qn3 = v[(size_t)(nx - 4) * P];
-            const double rc = M5(dcbx, chv1, 1) * gn + M5(dcbx, chv1, 2) * gn1 + M5(dcbx, chv1, 3) * gn2 +
+            const T qn2 = v[(size_t)(nx - 3) * P], qn3 = v[(size_t)(nx - 4) * P];
+            const T rc = M5(dcbx, chv1, 1) * gn + M5(dcbx, chv1, 2) * gn1 + M5(dcbx, chv1, 3) * gn2 +
                               M5(dcbx, chv1, 4) * gn3;
-            const double rv = M4(cofvbx, ic + 1, 1) * qi + M4(cofvbx, ic + 1, 2) * qim1 +
+            const T rv = M4(cofvbx, ic + 1, 1) * qi + M4(cofvbx, ic + 1, 2) * qim1 +
                               M4(cofvbx, ic + 1, 3) * qn2 + M4(cofvbx, ic + 1, 4) * qn3;
             yc = rc + C3(icon, 1, nx) * yc;
             yv = rv + C3(iconv, 1, nx) * yv;
@@ -156,7 +160,7 @@ difvisxx_w(const double *__restrict__ gp, const double *__restrict__ q, double *
       }
       if (qq < nb - 1) {
         if (qq >= XK - 1) {
-          double xc = 0.0, xv = 0.0;
+          T xc = 0, xv = 0;
 #pragma unroll
           for (int d = 0; d < XK; ++d) {
             const int slot = (kk - d + XK) % XK;
@@ -166,7 +170,7 @@ difvisxx_w(const double *__restrict__ gp, const double *__restrict__ q, double *
               xc = (rc_[slot][b] - C3(icon, 3, i) * xc) * C3(icon, 2, i);
               xv = (rv_[slot][b] - C3(iconv, 3, i) * xv) * C3(iconv, 2, i);
               if (d == XK - 1) {
-                double r = xc + rex * xv;
+                T r = xc + rex * xv;
                 if (mode == 1) r = o[(size_t)(i - 1) * P] + r;
                 o[(size_t)(i - 1) * P] = r;
               }
@@ -174,7 +178,7 @@ difvisxx_w(const double *__restrict__ gp, const double *__restrict__ q, double *
           }
         }
       } else {
-        double xc = 0.0, xv = 0.0;
+        T xc = 0, xv = 0;
         bool started = false;
 #pragma unroll
         for (int d = 0; d < XK; ++d) {
@@ -189,12 +193,12 @@ difvisxx_w(const double *__restrict__ gp, const double *__restrict__ q, double *
                   xc = rc_[slot][b] * C3(icon, 2, nx);
                   xv = rv_[slot][b] * C3(iconv, 2, nx);
                   // outflow row: the velocity difference qi - qim1 = q(nx) - q(nx-1)
-                  o[oi] = 1.0 * (qi - qim1) * idx + rex * 0.0;
+                  o[oi] = T(1) * (qi - qim1) * idx + rex * T(0);
                   started = true;
                 } else {
                   xc = (rc_[slot][b] - C3(icon, 3, i) * xc) * C3(icon, 2, i);
                   xv = (rv_[slot][b] - C3(iconv, 3, i) * xv) * C3(iconv, 2, i);
-                  double r = xc + rex * xv;
+                  T r = xc + rex * xv;
                   if (mode == 1) r = o[oi] + r;
                   o[oi] = r;
                 }
@@ -207,16 +211,25 @@ difvisxx_w(const double *__restrict__ gp, const double *__restrict__ q, double *
   }
 }
 
-void kx_interpxx(Box &b, const double *gen, double *geni, const double *icon, const double *econ, int chi,
-                 int chi2, size_t P) {
-  LAUNCH(b, "interpxx", nblk(P), TPB, 0, interpxx_w, gen, geni, icon, econ, b.dc.inbx, chi, chi2, b.nx, P);
+template <typename T>
+void kx_interpxx(Box &b, const T *gen, T *geni, const T *icon, const T *econ, int chi, int chi2, size_t P) {
+  auto kf = &interpxx_w<T>;
+  LAUNCH(b, "interpxx", nblk(P), TPB, 0, kf, gen, geni, icon, econ, coefs<T>(b).inbx, chi, chi2, b.nx, P);
 }
 
-void kx_difvisxx(Box &b, const double *prod, const double *vel, double *out, const double *icon,
-                 const double *iconv, const double *econ, const double *econv, int chv, int ic, size_t P,
-                 int mode) {
-  LAUNCH(b, "difvisxx", nblk(P), TPB, 0, difvisxx_w, prod, vel, out, icon, iconv, econ, econv, b.dc.dcbx,
-         b.dc.cofvbx, chv, ic, b.nx, P, b.hc.rex, b.hc.idx, mode);
+template <typename T>
+void kx_difvisxx(Box &b, const T *prod, const T *vel, T *out, const T *icon, const T *iconv, const T *econ,
+                 const T *econv, int chv, int ic, size_t P, int mode) {
+  auto kf = &difvisxx_w<T>;
+  LAUNCH(b, "difvisxx", nblk(P), TPB, 0, kf, prod, vel, out, icon, iconv, econ, econv, coefs<T>(b).dcbx,
+         coefs<T>(b).cofvbx, chv, ic, b.nx, P, b.hc.rex, b.hc.idx, mode);
 }
+
+template void kx_interpxx<double>(Box &, const double *, double *, const double *, const double *, int, int, size_t);
+template void kx_interpxx<float>(Box &, const float *, float *, const float *, const float *, int, int, size_t);
+template void kx_difvisxx<double>(Box &, const double *, const double *, double *, const double *, const double *,
+                                  const double *, const double *, int, int, size_t, int);
+template void kx_difvisxx<float>(Box &, const float *, const float *, float *, const float *, const float *,
+                                 const float *, const float *, int, int, size_t, int);
 
 } // namespace otbl

@@ -151,13 +151,15 @@ __device__ inline void ytab_impl(double *tab, const double *f, const double *ico
 
 // launchers (otbl_ycol.hip); valid when Box::yL > 0
 void k_ytab_fixed(Box &b);
-void ky_interpyy(Box &b, const double *gen, double *geni, int loguv, int i0, int nplanes, size_t ips,
-                 size_t ops);
-void ky_differzy(Box &b, const double *geni, double *res, int which, int nplanes, size_t ips, size_t ops);
-void ky_vistyy(Box &b, const double *u, double *res, int R, int nplanes);
-void ky_implzy(Box &b, double *u, const double *res, int R, int nplanes);
-void ky_rhsu_loop1(Box &b, const double *uv, const double *uw, double *rhsu);
-void ky_rhs_loop2(Box &b, const double *vw, const double *vv, const double *ww);
-void ky_rk_final(Box &b, double *u, const double *res, const double *rhs, double *rhsp, int jee);
+template <typename T>
+void ky_interpyy(Box &b, const T *gen, T *geni, int loguv, int i0, int nplanes, size_t ips, size_t ops);
+template <typename T> void ky_differzy(Box &b, const T *geni, T *res, int which, int nplanes, size_t ips, size_t ops);
+template <typename T> void ky_vistyy(Box &b, const T *u, T *res, int R, int nplanes);
+// the implicit factors (implu / implv) stay FP64; the table of the
+// factored system comes in the precision of the fields
+template <typename T> void ky_implzy(Box &b, T *u, const T *res, int R, int nplanes);
+template <typename T> void ky_rhsu_loop1(Box &b, const T *uv, const T *uw, T *rhsu);
+template <typename T> void ky_rhs_loop2(Box &b, const T *vw, const T *vv, const T *ww);
+template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, T *rhsp, int jee);
 
 } // namespace otbl

@@ -98,16 +98,16 @@ __device__ __forceinline__ size_t yopaque(size_t x) {
 static constexpr int YGRP = 4;
 
 // Copy one system's table into LDS (all threads); the caller synchronises.
-template <int L>
-__device__ __forceinline__ void ytab_stage(double *dst, const double *__restrict__ src) {
+template <int L, typename T>
+__device__ __forceinline__ void ytab_stage(T *dst, const T *__restrict__ src) {
   for (int k = threadIdx.x; k < (int)ytab_size(L); k += YTPB) dst[k] = src[k];
 }
 
 // sum_k cf(row, k) * G(j1 + dir*(k-1)), k = 1..4, over 1-based rows of col:
 // the one-sided closures of the compact y operators
-__device__ __forceinline__ double bnd4(const double *cf, int ld, int row, const double *col, size_t s,
-                                       int j1, int dir) {
-  double r = 0.0;
+template <typename T>
+__device__ __forceinline__ T bnd4(const T *cf, int ld, int row, const T *col, size_t s, int j1, int dir) {
+  T r = 0;
 #pragma unroll
   for (int k = 1; k <= 4; ++k) r += cf[ld * (k - 1) + row - 1] * col[(size_t)(j1 + dir * (k - 1) - 1) * s];
   return r;
@@ -119,20 +119,18 @@ __device__ __forceinline__ double bnd4(const double *cf, int ld, int row, const 
 // and its last row (n-1) `top`; add1 and addm are added to rows 1 and n-2
 // (the boundary columns of implzy).  Rows past n have a zero stencil.
 // Returns y at the chunk's last row (its forward carry-out).
-template <int L, class Fetch>
-__device__ __forceinline__ double yfwd(double (&v)[L], Fetch fetch, const double *tab, int p, int n, double bot,
-                                       double top, double add1, double addm) {
+template <int L, typename T, class Fetch>
+__device__ __forceinline__ T yfwd(T (&v)[L], Fetch fetch, const T *tab, int p, int n, T bot, T top, T add1, T addm) {
   constexpr int NL = YNCH * L;
-  const double *M = tab + p * L, *E1 = tab + 5 * NL + p * L, *E2 = tab + 6 * NL + p * L,
-               *E3 = tab + 7 * NL + p * L;
+  const T *M = tab + p * L, *E1 = tab + 5 * NL + p * L, *E2 = tab + 6 * NL + p * L, *E3 = tab + 7 * NL + p * L;
   const int j0 = p * L;
-  double gm = fetch(0), g0 = fetch(1), y = 0.0;
+  T gm = fetch(0), g0 = fetch(1), y = 0;
 #pragma unroll
   for (int r = 0; r < L; ++r) {
     if (r % YGRP == 0) YFENCE();
     const int j = j0 + r;
-    const double gp = fetch(r + 2);
-    double x = E1[r] * gm + E2[r] * g0 + E3[r] * gp;
+    const T gp = fetch(r + 2);
+    T x = E1[r] * gm + E2[r] * g0 + E3[r] * gp;
     x = j == 0 ? bot : x;
     x = j == n - 1 ? top : x;
     x = j == 1 ? x + add1 : x;
@@ -149,13 +147,12 @@ __device__ __forceinline__ double yfwd(double (&v)[L], Fetch fetch, const double
 // Backward local sweep, carry exchange between the chunks of each column
 // and the spike correction: on return v holds the solution of the chunk.
 // xch: LDS, 2*YNCH*64 doubles.
-template <int L>
-__device__ __forceinline__ void ybwd(double (&v)[L], double yend, const double *tab, int p, int lane,
-                                     double *xch) {
+template <int L, typename T>
+__device__ __forceinline__ void ybwd(T (&v)[L], T yend, const T *tab, int p, int lane, T *xch) {
   constexpr int NL = YNCH * L;
-  const double *A2 = tab + NL + p * L, *B = tab + 2 * NL + p * L;
-  const double *G = tab + 3 * NL + p * L, *S = tab + 4 * NL + p * L;
-  const double *PE = tab + 8 * NL, *G0 = PE + YNCH, *S0 = G0 + YNCH;
+  const T *A2 = tab + NL + p * L, *B = tab + 2 * NL + p * L;
+  const T *G = tab + 3 * NL + p * L, *S = tab + 4 * NL + p * L;
+  const T *PE = tab + 8 * NL, *G0 = PE + YNCH, *S0 = G0 + YNCH;
   v[L - 1] = A2[L - 1] * v[L - 1];
 #pragma unroll
   for (int r = L - 2; r >= 0; --r) {
@@ -170,9 +167,9 @@ __device__ __forceinline__ void ybwd(double (&v)[L], double yend, const double *
   // forward carry C_p = y(row above chunk p): C_{q+1} = yend_q + PE_q C_q.
   // backward carry D_p = x(row below chunk p) = sum over q > p of
   // (xstart_q + G0_q C_q) * S0_{p+1} ... S0_{q-1}, accumulated forwards.
-  double C = 0.0;
+  T C = 0;
   for (int q = 0; q < p; ++q) C = xch[q * 64 + lane] + PE[q] * C;
-  double Cq = C, D = 0.0, prod = 1.0;
+  T Cq = C, D = 0, prod = 1;
   for (int q = p + 1; q < YNCH; ++q) {
     Cq = xch[(q - 1) * 64 + lane] + PE[q - 1] * Cq;
     D += prod * (xch[(YNCH + q) * 64 + lane] + G0[q] * Cq);
@@ -187,26 +184,27 @@ __device__ __forceinline__ void ybwd(double (&v)[L], double yend, const double *
 }
 
 // solve for a chunk whose input rows are read from a global column
-template <int L>
-__device__ __forceinline__ void ysolve_col(double (&v)[L], const double *col, size_t s, int nin, const double *tab,
-                                           int p, int lane, int n, double bot, double top, double *xch) {
+template <int L, typename T>
+__device__ __forceinline__ void ysolve_col(T (&v)[L], const T *col, size_t s, int nin, const T *tab, int p, int lane,
+                                           int n, T bot, T top, T *xch) {
   const int j0 = p * L;
-  const double yend = yfwd<L>(
-      v, [&](int k) { return col[(size_t)yclamp(j0 - 1 + k, nin) * s]; }, tab, p, n, bot, top, 0.0, 0.0);
+  const T yend = yfwd<L>(
+      v, [&](int k) { return col[(size_t)yclamp(j0 - 1 + k, nin) * s]; }, tab, p, n, bot, top, T(0), T(0));
   ybwd<L>(v, yend, tab, p, lane, xch);
 }
 
 // one-sided closures of the operators (lip.f90), evaluated by the wave that
 // holds the row they belong to
-struct YBnd {
-  double bot, top;
+template <typename T> struct YBnd {
+  T bot, top;
 };
 
 // interpyy (lip.f90:359-440): loguv 0 u rows -> v rows, 1 v rows -> u rows;
 // n = ny
-__device__ __forceinline__ YBnd interpyy_bnd(int loguv, int j0, int L, int ny, const double *inby,
-                                             const double *g, size_t s) {
-  YBnd b{0.0, 0.0};
+template <typename T>
+__device__ __forceinline__ YBnd<T> interpyy_bnd(int loguv, int j0, int L, int ny, const T *inby, const T *g,
+                                                size_t s) {
+  YBnd<T> b{0, 0};
   if (j0 == 0 && loguv != 0) b.bot = bnd4(inby, 4, 3, g, s, 1, 1);
   if (ny - 1 >= j0 && ny - 1 < j0 + L)
     b.top = loguv == 0 ? bnd4(inby, 4, 1, g, s, ny + 1, -1) : bnd4(inby, 4, 4, g, s, ny, -1);
@@ -215,9 +213,10 @@ __device__ __forceinline__ YBnd interpyy_bnd(int loguv, int j0, int L, int ny, c
 
 // differzy (lip.f90:152-232): which 0 v rows -> u rows (n = ny+1), which 1
 // u rows -> v rows (n = ny)
-__device__ __forceinline__ YBnd differzy_bnd(int which, int j0, int L, int ny, const double *dcby,
-                                             const double *g, size_t s) {
-  YBnd b{0.0, 0.0};
+template <typename T>
+__device__ __forceinline__ YBnd<T> differzy_bnd(int which, int j0, int L, int ny, const T *dcby, const T *g,
+                                                size_t s) {
+  YBnd<T> b{0, 0};
   const int n = which == 0 ? ny + 1 : ny;
   if (j0 == 0) b.bot = bnd4(dcby, 4, which == 0 ? 3 : 1, g, s, 1, 1);
   if (n - 1 >= j0 && n - 1 < j0 + L) b.top = bnd4(dcby, 4, which == 0 ? 4 : 2, g, s, ny, -1);
@@ -225,9 +224,10 @@ __device__ __forceinline__ YBnd differzy_bnd(int which, int j0, int L, int ny, c
 }
 
 // vistyy (lip.f90:447-513), n = jee
-__device__ __forceinline__ YBnd vistyy_bnd(int jee, int j0, int L, int ny, const double *cofvby, const double *u,
-                                           size_t s) {
-  YBnd b{0.0, 0.0};
+template <typename T>
+__device__ __forceinline__ YBnd<T> vistyy_bnd(int jee, int j0, int L, int ny, const T *cofvby, const T *u,
+                                              size_t s) {
+  YBnd<T> b{0, 0};
   const int ic = (jee == ny) ? 1 : 3;
   if (j0 == 0) b.bot = bnd4(cofvby, 4, ic, u, s, 1, 1);
   if (jee - 1 >= j0 && jee - 1 < j0 + L) b.top = bnd4(cofvby, 4, ic + 1, u, s, jee, -1);
@@ -237,43 +237,43 @@ __device__ __forceinline__ YBnd vistyy_bnd(int jee, int j0, int L, int ny, const
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, typename T>
 __global__ void __launch_bounds__(YTPB)
-interpyy_y(const double *__restrict__ gen, double *__restrict__ geni, const double *__restrict__ tab,
-           const double *__restrict__ inby, int loguv, int ny, int nz1, int i0, int nplanes, size_t ips,
+interpyy_y(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict__ tab,
+           const T *__restrict__ inby, int loguv, int ny, int nz1, int i0, int nplanes, size_t ips,
            size_t ops) {
-  __shared__ double xch[2 * YNCH * 64];
-  __shared__ double tl[ytab_size(L)];
+  __shared__ T xch[2 * YNCH * 64];
+  __shared__ T tl[ytab_size(L)];
   ytab_stage<L>(tl, tab);
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, (size_t)(i0 - 1));
-  const double *g = gen + q.ip * ips + q.kr;
-  double *o = geni + q.ip * ops + q.kr;
+  const T *g = gen + q.ip * ips + q.kr;
+  T *o = geni + q.ip * ops + q.kr;
   const size_t s = nz1;
   const int j0 = q.p * L;
-  const YBnd bd = interpyy_bnd(loguv, j0, L, ny, inby, g, s);
+  const YBnd<T> bd = interpyy_bnd(loguv, j0, L, ny, inby, g, s);
   __syncthreads(); // table staged
-  double v[L];
+  T v[L];
   ysolve_col<L>(v, g, s, loguv == 0 ? ny + 1 : ny, tl, q.p, q.lane, ny, bd.bot, bd.top, xch);
 #pragma unroll
   for (int r = 0; r < L; ++r)
     if (q.ok && j0 + r < ny) o[(size_t)(j0 + r) * s] = v[r];
 }
 
-template <int L>
+template <int L, typename T>
 __global__ void __launch_bounds__(YTPB)
-differzy_y(const double *__restrict__ gen, double *__restrict__ res, const double *__restrict__ tab, DevCoef c,
+differzy_y(const T *__restrict__ gen, T *__restrict__ res, const T *__restrict__ tab, DevCoefT<T> c,
            int which, int ny, int nz1, int nplanes, size_t ips, size_t ops) {
-  __shared__ double xch[2 * YNCH * 64];
-  __shared__ double tl[ytab_size(L)];
+  __shared__ T xch[2 * YNCH * 64];
+  __shared__ T tl[ytab_size(L)];
   ytab_stage<L>(tl, tab);
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 0);
-  const double *g = gen + q.ip * ips + q.kr;
-  double *o = res + q.ip * ops + q.kr;
+  const T *g = gen + q.ip * ips + q.kr;
+  T *o = res + q.ip * ops + q.kr;
   const size_t s = nz1;
   const int j0 = q.p * L, n = which == 0 ? ny + 1 : ny;
-  const YBnd bd = differzy_bnd(which, j0, L, ny, c.dcby, g, s);
+  const YBnd<T> bd = differzy_bnd(which, j0, L, ny, c.dcby, g, s);
   __syncthreads();
-  double v[L];
+  T v[L];
   // differzy reads input rows 1..ny in both directions
   ysolve_col<L>(v, g, s, ny, tl, q.p, q.lane, n, bd.bot, bd.top, xch);
 #pragma unroll
@@ -281,19 +281,19 @@ differzy_y(const double *__restrict__ gen, double *__restrict__ res, const doubl
     if (q.ok && j0 + r < n) o[(size_t)(j0 + r) * s] = v[r];
 }
 
-template <int L>
+template <int L, typename T>
 __global__ void __launch_bounds__(YTPB)
-vistyy_y(const double *__restrict__ u, double *__restrict__ res, const double *__restrict__ tab, DevCoef c,
+vistyy_y(const T *__restrict__ u, T *__restrict__ res, const T *__restrict__ tab, DevCoefT<T> c,
          int jee, int ny, int nz1, int nplanes) {
-  __shared__ double xch[2 * YNCH * 64];
-  __shared__ double tl[ytab_size(L)];
+  __shared__ T xch[2 * YNCH * 64];
+  __shared__ T tl[ytab_size(L)];
   ytab_stage<L>(tl, tab);
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 0);
   const size_t s = nz1, off = q.ip * s * jee + q.kr;
   const int j0 = q.p * L;
-  const YBnd bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, u + off, s);
+  const YBnd<T> bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, u + off, s);
   __syncthreads();
-  double v[L];
+  T v[L];
   ysolve_col<L>(v, u + off, s, jee, tl, q.p, q.lane, jee, bd.bot, bd.top, xch);
 #pragma unroll
   for (int r = 0; r < L; ++r)
@@ -302,22 +302,22 @@ vistyy_y(const double *__restrict__ u, double *__restrict__ res, const double *_
 
 // implzy (lip.f90:528-597): rows 2..jee-1 of u from res; rows 1 and jee of
 // u are the boundary values already in place
-template <int L>
+template <int L, typename T>
 __global__ void __launch_bounds__(YTPB)
-implzy_y(double *u, const double *__restrict__ res, const double *__restrict__ tab, const double *__restrict__ f,
+implzy_y(T *u, const T *__restrict__ res, const T *__restrict__ tab, const double *__restrict__ f,
          int jee, int nz1, int nplanes) {
-  __shared__ double xch[2 * YNCH * 64];
-  __shared__ double tl[ytab_size(L)];
+  __shared__ T xch[2 * YNCH * 64];
+  __shared__ T tl[ytab_size(L)];
   ytab_stage<L>(tl, tab);
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 0);
   const size_t s = nz1, off = q.ip * s * jee + q.kr;
   const int j0 = q.p * L;
-  const double u1 = u[off], uj = u[off + (size_t)(jee - 1) * s];
-  const double a1b = f[3 * (jee + 1)], a3b = f[3 * (jee + 1) + 1];
+  const T u1 = u[off], uj = u[off + (size_t)(jee - 1) * s];
+  const T a1b = (T)f[3 * (jee + 1)], a3b = (T)f[3 * (jee + 1) + 1];
   __syncthreads();
-  double w[L];
-  const double *rc = res + off;
-  const double yend = yfwd<L>(
+  T w[L];
+  const T *rc = res + off;
+  const T yend = yfwd<L>(
       w, [&](int k) { return rc[(size_t)yclamp(j0 - 1 + k, jee) * s]; }, tl, q.p, jee, u1, uj, -a1b * u1,
       -a3b * uj);
   ybwd<L>(w, yend, tl, q.p, q.lane, xch);
@@ -329,59 +329,59 @@ implzy_y(double *u, const double *__restrict__ res, const double *__restrict__ t
 }
 
 // rhsu = d(uv)/dy + i kz (uw) on all planes (rhsp.F90:164-169)
-template <int L>
+template <int L, typename T>
 __global__ void __launch_bounds__(YTPB)
-rhsu_loop1_y(const double *__restrict__ uv, const double *__restrict__ uw, double *__restrict__ rhsu,
-             const double *__restrict__ tab, DevCoef c, int ny, int nz1, int nplanes) {
-  __shared__ double xch[2 * YNCH * 64];
-  __shared__ double tl[ytab_size(L)];
+rhsu_loop1_y(const T *__restrict__ uv, const T *__restrict__ uw, T *__restrict__ rhsu,
+             const T *__restrict__ tab, DevCoefT<T> c, int ny, int nz1, int nplanes) {
+  __shared__ T xch[2 * YNCH * 64];
+  __shared__ T tl[ytab_size(L)];
   ytab_stage<L>(tl, tab);
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 0);
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   const int j0 = q.p * L;
-  const double *g = uv + q.ip * pv + q.kr;
-  const YBnd bd = differzy_bnd(0, j0, L, ny, c.dcby, g, s);
+  const T *g = uv + q.ip * pv + q.kr;
+  const YBnd<T> bd = differzy_bnd(0, j0, L, ny, c.dcby, g, s);
   __syncthreads();
-  double v[L];
+  T v[L];
   ysolve_col<L>(v, g, s, ny, tl, q.p, q.lane, ny + 1, bd.bot, bd.top, xch);
-  const double kz = c.kaz[q.kr / 2];
-  const double sg = (q.kr & 1) ? 1.0 : -1.0;
-  const double *w = uw + q.ip * pu + (q.kr ^ 1);
-  double *o = rhsu + q.ip * pu + q.kr;
+  const T kz = c.kaz[q.kr / 2];
+  const T sg = (q.kr & 1) ? T(1) : T(-1);
+  const T *w = uw + q.ip * pu + (q.kr ^ 1);
+  T *o = rhsu + q.ip * pu + q.kr;
   const size_t s1 = yopaque(s);
 #pragma unroll
   for (int r = 0; r < L; ++r) {
     if (r % YGRP == 0) YFENCE();
     const int j = j0 + r;
     const size_t jc = (size_t)yclamp(j, ny + 1) * s1;
-    const double wp = w[jc];
+    const T wp = w[jc];
     if (q.ok && j <= ny) o[jc] = v[r] + sg * wp * kz;
   }
 }
 
 // z-viscous terms and the rest of rhsv/rhsw (rhsp.F90:355-415), planes
 // 2..nx-1
-template <int L>
+template <int L, typename T>
 __global__ void __launch_bounds__(YTPB)
-rhs_loop2_y(double *rhsu, double *rhsv, double *rhsw, const double *__restrict__ resu,
-            const double *__restrict__ resv, const double *__restrict__ resw, const double *__restrict__ vw,
-            const double *__restrict__ vv, const double *__restrict__ ww, const double *__restrict__ tabu,
-            const double *__restrict__ tabv, DevCoef c, int ny, int nz1, int nplanes) {
-  __shared__ double xch[2 * YNCH * 64];
-  __shared__ double tlu[ytab_size(L)];
-  __shared__ double tlv[ytab_size(L)];
+rhs_loop2_y(T *rhsu, T *rhsv, T *rhsw, const T *__restrict__ resu,
+            const T *__restrict__ resv, const T *__restrict__ resw, const T *__restrict__ vw,
+            const T *__restrict__ vv, const T *__restrict__ ww, const T *__restrict__ tabu,
+            const T *__restrict__ tabv, DevCoefT<T> c, int ny, int nz1, int nplanes) {
+  __shared__ T xch[2 * YNCH * 64];
+  __shared__ T tlu[ytab_size(L)];
+  __shared__ T tlv[ytab_size(L)];
   ytab_stage<L>(tlu, tabu);
   ytab_stage<L>(tlv, tabv);
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 1);
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   const int j0 = q.p * L;
-  const double kv = c.kvis[q.kr / 2], kz = c.kaz[q.kr / 2];
-  const double sg = (q.kr & 1) ? 1.0 : -1.0;
+  const T kv = c.kvis[q.kr / 2], kz = c.kaz[q.kr / 2];
+  const T sg = (q.kr & 1) ? T(1) : T(-1);
   const size_t ou = q.ip * pu + q.kr, ov = q.ip * pv + q.kr, op = q.ip * pu + (q.kr ^ 1);
-  const YBnd bv = differzy_bnd(1, j0, L, ny, c.dcby, vv + ou, s);
-  const YBnd bw = differzy_bnd(0, j0, L, ny, c.dcby, vw + ou, s);
+  const YBnd<T> bv = differzy_bnd(1, j0, L, ny, c.dcby, vv + ou, s);
+  const YBnd<T> bw = differzy_bnd(0, j0, L, ny, c.dcby, vw + ou, s);
   __syncthreads();
-  double d[L];
+  T d[L];
   // rhsv: -kv resv (rows 2..ny-1), + i kz vw (rows 1..ny), + d(vv)/dy
   ysolve_col<L>(d, vv + ou, s, ny, tlv, q.p, q.lane, ny, bv.bot, bv.top, xch);
   {
@@ -391,8 +391,8 @@ rhs_loop2_y(double *rhsu, double *rhsv, double *rhsw, const double *__restrict__
       if (r % YGRP == 0) YFENCE();
       const int j = j0 + r;
       const size_t jc = (size_t)yclamp(j, ny) * s1;
-      double x = rhsv[ov + jc];
-      const double sv = resv[ov + jc], pw = vw[op + jc];
+      T x = rhsv[ov + jc];
+      const T sv = resv[ov + jc], pw = vw[op + jc];
       if (j >= 1 && j <= ny - 2) x -= sv * kv;
       x += sg * pw * kz;
       if (q.ok && j < ny) rhsv[ov + jc] = x + d[r];
@@ -408,9 +408,9 @@ rhs_loop2_y(double *rhsu, double *rhsv, double *rhsw, const double *__restrict__
       if (r % YGRP == 0) YFENCE();
       const int j = j0 + r;
       const size_t jc = (size_t)yclamp(j, ny + 1) * s1;
-      double x = rhsw[ou + jc];
-      const double sw = resw[ou + jc];
-      const double pw = ww[op + (size_t)yclamp(j, ny) * s1]; // ww holds rows 1..ny
+      T x = rhsw[ou + jc];
+      const T sw = resw[ou + jc];
+      const T pw = ww[op + (size_t)yclamp(j, ny) * s1]; // ww holds rows 1..ny
       if (j >= 1 && j <= ny - 1) x -= sw * kv;
       x += d[r];
       if (j <= ny - 1) x += sg * pw * kz;
@@ -426,7 +426,7 @@ rhs_loop2_y(double *rhsu, double *rhsv, double *rhsw, const double *__restrict__
       if (r % YGRP == 0) YFENCE();
       const int j = j0 + r;
       const size_t jc = (size_t)yclamp(j, ny + 1) * s1;
-      const double x = rhsu[ou + jc], su = resu[ou + jc];
+      const T x = rhsu[ou + jc], su = resu[ou + jc];
       if (q.ok && j >= 1 && j <= ny - 1) rhsu[ou + jc] = x - su * kv;
     }
   }
@@ -435,29 +435,29 @@ rhs_loop2_y(double *rhsu, double *rhsv, double *rhsw, const double *__restrict__
 // Explicit RK3 combination with vistyy (rhsp.F90:448-475) and the implicit
 // y-viscous solve (rhsp.F90:479-484) for one component, planes 1..nx
 // (plane 1: rhs_prev = rhs only; plane nx: no viscous term).
-template <int L>
+template <int L, typename T>
 __global__ void __launch_bounds__(YTPB)
-rk_final_y(double *u, const double *__restrict__ res, const double *__restrict__ rhs, double *rhsp,
-           const double *__restrict__ tabv, const double *__restrict__ tabi, DevCoef c,
+rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, T *rhsp,
+           const T *__restrict__ tabv, const T *__restrict__ tabi, DevCoefT<T> c,
            const double *__restrict__ f, const DevScalars *__restrict__ ds, int jee, int ny, int nz1, int nx) {
-  __shared__ double xch[2 * YNCH * 64];
-  __shared__ double tlv[ytab_size(L)];
-  __shared__ double tli[ytab_size(L)];
-  __shared__ double hal[2 * YNCH * 64];
+  __shared__ T xch[2 * YNCH * 64];
+  __shared__ T tlv[ytab_size(L)];
+  __shared__ T tli[ytab_size(L)];
+  __shared__ T hal[2 * YNCH * 64];
   ytab_stage<L>(tlv, tabv);
   ytab_stage<L>(tli, tabi);
   const YCol q = ycol((size_t)nz1 * nx, nz1, 0);
   const size_t s = nz1, off = q.ip * s * jee + q.kr;
   const bool act = q.ok && q.ip > 0;
   const int j0 = q.p * L;
-  const YBnd bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, res + off, s);
-  const double u1 = u[off], uj = u[off + (size_t)(jee - 1) * s];
-  const double a1b = f[3 * (jee + 1)], a3b = f[3 * (jee + 1) + 1];
+  const YBnd<T> bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, res + off, s);
+  const T u1 = u[off], uj = u[off + (size_t)(jee - 1) * s];
+  const T a1b = (T)f[3 * (jee + 1)], a3b = (T)f[3 * (jee + 1) + 1];
   __syncthreads();
-  double t[L];
+  T t[L];
   ysolve_col<L>(t, res + off, s, jee, tlv, q.p, q.lane, jee, bd.bot, bd.top, xch);
-  const double var2 = (q.ip == (size_t)(nx - 1)) ? 0.0 : ds->var2;
-  const double var3 = ds->var3, var4 = ds->var4;
+  const T var2 = (q.ip == (size_t)(nx - 1)) ? T(0) : (T)ds->var2;
+  const T var3 = (T)ds->var3, var4 = (T)ds->var4;
   {
     const size_t s1 = yopaque(s);
 #pragma unroll
@@ -465,7 +465,7 @@ rk_final_y(double *u, const double *__restrict__ res, const double *__restrict__
       if (r % YGRP == 0) YFENCE();
       const int j = j0 + r;
       const size_t o = off + (size_t)yclamp(j, jee) * s1;
-      const double rh = rhs[o], rp = rhsp[o], uu = u[o];
+      const T rh = rhs[o], rp = rhsp[o], uu = u[o];
       if (act && j < jee) t[r] = uu - (var3 * rh + var2 * t[r] + var4 * rp);
       if (q.ok && j < jee) rhsp[o] = rh;
     }
@@ -475,10 +475,10 @@ rk_final_y(double *u, const double *__restrict__ res, const double *__restrict__
   hal[q.p * 64 + q.lane] = t[0];
   hal[(YNCH + q.p) * 64 + q.lane] = t[L - 1];
   __syncthreads();
-  const double tm = q.p > 0 ? hal[(YNCH + q.p - 1) * 64 + q.lane] : 0.0;
-  const double tp = q.p < YNCH - 1 ? hal[(q.p + 1) * 64 + q.lane] : 0.0;
+  const T tm = q.p > 0 ? hal[(YNCH + q.p - 1) * 64 + q.lane] : T(0);
+  const T tp = q.p < YNCH - 1 ? hal[(q.p + 1) * 64 + q.lane] : T(0);
   // implzy on t, in place (row r is overwritten after rows r+1 is read)
-  const double yend = yfwd<L>(
+  const T yend = yfwd<L>(
       t, [&](int k) { return k == 0 ? tm : (k == L + 1 ? tp : t[k - 1]); }, tli, q.p, jee, u1, uj, -a1b * u1,
       -a3b * uj);
   ybwd<L>(t, yend, tli, q.p, q.lane, xch);
@@ -493,65 +493,86 @@ rk_final_y(double *u, const double *__restrict__ res, const double *__restrict__
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-#define YDISPATCH(Lv, CALL)                                                    \
+#define YDISPATCH(Lv, ...)                                                     \
   switch (Lv) {                                                                \
-  case 4: { constexpr int L = 4; CALL; } break;                                \
-  case 8: { constexpr int L = 8; CALL; } break;                                \
-  case 12: { constexpr int L = 12; CALL; } break;                              \
-  case 16: { constexpr int L = 16; CALL; } break;                              \
-  case 20: { constexpr int L = 20; CALL; } break;                              \
-  case 24: { constexpr int L = 24; CALL; } break;                              \
+  case 4: { constexpr int L = 4; __VA_ARGS__; } break;                         \
+  case 8: { constexpr int L = 8; __VA_ARGS__; } break;                         \
+  case 12: { constexpr int L = 12; __VA_ARGS__; } break;                       \
+  case 16: { constexpr int L = 16; __VA_ARGS__; } break;                       \
+  case 20: { constexpr int L = 20; __VA_ARGS__; } break;                       \
+  case 24: { constexpr int L = 24; __VA_ARGS__; } break;                       \
   default: throw std::invalid_argument("y chunk length not instantiated");    \
   }
 
 static inline unsigned ygrid(size_t ncols) { return (unsigned)((ncols + 63) / 64); }
-static inline const double *ytab(const Box &b, int k) { return b.ytab + (size_t)k * ytab_size(b.yL); }
+template <typename T> static inline const T *ytab(const Box &b, int k);
+template <> inline const double *ytab<double>(const Box &b, int k) {
+  return b.ytab + (size_t)k * ytab_size(b.yL);
+}
+template <> inline const float *ytab<float>(const Box &b, int k) { return b.ytab_f + (size_t)k * ytab_size(b.yL); }
 
-void ky_interpyy(Box &b, const double *gen, double *geni, int loguv, int i0, int nplanes, size_t ips,
-                 size_t ops) {
+template <typename T>
+void ky_interpyy(Box &b, const T *gen, T *geni, int loguv, int i0, int nplanes, size_t ips, size_t ops) {
   const unsigned g = ygrid((size_t)b.nz1 * nplanes);
-  YDISPATCH(b.yL, LAUNCH(b, "interpyy", g, YTPB, 0, interpyy_y<L>, gen, geni,
-                         ytab(b, loguv == 0 ? YT_INYU : YT_INYV), b.dc.inby, loguv, b.ny, b.nz1, i0, nplanes,
-                         ips, ops));
+  YDISPATCH(b.yL, auto kf = &interpyy_y<L, T>;
+            LAUNCH(b, "interpyy", g, YTPB, 0, kf, gen, geni, ytab<T>(b, loguv == 0 ? YT_INYU : YT_INYV),
+                   coefs<T>(b).inby, loguv, b.ny, b.nz1, i0, nplanes, ips, ops));
 }
 
-void ky_differzy(Box &b, const double *geni, double *res, int which, int nplanes, size_t ips, size_t ops) {
+template <typename T> void ky_differzy(Box &b, const T *geni, T *res, int which, int nplanes, size_t ips, size_t ops) {
   const unsigned g = ygrid((size_t)b.nz1 * nplanes);
-  YDISPATCH(b.yL, LAUNCH(b, "differzy", g, YTPB, 0, differzy_y<L>, geni, res,
-                         ytab(b, which == 0 ? YT_DCYU : YT_DCYV), b.dc, which, b.ny, b.nz1, nplanes, ips, ops));
+  YDISPATCH(b.yL, auto kf = &differzy_y<L, T>;
+            LAUNCH(b, "differzy", g, YTPB, 0, kf, geni, res, ytab<T>(b, which == 0 ? YT_DCYU : YT_DCYV), coefs<T>(b),
+                   which, b.ny, b.nz1, nplanes, ips, ops));
 }
 
-void ky_vistyy(Box &b, const double *u, double *res, int R, int nplanes) {
+template <typename T> void ky_vistyy(Box &b, const T *u, T *res, int R, int nplanes) {
   const unsigned g = ygrid((size_t)b.nz1 * nplanes);
-  YDISPATCH(b.yL, LAUNCH(b, "vistyy", g, YTPB, 0, vistyy_y<L>, u, res,
-                         ytab(b, R == b.ny + 1 ? YT_VIYU : YT_VIYV), b.dc, R, b.ny, b.nz1, nplanes));
+  YDISPATCH(b.yL, auto kf = &vistyy_y<L, T>;
+            LAUNCH(b, "vistyy", g, YTPB, 0, kf, u, res, ytab<T>(b, R == b.ny + 1 ? YT_VIYU : YT_VIYV), coefs<T>(b), R,
+                   b.ny, b.nz1, nplanes));
 }
 
-void ky_implzy(Box &b, double *u, const double *res, int R, int nplanes) {
+template <typename T> void ky_implzy(Box &b, T *u, const T *res, int R, int nplanes) {
   const unsigned g = ygrid((size_t)b.nz1 * nplanes);
   const bool uu = R == b.ny + 1;
-  YDISPATCH(b.yL, LAUNCH(b, "implzy", g, YTPB, 0, implzy_y<L>, u, res, ytab(b, uu ? YT_IMPU : YT_IMPV),
-                         uu ? b.implu : b.implv, R, b.nz1, nplanes));
+  YDISPATCH(b.yL, auto kf = &implzy_y<L, T>;
+            LAUNCH(b, "implzy", g, YTPB, 0, kf, u, res, ytab<T>(b, uu ? YT_IMPU : YT_IMPV), uu ? b.implu : b.implv,
+                   R, b.nz1, nplanes));
 }
 
-void ky_rhsu_loop1(Box &b, const double *uv, const double *uw, double *rhsu) {
+template <typename T> void ky_rhsu_loop1(Box &b, const T *uv, const T *uw, T *rhsu) {
   const unsigned g = ygrid((size_t)b.nz1 * b.nx);
-  YDISPATCH(b.yL, LAUNCH(b, "rhsu_loop1", g, YTPB, 0, rhsu_loop1_y<L>, uv, uw, rhsu, ytab(b, YT_DCYU), b.dc,
-                         b.ny, b.nz1, b.nx));
+  YDISPATCH(b.yL, auto kf = &rhsu_loop1_y<L, T>;
+            LAUNCH(b, "rhsu_loop1", g, YTPB, 0, kf, uv, uw, rhsu, ytab<T>(b, YT_DCYU), coefs<T>(b), b.ny, b.nz1, b.nx));
 }
 
-void ky_rhs_loop2(Box &b, const double *vw, const double *vv, const double *ww) {
+template <typename T> void ky_rhs_loop2(Box &b, const T *vw, const T *vv, const T *ww) {
   const unsigned g = ygrid((size_t)b.nz1 * (b.nx - 2));
-  YDISPATCH(b.yL, LAUNCH(b, "rhs_loop2", g, YTPB, 0, rhs_loop2_y<L>, b.rhsu, b.rhsv, b.rhsw, b.resu, b.resv,
-                         b.resw, vw, vv, ww, ytab(b, YT_DCYU), ytab(b, YT_DCYV), b.dc, b.ny, b.nz1, b.nx - 2));
+  YDISPATCH(b.yL, auto kf = &rhs_loop2_y<L, T>;
+            LAUNCH(b, "rhs_loop2", g, YTPB, 0, kf, fld<T>(b.rhsu), fld<T>(b.rhsv), fld<T>(b.rhsw), fld<T>(b.resu),
+                   fld<T>(b.resv), fld<T>(b.resw), vw, vv, ww, ytab<T>(b, YT_DCYU), ytab<T>(b, YT_DCYV), coefs<T>(b),
+                   b.ny, b.nz1, b.nx - 2));
 }
 
-void ky_rk_final(Box &b, double *u, const double *res, const double *rhs, double *rhsp, int jee) {
+template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, T *rhsp, int jee) {
   const unsigned g = ygrid((size_t)b.nz1 * b.nx);
   const bool uu = jee == b.ny + 1;
-  YDISPATCH(b.yL, LAUNCH(b, "rk_final", g, YTPB, 0, rk_final_y<L>, u, res, rhs, rhsp,
-                         ytab(b, uu ? YT_VIYU : YT_VIYV), ytab(b, uu ? YT_IMPU : YT_IMPV), b.dc,
-                         uu ? b.implu : b.implv, b.ds, jee, b.ny, b.nz1, b.nx));
+  YDISPATCH(b.yL, auto kf = &rk_final_y<L, T>;
+            LAUNCH(b, "rk_final", g, YTPB, 0, kf, u, res, rhs, rhsp, ytab<T>(b, uu ? YT_VIYU : YT_VIYV),
+                   ytab<T>(b, uu ? YT_IMPU : YT_IMPV), coefs<T>(b), uu ? b.implu : b.implv, b.ds, jee, b.ny, b.nz1,
+                   b.nx));
 }
+
+#define YINST(T)                                                                                               \
+  template void ky_interpyy<T>(Box &, const T *, T *, int, int, int, size_t, size_t);                          \
+  template void ky_differzy<T>(Box &, const T *, T *, int, int, size_t, size_t);                               \
+  template void ky_vistyy<T>(Box &, const T *, T *, int, int);                                                 \
+  template void ky_implzy<T>(Box &, T *, const T *, int, int);                                                 \
+  template void ky_rhsu_loop1<T>(Box &, const T *, const T *, T *);                                            \
+  template void ky_rhs_loop2<T>(Box &, const T *, const T *, const T *);                                        \
+  template void ky_rk_final<T>(Box &, T *, const T *, const T *, T *, int);
+YINST(double)
+YINST(float)
 
 } // namespace otbl

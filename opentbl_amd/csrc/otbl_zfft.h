@@ -216,17 +216,19 @@ template <int N> __device__ __forceinline__ int zphys_index(int t, int e) {
 // row of zeros, never a null pointer, so the loads carry no branches): the
 // Hermitian-extended spectrum of x + i y at index k (pack of two real
 // inverse transforms into one complex one).
-__device__ __forceinline__ double2 zpack_one(const double *X, const double *Y, int k, int n, int nz2) {
+template <typename R>
+__device__ __forceinline__ typename CT<R>::c zpack_one(const R *X, const R *Y, int k, int n, int nz2) {
+  using C = typename CT<R>::c;
   const bool lo = k <= nz2, hi = k >= n - nz2;
   const int kk = lo ? k : (hi ? n - k : 0);
-  const double2 x = ((const double2 *)X)[kk];
-  const double2 y = ((const double2 *)Y)[kk];
+  const C x = ((const C *)X)[kk];
+  const C y = ((const C *)Y)[kk];
   // k <= nz2: X[k] + i Y[k]; mirrored: conj(X[n-k]) + i conj(Y[n-k]);
   // Im of mode 0 does not enter; zero between nz2 and n - nz2
-  const double sx = lo ? -1.0 : 1.0;
-  double2 r = make_double2(x.x + sx * y.y, -sx * x.y + y.x);
-  r = k == 0 ? make_double2(x.x, y.x) : r;
-  return (lo || hi) ? r : make_double2(0.0, 0.0);
+  const R sx = lo ? R(-1) : R(1);
+  C r = mkc<C>(x.x + sx * y.y, -sx * x.y + y.x);
+  r = k == 0 ? mkc<C>(x.x, y.x) : r;
+  return (lo || hi) ? r : mkc<C>(0.0, 0.0);
 }
 
 // The first forward pass's inputs x[j + r N / R0] straight from the spectral
@@ -234,29 +236,30 @@ __device__ __forceinline__ double2 zpack_one(const double *X, const double *Y, i
 // (nz1 = 2 (N / 3), nz2 = N / 3 - 1), so register slots whose indices all
 // fall in the zero band nz2 < k < N - nz2 are known at compile time and
 // load nothing, and slots entirely below or above the band need no select.
-template <int N>
-__device__ __forceinline__ void zload_spectral(double2 (&v)[N / (64 * ZPlan<N>::W)], int t, const double *X,
-                                               const double *Y) {
-  constexpr int G = 64 * ZPlan<N>::W, R = ZPlan<N>::rad(0), NB = N / (R * G);
+template <int N, typename R>
+__device__ __forceinline__ void zload_spectral(typename CT<R>::c (&v)[N / (64 * ZPlan<N>::W)], int t, const R *X,
+                                               const R *Y) {
+  using C = typename CT<R>::c;
+  constexpr int G = 64 * ZPlan<N>::W, RD = ZPlan<N>::rad(0), NB = N / (RD * G);
   constexpr int nz2 = N / 3 - 1;
-  const double2 *X2 = (const double2 *)X, *Y2 = (const double2 *)Y;
+  const C *X2 = (const C *)X, *Y2 = (const C *)Y;
 #pragma unroll
   for (int m = 0; m < NB; ++m)
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int base = G * m + r * (N / R); // k = base + t, t < G
+    for (int r = 0; r < RD; ++r) {
+      const int base = G * m + r * (N / RD); // k = base + t, t < G
       const int k = base + t;
-      double2 z = make_double2(0.0, 0.0);
+      C z = mkc<C>(0.0, 0.0);
       if (base + G - 1 <= nz2) { // all lanes below the band: X[k] + i Y[k]
-        const double2 x = X2[k], y = Y2[k];
-        z = k == 0 ? make_double2(x.x, y.x) : make_double2(x.x - y.y, x.y + y.x);
+        const C x = X2[k], y = Y2[k];
+        z = k == 0 ? mkc<C>(x.x, y.x) : mkc<C>(x.x - y.y, x.y + y.x);
       } else if (base >= N - nz2) { // all above: conj(X[N-k]) + i conj(Y[N-k])
-        const double2 x = X2[N - k], y = Y2[N - k];
-        z = make_double2(x.x + y.y, -x.y + y.x);
+        const C x = X2[N - k], y = Y2[N - k];
+        z = mkc<C>(x.x + y.y, -x.y + y.x);
       } else if (base <= nz2 || base + G - 1 >= N - nz2) { // straddles an edge
         z = zpack_one(X, Y, k, N, nz2);
       }
-      v[m * R + r] = z;
+      v[m * RD + r] = z;
     }
   __builtin_amdgcn_sched_barrier(0);
 }
@@ -264,26 +267,37 @@ __device__ __forceinline__ void zload_spectral(double2 (&v)[N / (64 * ZPlan<N>::
 // After the last inverse pass (natural order, R = first forward radix,
 // outputs y[j + q N / R]): through LDS, split the transform Z of a + i b
 // into the spectra of a and b, modes 0..nz2, scaled by 1/N (ffts.f90:51).
-template <int N>
-__device__ __forceinline__ void zstore_spectral(const double2 (&v)[N / (64 * ZPlan<N>::W)], int t, double2 *lds,
-                                                double *A, double *B, int nz2) {
-  constexpr int W = ZPlan<N>::W, G = 64 * W, R = ZPlan<N>::rad(0), NB = N / (R * G);
+// With XA, A also receives the mean-flow terms of a product whose mean was
+// left out of the transform: A_k += ck XA_k for k >= 1 and A_0 += c0 XA_0,
+// XA read before A is written (XA may be A itself).
+template <int N, typename R>
+__device__ __forceinline__ void zstore_spectral(const typename CT<R>::c (&v)[N / (64 * ZPlan<N>::W)], int t,
+                                                typename CT<R>::c *lds, R *A, R *B, int nz2,
+                                                const R *XA = nullptr, R c0 = R(0), R ck = R(0)) {
+  using C = typename CT<R>::c;
+  constexpr int W = ZPlan<N>::W, G = 64 * W, RD = ZPlan<N>::rad(0), NB = N / (RD * G);
 #pragma unroll
   for (int m = 0; m < NB; ++m)
 #pragma unroll
-    for (int q = 0; q < R; ++q) lds[zpad(t + G * m + q * (N / R))] = v[m * R + q];
+    for (int q = 0; q < RD; ++q) lds[zpad(t + G * m + q * (N / RD))] = v[m * RD + q];
   zsync<W>();
-  const double sc = 1.0 / N;
+  const R sc = R(1.0 / N);
   for (int k = t; k <= nz2; k += G) {
-    const double2 z = lds[zpad(k)];
-    const double2 w = lds[zpad(k == 0 ? 0 : N - k)];
+    const C z = lds[zpad(k)];
+    const C w = lds[zpad(k == 0 ? 0 : N - k)];
     if (A) {
-      A[2 * k] = 0.5 * (z.x + w.x) * sc;
-      A[2 * k + 1] = (k == 0) ? 0.0 : 0.5 * (z.y - w.y) * sc;
+      R ar = R(0.5) * (z.x + w.x) * sc, ai = (k == 0) ? R(0) : R(0.5) * (z.y - w.y) * sc;
+      if (XA) {
+        const R xr = XA[2 * k], xi = XA[2 * k + 1];
+        ar += (k == 0 ? c0 : ck) * xr;
+        ai += (k == 0) ? R(0) : ck * xi;
+      }
+      A[2 * k] = ar;
+      A[2 * k + 1] = ai;
     }
     if (B) {
-      B[2 * k] = 0.5 * (z.y + w.y) * sc;
-      B[2 * k + 1] = (k == 0) ? 0.0 : -0.5 * (z.x - w.x) * sc;
+      B[2 * k] = R(0.5) * (z.y + w.y) * sc;
+      B[2 * k + 1] = (k == 0) ? R(0) : R(-0.5) * (z.x - w.x) * sc;
     }
   }
   zsync<W>();

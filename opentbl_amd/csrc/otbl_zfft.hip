@@ -34,16 +34,17 @@ __device__ __forceinline__ double wave_max(double v) {
 // (uy) in spectral space form r2c(u_x^2) -> wki1, r2c(u w_x) -> wki3,
 // r2c(v_x uy) -> wki2 (rows <= ny), and the maxima |u|, |w_x|, |v_x|(j)
 // of the CFL time step.  Rows are numbered (i-1)(ny+1) + (j-1).
-template <int N>
+template <int N, typename R>
 __global__ void __launch_bounds__(ZTeam<N>::TPB_)
-products1_z(const double *__restrict__ u, double *wki1, double *wki2, double *wki3, const double *__restrict__ uy,
-            const double2 *__restrict__ tw, const double *__restrict__ zrow, int ny, int nz1, int nz2, int nrows,
+products1_z(const R *__restrict__ u, R *wki1, R *wki2, R *wki3, const R *__restrict__ uy,
+            const typename CT<R>::c *__restrict__ tw, const R *__restrict__ zrow, int ny, int nz1, int nz2, int nrows,
             double *rowmax) {
+  using C = typename CT<R>::c;
   using Z = ZTeam<N>;
-  __shared__ double2 lds_all[Z::TEAMS * zlds_elems<N>()];
+  __shared__ C lds_all[Z::TEAMS * zlds_elems<N>()];
   // the team (and so the row and every row pointer) is uniform per wavefront
   const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
-  double2 *lds = lds_all + team * zlds_elems<N>();
+  C *lds = lds_all + team * zlds_elems<N>();
   const int row = blockIdx.x * Z::TEAMS + team;
   if (row >= nrows) return; // a whole team (one wavefront when TEAMS > 1)
   const int j = row % (ny + 1) + 1, i = row / (ny + 1) + 1;
@@ -52,22 +53,34 @@ products1_z(const double *__restrict__ u, double *wki1, double *wki2, double *wk
   // Pairs chosen so that almost nothing is held across a transform:
   // (v_x, u_y) gives v_x u_y, inverted at once; (u, w_x) gives u w_x, kept
   // through the transform of u_x, which gives u_x^2.
-  double um = 0.0, wm = 0.0, vm = 0.0;
+  // In FP32 the mean flow (mode 0) of u_x and u_y, O(1) against O(1e-3)
+  // fluctuations, stays out of the transforms: their round-off scales with
+  // the signal, and in u_x^2 it would reach the mean mode and, through
+  // d/dx, the x-momentum balance.  With U the mean, u = U + u':
+  // u_x^2 = U^2 + 2 U u_x' + u_x'^2 and v_x u_y = U_y v_x + v_x u_y', the
+  // U terms added in spectral space.
+  constexpr bool SPLIT = sizeof(R) == 4;
+  R um = 0.0, wm = 0.0, vm = 0.0;
   if (vrow) {
-    double2 C[Z::E];
-    zload_spectral<N>(C, t, wki2 + ov, uy + ou);
-    zrun<N, +1, false>(C, t, tw, lds);
+    C Q[Z::E];
+    zload_spectral<N>(Q, t, wki2 + ov, uy + ou);
+    const R Uy = SPLIT ? uy[ou] : R(0);
+    if (SPLIT && t == 0) Q[0].y = R(0); // k = 0 sits in lane 0, slot 0
+    zrun<N, +1, false>(Q, t, tw, lds);
 #pragma unroll
     for (int e = 0; e < Z::E; ++e) {
-      vm = fmax(vm, fabs(C[e].x));
-      C[e] = make_double2(C[e].x * C[e].y, 0.0);
+      vm = fmax(vm, fabs(Q[e].x));
+      Q[e] = mkc<C>(Q[e].x * Q[e].y, R(0));
     }
-    zrun<N, -1, true>(C, t, tw, lds);
-    zstore_spectral<N>(C, t, lds, wki2 + ov, nullptr, nz2);
+    zrun<N, -1, true>(Q, t, tw, lds);
+    if constexpr (SPLIT)
+      zstore_spectral<N, R>(Q, t, lds, wki2 + ov, nullptr, nz2, wki2 + ov, Uy, Uy);
+    else
+      zstore_spectral<N, R>(Q, t, lds, wki2 + ov, nullptr, nz2);
   }
-  double s1[Z::E];
+  R s1[Z::E];
   {
-    double2 A[Z::E];
+    C A[Z::E];
     zload_spectral<N>(A, t, u + ou, wki3 + ou);
     zrun<N, +1, false>(A, t, tw, lds);
 #pragma unroll
@@ -78,13 +91,18 @@ products1_z(const double *__restrict__ u, double *wki1, double *wki2, double *wk
     }
   }
   {
-    double2 B[Z::E];
+    C B[Z::E];
     zload_spectral<N>(B, t, wki1 + ou, zrow);
+    const R Ux = SPLIT ? wki1[ou] : R(0);
+    if (SPLIT && t == 0) B[0].x = R(0);
     zrun<N, +1, false>(B, t, tw, lds);
 #pragma unroll
-    for (int e = 0; e < Z::E; ++e) B[e] = make_double2(B[e].x * B[e].x, s1[e]);
+    for (int e = 0; e < Z::E; ++e) B[e] = mkc<C>(B[e].x * B[e].x, s1[e]);
     zrun<N, -1, true>(B, t, tw, lds);
-    zstore_spectral<N>(B, t, lds, wki1 + ou, wki3 + ou, nz2);
+    if constexpr (SPLIT)
+      zstore_spectral<N, R>(B, t, lds, wki1 + ou, wki3 + ou, nz2, wki1 + ou, Ux, R(2) * Ux);
+    else
+      zstore_spectral<N, R>(B, t, lds, wki1 + ou, wki3 + ou, nz2);
   }
   // per-row maxima; rowmax_kernel reduces them (no atomics on shared
   // addresses from every row)
@@ -92,7 +110,7 @@ products1_z(const double *__restrict__ u, double *wki1, double *wki2, double *wk
   wm = wave_max(wm);
   vm = wave_max(vm);
   if constexpr (Z::W > 1) { // combine the team's wavefronts through its LDS buffer
-    if ((t & 63) == 0) lds[t / 64] = make_double2(um, wm), lds[Z::W + t / 64] = make_double2(vm, 0.0);
+    if ((t & 63) == 0) lds[t / 64] = mkc<C>(um, wm), lds[Z::W + t / 64] = mkc<C>(vm, R(0));
     zsync<Z::W>();
     if (t == 0)
       for (int q = 1; q < Z::W; ++q) {
@@ -104,7 +122,7 @@ products1_z(const double *__restrict__ u, double *wki1, double *wki2, double *wk
   if (t == 0) {
     rowmax[row] = um;
     rowmax[nrows + row] = wm;
-    rowmax[2 * (size_t)nrows + row] = vrow ? vm : 0.0;
+    rowmax[2 * (size_t)nrows + row] = vrow ? (double)vm : 0.0;
   }
 }
 
@@ -146,15 +164,16 @@ rowmax_kernel(const double *__restrict__ rowmax, DevScalars *ds, unsigned long l
 // planes 2..nx-1 (rows numbered (i-2) ny + (j-1)): vw = r2c(v w_v) -> ya
 // (w interpolated to v rows is read from ya), vv = r2c(v_u^2) -> yb (v
 // interpolated to u rows is read from yb), ww = r2c(w^2) -> yc.
-template <int N>
+template <int N, typename R>
 __global__ void __launch_bounds__(ZTeam<N>::TPB_)
-products2_z(const double *__restrict__ resv, const double *__restrict__ resw, double *ya, double *yb, double *yc,
-            const double2 *__restrict__ tw, int ny, int nz1, int nz2, int nrows) {
+products2_z(const R *__restrict__ resv, const R *__restrict__ resw, R *ya, R *yb, R *yc,
+            const typename CT<R>::c *__restrict__ tw, int ny, int nz1, int nz2, int nrows) {
+  using C = typename CT<R>::c;
   using Z = ZTeam<N>;
-  __shared__ double2 lds_all[Z::TEAMS * zlds_elems<N>()];
+  __shared__ C lds_all[Z::TEAMS * zlds_elems<N>()];
   // the team (and so the row and every row pointer) is uniform per wavefront
   const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
-  double2 *lds = lds_all + team * zlds_elems<N>();
+  C *lds = lds_all + team * zlds_elems<N>();
   const int row = blockIdx.x * Z::TEAMS + team;
   if (row >= nrows) return;
   const int j = row % ny + 1, i = row / ny + 2;
@@ -162,22 +181,22 @@ products2_z(const double *__restrict__ resv, const double *__restrict__ resw, do
   const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
   // (v_u, w) gives (v_u^2, w^2), inverted at once; (v, w_v) gives v w_v
   {
-    double2 B[Z::E];
+    C B[Z::E];
     zload_spectral<N>(B, t, yb + ou, resw + ou);
     zrun<N, +1, false>(B, t, tw, lds);
 #pragma unroll
-    for (int e = 0; e < Z::E; ++e) B[e] = make_double2(B[e].x * B[e].x, B[e].y * B[e].y);
+    for (int e = 0; e < Z::E; ++e) B[e] = mkc<C>(B[e].x * B[e].x, B[e].y * B[e].y);
     zrun<N, -1, true>(B, t, tw, lds);
-    zstore_spectral<N>(B, t, lds, yb + ou, yc + ou, nz2);
+    zstore_spectral<N, R>(B, t, lds, yb + ou, yc + ou, nz2);
   }
   {
-    double2 A[Z::E];
+    C A[Z::E];
     zload_spectral<N>(A, t, resv + ov, ya + ou);
     zrun<N, +1, false>(A, t, tw, lds);
 #pragma unroll
-    for (int e = 0; e < Z::E; ++e) A[e] = make_double2(A[e].x * A[e].y, 0.0);
+    for (int e = 0; e < Z::E; ++e) A[e] = mkc<C>(A[e].x * A[e].y, R(0));
     zrun<N, -1, true>(A, t, tw, lds);
-    zstore_spectral<N>(A, t, lds, ya + ou, nullptr, nz2);
+    zstore_spectral<N, R>(A, t, lds, ya + ou, nullptr, nz2);
   }
 }
 
@@ -206,7 +225,7 @@ fourxz_z(double *fou, double *phys, int iopt, const double2 *__restrict__ tw, co
 #pragma unroll
     for (int e = 0; e < Z::E; ++e) A[e] = make_double2(Ph[zphys_index<N>(t, e)], 0.0);
     zrun<N, -1, true>(A, t, tw, lds);
-    zstore_spectral<N>(A, t, lds, F, nullptr, nz2);
+    zstore_spectral<N, double>(A, t, lds, F, nullptr, nz2);
   }
 }
 
@@ -220,40 +239,42 @@ fourxz_z(double *fou, double *phys, int iopt, const double2 *__restrict__ tw, co
 // chirp and the Makhoul untwisting through the team's LDS buffer.
 // iopt < 0: DCT-II x 1/N, iopt > 0: DCT-III x 0.5; in place (src == dst)
 // allowed, each team reads its two lines before writing them.
-template <int M>
+template <int M, typename R>
 __global__ void __launch_bounds__(ZTeam<M>::TPB_)
-dct_rows_z(const double *src, double *dst, int Lp, int N, int iopt, const double2 *__restrict__ chirp,
-           const double2 *__restrict__ bf, const double2 *__restrict__ tw4, const double2 *__restrict__ tw) {
+dct_rows_z(const R *src, R *dst, int Lp, int N, int iopt, const typename CT<R>::c *__restrict__ chirp,
+           const typename CT<R>::c *__restrict__ bf, const typename CT<R>::c *__restrict__ tw4,
+           const typename CT<R>::c *__restrict__ tw) {
+  using C = typename CT<R>::c;
   using Z = ZTeam<M>;
-  __shared__ double2 lds_all[Z::TEAMS * zlds_elems<M>()];
+  __shared__ C lds_all[Z::TEAMS * zlds_elems<M>()];
   const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
-  double2 *lds = lds_all + team * zlds_elems<M>();
+  C *lds = lds_all + team * zlds_elems<M>();
   const int pair = blockIdx.x * Z::TEAMS + team;
   const int l0 = 2 * pair, l1 = l0 + 1;
   if (l0 >= Lp) return; // a whole team
   const bool has1 = l1 < Lp;
-  const double *x0 = src + (size_t)l0 * N, *x1 = src + (size_t)(has1 ? l1 : l0) * N;
+  const R *x0 = src + (size_t)l0 * N, *x1 = src + (size_t)(has1 ? l1 : l0) * N;
   const int half = (N + 1) / 2;
   constexpr int R0 = ZPlan<M>::rad(0), NB0 = M / (R0 * Z::G);
-  double2 v[Z::E];
+  C v[Z::E];
 #pragma unroll
   for (int m = 0; m < NB0; ++m)
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
       const int k = t + Z::G * m + r * (M / R0);
-      double2 a = make_double2(0.0, 0.0);
+      C a = mkc<C>(R(0), R(0));
       if (k < N) {
-        double2 z;
+        C z;
         if (iopt < 0) { // Makhoul reordering of the two real lines
           const int sidx = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
-          z = make_double2(x0[sidx], has1 ? x1[sidx] : 0.0);
+          z = mkc<C>(x0[sidx], has1 ? x1[sidx] : R(0));
         } else { // V = (X_k - i X_{N-k}) exp(+i pi k / 2N), Z = V0 + i V1, conj(Z)
-          double2 tt = tw4[k];
+          C tt = tw4[k];
           tt.y = -tt.y;
-          const double a0 = x0[k], b0 = k ? x0[N - k] : 0.0;
-          const double a1 = has1 ? x1[k] : 0.0, b1 = (has1 && k) ? x1[N - k] : 0.0;
-          const double2 v0 = cmulc(make_double2(a0, -b0), tt), v1 = cmulc(make_double2(a1, -b1), tt);
-          z = make_double2(v0.x - v1.y, -(v0.y + v1.x));
+          const R a0 = x0[k], b0 = k ? x0[N - k] : R(0);
+          const R a1 = has1 ? x1[k] : R(0), b1 = (has1 && k) ? x1[N - k] : R(0);
+          const C v0 = cmulc(mkc<C>(a0, -b0), tt), v1 = cmulc(mkc<C>(a1, -b1), tt);
+          z = mkc<C>(v0.x - v1.y, -(v0.y + v1.x));
         }
         a = cmulc(z, chirp[k]);
       }
@@ -272,23 +293,23 @@ dct_rows_z(const double *src, double *dst, int Lp, int N, int iopt, const double
       if (k < N) lds[zpad(k)] = cmulc(chirp[k], v[m * R0 + q]);
     }
   zsync<Z::W>();
-  double *y0 = dst + (size_t)l0 * N, *y1 = dst + (size_t)l1 * N;
+  R *y0 = dst + (size_t)l0 * N, *y1 = dst + (size_t)l1 * N;
   if (iopt < 0) {
-    const double sc = 2.0 / N;
+    const R sc = R(2.0 / N);
     for (int k = t; k < N; k += Z::G) {
-      const double2 z = lds[zpad(k)], w = lds[zpad(k ? N - k : 0)];
-      const double2 v0 = make_double2(0.5 * (z.x + w.x), 0.5 * (z.y - w.y));
-      const double2 v1 = make_double2(0.5 * (z.y + w.y), -0.5 * (z.x - w.x));
-      const double2 tt = tw4[k];
+      const C z = lds[zpad(k)], w = lds[zpad(k ? N - k : 0)];
+      const C v0 = mkc<C>(R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y));
+      const C v1 = mkc<C>(R(0.5) * (z.y + w.y), R(-0.5) * (z.x - w.x));
+      const C tt = tw4[k];
       y0[k] = sc * (tt.x * v0.x - tt.y * v0.y);
       if (has1) y1[k] = sc * (tt.x * v1.x - tt.y * v1.y);
     }
   } else {
     for (int k = t; k < N; k += Z::G) {
-      const double2 zc = lds[zpad(k)]; // conj of sum_k Z_k exp(+2 pi i k n / N)
+      const C zc = lds[zpad(k)]; // conj of sum_k Z_k exp(+2 pi i k n / N)
       const int o = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
-      y0[o] = 0.5 * zc.x;
-      if (has1) y1[o] = -0.5 * zc.y;
+      y0[o] = R(0.5) * zc.x;
+      if (has1) y1[o] = R(-0.5) * zc.y;
     }
   }
 }
@@ -296,13 +317,13 @@ dct_rows_z(const double *src, double *dst, int Lp, int N, int iopt, const double
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-#define ZDISPATCH(n, CALL)                                                     \
+#define ZDISPATCH(n, ...)                                                      \
   switch (n) {                                                                 \
-  case 256: { constexpr int N = 256; CALL; } break;                            \
-  case 512: { constexpr int N = 512; CALL; } break;                            \
-  case 1024: { constexpr int N = 1024; CALL; } break;                          \
-  case 2048: { constexpr int N = 2048; CALL; } break;                          \
-  case 4096: { constexpr int N = 4096; CALL; } break;                          \
+  case 256: { constexpr int N = 256; __VA_ARGS__; } break;                     \
+  case 512: { constexpr int N = 512; __VA_ARGS__; } break;                     \
+  case 1024: { constexpr int N = 1024; __VA_ARGS__; } break;                   \
+  case 2048: { constexpr int N = 2048; __VA_ARGS__; } break;                   \
+  case 4096: { constexpr int N = 4096; __VA_ARGS__; } break;                   \
   default: throw std::invalid_argument("z transform length not instantiated"); \
   }
 
@@ -316,32 +337,44 @@ static void zcheck(const Box &b) {
     throw std::invalid_argument("z transform: nz1/nz2 do not follow nz");
 }
 
-void kz_products1(Box &b) {
+template <typename R> static const typename CT<R>::c *ztw(const Box &b);
+template <> const double2 *ztw<double>(const Box &b) { return (const double2 *)b.ztw; }
+template <> const float2 *ztw<float>(const Box &b) { return (const float2 *)b.ztw_f; }
+
+template <typename R> void kz_products1(Box &b) {
   zcheck(b);
   const int nrows = (b.ny + 1) * b.nx;
-  ZDISPATCH(b.nz, LAUNCH(b, "products1", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, products1_z<N>, b.f[OTBL_U], b.wki1,
-                         b.wki2, b.wki3, b.yc, (const double2 *)b.ztw, b.zrow, b.ny, b.nz1, b.nz2, nrows,
-                         b.rowmax));
+  const R *zrow = fld<R>(b.zrow);
+  ZDISPATCH(b.nz, auto kf = &products1_z<N, R>;
+            LAUNCH(b, "products1", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, kf, fld<R>(b.f[OTBL_U]), fld<R>(b.wki1),
+                   fld<R>(b.wki2), fld<R>(b.wki3), fld<R>(b.yc), ztw<R>(b), zrow, b.ny, b.nz1, b.nz2, nrows,
+                   b.rowmax));
   LAUNCH(b, "rowmax", b.ny + 1, 256, 0, rowmax_kernel, b.rowmax, b.ds, b.vmbits, b.ny, b.nx);
 }
 
-void kz_products2(Box &b) {
+template <typename R> void kz_products2(Box &b) {
   zcheck(b);
   const int nrows = b.ny * (b.nx - 2);
-  ZDISPATCH(b.nz, LAUNCH(b, "products2", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, products2_z<N>, b.resv, b.resw, b.ya,
-                         b.yb, b.yc, (const double2 *)b.ztw, b.ny, b.nz1, b.nz2, nrows));
+  ZDISPATCH(b.nz, auto kf = &products2_z<N, R>;
+            LAUNCH(b, "products2", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, kf, fld<R>(b.resv), fld<R>(b.resw),
+                   fld<R>(b.ya), fld<R>(b.yb), fld<R>(b.yc), ztw<R>(b), b.ny, b.nz1, b.nz2, nrows));
 }
 
-void kz_dct_pencils(Box &b, double *P, int iopt) {
-  const int Lp = (b.ny - 1) * b.nz1, N = b.nx - 1, pairs = (Lp + 1) / 2;
-  const char *name = iopt < 0 ? "dct2_rows" : "dct3_rows";
+// Lp lines of N points from P (in place); the lines of the mean-flow mode
+// alone (Lp = 2 (ny-1)) are timed under their own names
+template <typename R> static void dct_launch(Box &b, R *P, int iopt, int Lp, const R *chirp, const R *bf,
+                                             const R *tw4, const R *tw) {
+  using C = typename CT<R>::c;
+  const int N = b.nx - 1, pairs = (Lp + 1) / 2;
+  const bool all = Lp == (b.ny - 1) * b.nz1;
+  const char *name = iopt < 0 ? (all ? "dct2_rows" : "dct2_mean") : (all ? "dct3_rows" : "dct3_mean");
   switch (b.dct_M) {
 #define ZDCT(MM)                                                                                              \
-  case MM:                                                                                                     \
-    LAUNCH(b, name, zgrid<MM>(pairs), ZTeam<MM>::TPB_, 0, dct_rows_z<MM>, P, P, Lp, N, iopt,                     \
-           (const double2 *)b.dct_chirp, (const double2 *)b.dct_bf, (const double2 *)b.dct_tw4,                 \
-           (const double2 *)b.dct_ztw);                                                                        \
-    break;
+  case MM: {                                                                                                   \
+    auto kf = &dct_rows_z<MM, R>;                                                                              \
+    LAUNCH(b, name, zgrid<MM>(pairs), ZTeam<MM>::TPB_, 0, kf, P, P, Lp, N, iopt, (const C *)chirp,             \
+           (const C *)bf, (const C *)tw4, (const C *)tw);                                                      \
+  } break;
     ZDCT(512)
     ZDCT(1024)
     ZDCT(2048)
@@ -352,10 +385,23 @@ void kz_dct_pencils(Box &b, double *P, int iopt) {
   }
 }
 
+template <> void kz_dct_pencils<double>(Box &b, double *P, int iopt, int Lp) {
+  dct_launch<double>(b, P, iopt, Lp, b.dct_chirp, b.dct_bf, b.dct_tw4, b.dct_ztw);
+}
+template <> void kz_dct_pencils<float>(Box &b, float *P, int iopt, int Lp) {
+  dct_launch<float>(b, P, iopt, Lp, b.dct_chirp_f, b.dct_bf_f, b.dct_tw4_f, b.dct_ztw_f);
+}
+
+template void kz_products1<double>(Box &);
+template void kz_products1<float>(Box &);
+template void kz_products2<double>(Box &);
+template void kz_products2<float>(Box &);
+
 void kz_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows) {
   zcheck(b);
-  ZDISPATCH(b.nz, LAUNCH(b, "fourxz", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, fourxz_z<N>, fou, phys, iopt,
-                         (const double2 *)b.ztw, b.zrow, b.nz1, b.nz2, nrows));
+  ZDISPATCH(b.nz, auto kf = &fourxz_z<N>;
+            LAUNCH(b, "fourxz", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, kf, fou, phys, iopt, (const double2 *)b.ztw,
+                   b.zrow, b.nz1, b.nz2, nrows));
 }
 
 } // namespace otbl
