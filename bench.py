@@ -79,15 +79,20 @@ TRACE_NAMES = {"dct2_rows": "dct_rows_z", "dct3_rows": "dct_rows_z", "div": "div
 
 def profiled_traffic(workload, precision, kname):
     """HBM bytes per launch of one kernel from the committed rocprofv3 summary
-    of this workload (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by
-    what the counter reads for a plain 1 GiB device copy); None if absent."""
+    of this workload and precision (FETCH_SIZE + WRITE_SIZE per dispatch,
+    each scaled by what the counter reads for a plain 1 GiB device copy); the
+    summary lists kernels by time, so the first row whose HIP function
+    matches is the launch timed here.  None if absent."""
     path = os.path.join(ROOT, "profiles", f"substep_kernels_{workload}_{precision}.md")
     if not os.path.exists(path):
         return None
     fn = TRACE_NAMES.get(kname, kname + "_kernel")
+    if precision == "mixed" and kname == "poisson_y":
+        fn = "poisson_y_mixed_kernel"
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 7 and cells[0] == fn and cells[5] and cells[6]:
+        name = cells[0] if cells else ""
+        if len(cells) >= 7 and (name == fn or name.startswith(fn + "<")) and cells[5] and cells[6]:
             return (float(cells[5]) + float(cells[6])) * 1e9
     return None
 

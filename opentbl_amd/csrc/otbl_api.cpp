@@ -289,8 +289,7 @@ static void free_box(Box *b) {
                     b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax, b->P0, b->S0, b->p0, b->um64, b->rm64, b->tm64};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
-  float *fbufs[] = {b->coefbuf_f, b->ytab_f, b->ztw_f, b->dct_chirp_f, b->dct_bf_f, b->dct_tw4_f, b->dct_ztw_f,
-                    b->pdl, b->pdu};
+  float *fbufs[] = {b->coefbuf_f, b->ytab_f, b->ztw_f, b->dct_chirp_f, b->dct_bf_f, b->dct_tw4_f, b->dct_ztw_f};
   for (float *p : fbufs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
@@ -367,10 +366,18 @@ static void need(bool ok, const char *msg) {
   if (!ok) throw std::invalid_argument(msg);
 }
 
+// Slot in the FP64 mean-flow array of row r = (i-1) R + (j-1) of field id:
+// p0 is [nx][ny] (the same order), um64 [ny+1][nx].
+static size_t mean_slot(const Box &b, int id, size_t r) {
+  if (id == OTBL_P) return r;
+  const size_t R = b.ny + 1, i = r / R, j = r % R;
+  return j * b.nx + i;
+}
+
 // FP32 copies of the factored FP64 coefficients and tables, the Poisson
 // pivots and the FP64 mean-flow pencils of the mixed-precision substep.
 static void setup_mixed(Box &b) {
-  need(b.yL && b.zfast && b.dct_zfast,
+  need(b.yL && b.zfast && b.dct_zfast && b.pchunk,
        "OTBL_MIXED needs the chunked y solver (ny <= 24*16), a z length of 256..4096 and a power-of-two "
        "Bluestein length of 512..8192 (nx-1)");
   need(b.xwin, "OTBL_MIXED needs x lines that the windowed sweep solves (nx <= 32 or decaying x systems)");
@@ -397,10 +404,6 @@ static void setup_mixed(Box &b) {
   k_d2f(b, b.dct_tw4, b.dct_tw4_f, 2 * N);
   k_d2f(b, b.dct_bf, b.dct_bf_f, 2 * M);
   k_d2f(b, b.dct_ztw, b.dct_ztw_f, 2 * M);
-  const size_t npiv = (size_t)(b.ny - 1) * (b.nz1 / 2) * N;
-  dalloc(b.pdl, npiv);
-  dalloc(b.pdu, npiv);
-  k_poisson_pivots(b);
   dalloc(b.P0, 2 * (size_t)(b.ny - 1) * N);
   dalloc(b.S0, 2 * (size_t)(b.ny - 1) * N);
   dalloc(b.p0, (size_t)b.nx * b.ny);
@@ -445,6 +448,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   b->pv = (size_t)b->nz1 * b->ny;
   b->nu = b->pu * b->nx;
   b->nv = b->pv * b->nx;
+  need(b->nu < (size_t(1) << 32), "a field must have fewer than 2^32 elements");
   OTBL_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
   build_coef(b->hc, b->nx, b->ny, b->nz, b->re, b->ax, b->az, b->cfl, y);
   upload_coef(*b);
@@ -510,6 +514,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   // chunked y-line solver unless the columns are longer than its
   // instantiations (or OTBL_Y_MARCH asks for one thread per column)
   b->yL = std::getenv("OTBL_Y_MARCH") ? 0 : ytab_rows(b->ny);
+  b->pchunk = std::getenv("OTBL_Y_MARCH") ? 0 : kp_chunk_rows(b->ny);
   if (b->yL) {
     dalloc(b->ytab, YT_COUNT * ytab_size(b->yL));
     k_ytab_fixed(*b);
@@ -561,12 +566,12 @@ int otbl_set_field(otbl_ctx *ctx, int id, const double *host) {
     h2d(b, b.ya, host, n * sizeof(double));
     k_d2f(b, b.ya, fld<float>(b.f[id]), n);
     OTBL_HIP(hipStreamSynchronize(b.s));
-    double *mean = id == OTBL_P ? b.p0 : (id == OTBL_U ? b.um64 : nullptr);
-    if (mean) {
-      // the mean-flow pressure and u (real part of mode 0) are kept in FP64
+    if (id == OTBL_P || id == OTBL_U) {
+      // the mean-flow pressure ([nx][ny]) and u ([ny+1][nx]) (real parts of
+      // mode 0) are kept in FP64
       std::vector<double> m(n / b.nz1);
-      for (size_t r = 0; r < m.size(); ++r) m[r] = host[r * b.nz1];
-      h2d(b, mean, m.data(), m.size() * sizeof(double));
+      for (size_t r = 0; r < m.size(); ++r) m[mean_slot(b, id, r)] = host[r * b.nz1];
+      h2d(b, id == OTBL_P ? b.p0 : b.um64, m.data(), m.size() * sizeof(double));
     }
   }
   API_END
@@ -584,11 +589,10 @@ int otbl_get_field(otbl_ctx *ctx, int id, double *host) {
     k_f2d(b, fld<float>(b.f[id]), b.ya, n);
     OTBL_HIP(hipStreamSynchronize(b.s));
     d2h(b, host, b.ya, n * sizeof(double));
-    const double *mean = id == OTBL_P ? b.p0 : (id == OTBL_U ? b.um64 : nullptr);
-    if (mean) {
+    if (id == OTBL_P || id == OTBL_U) {
       std::vector<double> m(n / b.nz1);
-      d2h(b, m.data(), mean, m.size() * sizeof(double));
-      for (size_t r = 0; r < m.size(); ++r) host[r * b.nz1] = m[r];
+      d2h(b, m.data(), id == OTBL_P ? b.p0 : b.um64, m.size() * sizeof(double));
+      for (size_t r = 0; r < m.size(); ++r) host[r * b.nz1] = m[mean_slot(b, id, r)];
     }
   }
   API_END
@@ -860,9 +864,9 @@ int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len) {
   Box &b = box_of(ctx);
   need(buf && len > 0, "otbl_get_paths: null buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof tmp, "y:%s%d z:%s%d dct:%s%d x:%s", b.yL ? "chunk" : "march", b.yL,
+  std::snprintf(tmp, sizeof tmp, "y:%s%d z:%s%d dct:%s%d x:%s pois:%s%d", b.yL ? "chunk" : "march", b.yL,
                 b.zfast ? "engine" : "stockham", b.nz, b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"),
-                b.dct_M, b.xwin ? "window" : "march");
+                b.dct_M, b.xwin ? "window" : "march", b.pchunk ? "chunk" : "march", b.pchunk);
   std::snprintf(buf, len, "%s", tmp);
   API_END
 }

@@ -79,16 +79,16 @@ struct Box {
   DevCoefT<float> dcf{};
   float *coefbuf_f = nullptr, *ytab_f = nullptr, *ztw_f = nullptr;
   float *dct_chirp_f = nullptr, *dct_bf_f = nullptr, *dct_tw4_f = nullptr, *dct_ztw_f = nullptr;
-  // mixed Poisson: FP32 pivots of every (kz, kx) y operator (top-down and
-  // bottom-up eliminations, [ny-1][nz1/2][nx-1]) and the FP64 pencils of the
-  // mean-flow (kz = 0: kr = 0, 1) lines with their scratch (2(ny-1)(nx-1)
-  // each)
-  float *pdl = nullptr, *pdu = nullptr;
+  // Poisson y solves of the modes kz >= 1 partitioned in registers: rows
+  // per chunk (0: one thread marches each column)
+  int pchunk = 0;
+  // mixed Poisson: the FP64 pencils of the mean-flow (kz = 0: kr = 0, 1)
+  // lines with their scratch (2(ny-1)(nx-1) each)
   double *P0 = nullptr, *S0 = nullptr;
   // mixed: the mean-flow pressure (real part of mode 0 of p) in FP64,
   // [nx][ny]; its gradients feed the predictor and its phi the projection
   double *p0 = nullptr;
-  // mixed: the mean-flow u (real part of mode 0 of u) in FP64, [nx][ny+1],
+  // mixed: the mean-flow u (real part of mode 0 of u) in FP64, [ny+1][nx],
   // updated in FP64 wherever the substep updates u (the FP32 field holds its
   // rounding); its saved copy and a scratch of the same size
   double *um64 = nullptr, *rm64 = nullptr, *tm64 = nullptr;
@@ -165,7 +165,10 @@ void k_substep_rhsp(Box &b, int m);
 void k_outflow(Box &b);
 void k_pois(Box &b, int m, double varstep, bool use_m);
 void k_end_step(Box &b);
-void k_poisson_pivots(Box &b);
+// partitioned Poisson y solves of the modes kz >= 1 on pencils
+// (otbl_poisson.hip); REF: one FP64-residual refinement step
+int kp_chunk_rows(int ny);
+template <typename T, bool REF> void kp_poisson_y(Box &b, T *P);
 // element conversions between FP64 and FP32 buffers on b.s
 void k_d2f(Box &b, const double *a, float *o, size_t n);
 void k_f2d(Box &b, const float *a, double *o, size_t n);

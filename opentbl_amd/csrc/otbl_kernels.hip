@@ -836,6 +836,18 @@ void k_cosftx(Box &b, const double *in, double *out, int iopt) {
 // ---------------------------------------------------------------------------
 // rhsp pieces
 // ---------------------------------------------------------------------------
+// Element-wise kernels over u-type fields: element e = ((i-1)(ny+1) +
+// j-1) nz1 + kr of a 1-D grid, split with 32-bit divisions (a field has
+// fewer than 2^32 elements, checked at setup).
+__device__ __forceinline__ bool uelem(int nz1, int R, int nx, size_t &e, int &kr, int &j, int &i) {
+  const unsigned ee = blockIdx.x * blockDim.x + threadIdx.x;
+  e = ee;
+  const unsigned row = ee / (unsigned)nz1;
+  kr = (int)(ee - row * (unsigned)nz1);
+  i = (int)(row / (unsigned)R) + 1;
+  j = (int)(row - (unsigned)(i - 1) * (unsigned)R) + 1;
+  return i <= nx;
+}
 __global__ void reset_max_kernel(DevScalars *ds, unsigned long long *vmbits, int ny) {
   for (int j = threadIdx.x; j < ny; j += blockDim.x) vmbits[j] = 0ull;
   if (threadIdx.x == 0) {
@@ -909,20 +921,17 @@ __global__ void rhsu_loop1_kernel(const double *__restrict__ uv, const double *_
 // predictor (rhsp.F90:291-339) in one pass over u-type elements.
 // With FP32 fields: p0, the FP64 mean-flow pressure [nx][ny], whose
 // gradients replace those of p's mode-0 real part; um, the FP64 mean-flow u
-// [nx][ny+1] (mode-0 real part), updated in FP64 and saved to rm (else
+// [ny+1][nx] (mode-0 real part), updated in FP64 and saved to rm (else
 // all null).
 template <typename T>
 __global__ void save_predict_kernel(T *u, T *v, T *w, const T *__restrict__ p, T *resu, T *resv, T *resw,
                                     const T *idyy, const T *kaz, const DevScalars *ds, double idxx_d, int nx,
                                     int ny, int nz1, const double *__restrict__ p0, const double *idyy_d,
                                     double *um, double *rm) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t pu = (size_t)nz1 * (ny + 1), pv = (size_t)nz1 * ny;
-  if (e >= pu * nx) return;
-  int i = (int)(e / pu) + 1;
-  size_t r = e % pu;
-  int j = (int)(r / nz1) + 1;
-  int kr = (int)(r % nz1);
+  size_t e;
+  int kr, j, i;
+  if (!uelem(nz1, ny + 1, nx, e, kr, j, i)) return;
+  const size_t pu = (size_t)nz1 * (ny + 1), pv = (size_t)nz1 * ny;
   const T var1 = (T)ds->var1, idxx = (T)idxx_d;
   size_t ev = (size_t)(i - 1) * pv + (size_t)(j - 1) * nz1 + kr;
   T uu = u[e], ww = w[e];
@@ -932,7 +941,7 @@ __global__ void save_predict_kernel(T *u, T *v, T *w, const T *__restrict__ p, T
   const double *pm = mean ? p0 + (size_t)(i - 1) * ny + (j - 1) : nullptr; // mean pressure at (i, j)
   const bool upred = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny;
   if (um && kr == 0) {
-    const size_t q = (size_t)(i - 1) * (ny + 1) + (j - 1);
+    const size_t q = (size_t)(j - 1) * nx + (i - 1);
     double U = um[q];
     rm[q] = U;
     if (upred) {
@@ -1099,7 +1108,7 @@ __global__ void genflu_stats_kernel(const T *resu, double *um, int *ibuf, double
 template <typename T>
 __global__ void genflu_plane_kernel(T *u, T *v, T *w, const T *resu, const T *resv, const T *resw, const double *u0,
                                     const double *v0, const int *ibuf, const double *dbuf, DevCoef c,
-                                    const DevScalars *ds, int ny, int nz1, int xout, double *um) {
+                                    const DevScalars *ds, int ny, int nz1, int xout, double *um, int nx1) {
   const int f = blockIdx.y;
   const int n = f == 2 ? ny : ny + 1, s = nz1;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1110,7 +1119,7 @@ __global__ void genflu_plane_kernel(T *u, T *v, T *w, const T *resu, const T *re
     double z = 0.0;
     if (k == 0 && f == 0) {
       z = u0[i - 1];
-      if (um) um[i - 1] = z; // plane 1 of the FP64 mean-flow u
+      if (um) um[(size_t)(i - 1) * nx1] = z; // plane 1 of the FP64 mean-flow u [ny+1][nx]
     }
     if (k == 0 && f == 2) z = v0[i - 1];
     out[e] = (T)z;
@@ -1183,7 +1192,7 @@ __global__ void rhs_loop2_kernel(double *rhsu, double *rhsv, double *rhsw, const
 
 // boun (boun.f90:7-56) and the right-hand-side boundary rows
 // (rhsp.F90:421-445) for every plane.  Columns (kr, i).
-// um: with FP32 fields, the FP64 mean-flow u [nx][ny+1], whose boundary
+// um: with FP32 fields, the FP64 mean-flow u [ny+1][nx], whose boundary
 // rows are extrapolated in FP64 from its own interior (else null).
 template <typename T>
 __global__ void boun_kernel(T *u, T *v, T *w, T *rhsu, T *rhsv, T *rhsw, DevCoefT<T> c, int ny, int nz1, int nx,
@@ -1201,12 +1210,13 @@ __global__ void boun_kernel(T *u, T *v, T *w, T *rhsu, T *rhsv, T *rhsw, DevCoef
   for (int q = 0; q < 2; ++q) {
     T *a = f[q];
     if (q == 0 && um && kr == 0) {
-      double *U = um + ip * (ny + 1);
-      U[0] = -1.0 / M4(cd.inby, 2, 1) * (M4(cd.inby, 2, 2) * U[1] + M4(cd.inby, 2, 3) * U[2] + M4(cd.inby, 2, 4) * U[3]);
-      U[ny] = -1.0 / M2(cd.ccon, 1, 1) *
-              (M2(cd.ccon, 1, 2) * U[ny - 1] + M2(cd.ccon, 1, 3) * U[ny - 2] + M2(cd.ccon, 1, 4) * U[ny - 3]);
-      a[0] = (T)U[0];
-      a[(size_t)ny * s] = (T)U[ny];
+      double *U = um + ip;
+      auto UR = [&](int r) -> double & { return U[(size_t)r * nx]; }; // row r+1
+      UR(0) = -1.0 / M4(cd.inby, 2, 1) * (M4(cd.inby, 2, 2) * UR(1) + M4(cd.inby, 2, 3) * UR(2) + M4(cd.inby, 2, 4) * UR(3));
+      UR(ny) = -1.0 / M2(cd.ccon, 1, 1) *
+               (M2(cd.ccon, 1, 2) * UR(ny - 1) + M2(cd.ccon, 1, 3) * UR(ny - 2) + M2(cd.ccon, 1, 4) * UR(ny - 3));
+      a[0] = (T)UR(0);
+      a[(size_t)ny * s] = (T)UR(ny);
       continue;
     }
     a[0] = i21 * (M4(c.inby, 2, 2) * a[s] + M4(c.inby, 2, 3) * a[2 * s] + M4(c.inby, 2, 4) * a[3 * s]);
@@ -1266,8 +1276,8 @@ __global__ void rk_final_kernel(double *u, const double *res, const double *rhs,
 // of u of every plane, from the FP64 state um and its saved copy rm, with
 // the FP32 rhs and previous rhs (mode-0 real parts).  Runs before the FP32
 // rk_final of u, which then overwrites rhsp; mean_to_field_kernel copies the
-// result into the FP32 field afterwards.  One thread per plane; tm: scratch
-// [nx][ny+1].
+// result into the FP32 field afterwards.  One thread per plane, arrays
+// [ny+1][nx] (adjacent planes adjacent); tm: scratch of that size.
 template <typename T>
 __global__ void mean_rk_final_kernel(double *um, const double *rm, const T *rhs, const T *rhsp, double *tm,
                                      DevCoef c, const double *f, const DevScalars *ds, int ny, int nz1, int nx) {
@@ -1275,20 +1285,23 @@ __global__ void mean_rk_final_kernel(double *um, const double *rm, const T *rhs,
   if (ip >= nx || ip == 0) return; // plane 1 keeps u
   const int jee = ny + 1;
   const size_t pu = (size_t)nz1 * jee;
-  double *U = um + (size_t)ip * jee, *t = tm + (size_t)ip * jee;
+  double *U = um + ip, *t = tm + ip;
   const T *rh = rhs + ip * pu, *rp = rhsp + ip * pu;
   const double var2 = (ip == nx - 1) ? 0.0 : ds->var2, var3 = ds->var3, var4 = ds->var4;
-  vistyy_col(rm + (size_t)ip * jee, t, 1, jee, ny, c);
-  for (int j = 0; j < jee; ++j)
-    t[j] = U[j] - (var3 * (double)rh[(size_t)j * nz1] + var2 * t[j] + var4 * (double)rp[(size_t)j * nz1]);
-  implzy_col(U, t, 1, jee, c.vyui, f);
+  vistyy_col(rm + ip, t, nx, jee, ny, c);
+  for (int j = 0; j < jee; ++j) {
+    const size_t o = (size_t)j * nx;
+    t[o] = U[o] - (var3 * (double)rh[(size_t)j * nz1] + var2 * t[o] + var4 * (double)rp[(size_t)j * nz1]);
+  }
+  implzy_col(U, t, nx, jee, c.vyui, f);
 }
 
 template <typename T>
 __global__ void mean_to_field_kernel(T *u, const double *um, int nx, int jee, int nz1) {
-  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (size_t)nx * jee) return;
-  u[q * nz1] = (T)um[q];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nx * jee) return;
+  const int i = q % nx, j = q / nx;
+  u[((size_t)i * jee + j) * nz1] = (T)um[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1300,7 +1313,7 @@ __global__ void outflow_kernel(T *u, const T *v, T *rhsupa, DevCoef c, DevScalar
   __shared__ double red[3][TPB];
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   double a = 0.0, b1 = 0.0, b2 = 0.0;
-  // With FP32 fields: u from the FP64 mean-flow u um [nx][ny+1]; a top-row
+  // With FP32 fields: u from the FP64 mean-flow u um [ny+1][nx]; a top-row
   // v that is the FP32 image of its boundary value (vmagic, set by boun)
   // enters as that FP64 value (the rounding of this smooth profile is
   // correlated along the boundary and would bias the flux).
@@ -1310,8 +1323,8 @@ __global__ void outflow_kernel(T *u, const T *v, T *rhsupa, DevCoef c, DevScalar
     a += (F32 && vt == (T)c.vmagic[i]) ? c.vmagic[i] : (double)vt;
   }
   for (int j = threadIdx.x + 2; j <= ny; j += blockDim.x) {
-    const double u1 = um ? um[j - 1] : (double)u[(size_t)(j - 1) * s];
-    const double un = um ? um[(size_t)(nx - 1) * (ny + 1) + (j - 1)] : (double)u[(size_t)(nx - 1) * pu + (size_t)(j - 1) * s];
+    const double u1 = um ? um[(size_t)(j - 1) * nx] : (double)u[(size_t)(j - 1) * s];
+    const double un = um ? um[(size_t)(j - 1) * nx + (nx - 1)] : (double)u[(size_t)(nx - 1) * pu + (size_t)(j - 1) * s];
     b1 += u1 * c.dy[j - 1];
     b2 += un * c.dy[j - 1];
   }
@@ -1332,7 +1345,7 @@ __global__ void outflow_kernel(T *u, const T *v, T *rhsupa, DevCoef c, DevScalar
   for (int j = threadIdx.x + 2; j <= ny; j += blockDim.x) {
     size_t e = (size_t)(nx - 1) * pu + (size_t)(j - 1) * s;
     if (um) {
-      double &U = um[(size_t)(nx - 1) * (ny + 1) + (j - 1)];
+      double &U = um[(size_t)(j - 1) * nx + (nx - 1)];
       U += alpha;
       u[e] = (T)U;
     } else {
@@ -1443,7 +1456,7 @@ div_pencils_kernel(const T *u, const T *v, const T *w, T *P, DevCoefT<T> c, doub
         // the FP64 mean-flow u um); line (j-1, kr) at P0[(2 (j-1) + kr) N]
         const double wp = w[ow + (kr ^ 1)], kz = cd.kaz[0];
         const double d = (kr ? wp * kz : -wp * kz) + ((double)v[ov + s + kr] - (double)v[ov + kr]) * cd.idy[j];
-        const double du = (um && kr == 0) ? um[(size_t)(i - 1) * (ny + 1) + j] - um[(size_t)(i - 2) * (ny + 1) + j]
+        const double du = (um && kr == 0) ? um[(size_t)j * nx + (i - 1)] - um[(size_t)j * nx + (i - 2)]
                                           : (double)u[ow + kr] - (double)u[ow - pu + kr];
         P0[(size_t)(2 * (j - 1) + kr) * N + x] = d + idx_d * du;
       }
@@ -1464,10 +1477,10 @@ div_pencils_kernel(const T *u, const T *v, const T *w, T *P, DevCoefT<T> c, doub
 // per-mode y tridiagonal on pencils (pois.f90:102-117, soltriy
 // pois.f90:211-239): one thread per (kr, kx-mode x), rows at stride nz1*N.
 __global__ void __launch_bounds__(TPB)
-poisson_y_pencil_kernel(double *P, double *S, DevCoef c, int nx, int ny, int nz1) {
+poisson_y_pencil_kernel(double *P, double *S, DevCoef c, int nx, int ny, int nz1, int kr_end) {
   const int N = nx - 1;
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nz1 * N) return;
+  if (t >= (size_t)kr_end * N) return; // columns kr < kr_end
   const int x = (int)(t % N), kr = (int)(t / N), i = x + 2;
   const size_t rs = (size_t)nz1 * N;
   double *col = P + (size_t)kr * N + x;
@@ -1502,126 +1515,6 @@ poisson_y_pencil_kernel(double *P, double *S, DevCoef c, int nx, int ny, int nz1
   }
 }
 
-// ---------------------------------------------------------------------------
-// mixed-precision Poisson y solves
-// ---------------------------------------------------------------------------
-// The y operator of mode (kz, kx) (pois.f90:102-117) depends on the grid
-// only, so its pivots are factored once, in FP64, and kept as FP32: dl[j] =
-// 1/a2_j of the elimination from row 1 down (soltriy's order, pois.f90:
-// 211-239) and du[j] = 1/c_j of the elimination from row ny-1 up.  Tables
-// [j-1][kz][x] for kz = 0..nz1/2-1, x = kx mode 0..nx-2.
-__global__ void __launch_bounds__(TPB)
-poisson_pivots_kernel(float *dl, float *du, DevCoef c, int nx, int ny, int nzh) {
-  const int N = nx - 1, ny1 = ny - 1;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nzh * N) return;
-  const int x = (int)(t % N), kz = (int)(t / N), i = x + 2;
-  const size_t rs = (size_t)nzh * N;
-  const double sh = c.kaz2[kz] + c.kmod[i - 1];
-  double a2 = C3(c.ayp, 2, 1) - sh;
-  dl[t] = (float)(1.0 / a2);
-  for (int j = 2; j <= ny1; ++j) {
-    const double d = 1.0 / a2;
-    a2 = (C3(c.ayp, 2, j) - sh) - C3(c.ayp, 1, j) * d * C3(c.ayp, 3, j - 1);
-    dl[(size_t)(j - 1) * rs + t] = (float)(1.0 / a2);
-  }
-  double cc = C3(c.ayp, 2, ny1) - sh;
-  du[(size_t)(ny1 - 1) * rs + t] = (float)(1.0 / cc);
-  for (int j = ny1 - 1; j >= 1; --j) {
-    const double d = 1.0 / cc;
-    cc = (C3(c.ayp, 2, j) - sh) - C3(c.ayp, 3, j) * C3(c.ayp, 1, j + 1) * d;
-    du[(size_t)(j - 1) * rs + t] = (float)(1.0 / cc);
-  }
-}
-
-// y solve of the modes kz >= 1 on FP32 pencils P (line (j-1) nz1 + kr, x
-// contiguous): one thread per (kz, kx) solves the Re and Im columns with the
-// FP32 pivots, forms the residual b - A x of that solution in FP64 and adds
-// the FP32 solution of A d = r through the bottom-up factors: one step of
-// iterative refinement.  Rows: a(1,j) x(j-1) + (a(2,j) - kz^2 - kmod) x(j) +
-// a(3,j) x(j+1) = b(j), a = ayp.  S: FP32 scratch of the pencils' size.
-// Threads of kz = 0 clear the Im column (the zero mode is real,
-// pois.f90:102); the Re column of kz = 0 is solved in FP64 on its own pencils.
-__global__ void __launch_bounds__(TPB)
-poisson_y_mixed_kernel(float *P, float *S, const float *__restrict__ dl, const float *__restrict__ du,
-                       DevCoefT<float> cf, DevCoef cd, int nx, int ny, int nz1) {
-  const int N = nx - 1, nzh = nz1 / 2, ny1 = ny - 1;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nzh * N) return;
-  const int x = (int)(t % N), kz = (int)(t / N), i = x + 2;
-  const size_t rs = (size_t)nz1 * N, prs = (size_t)nzh * N;
-  float *pr = P + (size_t)(2 * kz) * N + x, *pim = pr + N;
-  float *sr = S + (size_t)(2 * kz) * N + x, *sim = sr + N;
-  const float *dlp = dl + t, *dup = du + t;
-  if (kz == 0) {
-    for (int j = 0; j < ny1; ++j) pim[(size_t)j * rs] = 0.f;
-    return;
-  }
-  const double shd = cd.kaz2[kz] + cd.kmod[i - 1];
-  // forward elimination, rows ascending: y -> S, b stays in P
-  float yr = pr[0], yi = pim[0];
-  sr[0] = yr;
-  sim[0] = yi;
-  for (int j = 2; j <= ny1; ++j) {
-    const size_t o = (size_t)(j - 1) * rs;
-    const float a1 = -C3(cf.ayp, 1, j) * dlp[(size_t)(j - 2) * prs];
-    yr = pr[o] + a1 * yr;
-    yi = pim[o] + a1 * yi;
-    sr[o] = yr;
-    sim[o] = yi;
-  }
-  // back substitution, rows descending: x -> S; with x(j) known, the FP64
-  // residual of row j+1 and its bottom-up elimination z -> P
-  float xr1 = 0.f, xi1 = 0.f, xr2 = 0.f, xi2 = 0.f; // x(j+1), x(j+2)
-  float br1 = 0.f, bi1 = 0.f;                       // b(j+1)
-  float zr = 0.f, zi = 0.f;                         // z(j+2), then z(j+1)
-  for (int j = ny1; j >= 1; --j) {
-    const size_t o = (size_t)(j - 1) * rs;
-    const float d = dlp[(size_t)(j - 1) * prs];
-    const float a3 = j < ny1 ? C3(cf.ayp, 3, j) : 0.f;
-    const float br = pr[o], bi = pim[o];
-    const float xr = (sr[o] - a3 * xr1) * d, xi = (sim[o] - a3 * xi1) * d;
-    sr[o] = xr;
-    sim[o] = xi;
-    if (j < ny1) {
-      const int q = j + 1;
-      const double a1q = C3(cd.ayp, 1, q), dq = C3(cd.ayp, 2, q) - shd, a3q = q < ny1 ? C3(cd.ayp, 3, q) : 0.0;
-      const double rr = (double)br1 - (a1q * (double)xr + dq * (double)xr1 + a3q * (double)xr2);
-      const double ri = (double)bi1 - (a1q * (double)xi + dq * (double)xi1 + a3q * (double)xi2);
-      const float f = q < ny1 ? C3(cf.ayp, 3, q) * dup[(size_t)q * prs] : 0.f;
-      zr = (float)rr - f * zr;
-      zi = (float)ri - f * zi;
-      pr[o + rs] = zr;
-      pim[o + rs] = zi;
-    }
-    xr2 = xr1;
-    xi2 = xi1;
-    xr1 = xr;
-    xi1 = xi;
-    br1 = br;
-    bi1 = bi;
-  }
-  {
-    const double dq = C3(cd.ayp, 2, 1) - shd, a3q = ny1 > 1 ? C3(cd.ayp, 3, 1) : 0.0;
-    const double rr = (double)br1 - (dq * (double)xr1 + a3q * (double)xr2);
-    const double ri = (double)bi1 - (dq * (double)xi1 + a3q * (double)xi2);
-    const float f = ny1 > 1 ? C3(cf.ayp, 3, 1) * dup[prs] : 0.f;
-    pr[0] = (float)rr - f * zr;
-    pim[0] = (float)ri - f * zi;
-  }
-  // the correction, rows ascending, added to x -> P
-  float dr = 0.f, di = 0.f;
-  for (int j = 1; j <= ny1; ++j) {
-    const size_t o = (size_t)(j - 1) * rs;
-    const float u = dup[(size_t)(j - 1) * prs];
-    const float a1 = j > 1 ? C3(cf.ayp, 1, j) : 0.f;
-    dr = (pr[o] - a1 * dr) * u;
-    di = (pim[o] - a1 * di) * u;
-    pr[o] = sr[o] + dr;
-    pim[o] = sim[o] + di;
-  }
-}
-
 // projection (pois.f90:144-162): p, w, v, u from phi = rest
 // With FP32 fields, P0 holds phi of the mean-flow lines in FP64 (line
 // (j-1, kr) at P0[(2 (j-1) + kr)(nx-1)], planes 2..nx), p0 the FP64
@@ -1631,14 +1524,10 @@ template <typename T>
 __global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T> c, const DevScalars *ds,
                                double vstep, int use_m, double idxx_d, int nx, int ny, int nz1,
                                const double *__restrict__ P0, double *p0, const double *idyy_d, double *um) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t e;
+  int kr, j, i;
+  if (!uelem(nz1, ny + 1, nx, e, kr, j, i) || i < 2) return;
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
-  if (e >= pu * nx) return;
-  int i = (int)(e / pu) + 1;
-  size_t r = e % pu;
-  int j = (int)(r / s) + 1;
-  int kr = (int)(r % s);
-  if (i < 2) return;
   const double vs = use_m ? ds->varstep : vstep;
   if (P0 && kr == 0) {
     // phi(row r, plane q) = P0[2 (r-1) N + q-2]; kaz(0) = 0 leaves w
@@ -1654,7 +1543,7 @@ __global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T
       if (i <= nx - 1) {
         const double du = idxx_d * (P0[(size_t)(j - 2) * rN + (i - 1)] - phm);
         if (um) {
-          double &U = um[(size_t)(i - 1) * (ny + 1) + (j - 1)];
+          double &U = um[(size_t)(j - 1) * nx + (i - 1)];
           U -= du;
           u[e] = (T)U;
         } else {
@@ -1694,9 +1583,8 @@ static void pois_mixed(Box &b, double varstep, bool use_m) {
          fld<float>(b.f[OTBL_W]), P, b.dcf, b.hc.idx, nx, ny, nz1, b.P0, b.dc, (const double *)b.um64);
   kz_dct_pencils<float>(b, P, -1, Lp);
   kz_dct_pencils<double>(b, b.P0, -1, 2 * (ny - 1));
-  LAUNCH(b, "poisson_y", nblk((size_t)(nz1 / 2) * N), TPB, 0, poisson_y_mixed_kernel, P, F, b.pdl, b.pdu, b.dcf,
-         b.dc, nx, ny, nz1);
-  LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.P0, b.S0, b.dc, nx, ny, 2);
+  kp_poisson_y<float, true>(b, P);
+  LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.P0, b.S0, b.dc, nx, ny, 2, 2);
   kz_dct_pencils<float>(b, P, 1, Lp);
   kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1));
   LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pv, b.P0, nz1);
@@ -1717,8 +1605,16 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
     LAUNCH(b, "div", tgrid(b), TPB, 0, div_pencils_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest2,
            b.dc, b.hc.idx, nx, ny, nz1, (double *)nullptr, b.dc, (const double *)nullptr);
     dct_pencils(b, b.rest2, -1);
-    LAUNCH(b, "poisson_y", nblk((size_t)nz1 * N), TPB, 0, poisson_y_pencil_kernel, b.rest2, b.rest, b.dc, nx, ny,
-           nz1);
+    if (b.pchunk) {
+      // modes kz >= 1 partitioned in registers; the pinned mean-flow mode
+      // (kr = 0, 1) marching, one thread per column
+      kp_poisson_y<double, false>(b, b.rest2);
+      LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.rest2, b.rest, b.dc, nx, ny,
+             nz1, 2);
+    } else {
+      LAUNCH(b, "poisson_y", nblk((size_t)nz1 * N), TPB, 0, poisson_y_pencil_kernel, b.rest2, b.rest, b.dc, nx, ny,
+             nz1, nz1);
+    }
     dct_pencils(b, b.rest2, 1);
     LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<double>, b.rest2, b.rest, Lp, N, b.pv,
            (const double *)nullptr, nz1);
@@ -1734,12 +1630,6 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
   LAUNCH(b, "project", nblk(b.nu), TPB, 0, project_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W],
          b.f[OTBL_P], b.rest, b.dc, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1, (const double *)nullptr,
          (double *)nullptr, b.dc.idyy, (double *)nullptr);
-}
-
-void k_poisson_pivots(Box &b) {
-  const int nzh = b.nz1 / 2;
-  LAUNCH(b, "poisson_pivots", nblk((size_t)nzh * (b.nx - 1)), TPB, 0, poisson_pivots_kernel, b.pdl, b.pdu, b.dc,
-         b.nx, b.ny, nzh);
 }
 
 // ---------------------------------------------------------------------------
@@ -1815,7 +1705,7 @@ template <typename T> static void rhsp_t(Box &b, int m) {
     LAUNCH(b, "genflu", 1, TPB, 4 * (size_t)(ny + 2) * sizeof(double), genflu_stats_kernel<T>, resu, b.um, ibuf,
            dbuf, b.dc, b.ds, ny, nz1, b.xout, b.re);
     LAUNCH(b, "genflu", dim3(nblk((size_t)nz1 * (ny + 1)), 3), TPB, 0, genflu_plane_kernel<T>, u, v, w, resu, resv,
-           resw, b.u0, b.v0, ibuf, dbuf, b.dc, b.ds, ny, nz1, b.xout, um);
+           resw, b.u0, b.v0, ibuf, dbuf, b.dc, b.ds, ny, nz1, b.xout, um, nx);
   }
   // plane loop 2 (planes 2..nx-1)
   if (nx > 2) {

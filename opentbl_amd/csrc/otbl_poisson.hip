@@ -1,0 +1,254 @@
+// Per-mode y Poisson solves of pois (pois.f90:102-117, soltriy
+// pois.f90:211-239) for the modes kz >= 1, as partitioned tridiagonal solves
+// held in registers.
+//
+// The pencils hold the x-transformed divergence: row j of line (kz, kx) (Re
+// and Im parts, kr = 2 kz, 2 kz + 1) at P[(j-1) nz1 N + kr N + kx],
+// N = nx - 1.  Row j of the operator of mode (kz, kx):
+//   a(1,j) x(j-1) + (a(2,j) - kz^2 - kmod(kx)) x(j) + a(3,j) x(j+1) = b(j),
+// a = ayp, rows 1..ny-1 (no a(1,1), no a(3,ny-1) term).  For kz >= 1 the
+// operator is strictly diagonally dominant, so chunks factor without
+// pivoting.
+//
+// A workgroup takes 16 consecutive (kz, kx) modes, i.e. 32 real columns
+// (Re and Im, one per lane of a half-wavefront); each of its 32 half-
+// wavefronts takes L consecutive rows of them (chunk k: rows kL+1 .. kL+L;
+// rows past ny-1 are identity rows).  A chunk factors its rows, solves them
+// for the right-hand side y and for the two coupling columns V (the
+// sub-diagonal entry of its first row times e_1) and W (the super-diagonal
+// entry of its last row times e_L), so that x = y - V X(k-1) - W X'(k+1)
+// with X(k-1) the last unknown of chunk k-1 and X'(k+1) the first of chunk
+// k+1.  The first and last rows of every chunk form a 2 x 32 interface
+// system, eliminated by one lane per column through LDS.  Inputs are read
+// once and outputs written once.
+//
+// REF (FP32 pencils): after the solve, the residual b - A x of every row is
+// formed in FP64 (rows of the neighbouring chunks from the interface
+// values) and the FP32 solution of A d = r, through the same factors and
+// spikes, is added: one step of iterative refinement.
+#include "otbl_kcommon.h"
+
+namespace otbl {
+
+static constexpr int PK = 32;     // chunks per column = columns per workgroup
+static constexpr int PROWS = 512; // rows the LDS coefficient copy holds (ny - 1 <= PROWS)
+
+template <int L, typename T, bool REF>
+__global__ void __launch_bounds__(1024)
+poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, int nz1) {
+  // interface system per chunk and lane column: yF, yL, VF, VL, WF, WL; then
+  // the interface unknowns F, L
+  __shared__ T red[PK][6][PK];
+  __shared__ T itf[PK][2][PK];
+  // row coefficients of the operator, rows 0 .. PK L: a(1,r) (0 where absent),
+  // a(2,r) (1 past ny-1), a(3,r) (0 where absent); FP64 copies for REF
+  __shared__ T ca1[PK * L + 2], ca2[PK * L + 2], ca3[PK * L + 2];
+  __shared__ double da1[REF ? PK * L + 2 : 1], da2[REF ? PK * L + 2 : 1], da3[REF ? PK * L + 2 : 1];
+  const int N = nx - 1, nzh = nz1 / 2, ny1 = ny - 1;
+  for (int r = threadIdx.x; r <= PK * L; r += blockDim.x) {
+    const bool in = r >= 1 && r <= ny1;
+    ca1[r] = (in && r >= 2) ? C3(c.ayp, 1, r) : T(0);
+    ca2[r] = in ? C3(c.ayp, 2, r) : T(1);
+    ca3[r] = (in && r < ny1) ? C3(c.ayp, 3, r) : T(0);
+    if constexpr (REF) {
+      da1[r] = (in && r >= 2) ? C3(cd.ayp, 1, r) : 0.0;
+      da2[r] = in ? C3(cd.ayp, 2, r) : 1.0;
+      da3[r] = (in && r < ny1) ? C3(cd.ayp, 3, r) : 0.0;
+    }
+  }
+  __syncthreads();
+  // lane column: the Re (even) or Im (odd) part of one of 16 (kz, kx) modes
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & (PK - 1), k = wave * 2 + (lane >> 5);
+  const size_t t = (size_t)N + (size_t)blockIdx.x * (PK / 2) + (col >> 1); // kz >= 1 only
+  const bool live = t < (size_t)nzh * N;
+  const int kz = live ? (int)(t / N) : 1, x = live ? (int)(t % N) : 0;
+  // 32-bit byte offsets from the (uniform) pencil base: the pencils are
+  // smaller than 4 GiB (checked by the launcher)
+  const unsigned rs = (unsigned)(nz1 * N * sizeof(T));
+  const unsigned base = (unsigned)(((2 * kz + (col & 1)) * N + x) * sizeof(T));
+  const int r0 = k * L + 1;
+  char *Pc = reinterpret_cast<char *>(P);
+  auto AT = [&](int q) -> T & { return *reinterpret_cast<T *>(Pc + (base + (unsigned)(r0 + q - 1) * rs)); };
+  const T sh = c.kaz2[kz] + c.kmod[x + 1];
+  auto A1 = [&](int r) -> T { return ca1[r]; };
+  auto A3 = [&](int r) -> T { return ca3[r]; };
+  auto DG = [&](int r) -> T { return r <= ny1 ? ca2[r] - sh : T(1); };
+
+  // chunk factorisation: inverse pivots
+  T ip[L];
+#pragma unroll
+  for (int q = 0; q < L; ++q) {
+    const int r = r0 + q;
+    T piv = DG(r);
+    if (q > 0) piv -= A1(r) * ip[q - 1] * A3(r - 1);
+    ip[q] = T(1) / piv;
+  }
+  // right-hand side, read once
+  T xv[L];
+#pragma unroll
+  for (int q = 0; q < L; ++q) xv[q] = (live && r0 + q <= ny1) ? AT(q) : T(0);
+  // local solve in place
+  auto local_solve = [&](T(&v)[L]) {
+#pragma unroll
+    for (int q = 1; q < L; ++q) v[q] -= A1(r0 + q) * ip[q - 1] * v[q - 1];
+    v[L - 1] *= ip[L - 1];
+#pragma unroll
+    for (int q = L - 2; q >= 0; --q) v[q] = (v[q] - A3(r0 + q) * v[q + 1]) * ip[q];
+  };
+  local_solve(xv);
+  __builtin_amdgcn_sched_barrier(0);
+  // spikes V (left coupling) and W (right coupling): only their end values
+  // are kept; the arrays are formed again where they are applied
+  auto spike_v = [&](T(&v)[L]) {
+    v[0] = A1(r0);
+#pragma unroll
+    for (int q = 1; q < L; ++q) v[q] = -A1(r0 + q) * ip[q - 1] * v[q - 1];
+    v[L - 1] *= ip[L - 1];
+#pragma unroll
+    for (int q = L - 2; q >= 0; --q) v[q] = (v[q] - A3(r0 + q) * v[q + 1]) * ip[q];
+  };
+  auto spike_w = [&](T(&w)[L]) {
+    w[L - 1] = A3(r0 + L - 1) * ip[L - 1];
+#pragma unroll
+    for (int q = L - 2; q >= 0; --q) w[q] = -A3(r0 + q) * w[q + 1] * ip[q];
+  };
+  T VF, VL, WF, WL;
+  {
+    T v[L];
+    spike_v(v);
+    VF = v[0];
+    VL = v[L - 1];
+  }
+  {
+    T w[L];
+    spike_w(w);
+    WF = w[0];
+    WL = w[L - 1];
+  }
+  // x -= V XL + W XF, one spike array alive at a time
+  auto combine = [&](T(&v_)[L], T XL, T XF) {
+    {
+      T v[L];
+      spike_v(v);
+#pragma unroll
+      for (int q = 0; q < L; ++q) v_[q] -= v[q] * XL;
+    }
+    {
+      T w[L];
+      spike_w(w);
+#pragma unroll
+      for (int q = 0; q < L; ++q) v_[q] -= w[q] * XF;
+    }
+  };
+  // interface system: one lane per lane column (wave 0, lanes 0..31)
+  // eliminates over the chunks; every chunk then reads the unknowns it
+  // couples to
+  auto interface_solve = [&](T yF, T yL, T &XL, T &XF) {
+    red[k][0][col] = yF;
+    red[k][1][col] = yL;
+    red[k][2][col] = VF;
+    red[k][3][col] = VL;
+    red[k][4][col] = WF;
+    red[k][5][col] = WL;
+    __syncthreads();
+    if (wave == 0 && lane < PK) {
+      // L(q) = al(q) + be(q) F(q+1), F(q) = ga(q) + de(q) F(q+1)
+      T al = 0, be = 0;
+#pragma unroll 1
+      for (int q = 0; q < PK; ++q) {
+        const T qyF = red[q][0][col], qyL = red[q][1][col], qVF = red[q][2][col], qVL = red[q][3][col];
+        const T qWF = red[q][4][col], qWL = red[q][5][col];
+        const T den = T(1) / (T(1) + qVF * be);
+        const T ga = (qyF - qVF * al) * den, de = -qWF * den;
+        const T nal = qyL - qVL * (al + be * ga), nbe = -qVL * be * de - qWL;
+        red[q][0][col] = ga;
+        red[q][1][col] = de;
+        red[q][2][col] = nal;
+        red[q][3][col] = nbe;
+        al = nal;
+        be = nbe;
+      }
+      T Fn = 0; // F(q+1)
+#pragma unroll 1
+      for (int q = PK - 1; q >= 0; --q) {
+        itf[q][1][col] = red[q][2][col] + red[q][3][col] * Fn; // L(q)
+        Fn = red[q][0][col] + red[q][1][col] * Fn;
+        itf[q][0][col] = Fn; // F(q)
+      }
+    }
+    __syncthreads();
+    XL = k > 0 ? itf[k - 1][1][col] : T(0);
+    XF = k < PK - 1 ? itf[k + 1][0][col] : T(0);
+    __syncthreads(); // red / itf are reused by a second solve
+  };
+  T XL, XF;
+  interface_solve(xv[0], xv[L - 1], XL, XF);
+  combine(xv, XL, XF);
+  if constexpr (REF) {
+    // residual of every row in FP64, b read again
+    const double shd = cd.kaz2[kz] + cd.kmod[x + 1];
+    T d[L];
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+      // one row at a time: its loads are not hoisted above the previous row
+      __builtin_amdgcn_sched_barrier(0);
+      const int r = r0 + q;
+      const bool in = live && r <= ny1;
+      const double b = in ? (double)AT(q) : 0.0;
+      const double pm = q > 0 ? (double)xv[q - 1] : (double)XL, pp = q < L - 1 ? (double)xv[q + 1] : (double)XF;
+      const double dg = r <= ny1 ? da2[r] - shd : 1.0;
+      d[q] = in ? (T)(b - (da1[r] * pm + dg * (double)xv[q] + da3[r] * pp)) : T(0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    local_solve(d);
+    T dL, dF;
+    interface_solve(d[0], d[L - 1], dL, dF);
+#pragma unroll
+    for (int q = 0; q < L; ++q) xv[q] += d[q];
+    combine(xv, dL, dF);
+  }
+#pragma unroll
+  for (int q = 0; q < L; ++q)
+    if (live && r0 + q <= ny1) AT(q) = xv[q];
+}
+
+// rows per chunk for ny-1 rows (0: not instantiated)
+int kp_chunk_rows(int ny) {
+  if (ny - 1 > PROWS) return 0;
+  const int need = (ny - 1 + PK - 1) / PK;
+  for (int L : {2, 4, 6, 8, 10, 12, 14, 16})
+    if (L >= need) return L;
+  return 0;
+}
+
+template <typename T, bool REF> void kp_poisson_y(Box &b, T *P) {
+  const int N = b.nx - 1, nzh = b.nz1 / 2;
+  const size_t cols = (size_t)(nzh - 1) * N;
+  if (cols == 0) return;
+  if ((size_t)(b.ny - 1) * b.nz1 * N * sizeof(T) >= (size_t(1) << 32))
+    throw std::invalid_argument("Poisson pencils of 4 GiB or more");
+  const unsigned grid = (unsigned)((cols + PK / 2 - 1) / (PK / 2)); // 16 modes (Re, Im) per workgroup
+  switch (kp_chunk_rows(b.ny)) {
+#define PCASE(LL)                                                                                          \
+  case LL: {                                                                                               \
+    auto kf = &poisson_y_chunk<LL, T, REF>;                                                                \
+    LAUNCH(b, "poisson_y", grid, 1024, 0, kf, P, coefs<T>(b), b.dc, b.nx, b.ny, b.nz1);                    \
+  } break;
+    PCASE(2)
+    PCASE(4)
+    PCASE(6)
+    PCASE(8)
+    PCASE(10)
+    PCASE(12)
+    PCASE(14)
+    PCASE(16)
+#undef PCASE
+  default: throw std::invalid_argument("Poisson chunk length not instantiated");
+  }
+}
+
+template void kp_poisson_y<double, false>(Box &, double *);
+template void kp_poisson_y<float, true>(Box &, float *);
+
+} // namespace otbl

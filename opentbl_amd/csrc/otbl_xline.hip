@@ -36,7 +36,22 @@ interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict_
     for (int kk = 0; kk < XK; ++kk) {
       const int qq = it + kk;
       if (qq >= nb) break;
-      // forward elimination over block qq
+      // forward elimination over block qq; a block of interior planes
+      // issues its XB loads before any use (one basic block)
+      const int i0 = qq * XB + 1;
+      if (i0 >= 2 && i0 + XB - 1 <= nx - 1) {
+        T gn[XB];
+#pragma unroll
+        for (int b = 0; b < XB; ++b) gn[b] = g[(size_t)(i0 + b) * P]; // g(i+1)
+#pragma unroll
+        for (int b = 0; b < XB; ++b) {
+          const int i = i0 + b;
+          const T rhs = C2(econ, 1, i) * gi + C2(econ, 2, i) * gn[b];
+          gi = gn[b];
+          y = rhs + C3(icon, 1, i) * y;
+          ring[kk][b] = y;
+        }
+      } else
 #pragma unroll
       for (int b = 0; b < XB; ++b) {
         const int i = qq * XB + b + 1;
@@ -124,6 +139,29 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
     for (int kk = 0; kk < XK; ++kk) {
       const int qq = it + kk;
       if (qq >= nb) break;
+      const int i0 = qq * XB + 1;
+      if (i0 >= 2 && i0 + XB - 1 <= nx - 1) {
+        // interior block: its 2 XB loads issued before any use
+        T gb[XB], qb[XB];
+#pragma unroll
+        for (int b = 0; b < XB; ++b) {
+          gb[b] = g[(size_t)(i0 + b - 1) * P]; // g(i)
+          qb[b] = v[(size_t)(i0 + b) * P];     // q(i+1)
+        }
+#pragma unroll
+        for (int b = 0; b < XB; ++b) {
+          const int i = i0 + b;
+          const T rc = C2(econ, 1, i) * gim1 + C2(econ, 2, i) * gb[b];
+          const T rv = C3(econv, 1, i) * qb[b] + C3(econv, 2, i) * qi + C3(econv, 3, i) * qim1;
+          yc = rc + C3(icon, 1, i) * yc;
+          yv = rv + C3(iconv, 1, i) * yv;
+          gim1 = gb[b];
+          qim1 = qi;
+          qi = qb[b];
+          rc_[kk][b] = yc;
+          rv_[kk][b] = yv;
+        }
+      } else
 #pragma unroll
       for (int b = 0; b < XB; ++b) {
         const int i = qq * XB + b + 1;

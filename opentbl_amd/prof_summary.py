@@ -19,8 +19,19 @@ import re
 
 
 def _short(name):
-    m = re.match(r"(?:otbl::)?([A-Za-z_][\w]*)", name)
-    return m.group(1) if m else name
+    """Kernel name without return type, namespace and argument list, with
+    its template arguments: "rk_final_y<20, float>"."""
+    n = re.sub(r"^void\s+", "", name.strip())
+    n = n.replace("otbl::", "")
+    depth = 0
+    for k, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return n[:k].strip()
+    return n
 
 
 def _one(d, pattern):
