@@ -55,7 +55,7 @@ def kernel_model(cfg, s=8):
     m["rhsu_loop1"] = ((Rv + 2 * Ru) * f, 0.0, 1)
     m["rhs_loop2"] = ((X - 2) * z * s * (2 * Ru + 2 * Rv + 2 * Ru + Ru + Rv + Ru + Ru + Rv), 0.0, 1)
     m["rk_final"] = ((6 * Ru + 6 * Rv + 6 * Ru) * f / 3, 0.0, 3)
-    m["save_predict"] = ((3 * 3 * Ru + Rv) * f / 1, 0.0, 1)  # u,v,w read, res write, u,v,w write, p
+    m["save_predict"] = ((4 * Ru + 3 * Rv) * f, 0.0, 1)  # u, v, w, p read; predicted u, v, w written
     N, L = cfg.nx - 1, (cfg.ny - 1) * z
     M = 1 << (2 * N - 2).bit_length()  # power-of-two Bluestein length >= 2N-1
     # Poisson x transforms: one pencil read and written; Bluestein FFTs of

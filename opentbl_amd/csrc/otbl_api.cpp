@@ -282,7 +282,8 @@ static double window_decay(std::vector<double> a, int n) {
 static void free_box(Box *b) {
   if (!b) return;
   for (int q = 0; q < 4; ++q)
-    if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
+    for (int f = 0; f < 2; ++f)
+      if (b->gexec[q][f]) (void)hipGraphExecDestroy(b->gexec[q][f]);
   double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
                     b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
@@ -315,7 +316,11 @@ static void run_substep(Box &b, int m) {
     enqueue_substep(b, m);
     return;
   }
-  if (!b.gexec[m]) {
+  // the substep swaps the velocity fields with their saved copies: one graph
+  // per role assignment, and the host-side swap done again on replay
+  const int fl = b.flip;
+  hipGraphExec_t &ge = b.gexec[m][fl];
+  if (!ge) {
     hipGraph_t g;
     OTBL_HIP(hipStreamBeginCapture(b.s, hipStreamCaptureModeThreadLocal));
     try {
@@ -323,13 +328,16 @@ static void run_substep(Box &b, int m) {
     } catch (...) {
       hipGraph_t dummy;
       (void)hipStreamEndCapture(b.s, &dummy);
+      if (b.flip != fl) k_swap_saved(b);
       throw;
     }
     OTBL_HIP(hipStreamEndCapture(b.s, &g));
-    OTBL_HIP(hipGraphInstantiate(&b.gexec[m], g, nullptr, nullptr, 0));
+    OTBL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     OTBL_HIP(hipGraphDestroy(g));
+    if (b.flip != fl) k_swap_saved(b); // capture ran the host side only
   }
-  OTBL_HIP(hipGraphLaunch(b.gexec[m], b.s));
+  OTBL_HIP(hipGraphLaunch(ge, b.s));
+  k_swap_saved(b);
 }
 
 } // namespace otbl

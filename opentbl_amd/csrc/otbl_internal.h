@@ -134,7 +134,10 @@ struct Box {
   double *dct_chirp = nullptr, *dct_bf = nullptr, *dct_tw4 = nullptr;
   // graphs
   bool use_graphs = false;
-  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  // one graph per substep m and buffer role assignment (flip: the velocity
+  // fields and their saved copies swap roles at every substep)
+  hipGraphExec_t gexec[4][2] = {};
+  int flip = 0;
   // measurement
   bool kprof = false;
   struct KStat { int n = 0; double ms = 0.0; };
@@ -165,6 +168,7 @@ void k_substep_rhsp(Box &b, int m);
 void k_outflow(Box &b);
 void k_pois(Box &b, int m, double varstep, bool use_m);
 void k_end_step(Box &b);
+void k_swap_saved(Box &b);
 // partitioned Poisson y solves of the modes kz >= 1 on pencils
 // (otbl_poisson.hip); REF: one FP64-residual refinement step
 int kp_chunk_rows(int ny);

@@ -918,28 +918,31 @@ __global__ void rhsu_loop1_kernel(const double *__restrict__ uv, const double *_
 }
 
 // Save the velocities (rhsp.F90:281-285) and apply the pressure-gradient
-// predictor (rhsp.F90:291-339) in one pass over u-type elements.
+// predictor (rhsp.F90:291-339) in one pass over u-type elements.  The
+// velocities u, v, w are left in place as the saved copy; the predicted
+// velocities (or the copies, outside the predictor's rows and planes) are
+// written to un, vn, wn, and the caller swaps the roles of the two sets.
 // With FP32 fields: p0, the FP64 mean-flow pressure [nx][ny], whose
 // gradients replace those of p's mode-0 real part; um, the FP64 mean-flow u
 // [ny+1][nx] (mode-0 real part), updated in FP64 and saved to rm (else
 // all null).
 template <typename T>
-__global__ void save_predict_kernel(T *u, T *v, T *w, const T *__restrict__ p, T *resu, T *resv, T *resw,
-                                    const T *idyy, const T *kaz, const DevScalars *ds, double idxx_d, int nx,
-                                    int ny, int nz1, const double *__restrict__ p0, const double *idyy_d,
-                                    double *um, double *rm) {
+__global__ void save_predict_kernel(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+                                    const T *__restrict__ p, T *__restrict__ un, T *__restrict__ vn,
+                                    T *__restrict__ wn, const T *idyy, const T *kaz, const DevScalars *ds,
+                                    double idxx_d, int nx, int ny, int nz1, const double *__restrict__ p0,
+                                    const double *idyy_d, double *um, double *rm) {
   size_t e;
   int kr, j, i;
   if (!uelem(nz1, ny + 1, nx, e, kr, j, i)) return;
   const size_t pu = (size_t)nz1 * (ny + 1), pv = (size_t)nz1 * ny;
   const T var1 = (T)ds->var1, idxx = (T)idxx_d;
   size_t ev = (size_t)(i - 1) * pv + (size_t)(j - 1) * nz1 + kr;
-  T uu = u[e], ww = w[e];
-  resu[e] = uu;
-  resw[e] = ww;
+  const T uu = u[e], ww = w[e];
   const bool mean = p0 && kr == 0;
   const double *pm = mean ? p0 + (size_t)(i - 1) * ny + (j - 1) : nullptr; // mean pressure at (i, j)
   const bool upred = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny;
+  T uo = uu;
   if (um && kr == 0) {
     const size_t q = (size_t)(j - 1) * nx + (i - 1);
     double U = um[q];
@@ -947,30 +950,34 @@ __global__ void save_predict_kernel(T *u, T *v, T *w, const T *__restrict__ p, T
     if (upred) {
       U -= ds->var1 * idxx_d * (pm[ny] - pm[0]);
       um[q] = U;
-      u[e] = (T)U;
+      uo = (T)U;
     }
   } else if (upred) {
     if (mean)
-      u[e] = (T)((double)uu - ds->var1 * idxx_d * (pm[ny] - pm[0]));
+      uo = (T)((double)uu - ds->var1 * idxx_d * (pm[ny] - pm[0]));
     else
-      u[e] = uu - var1 * idxx * (p[ev + pv] - p[ev]);
+      uo = uu - var1 * idxx * (p[ev + pv] - p[ev]);
   }
+  un[e] = uo;
+  T wo = ww;
   if (i >= 2 && j >= 2 && j <= ny) {
     // w -= var1 * (i kz p): Re gets -(-kz Im p), Im gets -(kz Re p)
     T kz = kaz[kr / 2];
     T pp = p[ev ^ 1];
     T t = (kr & 1) ? (kz * pp) : (-kz * pp);
-    w[e] = ww - var1 * t;
+    wo = ww - var1 * t;
   }
+  wn[e] = wo;
   if (j <= ny) {
-    T vv = v[ev];
-    resv[ev] = vv;
+    const T vv = v[ev];
+    T vo = vv;
     if (i >= 2 && j >= 2 && j <= ny - 1) {
       if (mean)
-        v[ev] = (T)((double)vv - ds->var1 * idyy_d[j] * (pm[1] - pm[0]));
+        vo = (T)((double)vv - ds->var1 * idyy_d[j] * (pm[1] - pm[0]));
       else
-        v[ev] = vv - var1 * idyy[j] * (p[ev + nz1] - p[ev]);
+        vo = vv - var1 * idyy[j] * (p[ev + nz1] - p[ev]);
     }
+    vn[ev] = vo;
   }
 }
 
@@ -1635,6 +1642,15 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
 // ---------------------------------------------------------------------------
 // rhsp (rhsp.F90:22-489)
 // ---------------------------------------------------------------------------
+// Swap the roles of the velocity fields and their saved copies (after
+// save_predict_kernel); a captured substep applies the same swap on replay.
+void k_swap_saved(Box &b) {
+  std::swap(b.f[OTBL_U], b.resu);
+  std::swap(b.f[OTBL_V], b.resv);
+  std::swap(b.f[OTBL_W], b.resw);
+  b.flip ^= 1;
+}
+
 // The substep in the element type T of the fields: double (OTBL_FP64, with
 // the one-thread-per-line fallbacks) or float (OTBL_MIXED, fast paths only;
 // statistics, CFL maxima and the recycling profiles stay FP64).
@@ -1698,6 +1714,11 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   double *um = F64 ? nullptr : b.um64; // the FP64 mean-flow u of the mixed substep
   LAUNCH(b, "save_predict", nblk(b.nu), TPB, 0, save_predict_kernel<T>, u, v, w, p, resu, resv, resw, c.idyy, c.kaz,
          b.ds, b.hc.idxx, nx, ny, nz1, F64 ? nullptr : (const double *)b.p0, b.dc.idyy, um, b.rm64);
+  // the predicted velocities now live in the former save buffers
+  k_swap_saved(b);
+  std::swap(u, resu);
+  std::swap(v, resv);
+  std::swap(w, resw);
   // recycling inflow (rhsp.F90:287)
   {
     int *ibuf = (int *)(b.gwork + (size_t)nz1 * (ny + 1));
