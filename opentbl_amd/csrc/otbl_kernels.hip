@@ -1044,64 +1044,99 @@ __global__ void genflu_stats_kernel(const T *resu, double *um, int *ibuf, double
   int *kl = ibuf;                  // kl[2][ny+1]
   int *ntop = ibuf + 2 * (ny + 1); // ntop[2]
   double *fsca = dbuf;             // fsca[2][ny+1]
-  // the sequential profile work below runs in LDS: y, um, ysca, wlwa
+  // the profile work runs in LDS: y, um and, per profile kk, ysca, wlwa and
+  // the dobleu weights yyy ([2][ny+2] each)
   extern __shared__ double gsh[];
-  double *y = gsh, *ums = gsh + (ny + 2), *ysca = gsh + 2 * (ny + 2), *wlwa = gsh + 3 * (ny + 2);
+  const int S_ = ny + 2;
+  double *y = gsh, *ums = gsh + S_, *ysca = gsh + 2 * S_, *wlwa = gsh + 4 * S_, *yyy = gsh + 6 * S_;
+  __shared__ double sh_g[3], sh_www1[2];
   for (int j = threadIdx.x; j < ny + 2; j += blockDim.x) {
     y[j] = c.y[j];
     ums[j] = j <= ny ? um[j] : 0.0;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double *dy = c.dy;
-  double utau = sqrt(fabs((ums[1] - ums[0]) / (y[2] - y[1]) / re));
-  double uinf = 0.0;
-  for (int j = ny + 1 - 20; j <= ny + 1 - 5; ++j) uinf += ums[j - 1];
-  uinf /= 16.0;
-  double u99 = (double)0.99f * uinf;
-  int jtop = 0;
-  for (int j = 1; j <= ny - 2; ++j) {
-    if (ums[j - 1] >= u99) break;
-    jtop = j + 1;
-  }
-  double sum = 0.0;
-  for (int j = 2; j <= jtop + 3; ++j) sum += (uinf - ums[j - 1]) * ums[j - 1] * dy[j - 1];
-  double rthout = sum * re / uinf;
-  double g2 = ds->rthin / rthout, g1 = pow(g2, 0.125);
-  ds->utauout = utau;
-  ds->dout = y[jtop];
-  ds->rthout = rthout;
-  ds->gamma1 = g1;
-  ds->gamma2 = g2;
-  for (int kk = 0; kk < 2; ++kk) {
-    for (int j = 1; j <= ny; ++j) ysca[j] = (kk == 0) ? 0.5 * (y[j] + y[j + 1]) / ds->din : y[j] / ds->din;
-    // dobleu (genflu.F90:173-199)
-    const double ag = 4.0, bg = 0.2;
-    double bg2 = 1.0 - 2.0 * bg, tag = 1.0 / tanh(ag), www1 = (double)0.7895f;
-    wlwa[1] = 0.0;
-    for (int q = 2; q <= ny; ++q) {
-      double xxx = 0.5 * (ysca[q] + ysca[q - 1]);
-      double yyy = (xxx >= 1.0) ? 1.0 : 0.5 * (1.0 + tag * tanh(ag * (xxx - bg) / (bg2 * xxx + bg)));
-      wlwa[q] = wlwa[q - 1] + yyy * (ysca[q] - ysca[q - 1]);
-      if (ysca[q] >= 1.0 && ysca[q - 1] < 1.0) www1 = wlwa[q - 1] + (1.0 - ysca[q - 1]) * yyy;
+  if (threadIdx.x == 0) {
+    const double *dy = c.dy;
+    double utau = sqrt(fabs((ums[1] - ums[0]) / (y[2] - y[1]) / re));
+    double uinf = 0.0;
+    for (int j = ny + 1 - 20; j <= ny + 1 - 5; ++j) uinf += ums[j - 1];
+    uinf /= 16.0;
+    double u99 = (double)0.99f * uinf;
+    int jtop = 0;
+    for (int j = 1; j <= ny - 2; ++j) {
+      if (ums[j - 1] >= u99) break;
+      jtop = j + 1;
     }
-    for (int q = 1; q <= ny; ++q) wlwa[q] = ysca[q] / g1 + (1.0 / g2 - 1.0 / g1) / www1 * wlwa[q];
+    double sum = 0.0;
+    for (int j = 2; j <= jtop + 3; ++j) sum += (uinf - ums[j - 1]) * ums[j - 1] * dy[j - 1];
+    double rthout = sum * re / uinf;
+    double g2 = ds->rthin / rthout, g1 = pow(g2, 0.125);
+    ds->utauout = utau;
+    ds->dout = y[jtop];
+    ds->rthout = rthout;
+    ds->gamma1 = g1;
+    ds->gamma2 = g2;
+    sh_g[0] = g1;
+    sh_g[1] = g2;
+    sh_g[2] = ds->din;
+  }
+  __syncthreads();
+  const double g1 = sh_g[0], g2 = sh_g[1], din = sh_g[2];
+  // both profiles' scaled rows
+  for (int e = threadIdx.x; e < 2 * ny; e += blockDim.x) {
+    const int kk = e / ny, j = e % ny + 1;
+    ysca[kk * S_ + j] = (kk == 0) ? 0.5 * (y[j] + y[j + 1]) / din : y[j] / din;
+  }
+  __syncthreads();
+  // dobleu (genflu.F90:173-199): the blending weight of every interval,
+  // then its running integral and w1 per profile in row order
+  const double ag = 4.0, bg = 0.2;
+  const double bg2 = 1.0 - 2.0 * bg, tag = 1.0 / tanh(ag);
+  for (int e = threadIdx.x; e < 2 * (ny - 1); e += blockDim.x) {
+    const int kk = e / (ny - 1), q = e % (ny - 1) + 2;
+    const double *ys = ysca + kk * S_;
+    const double xxx = 0.5 * (ys[q] + ys[q - 1]);
+    yyy[kk * S_ + q] = (xxx >= 1.0) ? 1.0 : 0.5 * (1.0 + tag * tanh(ag * (xxx - bg) / (bg2 * xxx + bg)));
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int kk = threadIdx.x;
+    const double *ys = ysca + kk * S_, *yy = yyy + kk * S_;
+    double *wl = wlwa + kk * S_;
+    double www1 = (double)0.7895f;
+    wl[1] = 0.0;
+    for (int q = 2; q <= ny; ++q) {
+      wl[q] = wl[q - 1] + yy[q] * (ys[q] - ys[q - 1]);
+      if (ys[q] >= 1.0 && ys[q - 1] < 1.0) www1 = wl[q - 1] + (1.0 - ys[q - 1]) * yy[q];
+    }
+    sh_www1[kk] = www1;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 2 * ny; e += blockDim.x) {
+    const int kk = e / ny, q = e % ny + 1;
+    double *wl = wlwa + kk * S_;
+    wl[q] = ysca[kk * S_ + q] / g1 + (1.0 / g2 - 1.0 / g1) / sh_www1[kk] * wl[q];
+    kl[kk * (ny + 1) + q] = 0;
+    fsca[kk * (ny + 1) + q] = 0.0;
+  }
+  __syncthreads();
+  // the interpolation tables: rows in order, the bisection carrying its
+  // lower bound from row to row (one thread per profile)
+  if (threadIdx.x < 2) {
+    const int kk = threadIdx.x;
+    const double *ys = ysca + kk * S_, *wl = wlwa + kk * S_;
     int kll = 1, khh;
     ntop[kk] = 0;
     for (int q = 1; q <= ny; ++q) {
-      kl[kk * (ny + 1) + q] = 0;
-      fsca[kk * (ny + 1) + q] = 0.0;
-    }
-    for (int q = 1; q <= ny; ++q) {
-      if (wlwa[q] <= ysca[ny]) {
+      if (wl[q] <= ys[ny]) {
         ntop[kk] = q;
         khh = ny;
         while (khh - kll > 1) {
           int km = (khh + kll) / 2;
-          if (ysca[km] >= wlwa[q]) khh = km; else kll = km;
+          if (ys[km] >= wl[q]) khh = km; else kll = km;
         }
         kl[kk * (ny + 1) + q] = kll;
-        fsca[kk * (ny + 1) + q] = (wlwa[q] - ysca[kll]) / (ysca[khh] - ysca[kll]);
+        fsca[kk * (ny + 1) + q] = (wl[q] - ys[kll]) / (ys[khh] - ys[kll]);
       }
     }
   }
@@ -1286,21 +1321,92 @@ __global__ void rk_final_kernel(double *u, const double *res, const double *rhs,
 // result into the FP32 field afterwards.  One thread per plane, arrays
 // [ny+1][nx] (adjacent planes adjacent); tm: scratch of that size.
 template <typename T>
-__global__ void mean_rk_final_kernel(double *um, const double *rm, const T *rhs, const T *rhsp, double *tm,
-                                     DevCoef c, const double *f, const DevScalars *ds, int ny, int nz1, int nx) {
+__global__ void mean_rk_final_kernel(double *__restrict__ um, const double *__restrict__ rm, const T *rhs,
+                                     const T *rhsp, double *__restrict__ tm, DevCoef c, const double *f,
+                                     const DevScalars *ds, int ny, int nz1, int nx) {
+  constexpr int RB = 8; // rows whose loads are issued together
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= nx || ip == 0) return; // plane 1 keeps u
   const int jee = ny + 1;
   const size_t pu = (size_t)nz1 * jee;
   double *U = um + ip, *t = tm + ip;
+  const double *R = rm + ip;
   const T *rh = rhs + ip * pu, *rp = rhsp + ip * pu;
   const double var2 = (ip == nx - 1) ? 0.0 : ds->var2, var3 = ds->var3, var4 = ds->var4;
-  vistyy_col(rm + ip, t, nx, jee, ny, c);
-  for (int j = 0; j < jee; ++j) {
-    const size_t o = (size_t)j * nx;
-    t[o] = U[o] - (var3 * (double)rh[(size_t)j * nz1] + var2 * t[o] + var4 * (double)rp[(size_t)j * nz1]);
+  auto O = [&](int j) { return (size_t)(j - 1) * nx; }; // row j of a [ny+1][nx] array
+  // batched sweep over rows j = from, from + dir, ..., to (inclusive): the
+  // loads of RB rows first, then their updates in order
+  auto sweep = [&](int from, int to, int dir, auto load, auto step) {
+    using V = decltype(load(from));
+    int j = from;
+    const int n = (to - from) * dir + 1;
+    int done = 0;
+    for (; done + RB <= n; done += RB, j += RB * dir) {
+      V v[RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) v[q] = load(j + q * dir);
+#pragma unroll
+      for (int q = 0; q < RB; ++q) step(j + q * dir, v[q]);
+    }
+    for (; done < n; ++done, j += dir) step(j, load(j));
+  };
+  // vistyy of the saved u (lip.f90:447-513): right-hand side and forward
+  // elimination, rows ascending (rm rows j-1, j, j+1 rolled in registers)
+  const double *icon = c.viyu, *econ = c.cofvyu;
+  double rjm = R[O(1)], rj = R[O(2)];
+  double y = M4(c.cofvby, 3, 1) * R[O(1)] + M4(c.cofvby, 3, 2) * R[O(2)] + M4(c.cofvby, 3, 3) * R[O(3)] +
+             M4(c.cofvby, 3, 4) * R[O(4)];
+  t[O(1)] = y;
+  sweep(2, jee - 1, 1, [&](int j) { return R[O(j + 1)]; }, [&](int j, double rjp) {
+    const double o = C3(econ, 1, j) * rjp + C3(econ, 2, j) * rj + C3(econ, 3, j) * rjm;
+    y = o + C3(icon, 1, j) * y;
+    t[O(j)] = y;
+    rjm = rj;
+    rj = rjp;
+  });
+  {
+    const double o = M4(c.cofvby, 4, 1) * R[O(jee)] + M4(c.cofvby, 4, 2) * R[O(jee - 1)] +
+                     M4(c.cofvby, 4, 3) * R[O(jee - 2)] + M4(c.cofvby, 4, 4) * R[O(jee - 3)];
+    y = o + C3(icon, 1, jee) * y;
   }
-  implzy_col(U, t, nx, jee, c.vyui, f);
+  // back substitution (rows descending) and the explicit RK3 combination
+  // (rhsp.F90:448-475): t = u - (var3 rhs + var2 vistyy + var4 rhs_prev)
+  double x = y * C3(icon, 2, jee);
+  t[O(jee)] = U[O(jee)] - (var3 * (double)rh[(size_t)(jee - 1) * nz1] + var2 * x +
+                           var4 * (double)rp[(size_t)(jee - 1) * nz1]);
+  struct Row4 { double y, u, r, q; };
+  sweep(jee - 1, 1, -1,
+        [&](int j) {
+          return Row4{t[O(j)], U[O(j)], (double)rh[(size_t)(j - 1) * nz1], (double)rp[(size_t)(j - 1) * nz1]};
+        },
+        [&](int j, Row4 w) {
+          x = (w.y - C3(icon, 3, j) * x) * C3(icon, 2, j);
+          t[O(j)] = w.u - (var3 * w.r + var2 * x + var4 * w.q);
+        });
+  // implzy (lip.f90:528-597): interior rows 2..jee-1 of u; rows 1 and jee
+  // are the boundary values in place
+  const double *a1 = f, *a2 = f + (jee + 1), *a3 = f + 2 * (jee + 1);
+  const double a1b = f[3 * (jee + 1)], a3b = f[3 * (jee + 1) + 1];
+  const double *vic = c.vyui;
+  const double u1 = U[O(1)], ujee = U[O(jee)];
+  double tm1 = t[O(1)], t0 = t[O(2)];
+  y = 0.0;
+  sweep(2, jee - 1, 1, [&](int j) { return t[O(j + 1)]; }, [&](int j, double tp) {
+    double r = C3(vic, 1, j) * tm1 + C3(vic, 2, j) * t0 + C3(vic, 3, j) * tp;
+    if (j == 2) r -= a1b * u1;
+    if (j == jee - 1) r -= a3b * ujee;
+    y = (j == 2) ? r : r + a1[j] * y;
+    U[O(j)] = y;
+    tm1 = t0;
+    t0 = tp;
+  });
+  x = y * a2[jee - 1];
+  U[O(jee - 1)] = x;
+  if (jee - 2 >= 2)
+    sweep(jee - 2, 2, -1, [&](int j) { return U[O(j)]; }, [&](int j, double uy) {
+      x = (uy - a3[j] * x) * a2[j];
+      U[O(j)] = x;
+    });
 }
 
 template <typename T>
@@ -1484,7 +1590,9 @@ div_pencils_kernel(const T *u, const T *v, const T *w, T *P, DevCoefT<T> c, doub
 // per-mode y tridiagonal on pencils (pois.f90:102-117, soltriy
 // pois.f90:211-239): one thread per (kr, kx-mode x), rows at stride nz1*N.
 __global__ void __launch_bounds__(TPB)
-poisson_y_pencil_kernel(double *P, double *S, DevCoef c, int nx, int ny, int nz1, int kr_end) {
+poisson_y_pencil_kernel(double *__restrict__ P, double *__restrict__ S, DevCoef c, int nx, int ny, int nz1,
+                        int kr_end) {
+  constexpr int RB = 8; // rows whose loads are issued together
   const int N = nx - 1;
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)kr_end * N) return; // columns kr < kr_end
@@ -1498,28 +1606,52 @@ poisson_y_pencil_kernel(double *P, double *S, DevCoef c, int nx, int ny, int nz1
     return;
   }
   const double kz2 = c.kaz2[kr / 2], km = c.kmod[i - 1];
-  const bool pin = (i == 2 && kr == 0);
-  if (pin) col[(size_t)(ny1 - 1) * rs] = 0.0;
+  const bool pin = (i == 2 && kr == 0); // its last row: phi = 0 with a doubled diagonal
   double a2prev = C3(c.ayp, 2, 1) - kz2 - km;
   double a3prev = C3(c.ayp, 3, 1);
-  double y = col[0];
-  for (int j = 2; j <= ny1; ++j) {
-    double d = 1.0 / a2prev;
+  double y = (pin && ny1 == 1) ? 0.0 : col[0];
+  if (pin && ny1 == 1) col[0] = 0.0;
+  // forward elimination as soltriy, rows 2..ny-1
+  auto fwd = [&](int j, double b) {
+    if (pin && j == ny1) b = 0.0;
+    const double d = 1.0 / a2prev;
     d2[(size_t)(j - 2) * rs] = d;
-    double a1 = -C3(c.ayp, 1, j) * d;
+    const double a1 = -C3(c.ayp, 1, j) * d;
     double a2 = (pin && j == ny1) ? 2.0 * C3(c.ayp, 2, j) : C3(c.ayp, 2, j) - kz2 - km;
     a2 = a2 + a1 * a3prev;
-    y = col[(size_t)(j - 1) * rs] + a1 * y;
+    y = b + a1 * y;
     col[(size_t)(j - 1) * rs] = y;
     a2prev = a2;
     a3prev = C3(c.ayp, 3, j);
+  };
+  int j = 2;
+  for (; j + RB - 1 <= ny1; j += RB) {
+    double bv[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) bv[q] = col[(size_t)(j + q - 1) * rs];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) fwd(j + q, bv[q]);
   }
+  for (; j <= ny1; ++j) fwd(j, col[(size_t)(j - 1) * rs]);
   double x_ = y * (1.0 / a2prev);
   col[(size_t)(ny1 - 1) * rs] = x_;
-  for (int j = ny1 - 1; j >= 1; --j) {
-    x_ = (col[(size_t)(j - 1) * rs] - C3(c.ayp, 3, j) * x_) * d2[(size_t)(j - 1) * rs];
+  // back substitution, rows ny-2..1
+  auto bwd = [&](int j, double yv, double d) {
+    x_ = (yv - C3(c.ayp, 3, j) * x_) * d;
     col[(size_t)(j - 1) * rs] = x_;
+  };
+  j = ny1 - 1;
+  for (; j - RB + 1 >= 1; j -= RB) {
+    double yv[RB], dv[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      yv[q] = col[(size_t)(j - q - 1) * rs];
+      dv[q] = d2[(size_t)(j - q - 1) * rs];
+    }
+#pragma unroll
+    for (int q = 0; q < RB; ++q) bwd(j - q, yv[q], dv[q]);
   }
+  for (; j >= 1; --j) bwd(j, col[(size_t)(j - 1) * rs], d2[(size_t)(j - 1) * rs]);
 }
 
 // projection (pois.f90:144-162): p, w, v, u from phi = rest
@@ -1723,7 +1855,7 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   {
     int *ibuf = (int *)(b.gwork + (size_t)nz1 * (ny + 1));
     double *dbuf = b.gwork + (size_t)nz1 * (ny + 1) + 4 * (ny + 2);
-    LAUNCH(b, "genflu", 1, TPB, 4 * (size_t)(ny + 2) * sizeof(double), genflu_stats_kernel<T>, resu, b.um, ibuf,
+    LAUNCH(b, "genflu", 1, TPB, 8 * (size_t)(ny + 2) * sizeof(double), genflu_stats_kernel<T>, resu, b.um, ibuf,
            dbuf, b.dc, b.ds, ny, nz1, b.xout, b.re);
     LAUNCH(b, "genflu", dim3(nblk((size_t)nz1 * (ny + 1)), 3), TPB, 0, genflu_plane_kernel<T>, u, v, w, resu, resv,
            resw, b.u0, b.v0, ibuf, dbuf, b.dc, b.ds, ny, nz1, b.xout, um, nx);
