@@ -136,7 +136,7 @@ int otbl_d2d(void *dst, const void *src, size_t bytes);
 
 /* Kernel paths the context chose at setup, as text: "y:chunk<L>|march0
  * z:engine<nz>|stockham<nz> dct:engine<M>|stockham<M>|gemm<M> x:window|march
- * pois:chunk<L>|march0"
+ * xseg:<threads per x line> pois:chunk<L>|march0"
  * (no counterpart in the reference; for tests and benchmark records). */
 int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len);
 

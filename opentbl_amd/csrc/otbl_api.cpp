@@ -279,6 +279,26 @@ static double window_decay(std::vector<double> a, int n) {
   return worst;
 }
 
+// Largest product of the factored multipliers |a(1,k)| (coeft.f90:1338-1353)
+// over XLINE_LEAD consecutive rows: the factor by which a segment's forward
+// elimination damps its zero start.
+static double lead_decay(std::vector<double> a, int n) {
+  auto A = [&](int r, int j) -> double & { return a[3 * (size_t)j + r - 1]; };
+  for (int j = 2; j <= n; ++j) {
+    double d = 1.0 / A(2, j - 1);
+    A(2, j - 1) = d;
+    A(1, j) = -A(1, j) * d;
+    A(2, j) = A(2, j) + A(1, j) * A(3, j - 1);
+  }
+  double worst = 0.0;
+  for (int i = 2; i + XLINE_LEAD - 1 <= n; ++i) {
+    double p = 1.0;
+    for (int k = i; k < i + XLINE_LEAD; ++k) p *= std::fabs(A(1, k));
+    worst = std::max(worst, p);
+  }
+  return worst;
+}
+
 static void free_box(Box *b) {
   if (!b) return;
   for (int q = 0; q < 4; ++q)
@@ -518,6 +538,19 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     // the mixed substep has no marching fallback: short lines, which the
     // windowed kernel solves exactly, or any decaying x systems
     if (b->precision == OTBL_MIXED) b->xwin = b->nx <= XLINE_EXACT || worst < 1e-17;
+    // segmented lines: enough threads in flight (about 6e5) when the
+    // forward recurrences forget their start as well and the segments stay
+    // longer than their lead-in and backward window
+    double lead = 0.0;
+    for (const std::vector<double> *a : {&h.inxu, &h.inxv, &h.dcxu, &h.dcxv, &h.vixu, &h.vixv})
+      lead = std::max(lead, lead_decay(*a, b->nx));
+    const size_t lines = b->pu;
+    int nseg = (int)std::min<size_t>(4, std::max<size_t>(1, (600000 + lines / 2) / lines));
+    while (nseg > 1 && b->nx / nseg < 4 * (XLINE_LEAD + XLINE_WINDOW)) --nseg;
+    // OTBL_XSEG=n forces n segments (at most one per group of XLINE_LEAD planes)
+    if (const char *e = std::getenv("OTBL_XSEG"))
+      nseg = std::max(1, std::min(std::atoi(e), (b->nx + XLINE_LEAD - 1) / XLINE_LEAD));
+    b->xseg = (b->xwin && lead < 1e-17) ? nseg : 1;
   }
   // chunked y-line solver unless the columns are longer than its
   // instantiations (or OTBL_Y_MARCH asks for one thread per column)
@@ -872,9 +905,9 @@ int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len) {
   Box &b = box_of(ctx);
   need(buf && len > 0, "otbl_get_paths: null buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof tmp, "y:%s%d z:%s%d dct:%s%d x:%s pois:%s%d", b.yL ? "chunk" : "march", b.yL,
+  std::snprintf(tmp, sizeof tmp, "y:%s%d z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d", b.yL ? "chunk" : "march", b.yL,
                 b.zfast ? "engine" : "stockham", b.nz, b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"),
-                b.dct_M, b.xwin ? "window" : "march", b.pchunk ? "chunk" : "march", b.pchunk);
+                b.dct_M, b.xwin ? "window" : "march", b.xseg, b.pchunk ? "chunk" : "march", b.pchunk);
   std::snprintf(buf, len, "%s", tmp);
   API_END
 }

@@ -108,6 +108,10 @@ struct Box {
   // x lines with the windowed backward sweep (enabled when every window
   // product of the factored x matrices is below round-off)
   bool xwin = false;
+  // segments per x line (threads per line) of the windowed kernels, > 1
+  // when every forward multiplier product over XLINE_LEAD planes is below
+  // round-off as well
+  int xseg = 1;
   // chunked y-line solver: rows per chunk (0: one thread marches each
   // column) and the tables of its eight systems
   int yL = 0;
@@ -183,6 +187,7 @@ void kz_twiddles(Box &b, double *tw, int n);
 // windowed x lines (otbl_xline.hip), valid when Box::xwin
 constexpr int XLINE_WINDOW = 24; // planes of the backward window
 constexpr int XLINE_EXACT = 32;  // lines up to this length are solved exactly
+constexpr int XLINE_LEAD = 32;   // forward lead-in of a line segment, planes
 template <typename T>
 void kx_interpxx(Box &b, const T *gen, T *geni, const T *icon, const T *econ, int chi, int chi2, size_t P);
 template <typename T>

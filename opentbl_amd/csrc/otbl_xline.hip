@@ -10,6 +10,15 @@
 // line are solved exactly from plane nx.  The y values of the last XK blocks
 // stay in registers (a ring of XK x XB), so every input is read once and
 // every output written once, with no line scratch in HBM.
+//
+// Segments: with nseg > 1 a line is split into nseg runs of groups of XK
+// blocks, one thread each (more threads in flight).  A segment's forward
+// elimination starts one group (XLINE_LEAD planes) before its first block,
+// from y = 0; the forward recurrence damps that start error by |m_i| per
+// plane (the host checks every product over XLINE_LEAD planes), so the
+// segment's own planes see the exact y.  Its backward windows reach XK - 1
+// blocks past its last block.  Segments start on a group boundary, so the
+// ring slot of a block is its index mod XK in every segment.
 #include "otbl_kcommon.h"
 
 namespace otbl {
@@ -17,25 +26,48 @@ namespace otbl {
 static constexpr int XB = 8, XK = 4; // block of planes, ring length in blocks
 static_assert((XK - 1) * XB == XLINE_WINDOW, "window length");
 static_assert(XK * XB == XLINE_EXACT, "lines of at most XK blocks take the exact sweep alone");
+static_assert(XK * XB == XLINE_LEAD, "a segment's forward lead-in is one group of XK blocks");
+
+// block range of segment sg of nseg over nb blocks (whole groups of XK
+// blocks): emitted blocks [bs0, bs1), swept blocks [qs, qe], qs a multiple of
+// XK
+struct XSeg {
+  int bs0, bs1, qs, qe;
+  __device__ XSeg(int sg, int nseg, int nb) {
+    const int ng = (nb + XK - 1) / XK;
+    bs0 = (int)((long)sg * ng / nseg) * XK;
+    bs1 = min((int)((long)(sg + 1) * ng / nseg) * XK, nb);
+    qs = bs0 > XK ? bs0 - XK : 0;
+    qe = min(bs1 + XK - 2, nb - 1);
+  }
+  __device__ bool emits(int e) const { return e >= bs0 && e < bs1; }
+};
 
 // interpxx (lip.f90:248-300): midpoint interpolation along x
 template <typename T>
 __global__ void __launch_bounds__(TPB)
 interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict__ icon,
-           const T *__restrict__ econ, const T *__restrict__ inbx, int chi, int chi2, int nx, size_t P) {
-  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+           const T *__restrict__ econ, const T *__restrict__ inbx, int chi, int chi2, int nx, size_t P, int nseg) {
+  // the segment is a function of the block (uniform): the plane loop and
+  // its coefficient indices stay scalar
+  const int nbp = (int)((P + TPB - 1) / TPB);
+  const int sg = (int)blockIdx.x / nbp;
+  const size_t l = (size_t)((int)blockIdx.x - sg * nbp) * TPB + threadIdx.x;
   if (l >= P) return;
+  const int nb = (nx + XB - 1) / XB;
+  const XSeg sgm(sg, nseg, nb);
   const T *g = gen + l;
   T *o = geni + l;
-  const int nb = (nx + XB - 1) / XB;
   T ring[XK][XB];
   T y = 0;
-  T gi = g[P]; // g(i) of the interior plane i being processed, from i = 2
-  for (int it = 0; it < nb; it += XK) {
+  // g(i) of the interior plane i being processed: from i = 2, or from the
+  // first plane of the segment's lead-in
+  T gi = sgm.qs > 0 ? g[(size_t)(sgm.qs * XB) * P] : g[P];
+  for (int it = sgm.qs; it <= sgm.qe; it += XK) {
 #pragma unroll
     for (int kk = 0; kk < XK; ++kk) {
       const int qq = it + kk;
-      if (qq >= nb) break;
+      if (qq > sgm.qe) break;
       // forward elimination over block qq; a block of interior planes
       // issues its XB loads before any use (one basic block)
       const int i0 = qq * XB + 1;
@@ -75,7 +107,7 @@ interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict_
         }
       }
       if (qq < nb - 1) {
-        if (qq >= XK - 1) {
+        if (qq >= XK - 1 && sgm.emits(qq - (XK - 1))) {
           // windowed backward sweep: blocks qq .. qq-XK+1, emit the oldest
           T x = 0;
 #pragma unroll
@@ -91,13 +123,13 @@ interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict_
         }
       } else {
         // last block: exact backward sweep from plane nx over the blocks
-        // not emitted yet
+        // not emitted yet (those of this segment)
         T x = 0;
         bool started = false;
 #pragma unroll
         for (int d = 0; d < XK; ++d) {
           const int slot = (kk - d + XK) % XK;
-          if (qq - d >= 0) {
+          if (qq - d >= sgm.qs) {
 #pragma unroll
             for (int b = XB - 1; b >= 0; --b) {
               const int i = (qq - d) * XB + b + 1;
@@ -105,7 +137,7 @@ interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict_
                 const T yy = ring[slot][b];
                 x = started ? (yy - C3(icon, 3, i) * x) * C3(icon, 2, i) : yy * C3(icon, 2, nx);
                 started = true;
-                o[(size_t)(i - 1) * P] = x;
+                if (sgm.emits(qq - d)) o[(size_t)(i - 1) * P] = x;
               }
             }
           }
@@ -123,22 +155,30 @@ __global__ void __launch_bounds__(TPB)
 difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
            const T *__restrict__ icon, const T *__restrict__ iconv, const T *__restrict__ econ,
            const T *__restrict__ econv, const T *__restrict__ dcbx, const T *__restrict__ cofvbx,
-           int chv, int ic, int nx, size_t P, double rex_d, double idx_d, int mode) {
-  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+           int chv, int ic, int nx, size_t P, double rex_d, double idx_d, int mode, int nseg) {
+  // the segment is a function of the block (uniform): the plane loop and
+  // its coefficient indices stay scalar
+  const int nbp = (int)((P + TPB - 1) / TPB);
+  const int sg = (int)blockIdx.x / nbp;
+  const size_t l = (size_t)((int)blockIdx.x - sg * nbp) * TPB + threadIdx.x;
   if (l >= P) return;
+  const int nb = (nx + XB - 1) / XB;
+  const XSeg sgm(sg, nseg, nb);
   const T *g = gp + l, *v = q + l;
   T *o = out + l;
   const int chv1 = chv + 2;
   const T rex = (T)rex_d, idx = (T)idx_d;
-  const int nb = (nx + XB - 1) / XB;
   T rc_[XK][XB], rv_[XK][XB];
   T yc = 0, yv = 0;
-  T gim1 = g[0], qim1 = v[0], qi = v[P];
-  for (int it = 0; it < nb; it += XK) {
+  // g(i-1), q(i-1), q(i) before the first interior plane i processed (2, or
+  // the first plane of the segment's lead-in)
+  const int i0s = sgm.qs > 0 ? sgm.qs * XB + 1 : 2;
+  T gim1 = g[(size_t)(i0s - 2) * P], qim1 = v[(size_t)(i0s - 2) * P], qi = v[(size_t)(i0s - 1) * P];
+  for (int it = sgm.qs; it <= sgm.qe; it += XK) {
 #pragma unroll
     for (int kk = 0; kk < XK; ++kk) {
       const int qq = it + kk;
-      if (qq >= nb) break;
+      if (qq > sgm.qe) break;
       const int i0 = qq * XB + 1;
       if (i0 >= 2 && i0 + XB - 1 <= nx - 1) {
         // interior block: its 2 XB loads issued before any use
@@ -197,7 +237,7 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
         }
       }
       if (qq < nb - 1) {
-        if (qq >= XK - 1) {
+        if (qq >= XK - 1 && sgm.emits(qq - (XK - 1))) {
           T xc = 0, xv = 0;
 #pragma unroll
           for (int d = 0; d < XK; ++d) {
@@ -221,7 +261,8 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
 #pragma unroll
         for (int d = 0; d < XK; ++d) {
           const int slot = (kk - d + XK) % XK;
-          if (qq - d >= 0) {
+          if (qq - d >= sgm.qs) {
+            const bool em = sgm.emits(qq - d);
 #pragma unroll
             for (int b = XB - 1; b >= 0; --b) {
               const int i = (qq - d) * XB + b + 1;
@@ -231,14 +272,16 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
                   xc = rc_[slot][b] * C3(icon, 2, nx);
                   xv = rv_[slot][b] * C3(iconv, 2, nx);
                   // outflow row: the velocity difference qi - qim1 = q(nx) - q(nx-1)
-                  o[oi] = T(1) * (qi - qim1) * idx + rex * T(0);
+                  if (em) o[oi] = T(1) * (qi - qim1) * idx + rex * T(0);
                   started = true;
                 } else {
                   xc = (rc_[slot][b] - C3(icon, 3, i) * xc) * C3(icon, 2, i);
                   xv = (rv_[slot][b] - C3(iconv, 3, i) * xv) * C3(iconv, 2, i);
-                  T r = xc + rex * xv;
-                  if (mode == 1) r = o[oi] + r;
-                  o[oi] = r;
+                  if (em) {
+                    T r = xc + rex * xv;
+                    if (mode == 1) r = o[oi] + r;
+                    o[oi] = r;
+                  }
                 }
               }
             }
@@ -252,15 +295,16 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
 template <typename T>
 void kx_interpxx(Box &b, const T *gen, T *geni, const T *icon, const T *econ, int chi, int chi2, size_t P) {
   auto kf = &interpxx_w<T>;
-  LAUNCH(b, "interpxx", nblk(P), TPB, 0, kf, gen, geni, icon, econ, coefs<T>(b).inbx, chi, chi2, b.nx, P);
+  LAUNCH(b, "interpxx", (unsigned)(nblk(P) * b.xseg), TPB, 0, kf, gen, geni, icon, econ, coefs<T>(b).inbx, chi, chi2, b.nx, P,
+         b.xseg);
 }
 
 template <typename T>
 void kx_difvisxx(Box &b, const T *prod, const T *vel, T *out, const T *icon, const T *iconv, const T *econ,
                  const T *econv, int chv, int ic, size_t P, int mode) {
   auto kf = &difvisxx_w<T>;
-  LAUNCH(b, "difvisxx", nblk(P), TPB, 0, kf, prod, vel, out, icon, iconv, econ, econv, coefs<T>(b).dcbx,
-         coefs<T>(b).cofvbx, chv, ic, b.nx, P, b.hc.rex, b.hc.idx, mode);
+  LAUNCH(b, "difvisxx", (unsigned)(nblk(P) * b.xseg), TPB, 0, kf, prod, vel, out, icon, iconv, econ, econv, coefs<T>(b).dcbx,
+         coefs<T>(b).cofvbx, chv, ic, b.nx, P, b.hc.rex, b.hc.idx, mode, b.xseg);
 }
 
 template void kx_interpxx<double>(Box &, const double *, double *, const double *, const double *, int, int, size_t);

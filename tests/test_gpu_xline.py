@@ -96,3 +96,44 @@ def test_windowed_matches_marching(xb):
     gw.close()
     gm.close()
     assert all(e < STEP_TOL for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("nx,nseg", [(257, 3), (600, 2), (600, 4)])
+def test_segmented_lines(oracle_lib, nx, nseg):
+    """Lines split into segments whose forward sweep starts XLINE_LEAD planes
+    early from zero: operators and whole steps against one thread per line
+    (OTBL_XSEG=1) and the steps against the oracle."""
+    cfg = config.small(name=f"xs{nx}", nx=nx, ny=40, nz=24, Lx=8.0 * nx / 32)
+    st = ic.make_state(cfg)
+    boxes = {}
+    for n in (1, nseg):
+        os.environ["OTBL_XSEG"] = str(n)
+        try:
+            boxes[n] = GpuBox(cfg, st)
+        finally:
+            del os.environ["OTBL_XSEG"]
+    g1, gs = boxes[1], boxes[nseg]
+    assert g1.paths()["xseg"] == "1" and gs.paths()["xseg"] == str(nseg), gs.paths()
+    R = cfg.ny + 1
+    g = field(cfg, R, 5)
+    q = field(cfg, R, 6)
+    for name, args in (("interpxx", (R, 1)), ("difvisxx", (R, 1))):
+        outs = []
+        for gb in (g1, gs):
+            dg, dq, do = DevArray.from_host(g), DevArray.from_host(q), DevArray(g.size)
+            if name == "interpxx":
+                gb.op(name, dg.ptr, do.ptr, *args)
+            else:
+                gb.op(name, dg.ptr, dq.ptr, do.ptr, *args)
+            outs.append(do.to_host())
+        assert rel(outs[1], outs[0]) < 1e-13, name
+    ob = oracle_lib.OracleBox(cfg, st)
+    ob.step()
+    gs.step(1)
+    g1.step(1)
+    errs = state_errors(gs, ob)
+    ediff = {n: rel(gs.get(n), g1.get(n)) for n in ("u", "v", "w")}
+    g1.close()
+    gs.close()
+    assert_state(errs)
+    assert all(e < 1e-12 for e in ediff.values()), ediff
