@@ -18,6 +18,18 @@ static constexpr int TPB = 256;
 
 static inline unsigned nblk(size_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
 
+// XCD-contiguous block order.  The dispatcher hands workgroup b of a 1-D
+// grid to XCD b % 8, and each XCD has its own L2; a kernel whose neighbouring
+// blocks share cache lines (row chunks that do not start on a line) takes
+// its logical block from here, so that XCD x works through the contiguous
+// run of logical blocks [x G/8, (x+1) G/8) and the shared lines meet in one
+// L2.  A bijection of [0, gridDim.x) for every grid size.
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned per = G >> 3, rem = G & 7, x = b & 7, k = b >> 3;
+  return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
 #define LAUNCH(b, name, grid, block, shm, kern, ...)                          \
   do {                                                                         \
     kbegin(b, name);                                                           \

@@ -60,7 +60,9 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
   // lane column: the Re (even) or Im (odd) part of one of 16 (kz, kx) modes
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & (PK - 1), k = wave * 2 + (lane >> 5);
-  const size_t t = (size_t)N + (size_t)blockIdx.x * (PK / 2) + (col >> 1); // kz >= 1 only
+  // XCD-contiguous block order: neighbouring workgroups share the cache
+  // lines at the ends of their 16-mode runs
+  const size_t t = (size_t)N + (size_t)xcd_block() * (PK / 2) + (col >> 1); // kz >= 1 only
   const bool live = t < (size_t)nzh * N;
   const int kz = live ? (int)(t / N) : 1, x = live ? (int)(t % N) : 0;
   // 32-bit byte offsets from the (uniform) pencil base: the pencils are

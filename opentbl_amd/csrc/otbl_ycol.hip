@@ -74,7 +74,7 @@ __device__ inline YCol ycol(size_t ncols, int nz1, size_t ip0) {
   YCol y;
   y.lane = threadIdx.x & 63;
   y.p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  size_t c = (size_t)blockIdx.x * 64 + y.lane;
+  size_t c = (size_t)xcd_block() * 64 + y.lane;
   y.ok = c < ncols;
   if (!y.ok) c = 0; // a dead lane addresses column 0 of the first plane
   y.kr = (int)(c % nz1);
@@ -96,6 +96,13 @@ __device__ __forceinline__ size_t yopaque(size_t x) {
 // ahead of its dependency chain (which runs out of registers and spills).
 #define YFENCE() __builtin_amdgcn_sched_barrier(0)
 static constexpr int YGRP = 4;
+
+// the largest divisor of L that is at most g: rows per load group
+__host__ __device__ constexpr int yrows_group(int L, int g) {
+  for (int k = g; k > 1; --k)
+    if (L % k == 0) return k;
+  return 1;
+}
 
 // Copy one system's table into LDS (all threads); the caller synchronises.
 template <int L, typename T>
@@ -349,13 +356,19 @@ rhsu_loop1_y(const T *__restrict__ uv, const T *__restrict__ uw, T *__restrict__
   const T *w = uw + q.ip * pu + (q.kr ^ 1);
   T *o = rhsu + q.ip * pu + q.kr;
   const size_t s1 = yopaque(s);
+  // a group's loads are all issued before its first store
+  constexpr int GG = sizeof(T) == 4 ? yrows_group(L, 10) : 1; // FP64: no spare registers
 #pragma unroll
-  for (int r = 0; r < L; ++r) {
-    if (r % YGRP == 0) YFENCE();
-    const int j = j0 + r;
-    const size_t jc = (size_t)yclamp(j, ny + 1) * s1;
-    const T wp = w[jc];
-    if (q.ok && j <= ny) o[jc] = v[r] + sg * wp * kz;
+  for (int g0 = 0; g0 < L; g0 += GG) {
+    T wp[GG];
+#pragma unroll
+    for (int r = 0; r < GG; ++r) wp[r] = w[(size_t)yclamp(j0 + g0 + r, ny + 1) * s1];
+#pragma unroll
+    for (int r = 0; r < GG; ++r) {
+      const int j = j0 + g0 + r;
+      if (q.ok && j <= ny) o[(size_t)j * s1] = v[g0 + r] + sg * wp[r] * kz;
+    }
+    YFENCE();
   }
 }
 
@@ -380,6 +393,10 @@ rhs_loop2_y(T *rhsu, T *rhsv, T *rhsw, const T *__restrict__ resu,
   const size_t ou = q.ip * pu + q.kr, ov = q.ip * pv + q.kr, op = q.ip * pu + (q.kr ^ 1);
   const YBnd<T> bv = differzy_bnd(1, j0, L, ny, c.dcby, vv + ou, s);
   const YBnd<T> bw = differzy_bnd(0, j0, L, ny, c.dcby, vw + ou, s);
+  // rows per group of the explicit terms: a group's loads are all issued
+  // before its first store (the stores go to arrays that are also read, at
+  // clamped rows the compiler cannot tell apart)
+  constexpr int GG = sizeof(T) == 4 ? yrows_group(L, 10) : 1; // FP64: no spare registers
   __syncthreads();
   T d[L];
   // rhsv: -kv resv (rows 2..ny-1), + i kz vw (rows 1..ny), + d(vv)/dy
@@ -387,47 +404,70 @@ rhs_loop2_y(T *rhsu, T *rhsv, T *rhsw, const T *__restrict__ resu,
   {
     const size_t s1 = yopaque(s);
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
-      if (r % YGRP == 0) YFENCE();
-      const int j = j0 + r;
-      const size_t jc = (size_t)yclamp(j, ny) * s1;
-      T x = rhsv[ov + jc];
-      const T sv = resv[ov + jc], pw = vw[op + jc];
-      if (j >= 1 && j <= ny - 2) x -= sv * kv;
-      x += sg * pw * kz;
-      if (q.ok && j < ny) rhsv[ov + jc] = x + d[r];
+    for (int g0 = 0; g0 < L; g0 += GG) {
+      T x[GG], sv[GG], pw[GG];
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const size_t jc = (size_t)yclamp(j0 + g0 + r, ny) * s1;
+        x[r] = rhsv[ov + jc];
+        sv[r] = resv[ov + jc];
+        pw[r] = vw[op + jc];
+      }
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const int j = j0 + g0 + r;
+        T e = x[r];
+        if (j >= 1 && j <= ny - 2) e -= sv[r] * kv;
+        e += sg * pw[r] * kz;
+        if (q.ok && j < ny) rhsv[ov + (size_t)j * s1] = e + d[g0 + r];
+      }
+      YFENCE();
     }
   }
-  YFENCE();
   // rhsw: -kv resw (rows 2..ny), + d(vw)/dy, + i kz ww (rows 1..ny)
   ysolve_col<L>(d, vw + ou, s, ny, tlu, q.p, q.lane, ny + 1, bw.bot, bw.top, xch);
   {
     const size_t s1 = yopaque(s);
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
-      if (r % YGRP == 0) YFENCE();
-      const int j = j0 + r;
-      const size_t jc = (size_t)yclamp(j, ny + 1) * s1;
-      T x = rhsw[ou + jc];
-      const T sw = resw[ou + jc];
-      const T pw = ww[op + (size_t)yclamp(j, ny) * s1]; // ww holds rows 1..ny
-      if (j >= 1 && j <= ny - 1) x -= sw * kv;
-      x += d[r];
-      if (j <= ny - 1) x += sg * pw * kz;
-      if (q.ok && j <= ny) rhsw[ou + jc] = x;
+    for (int g0 = 0; g0 < L; g0 += GG) {
+      T x[GG], sw[GG], pw[GG];
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const int j = j0 + g0 + r;
+        x[r] = rhsw[ou + (size_t)yclamp(j, ny + 1) * s1];
+        sw[r] = resw[ou + (size_t)yclamp(j, ny + 1) * s1];
+        pw[r] = ww[op + (size_t)yclamp(j, ny) * s1]; // ww holds rows 1..ny
+      }
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const int j = j0 + g0 + r;
+        T e = x[r];
+        if (j >= 1 && j <= ny - 1) e -= sw[r] * kv;
+        e += d[g0 + r];
+        if (j <= ny - 1) e += sg * pw[r] * kz;
+        if (q.ok && j <= ny) rhsw[ou + (size_t)j * s1] = e;
+      }
+      YFENCE();
     }
   }
-  YFENCE();
   // rhsu: -kv resu (rows 2..ny)
   {
     const size_t s1 = yopaque(s);
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
-      if (r % YGRP == 0) YFENCE();
-      const int j = j0 + r;
-      const size_t jc = (size_t)yclamp(j, ny + 1) * s1;
-      const T x = rhsu[ou + jc], su = resu[ou + jc];
-      if (q.ok && j >= 1 && j <= ny - 1) rhsu[ou + jc] = x - su * kv;
+    for (int g0 = 0; g0 < L; g0 += GG) {
+      T x[GG], su[GG];
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const size_t jc = (size_t)yclamp(j0 + g0 + r, ny + 1) * s1;
+        x[r] = rhsu[ou + jc];
+        su[r] = resu[ou + jc];
+      }
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const int j = j0 + g0 + r;
+        if (q.ok && j >= 1 && j <= ny - 1) rhsu[ou + (size_t)j * s1] = x[r] - su[r] * kv;
+      }
+      YFENCE();
     }
   }
 }
@@ -453,21 +493,60 @@ rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, T *rhsp,
   const YBnd<T> bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, res + off, s);
   const T u1 = u[off], uj = u[off + (size_t)(jee - 1) * s];
   const T a1b = (T)f[3 * (jee + 1)], a3b = (T)f[3 * (jee + 1) + 1];
-  __syncthreads();
-  T t[L];
-  ysolve_col<L>(t, res + off, s, jee, tlv, q.p, q.lane, jee, bd.bot, bd.top, xch);
   const T var2 = (q.ip == (size_t)(nx - 1)) ? T(0) : (T)ds->var2;
   const T var3 = (T)ds->var3, var4 = (T)ds->var4;
-  {
-    const size_t s1 = yopaque(s);
+  T t[L];
+  if constexpr (sizeof(T) == 4) {
+    // FP32: the explicit part w = u - (var3 rhs + var4 rhs_prev) is formed
+    // as soon as its 3L rows arrive (all issued at once) and rhs is saved as
+    // the next rhs_prev, so only w stays live through the vistyy solve
+    T w[L];
+    {
+      T rh[L], rp[L], uu[L];
+      const size_t s1 = yopaque(s);
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
-      if (r % YGRP == 0) YFENCE();
-      const int j = j0 + r;
-      const size_t o = off + (size_t)yclamp(j, jee) * s1;
-      const T rh = rhs[o], rp = rhsp[o], uu = u[o];
-      if (act && j < jee) t[r] = uu - (var3 * rh + var2 * t[r] + var4 * rp);
-      if (q.ok && j < jee) rhsp[o] = rh;
+      for (int r = 0; r < L; ++r) {
+        const size_t o = off + (size_t)yclamp(j0 + r, jee) * s1;
+        rh[r] = rhs[o];
+        rp[r] = rhsp[o];
+        uu[r] = u[o];
+      }
+#pragma unroll
+      for (int r = 0; r < L; ++r) {
+        const int j = j0 + r;
+        w[r] = uu[r] - (var3 * rh[r] + var4 * rp[r]);
+        if (q.ok && j < jee) rhsp[off + (size_t)j * s1] = rh[r];
+      }
+    }
+    YFENCE();
+    __syncthreads();
+    ysolve_col<L>(t, res + off, s, jee, tlv, q.p, q.lane, jee, bd.bot, bd.top, xch);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+      if (act && j0 + r < jee) t[r] = w[r] - var2 * t[r];
+  } else {
+    __syncthreads();
+    ysolve_col<L>(t, res + off, s, jee, tlv, q.p, q.lane, jee, bd.bot, bd.top, xch);
+    const size_t s1 = yopaque(s);
+    // a group's loads are all issued before its first store
+    constexpr int GG = 1; // FP64 holds no spare registers for a wider group
+#pragma unroll
+    for (int g0 = 0; g0 < L; g0 += GG) {
+      T rh[GG], rp[GG], uu[GG];
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const size_t o = off + (size_t)yclamp(j0 + g0 + r, jee) * s1;
+        rh[r] = rhs[o];
+        rp[r] = rhsp[o];
+        uu[r] = u[o];
+      }
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const int j = j0 + g0 + r;
+        if (act && j < jee) t[g0 + r] = uu[r] - (var3 * rh[r] + var2 * t[g0 + r] + var4 * rp[r]);
+        if (q.ok && j < jee) rhsp[off + (size_t)j * s1] = rh[r];
+      }
+      YFENCE();
     }
   }
   YFENCE();
