@@ -24,10 +24,16 @@ static inline unsigned nblk(size_t n, int t = TPB) { return (unsigned)((n + t - 
 // its logical block from here, so that XCD x works through the contiguous
 // run of logical blocks [x G/8, (x+1) G/8) and the shared lines meet in one
 // L2.  A bijection of [0, gridDim.x) for every grid size.
-__device__ __forceinline__ unsigned xcd_block() {
-  const unsigned G = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned G) {
   const unsigned per = G >> 3, rem = G & 7, x = b & 7, k = b >> 3;
   return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+__device__ __forceinline__ unsigned xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
+// the same for a 2-D grid (linear block id x-fastest): logical (bx, by)
+__device__ __forceinline__ void xcd_block2(unsigned &bx, unsigned &by) {
+  const unsigned lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  by = lb / gridDim.x;
+  bx = lb - by * gridDim.x;
 }
 
 #define LAUNCH(b, name, grid, block, shm, kern, ...)                          \

@@ -718,7 +718,9 @@ __global__ void __launch_bounds__(TPB) dct_rows_kernel(const double *src, double
 // 0..Lp-1, plane stride ps) and x-contiguous pencils P[l*N + x].
 __global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F, double *P, int Lp, int N, size_t ps) {
   __shared__ double tile[64][65];
-  const int l0 = blockIdx.x * 64, x0 = blockIdx.y * 64, t = threadIdx.x;
+  unsigned bx, by;
+  xcd_block2(bx, by); // XCD-contiguous tiles: neighbouring rows' tiles share lines
+  const int l0 = (int)bx * 64, x0 = (int)by * 64, t = threadIdx.x;
   for (int xx = t / 64; xx < 64; xx += TPB / 64) {
     int l = l0 + (t & 63), x = x0 + xx;
     if (l < Lp && x < N) tile[xx][t & 63] = F[(size_t)(x + 1) * ps + l];
@@ -736,7 +738,9 @@ template <typename T>
 __global__ void __launch_bounds__(TPB)
 pencils_to_planes_kernel(const T *P, T *F, int Lp, int N, size_t ps, const double *P0, int nz1) {
   __shared__ T tile[64][65];
-  const int l0 = blockIdx.x * 64, x0 = blockIdx.y * 64, t = threadIdx.x;
+  unsigned bx, by;
+  xcd_block2(bx, by); // XCD-contiguous tiles: neighbouring rows' tiles share lines
+  const int l0 = (int)bx * 64, x0 = (int)by * 64, t = threadIdx.x;
   for (int ll = t / 64; ll < 64; ll += TPB / 64) {
     int l = l0 + ll, x = x0 + (t & 63);
     if (l < Lp && x < N)
@@ -838,7 +842,8 @@ void k_cosftx(Box &b, const double *in, double *out, int iopt) {
 // ---------------------------------------------------------------------------
 // Element-wise kernels over u-type fields: element e = ((i-1)(ny+1) +
 // j-1) nz1 + kr of a 1-D grid, split with 32-bit divisions (a field has
-// fewer than 2^32 elements, checked at setup).
+// fewer than 2^32 elements, checked at setup).  (Plain block order: an
+// XCD-contiguous order measured slower for these streaming kernels.)
 __device__ __forceinline__ bool uelem(int nz1, int R, int nx, size_t &e, int &kr, int &j, int &i) {
   const unsigned ee = blockIdx.x * blockDim.x + threadIdx.x;
   e = ee;
@@ -1557,7 +1562,9 @@ div_pencils_kernel(const T *u, const T *v, const T *w, T *P, DevCoefT<T> c, doub
   const T idx = (T)idx_d;
   const int Lp = (ny - 1) * nz1, N = nx - 1;
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
-  const int l0 = blockIdx.x * 64, x0 = blockIdx.y * 64, t = threadIdx.x;
+  unsigned bx, by;
+  xcd_block2(bx, by); // XCD-contiguous tiles: neighbouring rows' tiles share lines
+  const int l0 = (int)bx * 64, x0 = (int)by * 64, t = threadIdx.x;
   for (int xx = t / 64; xx < 64; xx += TPB / 64) {
     int l = l0 + (t & 63), x = x0 + xx;
     if (l < Lp && x < N) {

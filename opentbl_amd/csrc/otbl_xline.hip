@@ -51,8 +51,9 @@ interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict_
   // the segment is a function of the block (uniform): the plane loop and
   // its coefficient indices stay scalar
   const int nbp = (int)((P + TPB - 1) / TPB);
-  const int sg = (int)blockIdx.x / nbp;
-  const size_t l = (size_t)((int)blockIdx.x - sg * nbp) * TPB + threadIdx.x;
+  const int bk = (int)xcd_block(); // XCD-contiguous: neighbouring blocks share lines
+  const int sg = bk / nbp;
+  const size_t l = (size_t)(bk - sg * nbp) * TPB + threadIdx.x;
   if (l >= P) return;
   const int nb = (nx + XB - 1) / XB;
   const XSeg sgm(sg, nseg, nb);
@@ -159,8 +160,9 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
   // the segment is a function of the block (uniform): the plane loop and
   // its coefficient indices stay scalar
   const int nbp = (int)((P + TPB - 1) / TPB);
-  const int sg = (int)blockIdx.x / nbp;
-  const size_t l = (size_t)((int)blockIdx.x - sg * nbp) * TPB + threadIdx.x;
+  const int bk = (int)xcd_block(); // XCD-contiguous: neighbouring blocks share lines
+  const int sg = bk / nbp;
+  const size_t l = (size_t)(bk - sg * nbp) * TPB + threadIdx.x;
   if (l >= P) return;
   const int nb = (nx + XB - 1) / XB;
   const XSeg sgm(sg, nseg, nb);
@@ -238,6 +240,12 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
       }
       if (qq < nb - 1) {
         if (qq >= XK - 1 && sgm.emits(qq - (XK - 1))) {
+          // mode 1: the emitted block's previous values, loaded before the
+          // sweep (a load after a store to the same array would wait for it)
+          T prev[XB];
+          const int ie = (qq - (XK - 1)) * XB; // 0-based first plane emitted
+#pragma unroll
+          for (int b = 0; b < XB; ++b) prev[b] = mode == 1 ? o[(size_t)(ie + b) * P] : T(0);
           T xc = 0, xv = 0;
 #pragma unroll
           for (int d = 0; d < XK; ++d) {
@@ -247,11 +255,7 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
               const int i = (qq - d) * XB + b + 1;
               xc = (rc_[slot][b] - C3(icon, 3, i) * xc) * C3(icon, 2, i);
               xv = (rv_[slot][b] - C3(iconv, 3, i) * xv) * C3(iconv, 2, i);
-              if (d == XK - 1) {
-                T r = xc + rex * xv;
-                if (mode == 1) r = o[(size_t)(i - 1) * P] + r;
-                o[(size_t)(i - 1) * P] = r;
-              }
+              if (d == XK - 1) o[(size_t)(i - 1) * P] = prev[b] + (xc + rex * xv);
             }
           }
         }
@@ -263,6 +267,12 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
           const int slot = (kk - d + XK) % XK;
           if (qq - d >= sgm.qs) {
             const bool em = sgm.emits(qq - d);
+            T prev[XB];
+#pragma unroll
+            for (int b = 0; b < XB; ++b) {
+              const int i = (qq - d) * XB + b + 1;
+              prev[b] = (mode == 1 && em && i <= nx) ? o[(size_t)(i - 1) * P] : T(0);
+            }
 #pragma unroll
             for (int b = XB - 1; b >= 0; --b) {
               const int i = (qq - d) * XB + b + 1;
@@ -277,11 +287,7 @@ difvisxx_w(const T *__restrict__ gp, const T *__restrict__ q, T *out,
                 } else {
                   xc = (rc_[slot][b] - C3(icon, 3, i) * xc) * C3(icon, 2, i);
                   xv = (rv_[slot][b] - C3(iconv, 3, i) * xv) * C3(iconv, 2, i);
-                  if (em) {
-                    T r = xc + rex * xv;
-                    if (mode == 1) r = o[oi] + r;
-                    o[oi] = r;
-                  }
+                  if (em) o[oi] = prev[b] + (xc + rex * xv);
                 }
               }
             }
