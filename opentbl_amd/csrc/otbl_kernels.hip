@@ -1289,8 +1289,9 @@ __global__ void boun_kernel(T *u, T *v, T *w, T *rhsu, T *rhsv, T *rhsw, DevCoef
 
 // Explicit RK3 combination with vistyy (rhsp.F90:448-475) and the implicit
 // y-viscous solve (rhsp.F90:479-484) for one component, columns (kr, i) over
-// planes 1..nx (plane 1: rhs_prev = rhs only).  wk: scratch of field size.
-__global__ void rk_final_kernel(double *u, const double *res, const double *rhs, double *rhsp,
+// planes 2..nx.  The reference then copies rhs into rhs_prev; here the two
+// buffers swap roles instead (k_swap_saved).  wk: scratch of field size.
+__global__ void rk_final_kernel(double *u, const double *res, const double *rhs, const double *rhsp,
                                 double *wk, DevCoef c, const double *f, const DevScalars *ds, int jee,
                                 int ny, int nz1, int nx) {
   size_t cidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1299,12 +1300,8 @@ __global__ void rk_final_kernel(double *u, const double *res, const double *rhs,
   size_t ip = cidx / nz1;
   const size_t s = nz1, ps = s * jee;
   const size_t off = ip * ps + kr;
-  const double *rh = rhs + off;
-  double *rp = rhsp + off;
-  if (ip == 0) {
-    for (int j = 0; j < jee; ++j) rp[(size_t)j * s] = rh[(size_t)j * s];
-    return;
-  }
+  const double *rh = rhs + off, *rp = rhsp + off;
+  if (ip == 0) return;
   double *uu = u + off, *t = wk + off;
   const double var2 = (ip == (size_t)(nx - 1)) ? 0.0 : ds->var2;
   const double var3 = ds->var3, var4 = ds->var4;
@@ -1313,7 +1310,6 @@ __global__ void rk_final_kernel(double *u, const double *res, const double *rhs,
     size_t o = (size_t)j * s;
     double r = rh[o];
     t[o] = uu[o] - (var3 * r + var2 * t[o] + var4 * rp[o]);
-    rp[o] = r;
   }
   implzy_col(uu, t, s, jee, jee == ny + 1 ? c.vyui : c.vyvi, f);
 }
@@ -1322,8 +1318,8 @@ __global__ void rk_final_kernel(double *u, const double *res, const double *rhs,
 // combination and implicit y solve of rk_final_kernel for the kz = 0 column
 // of u of every plane, from the FP64 state um and its saved copy rm, with
 // the FP32 rhs and previous rhs (mode-0 real parts).  Runs before the FP32
-// rk_final of u, which then overwrites rhsp; mean_to_field_kernel copies the
-// result into the FP32 field afterwards.  One thread per plane, arrays
+// rk_final of u; mean_to_field_kernel copies the result into the FP32 field
+// afterwards.  One thread per plane, arrays
 // [ny+1][nx] (adjacent planes adjacent); tm: scratch of that size.
 template <typename T>
 __global__ void mean_rk_final_kernel(double *__restrict__ um, const double *__restrict__ rm, const T *rhs,
@@ -1783,10 +1779,18 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
 // ---------------------------------------------------------------------------
 // Swap the roles of the velocity fields and their saved copies (after
 // save_predict_kernel); a captured substep applies the same swap on replay.
+// Role swap of a substep: the velocities with their saved copies (the
+// predictor wrote the new velocities into the save buffers) and the rhs work
+// fields with the previous-substep rhs fields (the rhs just formed is the
+// next substep's rhs_prev, rhsp.F90:475, and the old rhs_prev buffer is
+// rewritten whole by the next substep).
 void k_swap_saved(Box &b) {
   std::swap(b.f[OTBL_U], b.resu);
   std::swap(b.f[OTBL_V], b.resv);
   std::swap(b.f[OTBL_W], b.resw);
+  std::swap(b.f[OTBL_RHSUPA], b.rhsu);
+  std::swap(b.f[OTBL_RHSVPA], b.rhsv);
+  std::swap(b.f[OTBL_RHSWPA], b.rhsw);
   b.flip ^= 1;
 }
 
@@ -1800,6 +1804,8 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   T *u = fld<T>(b.f[OTBL_U]), *v = fld<T>(b.f[OTBL_V]), *w = fld<T>(b.f[OTBL_W]), *p = fld<T>(b.f[OTBL_P]);
   T *resu = fld<T>(b.resu), *resv = fld<T>(b.resv), *resw = fld<T>(b.resw);
   T *rhsu = fld<T>(b.rhsu), *rhsv = fld<T>(b.rhsv), *rhsw = fld<T>(b.rhsw);
+  // the previous substep's rhs (read before the role swap below)
+  const T *rp[3] = {fld<T>(b.f[OTBL_RHSUPA]), fld<T>(b.f[OTBL_RHSVPA]), fld<T>(b.f[OTBL_RHSWPA])};
   T *wki1 = fld<T>(b.wki1), *wki2 = fld<T>(b.wki2), *wki3 = fld<T>(b.wki3);
   T *ya = fld<T>(b.ya), *yb = fld<T>(b.yb), *yc = fld<T>(b.yc);
   const DevCoefT<T> &c = coefs<T>(b);
@@ -1853,7 +1859,8 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   double *um = F64 ? nullptr : b.um64; // the FP64 mean-flow u of the mixed substep
   LAUNCH(b, "save_predict", nblk(b.nu), TPB, 0, save_predict_kernel<T>, u, v, w, p, resu, resv, resw, c.idyy, c.kaz,
          b.ds, b.hc.idxx, nx, ny, nz1, F64 ? nullptr : (const double *)b.p0, b.dc.idyy, um, b.rm64);
-  // the predicted velocities now live in the former save buffers
+  // the predicted velocities now live in the former save buffers, and this
+  // substep's rhs buffers become the rhs_prev fields
   k_swap_saved(b);
   std::swap(u, resu);
   std::swap(v, resv);
@@ -1882,7 +1889,7 @@ template <typename T> static void rhsp_t(Box &b, int m) {
       LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, resv, resw, ya, yb, yc, plan, ny, nz1,
              b.nz2);
     if (b.yL)
-      ky_rhs_loop2<T>(b, ya, yb, yc);
+      ky_rhs_loop2<T>(b, rhsu, rhsv, rhsw, resu, resv, resw, ya, yb, yc);
     else if constexpr (F64)
       LAUNCH(b, "rhs_loop2", nblk((size_t)nz1 * (nx - 2)), TPB, 0, rhs_loop2_kernel, rhsu, rhsv, rhsw, resu, resv,
              resw, ya, yb, yc, wki1, b.dc, ny, nz1, nx - 2);
@@ -1892,7 +1899,6 @@ template <typename T> static void rhsp_t(Box &b, int m) {
          b.dc);
   // implicit factors for this dt, then the final update per component
   launch_impl_factor(b, 0.0, 0);
-  T *rp[3] = {fld<T>(b.f[OTBL_RHSUPA]), fld<T>(b.f[OTBL_RHSVPA]), fld<T>(b.f[OTBL_RHSWPA])};
   if (b.yL) {
     ky_rk_final<T>(b, v, resv, rhsv, rp[1], ny);
     if (um) // the FP64 mean-flow u first: it reads the previous rhs

@@ -159,7 +159,9 @@ template <typename T> void ky_vistyy(Box &b, const T *u, T *res, int R, int npla
 // factored system comes in the precision of the fields
 template <typename T> void ky_implzy(Box &b, T *u, const T *res, int R, int nplanes);
 template <typename T> void ky_rhsu_loop1(Box &b, const T *uv, const T *uw, T *rhsu);
-template <typename T> void ky_rhs_loop2(Box &b, const T *vw, const T *vv, const T *ww);
-template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, T *rhsp, int jee);
+template <typename T>
+void ky_rhs_loop2(Box &b, T *rhsu, T *rhsv, T *rhsw, const T *resu, const T *resv, const T *resw, const T *vw,
+                  const T *vv, const T *ww);
+template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, const T *rhsp, int jee);
 
 } // namespace otbl

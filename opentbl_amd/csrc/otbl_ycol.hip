@@ -473,11 +473,12 @@ rhs_loop2_y(T *rhsu, T *rhsv, T *rhsw, const T *__restrict__ resu,
 }
 
 // Explicit RK3 combination with vistyy (rhsp.F90:448-475) and the implicit
-// y-viscous solve (rhsp.F90:479-484) for one component, planes 1..nx
-// (plane 1: rhs_prev = rhs only; plane nx: no viscous term).
+// y-viscous solve (rhsp.F90:479-484) for one component, planes 2..nx
+// (plane nx: no viscous term).  The reference then copies rhs into rhs_prev
+// (rhsp.F90:475); here the buffers swap roles instead (k_swap_saved).
 template <int L, typename T>
 __global__ void __launch_bounds__(YTPB)
-rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, T *rhsp,
+rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, const T *__restrict__ rhsp,
            const T *__restrict__ tabv, const T *__restrict__ tabi, DevCoefT<T> c,
            const double *__restrict__ f, const DevScalars *__restrict__ ds, int jee, int ny, int nz1, int nx) {
   __shared__ T xch[2 * YNCH * 64];
@@ -498,8 +499,8 @@ rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, T *rhsp,
   T t[L];
   if constexpr (sizeof(T) == 4) {
     // FP32: the explicit part w = u - (var3 rhs + var4 rhs_prev) is formed
-    // as soon as its 3L rows arrive (all issued at once) and rhs is saved as
-    // the next rhs_prev, so only w stays live through the vistyy solve
+    // as soon as its 3L rows arrive (all issued at once), so only w stays
+    // live through the vistyy solve
     T w[L];
     {
       T rh[L], rp[L], uu[L];
@@ -512,11 +513,7 @@ rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, T *rhsp,
         uu[r] = u[o];
       }
 #pragma unroll
-      for (int r = 0; r < L; ++r) {
-        const int j = j0 + r;
-        w[r] = uu[r] - (var3 * rh[r] + var4 * rp[r]);
-        if (q.ok && j < jee) rhsp[off + (size_t)j * s1] = rh[r];
-      }
+      for (int r = 0; r < L; ++r) w[r] = uu[r] - (var3 * rh[r] + var4 * rp[r]);
     }
     YFENCE();
     __syncthreads();
@@ -528,25 +525,13 @@ rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, T *rhsp,
     __syncthreads();
     ysolve_col<L>(t, res + off, s, jee, tlv, q.p, q.lane, jee, bd.bot, bd.top, xch);
     const size_t s1 = yopaque(s);
-    // a group's loads are all issued before its first store
-    constexpr int GG = 1; // FP64 holds no spare registers for a wider group
 #pragma unroll
-    for (int g0 = 0; g0 < L; g0 += GG) {
-      T rh[GG], rp[GG], uu[GG];
-#pragma unroll
-      for (int r = 0; r < GG; ++r) {
-        const size_t o = off + (size_t)yclamp(j0 + g0 + r, jee) * s1;
-        rh[r] = rhs[o];
-        rp[r] = rhsp[o];
-        uu[r] = u[o];
-      }
-#pragma unroll
-      for (int r = 0; r < GG; ++r) {
-        const int j = j0 + g0 + r;
-        if (act && j < jee) t[g0 + r] = uu[r] - (var3 * rh[r] + var2 * t[g0 + r] + var4 * rp[r]);
-        if (q.ok && j < jee) rhsp[off + (size_t)j * s1] = rh[r];
-      }
-      YFENCE();
+    for (int r = 0; r < L; ++r) {
+      if (r % YGRP == 0) YFENCE();
+      const int j = j0 + r;
+      const size_t o = off + (size_t)yclamp(j, jee) * s1;
+      const T rh = rhs[o], rp = rhsp[o], uu = u[o];
+      if (act && j < jee) t[r] = uu - (var3 * rh + var2 * t[r] + var4 * rp);
     }
   }
   YFENCE();
@@ -626,15 +611,16 @@ template <typename T> void ky_rhsu_loop1(Box &b, const T *uv, const T *uw, T *rh
             LAUNCH(b, "rhsu_loop1", g, YTPB, 0, kf, uv, uw, rhsu, ytab<T>(b, YT_DCYU), coefs<T>(b), b.ny, b.nz1, b.nx));
 }
 
-template <typename T> void ky_rhs_loop2(Box &b, const T *vw, const T *vv, const T *ww) {
+template <typename T>
+void ky_rhs_loop2(Box &b, T *rhsu, T *rhsv, T *rhsw, const T *resu, const T *resv, const T *resw, const T *vw,
+                  const T *vv, const T *ww) {
   const unsigned g = ygrid((size_t)b.nz1 * (b.nx - 2));
   YDISPATCH(b.yL, auto kf = &rhs_loop2_y<L, T>;
-            LAUNCH(b, "rhs_loop2", g, YTPB, 0, kf, fld<T>(b.rhsu), fld<T>(b.rhsv), fld<T>(b.rhsw), fld<T>(b.resu),
-                   fld<T>(b.resv), fld<T>(b.resw), vw, vv, ww, ytab<T>(b, YT_DCYU), ytab<T>(b, YT_DCYV), coefs<T>(b),
-                   b.ny, b.nz1, b.nx - 2));
+            LAUNCH(b, "rhs_loop2", g, YTPB, 0, kf, rhsu, rhsv, rhsw, resu, resv, resw, vw, vv, ww,
+                   ytab<T>(b, YT_DCYU), ytab<T>(b, YT_DCYV), coefs<T>(b), b.ny, b.nz1, b.nx - 2));
 }
 
-template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, T *rhsp, int jee) {
+template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, const T *rhsp, int jee) {
   const unsigned g = ygrid((size_t)b.nz1 * b.nx);
   const bool uu = jee == b.ny + 1;
   YDISPATCH(b.yL, auto kf = &rk_final_y<L, T>;
@@ -649,8 +635,9 @@ template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs,
   template void ky_vistyy<T>(Box &, const T *, T *, int, int);                                                 \
   template void ky_implzy<T>(Box &, T *, const T *, int, int);                                                 \
   template void ky_rhsu_loop1<T>(Box &, const T *, const T *, T *);                                            \
-  template void ky_rhs_loop2<T>(Box &, const T *, const T *, const T *);                                        \
-  template void ky_rk_final<T>(Box &, T *, const T *, const T *, T *, int);
+  template void ky_rhs_loop2<T>(Box &, T *, T *, T *, const T *, const T *, const T *, const T *, const T *,     \
+                                const T *);                                                                    \
+  template void ky_rk_final<T>(Box &, T *, const T *, const T *, const T *, int);
 YINST(double)
 YINST(float)
 
