@@ -54,10 +54,12 @@ def kernel_model(cfg, s=8):
     m["interpyy"] = ((2 * Ru * X + (Ru + Rv) * (X - 2) + (Rv + Ru) * (X - 2)) * z * s / 3, 0.0, 3)
     m["rhsu_loop1"] = ((Rv + 2 * Ru) * f, 0.0, 1)
     m["rhs_loop2"] = ((X - 2) * z * s * (2 * Ru + 2 * Rv + 2 * Ru + Ru + Rv + Ru + Ru + Rv), 0.0, 1)
-    m["rk_final"] = ((6 * Ru + 6 * Rv + 6 * Ru) * f / 3, 0.0, 3)
+    # res, rhs, rhs_prev, u read; u written (rhs becomes rhs_prev by a role swap)
+    m["rk_final"] = ((5 * Ru + 5 * Rv + 5 * Ru) * f / 3, 0.0, 3)
     m["save_predict"] = ((4 * Ru + 3 * Rv) * f, 0.0, 1)  # u, v, w, p read; predicted u, v, w written
     N, L = cfg.nx - 1, (cfg.ny - 1) * z
-    M = 1 << (2 * N - 2).bit_length()  # power-of-two Bluestein length >= 2N-1
+    # Bluestein length: the shortest engine length >= 2N-1 (3 x 1024 or a power of two)
+    M = next(m for m in (512, 1024, 2048, 3072, 4096, 8192) if m >= 2 * N - 1)
     # Poisson x transforms: one pencil read and written; Bluestein FFTs of
     # length M, two lines per complex transform pair (forward + inverse)
     for nm in ("dct2_rows", "dct3_rows"):
@@ -71,7 +73,7 @@ def kernel_model(cfg, s=8):
 
 # kernel-event names -> HIP kernel function names in rocprofv3 output
 TRACE_NAMES = {"dct2_rows": "dct_rows_z", "dct3_rows": "dct_rows_z", "div": "div_pencils_kernel",
-               "poisson_y": "poisson_y_pencil_kernel", "to_planes": "pencils_to_planes_kernel",
+               "poisson_y": "poisson_y_chunk", "to_planes": "pencils_to_planes_kernel",
                "products1": "products1_z", "products2": "products2_z", "interpxx": "interpxx_w",
                "difvisxx": "difvisxx_w", "interpyy": "interpyy_y", "rhsu_loop1": "rhsu_loop1_y",
                "rhs_loop2": "rhs_loop2_y", "rk_final": "rk_final_y"}
@@ -87,8 +89,6 @@ def profiled_traffic(workload, precision, kname):
     if not os.path.exists(path):
         return None
     fn = TRACE_NAMES.get(kname, kname + "_kernel")
-    if precision == "mixed" and kname == "poisson_y":
-        fn = "poisson_y_mixed_kernel"
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         name = cells[0] if cells else ""

@@ -186,9 +186,13 @@ static FFTPlan make_plan(int n) {
 // conjugate chirp wrapped to length M (divided by M), and exp(-i pi k/2N).
 static void build_dct_fft(Box &b) {
   const int N = b.nx - 1;
-  // a power-of-two convolution length runs on the register engine
-  int M2 = 1;
-  while (M2 < 2 * N - 1) M2 *= 2;
+  // the register engine takes the shortest of its lengths that holds the
+  // convolution (OTBL_DCT_POW2: the shortest power of two)
+  int M2 = zdct_length(2 * N - 1);
+  if (std::getenv("OTBL_DCT_POW2")) {
+    M2 = 1;
+    while (M2 < 2 * N - 1) M2 *= 2;
+  }
   b.dct_zfast = zdct_supported(M2) && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
   const int M = b.dct_zfast ? M2 : good_length(2 * N - 1);
   const double pi = b.hc.pi;
@@ -500,9 +504,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     // power-of-two convolution length is instantiated, else on the LDS
     // Stockham kernel when its buffers fit; dense matrices otherwise
     const int N = b->nx - 1;
-    int M2 = 1;
-    while (M2 < 2 * N - 1) M2 *= 2;
-    const bool zf = zdct_supported(M2) && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
+    const bool zf = zdct_length(2 * N - 1) != 0 && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
     const int M = good_length(2 * N - 1);
     b->dct_fft = std::getenv("OTBL_DCT_GEMM") == nullptr && (zf || k_dct_shm_bytes(M) <= 160 * 1024);
   }

@@ -34,7 +34,7 @@ static constexpr int PK = 32;     // chunks per column = columns per workgroup
 static constexpr int PROWS = 512; // rows the LDS coefficient copy holds (ny - 1 <= PROWS)
 
 template <int L, typename T, bool REF>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1)
 poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, int nz1) {
   // interface system per chunk and lane column: yF, yL, VF, VL, WF, WL; then
   // the interface unknowns F, L
@@ -155,12 +155,23 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
     red[k][5][col] = WL;
     __syncthreads();
     if (wave == 0 && lane < PK) {
-      // L(q) = al(q) + be(q) F(q+1), F(q) = ga(q) + de(q) F(q+1)
+      // L(q) = al(q) + be(q) F(q+1), F(q) = ga(q) + de(q) F(q+1); the LDS
+      // reads of step q+1 are issued before step q's arithmetic (the sweep
+      // is a chain of dependent steps run by one half-wave)
       T al = 0, be = 0;
-#pragma unroll 1
+      T n0 = red[0][0][col], n1 = red[0][1][col], n2 = red[0][2][col], n3 = red[0][3][col];
+      T n4 = red[0][4][col], n5 = red[0][5][col];
+#pragma unroll 2
       for (int q = 0; q < PK; ++q) {
-        const T qyF = red[q][0][col], qyL = red[q][1][col], qVF = red[q][2][col], qVL = red[q][3][col];
-        const T qWF = red[q][4][col], qWL = red[q][5][col];
+        const T qyF = n0, qyL = n1, qVF = n2, qVL = n3, qWF = n4, qWL = n5;
+        if (q + 1 < PK) {
+          n0 = red[q + 1][0][col];
+          n1 = red[q + 1][1][col];
+          n2 = red[q + 1][2][col];
+          n3 = red[q + 1][3][col];
+          n4 = red[q + 1][4][col];
+          n5 = red[q + 1][5][col];
+        }
         const T den = T(1) / (T(1) + qVF * be);
         const T ga = (qyF - qVF * al) * den, de = -qWF * den;
         const T nal = qyL - qVL * (al + be * ga), nbe = -qVL * be * de - qWL;
@@ -172,10 +183,18 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
         be = nbe;
       }
       T Fn = 0; // F(q+1)
-#pragma unroll 1
+      T m0 = red[PK - 1][0][col], m1 = red[PK - 1][1][col], m2 = red[PK - 1][2][col], m3 = red[PK - 1][3][col];
+#pragma unroll 2
       for (int q = PK - 1; q >= 0; --q) {
-        itf[q][1][col] = red[q][2][col] + red[q][3][col] * Fn; // L(q)
-        Fn = red[q][0][col] + red[q][1][col] * Fn;
+        const T ga = m0, de = m1, nal = m2, nbe = m3;
+        if (q > 0) {
+          m0 = red[q - 1][0][col];
+          m1 = red[q - 1][1][col];
+          m2 = red[q - 1][2][col];
+          m3 = red[q - 1][3][col];
+        }
+        itf[q][1][col] = nal + nbe * Fn; // L(q)
+        Fn = ga + de * Fn;
         itf[q][0][col] = Fn; // F(q)
       }
     }

@@ -40,6 +40,11 @@ template <> struct ZPlan<2048> {
   static constexpr int W = 2, NP = 3;
   static constexpr int rad(int p) { return p < 2 ? 16 : 8; }
 };
+// 3 x 1024: the Bluestein length of the x transforms for N up to 1536
+template <> struct ZPlan<3072> {
+  static constexpr int W = 4, NP = 5;
+  static constexpr int rad(int p) { return p == 0 ? 12 : 4; }
+};
 template <> struct ZPlan<4096> {
   static constexpr int W = 4, NP = 3;
   static constexpr int rad(int p) { return 16; }
@@ -53,7 +58,17 @@ template <> struct ZPlan<8192> {
 // lengths of the z products and of the Poisson x transforms (Bluestein
 // convolution length) that the engine is instantiated for
 inline bool zfft_supported(int n) { return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096; }
-inline bool zdct_supported(int m) { return m == 512 || m == 1024 || m == 2048 || m == 4096 || m == 8192; }
+inline bool zdct_supported(int m) {
+  return m == 512 || m == 1024 || m == 2048 || m == 3072 || m == 4096 || m == 8192;
+}
+// the shortest engine length that holds a Bluestein convolution of length
+// n (0: none; convolutions of at most 256 points stay on the LDS kernels)
+inline int zdct_length(int n) {
+  if (n <= 256) return 0;
+  for (int m : {512, 1024, 2048, 3072, 4096, 8192})
+    if (m >= n) return m;
+  return 0;
+}
 inline int zfft_waves(int n) { return n <= 1024 ? 1 : n / 1024; }
 
 // padded LDS index of element a
@@ -113,8 +128,43 @@ template <int S, typename C> __device__ __forceinline__ void bfly16(C *v) {
   }
 }
 
+// 12-point DFT as 3 x 4: inner 4-point DFTs over stride-3 inputs
+// v[b + 3a], twiddles W12^{bq}, outer 3-point DFTs over b; X[q + 4p]
+template <int S, typename C> __device__ __forceinline__ void bfly12(C *v) {
+  using T = decltype(v[0].x);
+  C y[3][4];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    C t[4] = {v[b], v[b + 3], v[b + 6], v[b + 9]};
+    bfly4<S>(t);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[b][q] = t[q];
+  }
+  // W12^k = cos(pi k / 6) + S i sin(pi k / 6), k = b q in {1, 2, 3, 4, 6}
+  const T hr = (T)0.86602540378443864676;
+  auto tw = [&](C a, int k) -> C {
+    if (k == 3) return mul_si<S>(a);
+    if (k == 6) return mkc<C>(-a.x, -a.y);
+    const T cr = k == 1 ? hr : (k == 2 ? T(0.5) : T(-0.5));
+    const T si = S * (k == 1 ? T(0.5) : hr);
+    return mkc<C>(a.x * cr - a.y * si, a.x * si + a.y * cr);
+  };
+#pragma unroll
+  for (int b = 1; b < 3; ++b)
+#pragma unroll
+    for (int q = 1; q < 4; ++q) y[b][q] = tw(y[b][q], b * q);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    C t[3] = {y[0][q], y[1][q], y[2][q]};
+    bfly3<S>(t);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) v[q + 4 * p] = t[p];
+  }
+}
+
 template <int R, int S, typename C> __device__ __forceinline__ void zbfly(C *v) {
   if constexpr (R == 16) bfly16<S>(v);
+  else if constexpr (R == 12) bfly12<S>(v);
   else bfly<R, S>(v);
 }
 

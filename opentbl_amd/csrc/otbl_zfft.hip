@@ -378,6 +378,7 @@ template <typename R> static void dct_launch(Box &b, R *P, int iopt, int Lp, con
     ZDCT(512)
     ZDCT(1024)
     ZDCT(2048)
+    ZDCT(3072)
     ZDCT(4096)
     ZDCT(8192)
 #undef ZDCT

@@ -90,7 +90,8 @@ def test_engine_matches_stockham(zbox):
     assert all(e < STEP_TOL for e in errs.values()), errs
 
 
-@pytest.fixture(scope="module", params=[257, 1025, 1536], ids=["M512", "M2048", "M4096"])
+@pytest.fixture(scope="module", params=[257, 1025, 1536, 1537, 2049], ids=["M512", "M2048", "M3072", "M3072max",
+                                                                           "M4096"])
 def xbox(request, oracle_lib):
     nx = request.param
     cfg = config.small(name=f"x{nx}", nx=nx, ny=40, nz=24, Lx=8.0 * nx / 32)
@@ -100,11 +101,12 @@ def xbox(request, oracle_lib):
 
 @pytest.mark.parametrize("iopt", [-1, 1])
 def test_cosftx_long(xbox, oracle_lib, iopt):
-    """Poisson x transforms with power-of-two Bluestein lengths against the
-    oracle's direct DCT sums."""
+    """Poisson x transforms on the engine's Bluestein lengths (powers of two
+    and 3 x 1024) against the oracle's direct DCT sums."""
     cfg, st = xbox
     gb = GpuBox(cfg, st)
-    assert gb.paths()["dct"].startswith("engine"), gb.paths()
+    want = next(m for m in (512, 1024, 2048, 3072, 4096) if m >= 2 * (cfg.nx - 1) - 1)
+    assert gb.paths()["dct"] == f"engine{want}", gb.paths()
     f = np.ascontiguousarray(np.random.default_rng(cfg.nx + iopt).standard_normal((cfg.nx, cfg.ny, cfg.nz1)))
     lines = np.ascontiguousarray(f[1:, :cfg.ny - 1, :].transpose(1, 2, 0))
     oracle_lib.lib().oc_cosft(cfg.nx - 1, lines.shape[0] * lines.shape[1], optr(lines), iopt)
