@@ -311,7 +311,7 @@ static void free_box(Box *b) {
   double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
                     b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
-                    b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax, b->P0, b->S0, b->p0, b->um64, b->rm64, b->tm64};
+                    b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax, b->P0, b->S0, b->p0, b->um64, b->rm64};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
   float *fbufs[] = {b->coefbuf_f, b->ytab_f, b->ztw_f, b->dct_chirp_f, b->dct_bf_f, b->dct_tw4_f, b->dct_ztw_f};
@@ -441,7 +441,6 @@ static void setup_mixed(Box &b) {
   dalloc(b.p0, (size_t)b.nx * b.ny);
   dalloc(b.um64, (size_t)b.nx * (b.ny + 1));
   dalloc(b.rm64, (size_t)b.nx * (b.ny + 1));
-  dalloc(b.tm64, (size_t)b.nx * (b.ny + 1));
 }
 
 extern "C" {

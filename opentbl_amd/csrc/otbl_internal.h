@@ -90,8 +90,8 @@ struct Box {
   double *p0 = nullptr;
   // mixed: the mean-flow u (real part of mode 0 of u) in FP64, [ny+1][nx],
   // updated in FP64 wherever the substep updates u (the FP32 field holds its
-  // rounding); its saved copy and a scratch of the same size
-  double *um64 = nullptr, *rm64 = nullptr, *tm64 = nullptr;
+  // rounding) and its saved copy
+  double *um64 = nullptr, *rm64 = nullptr;
   // persistent state; every field and work-field pointer below holds
   // elements of esz bytes (read through fld<T>)
   double *f[OTBL_NFIELDS] = {nullptr};

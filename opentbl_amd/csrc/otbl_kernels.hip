@@ -1314,102 +1314,8 @@ __global__ void rk_final_kernel(double *u, const double *res, const double *rhs,
   implzy_col(uu, t, s, jee, jee == ny + 1 ? c.vyui : c.vyvi, f);
 }
 
-// FP64 mean-flow u of the mixed substep (fields FP32): the explicit RK3
-// combination and implicit y solve of rk_final_kernel for the kz = 0 column
-// of u of every plane, from the FP64 state um and its saved copy rm, with
-// the FP32 rhs and previous rhs (mode-0 real parts).  Runs before the FP32
-// rk_final of u; mean_to_field_kernel copies the result into the FP32 field
-// afterwards.  One thread per plane, arrays
-// [ny+1][nx] (adjacent planes adjacent); tm: scratch of that size.
-template <typename T>
-__global__ void mean_rk_final_kernel(double *__restrict__ um, const double *__restrict__ rm, const T *rhs,
-                                     const T *rhsp, double *__restrict__ tm, DevCoef c, const double *f,
-                                     const DevScalars *ds, int ny, int nz1, int nx) {
-  constexpr int RB = 8; // rows whose loads are issued together
-  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= nx || ip == 0) return; // plane 1 keeps u
-  const int jee = ny + 1;
-  const size_t pu = (size_t)nz1 * jee;
-  double *U = um + ip, *t = tm + ip;
-  const double *R = rm + ip;
-  const T *rh = rhs + ip * pu, *rp = rhsp + ip * pu;
-  const double var2 = (ip == nx - 1) ? 0.0 : ds->var2, var3 = ds->var3, var4 = ds->var4;
-  auto O = [&](int j) { return (size_t)(j - 1) * nx; }; // row j of a [ny+1][nx] array
-  // batched sweep over rows j = from, from + dir, ..., to (inclusive): the
-  // loads of RB rows first, then their updates in order
-  auto sweep = [&](int from, int to, int dir, auto load, auto step) {
-    using V = decltype(load(from));
-    int j = from;
-    const int n = (to - from) * dir + 1;
-    int done = 0;
-    for (; done + RB <= n; done += RB, j += RB * dir) {
-      V v[RB];
-#pragma unroll
-      for (int q = 0; q < RB; ++q) v[q] = load(j + q * dir);
-#pragma unroll
-      for (int q = 0; q < RB; ++q) step(j + q * dir, v[q]);
-    }
-    for (; done < n; ++done, j += dir) step(j, load(j));
-  };
-  // vistyy of the saved u (lip.f90:447-513): right-hand side and forward
-  // elimination, rows ascending (rm rows j-1, j, j+1 rolled in registers)
-  const double *icon = c.viyu, *econ = c.cofvyu;
-  double rjm = R[O(1)], rj = R[O(2)];
-  double y = M4(c.cofvby, 3, 1) * R[O(1)] + M4(c.cofvby, 3, 2) * R[O(2)] + M4(c.cofvby, 3, 3) * R[O(3)] +
-             M4(c.cofvby, 3, 4) * R[O(4)];
-  t[O(1)] = y;
-  sweep(2, jee - 1, 1, [&](int j) { return R[O(j + 1)]; }, [&](int j, double rjp) {
-    const double o = C3(econ, 1, j) * rjp + C3(econ, 2, j) * rj + C3(econ, 3, j) * rjm;
-    y = o + C3(icon, 1, j) * y;
-    t[O(j)] = y;
-    rjm = rj;
-    rj = rjp;
-  });
-  {
-    const double o = M4(c.cofvby, 4, 1) * R[O(jee)] + M4(c.cofvby, 4, 2) * R[O(jee - 1)] +
-                     M4(c.cofvby, 4, 3) * R[O(jee - 2)] + M4(c.cofvby, 4, 4) * R[O(jee - 3)];
-    y = o + C3(icon, 1, jee) * y;
-  }
-  // back substitution (rows descending) and the explicit RK3 combination
-  // (rhsp.F90:448-475): t = u - (var3 rhs + var2 vistyy + var4 rhs_prev)
-  double x = y * C3(icon, 2, jee);
-  t[O(jee)] = U[O(jee)] - (var3 * (double)rh[(size_t)(jee - 1) * nz1] + var2 * x +
-                           var4 * (double)rp[(size_t)(jee - 1) * nz1]);
-  struct Row4 { double y, u, r, q; };
-  sweep(jee - 1, 1, -1,
-        [&](int j) {
-          return Row4{t[O(j)], U[O(j)], (double)rh[(size_t)(j - 1) * nz1], (double)rp[(size_t)(j - 1) * nz1]};
-        },
-        [&](int j, Row4 w) {
-          x = (w.y - C3(icon, 3, j) * x) * C3(icon, 2, j);
-          t[O(j)] = w.u - (var3 * w.r + var2 * x + var4 * w.q);
-        });
-  // implzy (lip.f90:528-597): interior rows 2..jee-1 of u; rows 1 and jee
-  // are the boundary values in place
-  const double *a1 = f, *a2 = f + (jee + 1), *a3 = f + 2 * (jee + 1);
-  const double a1b = f[3 * (jee + 1)], a3b = f[3 * (jee + 1) + 1];
-  const double *vic = c.vyui;
-  const double u1 = U[O(1)], ujee = U[O(jee)];
-  double tm1 = t[O(1)], t0 = t[O(2)];
-  y = 0.0;
-  sweep(2, jee - 1, 1, [&](int j) { return t[O(j + 1)]; }, [&](int j, double tp) {
-    double r = C3(vic, 1, j) * tm1 + C3(vic, 2, j) * t0 + C3(vic, 3, j) * tp;
-    if (j == 2) r -= a1b * u1;
-    if (j == jee - 1) r -= a3b * ujee;
-    y = (j == 2) ? r : r + a1[j] * y;
-    U[O(j)] = y;
-    tm1 = t0;
-    t0 = tp;
-  });
-  x = y * a2[jee - 1];
-  U[O(jee - 1)] = x;
-  if (jee - 2 >= 2)
-    sweep(jee - 2, 2, -1, [&](int j) { return U[O(j)]; }, [&](int j, double uy) {
-      x = (uy - a3[j] * x) * a2[j];
-      U[O(j)] = x;
-    });
-}
-
+// The FP64 mean-flow u ([ny+1][nx]) rounded into the kz = 0 real part of
+// the FP32 field after its rk_final.
 template <typename T>
 __global__ void mean_to_field_kernel(T *u, const double *um, int nx, int jee, int nz1) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1901,9 +1807,8 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   launch_impl_factor(b, 0.0, 0);
   if (b.yL) {
     ky_rk_final<T>(b, v, resv, rhsv, rp[1], ny);
-    if (um) // the FP64 mean-flow u first: it reads the previous rhs
-      LAUNCH(b, "rk_mean", nblk(nx, 64), 64, 0, mean_rk_final_kernel<T>, um, (const double *)b.rm64, rhsu, rp[0],
-             b.tm64, b.dc, b.implu, b.ds, ny, nz1, nx);
+    if constexpr (!F64) // the FP64 mean-flow u (kz = 0 real part)
+      ky_mean_rk_final<T>(b, um, b.rm64, rhsu, rp[0]);
     ky_rk_final<T>(b, u, resu, rhsu, rp[0], ny + 1);
     if (um)
       LAUNCH(b, "rk_mean", nblk((size_t)nx * (ny + 1)), TPB, 0, mean_to_field_kernel<T>, u, (const double *)um, nx,

@@ -163,5 +163,7 @@ template <typename T>
 void ky_rhs_loop2(Box &b, T *rhsu, T *rhsv, T *rhsw, const T *resu, const T *resv, const T *resw, const T *vw,
                   const T *vv, const T *ww);
 template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, const T *rhsp, int jee);
+// rk_final of the FP64 mean-flow u (kz = 0 real part) of the mixed substep
+template <typename T> void ky_mean_rk_final(Box &b, double *um, const double *rm, const T *rhs, const T *rhsp);
 
 } // namespace otbl

@@ -554,6 +554,62 @@ rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, const T *
   }
 }
 
+// rk_final of the FP64 mean-flow u of the mixed substep (FP32 fields): the
+// explicit RK3 combination with vistyy (rhsp.F90:448-475) and the implicit y
+// solve (rhsp.F90:479-484) of the kz = 0 real part of u, every plane a
+// column.  um and its saved copy rm are [ny+1][nx] (plane fastest: column =
+// plane, row stride nx); rhs and rhs_prev are the FP32 fields (mode 0 of
+// plane ip at ip (ny+1) nz1, row stride nz1).  FP64 tables and arithmetic.
+template <int L, typename T>
+__global__ void __launch_bounds__(YTPB)
+mean_rk_final_y(double *um, const double *__restrict__ rm, const T *__restrict__ rhs, const T *__restrict__ rhsp,
+                const double *__restrict__ tabv, const double *__restrict__ tabi, DevCoef c,
+                const double *__restrict__ f, const DevScalars *__restrict__ ds, int ny, int nz1, int nx) {
+  __shared__ double xch[2 * YNCH * 64];
+  __shared__ double tlv[ytab_size(L)];
+  __shared__ double tli[ytab_size(L)];
+  __shared__ double hal[2 * YNCH * 64];
+  ytab_stage<L>(tlv, tabv);
+  ytab_stage<L>(tli, tabi);
+  const int jee = ny + 1;
+  const YCol q = ycol((size_t)nx, nx, 0); // column kr = plane index
+  const int ip = q.kr;
+  const size_t s = nx, off = ip;
+  const T *rh = rhs + (size_t)ip * nz1 * jee, *rp = rhsp + (size_t)ip * nz1 * jee;
+  const bool act = q.ok && ip > 0; // plane 1 keeps u
+  const int j0 = q.p * L;
+  const YBnd<double> bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, rm + off, s);
+  const double u1 = um[off], uj = um[off + (size_t)(jee - 1) * s];
+  const double a1b = f[3 * (jee + 1)], a3b = f[3 * (jee + 1) + 1];
+  const double var2 = ip == nx - 1 ? 0.0 : ds->var2, var3 = ds->var3, var4 = ds->var4;
+  __syncthreads();
+  double t[L];
+  ysolve_col<L>(t, rm + off, s, jee, tlv, q.p, q.lane, jee, bd.bot, bd.top, xch);
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    if (r % YGRP == 0) YFENCE();
+    const int j = j0 + r, jc = yclamp(j, jee);
+    const double rr = (double)rh[(size_t)jc * nz1], rq = (double)rp[(size_t)jc * nz1];
+    const double uu = um[off + (size_t)jc * s];
+    if (act && j < jee) t[r] = uu - (var3 * rr + var2 * t[r] + var4 * rq);
+  }
+  YFENCE();
+  hal[q.p * 64 + q.lane] = t[0];
+  hal[(YNCH + q.p) * 64 + q.lane] = t[L - 1];
+  __syncthreads();
+  const double tm = q.p > 0 ? hal[(YNCH + q.p - 1) * 64 + q.lane] : 0.0;
+  const double tp = q.p < YNCH - 1 ? hal[(q.p + 1) * 64 + q.lane] : 0.0;
+  const double yend = yfwd<L>(
+      t, [&](int k) { return k == 0 ? tm : (k == L + 1 ? tp : t[k - 1]); }, tli, q.p, jee, u1, uj, -a1b * u1,
+      -a3b * uj);
+  ybwd<L>(t, yend, tli, q.p, q.lane, xch);
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    const int j = j0 + r;
+    if (act && j >= 1 && j < jee - 1) um[off + (size_t)j * s] = t[r];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -628,6 +684,14 @@ template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs,
                    ytab<T>(b, uu ? YT_IMPU : YT_IMPV), coefs<T>(b), uu ? b.implu : b.implv, b.ds, jee, b.ny, b.nz1,
                    b.nx));
 }
+
+template <typename T> void ky_mean_rk_final(Box &b, double *um, const double *rm, const T *rhs, const T *rhsp) {
+  const unsigned g = ygrid((size_t)b.nx);
+  YDISPATCH(b.yL, auto kf = &mean_rk_final_y<L, T>;
+            LAUNCH(b, "rk_mean", g, YTPB, 0, kf, um, rm, rhs, rhsp, ytab<double>(b, YT_VIYU),
+                   ytab<double>(b, YT_IMPU), b.dc, b.implu, b.ds, b.ny, b.nz1, b.nx));
+}
+template void ky_mean_rk_final<float>(Box &, double *, const double *, const float *, const float *);
 
 #define YINST(T)                                                                                               \
   template void ky_interpyy<T>(Box &, const T *, T *, int, int, int, size_t, size_t);                          \
