@@ -840,19 +840,22 @@ void k_cosftx(Box &b, const double *in, double *out, int iopt) {
 // ---------------------------------------------------------------------------
 // rhsp pieces
 // ---------------------------------------------------------------------------
-// Element-wise kernels over u-type fields: element e = ((i-1)(ny+1) +
-// j-1) nz1 + kr of a 1-D grid, split with 32-bit divisions (a field has
-// fewer than 2^32 elements, checked at setup).  (Plain block order: an
-// XCD-contiguous order measured slower for these streaming kernels.)
-__device__ __forceinline__ bool uelem(int nz1, int R, int nx, size_t &e, int &kr, int &j, int &i) {
-  const unsigned ee = blockIdx.x * blockDim.x + threadIdx.x;
-  e = ee;
-  const unsigned row = ee / (unsigned)nz1;
-  kr = (int)(ee - row * (unsigned)nz1);
-  i = (int)(row / (unsigned)R) + 1;
-  j = (int)(row - (unsigned)(i - 1) * (unsigned)R) + 1;
-  return i <= nx;
+// Plane-marching element-wise kernels: a thread owns element (j, kr) of a
+// run of pm planes (blockIdx.y), so a value of plane i+1 it reads is its own
+// plane-i value of the next iteration (read from HBM once).  Run lengths as
+// measured fastest on C3 (runs of 1..32 planes timed): 4 planes for FP32
+// fields, 2 for FP64.
+template <typename T> static inline int prun() { return sizeof(T) == 4 ? 4 : 2; }
+__device__ __forceinline__ bool pcol(int nz1, int R, int &kr, int &j) {
+  const unsigned el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= (unsigned)nz1 * (unsigned)R) return false;
+  const unsigned row = el / (unsigned)nz1;
+  kr = (int)(el - row * (unsigned)nz1);
+  j = (int)row + 1;
+  return true;
 }
+static inline dim3 pgrid(const Box &b, int pm) { return dim3(nblk(b.pu), (unsigned)((b.nx + pm - 1) / pm)); }
+
 __global__ void reset_max_kernel(DevScalars *ds, unsigned long long *vmbits, int ny) {
   for (int j = threadIdx.x; j < ny; j += blockDim.x) vmbits[j] = 0ull;
   if (threadIdx.x == 0) {
@@ -936,53 +939,59 @@ __global__ void save_predict_kernel(const T *__restrict__ u, const T *__restrict
                                     const T *__restrict__ p, T *__restrict__ un, T *__restrict__ vn,
                                     T *__restrict__ wn, const T *idyy, const T *kaz, const DevScalars *ds,
                                     double idxx_d, int nx, int ny, int nz1, const double *__restrict__ p0,
-                                    const double *idyy_d, double *um, double *rm) {
-  size_t e;
-  int kr, j, i;
-  if (!uelem(nz1, ny + 1, nx, e, kr, j, i)) return;
+                                    const double *idyy_d, double *um, double *rm, int pm) {
+  int kr, j;
+  if (!pcol(nz1, ny + 1, kr, j)) return;
+  const int i0 = (int)blockIdx.y * pm + 1, i1 = min(nx, i0 + pm - 1);
   const size_t pu = (size_t)nz1 * (ny + 1), pv = (size_t)nz1 * ny;
   const T var1 = (T)ds->var1, idxx = (T)idxx_d;
-  size_t ev = (size_t)(i - 1) * pv + (size_t)(j - 1) * nz1 + kr;
-  const T uu = u[e], ww = w[e];
-  const bool mean = p0 && kr == 0;
-  const double *pm = mean ? p0 + (size_t)(i - 1) * ny + (j - 1) : nullptr; // mean pressure at (i, j)
-  const bool upred = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny;
-  T uo = uu;
-  if (um && kr == 0) {
-    const size_t q = (size_t)(j - 1) * nx + (i - 1);
-    double U = um[q];
-    rm[q] = U;
-    if (upred) {
-      U -= ds->var1 * idxx_d * (pm[ny] - pm[0]);
-      um[q] = U;
-      uo = (T)U;
-    }
-  } else if (upred) {
-    if (mean)
-      uo = (T)((double)uu - ds->var1 * idxx_d * (pm[ny] - pm[0]));
-    else
-      uo = uu - var1 * idxx * (p[ev + pv] - p[ev]);
-  }
-  un[e] = uo;
-  T wo = ww;
-  if (i >= 2 && j >= 2 && j <= ny) {
-    // w -= var1 * (i kz p): Re gets -(-kz Im p), Im gets -(kz Re p)
-    T kz = kaz[kr / 2];
-    T pp = p[ev ^ 1];
-    T t = (kr & 1) ? (kz * pp) : (-kz * pp);
-    wo = ww - var1 * t;
-  }
-  wn[e] = wo;
-  if (j <= ny) {
-    const T vv = v[ev];
-    T vo = vv;
-    if (i >= 2 && j >= 2 && j <= ny - 1) {
+  const bool mean = p0 && kr == 0, vrow = j <= ny;
+  size_t e = (size_t)(i0 - 1) * pu + (size_t)(j - 1) * nz1 + kr;
+  size_t ev = (size_t)(i0 - 1) * pv + (size_t)(j - 1) * nz1 + kr;
+  T pc = vrow ? p[ev] : T(0); // p(i, j), carried from plane to plane
+  for (int i = i0; i <= i1; ++i, e += pu, ev += pv) {
+    const T pn = (vrow && i < nx) ? p[ev + pv] : T(0); // p(i+1, j)
+    const T uu = u[e], ww = w[e];
+    const double *pm = mean ? p0 + (size_t)(i - 1) * ny + (j - 1) : nullptr; // mean pressure at (i, j)
+    const bool upred = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny;
+    T uo = uu;
+    if (um && kr == 0) {
+      const size_t q = (size_t)(j - 1) * nx + (i - 1);
+      double U = um[q];
+      rm[q] = U;
+      if (upred) {
+        U -= ds->var1 * idxx_d * (pm[ny] - pm[0]);
+        um[q] = U;
+        uo = (T)U;
+      }
+    } else if (upred) {
       if (mean)
-        vo = (T)((double)vv - ds->var1 * idyy_d[j] * (pm[1] - pm[0]));
+        uo = (T)((double)uu - ds->var1 * idxx_d * (pm[ny] - pm[0]));
       else
-        vo = vv - var1 * idyy[j] * (p[ev + nz1] - p[ev]);
+        uo = uu - var1 * idxx * (pn - pc);
     }
-    vn[ev] = vo;
+    un[e] = uo;
+    T wo = ww;
+    if (i >= 2 && j >= 2 && j <= ny) {
+      // w -= var1 * (i kz p): Re gets -(-kz Im p), Im gets -(kz Re p)
+      T kz = kaz[kr / 2];
+      T pp = p[ev ^ 1];
+      T t = (kr & 1) ? (kz * pp) : (-kz * pp);
+      wo = ww - var1 * t;
+    }
+    wn[e] = wo;
+    if (vrow) {
+      const T vv = v[ev];
+      T vo = vv;
+      if (i >= 2 && j >= 2 && j <= ny - 1) {
+        if (mean)
+          vo = (T)((double)vv - ds->var1 * idyy_d[j] * (pm[1] - pm[0]));
+        else
+          vo = vv - var1 * idyy[j] * (p[ev + nz1] - pc);
+      }
+      vn[ev] = vo;
+    }
+    pc = pn;
   }
 }
 
@@ -1125,9 +1134,41 @@ __global__ void genflu_stats_kernel(const T *resu, double *um, int *ibuf, double
     fsca[kk * (ny + 1) + q] = 0.0;
   }
   __syncthreads();
-  // the interpolation tables: rows in order, the bisection carrying its
-  // lower bound from row to row (one thread per profile)
-  if (threadIdx.x < 2) {
+  // the interpolation tables.  The reference bisects the rows in order,
+  // carrying the lower bound from row to row; where wl is non-decreasing
+  // (the rescaled coordinate of a monotone profile) a bisection from row 1
+  // finds the same interval, so the rows are searched in parallel, and in
+  // order (one thread per profile) otherwise.
+  __shared__ int sh_mono, sh_ntop[2];
+  if (threadIdx.x == 0) {
+    sh_mono = 1;
+    sh_ntop[0] = sh_ntop[1] = 0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 2 * (ny - 1); e += blockDim.x) {
+    const int kk = e / (ny - 1), q = e % (ny - 1) + 2;
+    if (wlwa[kk * S_ + q] < wlwa[kk * S_ + q - 1]) atomicAnd(&sh_mono, 0);
+  }
+  __syncthreads();
+  if (sh_mono) {
+    for (int e = threadIdx.x; e < 2 * ny; e += blockDim.x) {
+      const int kk = e / ny, q = e % ny + 1;
+      const double *ys = ysca + kk * S_;
+      const double w = wlwa[kk * S_ + q];
+      if (w <= ys[ny]) {
+        int kll = 1, khh = ny;
+        while (khh - kll > 1) {
+          const int km = (khh + kll) / 2;
+          if (ys[km] >= w) khh = km; else kll = km;
+        }
+        kl[kk * (ny + 1) + q] = kll;
+        fsca[kk * (ny + 1) + q] = (w - ys[kll]) / (ys[khh] - ys[kll]);
+        atomicMax(&sh_ntop[kk], q);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) ntop[threadIdx.x] = sh_ntop[threadIdx.x];
+  } else if (threadIdx.x < 2) {
     const int kk = threadIdx.x;
     const double *ys = ysca + kk * S_, *wl = wlwa + kk * S_;
     int kll = 1, khh;
@@ -1571,19 +1612,22 @@ poisson_y_pencil_kernel(double *__restrict__ P, double *__restrict__ S, DevCoef 
 template <typename T>
 __global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T> c, const DevScalars *ds,
                                double vstep, int use_m, double idxx_d, int nx, int ny, int nz1,
-                               const double *__restrict__ P0, double *p0, const double *idyy_d, double *um) {
-  size_t e;
-  int kr, j, i;
-  if (!uelem(nz1, ny + 1, nx, e, kr, j, i) || i < 2) return;
+                               const double *__restrict__ P0, double *p0, const double *idyy_d, double *um,
+                               int pm) {
+  int kr, j;
+  // rows 2..ny of planes 2..nx change
+  if (!pcol(nz1, ny + 1, kr, j) || j < 2 || j > ny) return;
+  const int i0 = max(2, (int)blockIdx.y * pm + 1), i1 = min(nx, (int)blockIdx.y * pm + pm);
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   const double vs = use_m ? ds->varstep : vstep;
+  size_t e = (size_t)(i0 - 1) * pu + (size_t)(j - 1) * s + kr;
+  size_t ov = (size_t)(i0 - 1) * pv + (size_t)(j - 1) * s + kr;
   if (P0 && kr == 0) {
     // phi(row r, plane q) = P0[2 (r-1) N + q-2]; kaz(0) = 0 leaves w
     const int N = nx - 1;
     const size_t rN = 2 * (size_t)N;
-    if (j >= 2 && j <= ny) {
+    for (int i = i0; i <= i1; ++i, e += pu, ov += pv) {
       const double phm = P0[(size_t)(j - 2) * rN + (i - 2)];
-      const size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s;
       double &pm = p0[(size_t)(i - 1) * ny + (j - 1)];
       pm = vs * phm + pm;
       p[ov] = (T)pm;
@@ -1602,21 +1646,20 @@ __global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T
     return;
   }
   const T varstep = (T)vs, idxx = (T)idxx_d;
-  const T *ph = rest + (size_t)(i - 1) * pv;
-  if (j >= 2 && j <= ny) {
-    size_t o = (size_t)(j - 2) * s; // phi row j-1
-    size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s;
-    p[ov + kr] = varstep * ph[o + kr] + p[ov + kr];
-    T kz = c.kaz[kr / 2];
-    T pp = ph[o + (kr ^ 1)];
-    T t = (kr & 1) ? kz * pp : -kz * pp;
+  const T kz = c.kaz[kr / 2];
+  const T iy = j <= ny - 1 ? c.idyy[j] : T(0);
+  const size_t o = (size_t)(j - 2) * s + kr; // phi row j-1 (v-type row j-2)
+  const T *ph = rest + (size_t)(i0 - 1) * pv;
+  T phc = i0 <= i1 ? ph[o] : T(0); // phi(i, row j-1), carried from plane to plane
+  for (int i = i0; i <= i1; ++i, e += pu, ov += pv, ph += pv) {
+    const T phn = i <= nx - 1 ? ph[pv + o] : T(0); // phi(i+1, row j-1)
+    p[ov] = varstep * phc + p[ov];
+    const T pp = ph[o - kr + (kr ^ 1)];
+    const T t = (kr & 1) ? kz * pp : -kz * pp;
     w[e] -= t;
-    if (j <= ny - 1) v[ov + kr] -= c.idyy[j] * (ph[(size_t)(j - 1) * s + kr] - ph[o + kr]);
-  }
-  if (i <= nx - 1 && j >= 2 && j <= ny) {
-    const T *pn = rest + (size_t)i * pv;
-    size_t o = (size_t)(j - 2) * s + kr; // phi row j-1 for u row j
-    u[e] -= idxx * (pn[o] - ph[o]);
+    if (j <= ny - 1) v[ov] -= iy * (ph[o + s] - phc);
+    if (i <= nx - 1) u[e] -= idxx * (phn - phc);
+    phc = phn;
   }
 }
 
@@ -1636,9 +1679,10 @@ static void pois_mixed(Box &b, double varstep, bool use_m) {
   kz_dct_pencils<float>(b, P, 1, Lp);
   kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1));
   LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pv, b.P0, nz1);
-  LAUNCH(b, "project", nblk(b.nu), TPB, 0, project_kernel<float>, fld<float>(b.f[OTBL_U]), fld<float>(b.f[OTBL_V]),
-         fld<float>(b.f[OTBL_W]), fld<float>(b.f[OTBL_P]), F, b.dcf, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx,
-         ny, nz1, (const double *)b.P0, b.p0, b.dc.idyy, b.um64);
+  const int pm = prun<float>();
+  LAUNCH(b, "project", pgrid(b, pm), TPB, 0, project_kernel<float>, fld<float>(b.f[OTBL_U]),
+         fld<float>(b.f[OTBL_V]), fld<float>(b.f[OTBL_W]), fld<float>(b.f[OTBL_P]), F, b.dcf, b.ds, varstep,
+         use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1, (const double *)b.P0, b.p0, b.dc.idyy, b.um64, pm);
 }
 
 void k_pois(Box &b, int m, double varstep, bool use_m) {
@@ -1675,9 +1719,10 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
     LAUNCH(b, "poisson_y", nblk(ncol), TPB, 0, poisson_y_kernel, b.rest2, b.rest, b.dc, nx, ny, nz1);
     k_cosftx(b, b.rest2, b.rest, 1);
   }
-  LAUNCH(b, "project", nblk(b.nu), TPB, 0, project_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W],
+  const int pm = prun<double>();
+  LAUNCH(b, "project", pgrid(b, pm), TPB, 0, project_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W],
          b.f[OTBL_P], b.rest, b.dc, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1, (const double *)nullptr,
-         (double *)nullptr, b.dc.idyy, (double *)nullptr);
+         (double *)nullptr, b.dc.idyy, (double *)nullptr, pm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1763,8 +1808,9 @@ template <typename T> static void rhsp_t(Box &b, int m) {
          b.hc.dxmin, b.hc.dtret, b.hc.rex, b.hc.rkc[m], b.hc.rkd[m], b.hc.rkcv[m], b.hc.rkdv[m]);
   // save velocities, pressure-gradient predictor
   double *um = F64 ? nullptr : b.um64; // the FP64 mean-flow u of the mixed substep
-  LAUNCH(b, "save_predict", nblk(b.nu), TPB, 0, save_predict_kernel<T>, u, v, w, p, resu, resv, resw, c.idyy, c.kaz,
-         b.ds, b.hc.idxx, nx, ny, nz1, F64 ? nullptr : (const double *)b.p0, b.dc.idyy, um, b.rm64);
+  const int pm = prun<T>();
+  LAUNCH(b, "save_predict", pgrid(b, pm), TPB, 0, save_predict_kernel<T>, u, v, w, p, resu, resv, resw, c.idyy,
+         c.kaz, b.ds, b.hc.idxx, nx, ny, nz1, F64 ? nullptr : (const double *)b.p0, b.dc.idyy, um, b.rm64, pm);
   // the predicted velocities now live in the former save buffers, and this
   // substep's rhs buffers become the rhs_prev fields
   k_swap_saved(b);
