@@ -64,6 +64,9 @@ interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict_
   // g(i) of the interior plane i being processed: from i = 2, or from the
   // first plane of the segment's lead-in
   T gi = sgm.qs > 0 ? g[(size_t)(sgm.qs * XB) * P] : g[P];
+  // the loads of the next block when it is interior, issued one block ahead
+  T pre[XB];
+  bool hp = false;
   for (int it = sgm.qs; it <= sgm.qe; it += XK) {
 #pragma unroll
     for (int kk = 0; kk < XK; ++kk) {
@@ -74,8 +77,19 @@ interpxx_w(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict_
       const int i0 = qq * XB + 1;
       if (i0 >= 2 && i0 + XB - 1 <= nx - 1) {
         T gn[XB];
+        if (hp) {
 #pragma unroll
-        for (int b = 0; b < XB; ++b) gn[b] = g[(size_t)(i0 + b) * P]; // g(i+1)
+          for (int b = 0; b < XB; ++b) gn[b] = pre[b];
+        } else {
+#pragma unroll
+          for (int b = 0; b < XB; ++b) gn[b] = g[(size_t)(i0 + b) * P]; // g(i+1)
+        }
+        // (FP32 only: FP64 holds no spare registers for it)
+        hp = sizeof(T) == 4 && qq + 1 <= sgm.qe && i0 + 2 * XB - 1 <= nx - 1;
+        if (hp) {
+#pragma unroll
+          for (int b = 0; b < XB; ++b) pre[b] = g[(size_t)(i0 + XB + b) * P];
+        }
 #pragma unroll
         for (int b = 0; b < XB; ++b) {
           const int i = i0 + b;

@@ -126,7 +126,10 @@ __device__ __forceinline__ T bnd4(const T *cf, int ld, int row, const T *col, si
 // and its last row (n-1) `top`; add1 and addm are added to rows 1 and n-2
 // (the boundary columns of implzy).  Rows past n have a zero stencil.
 // Returns y at the chunk's last row (its forward carry-out).
-template <int L, typename T, class Fetch>
+// G: rows per fenced group (a group's input loads are issued together);
+// the FP32 interpolation and rhsu kernels gain from whole-chunk groups, the
+// kernels with more live state lose.
+template <int L, int G = YGRP, typename T, class Fetch>
 __device__ __forceinline__ T yfwd(T (&v)[L], Fetch fetch, const T *tab, int p, int n, T bot, T top, T add1, T addm) {
   constexpr int NL = YNCH * L;
   const T *M = tab + p * L, *E1 = tab + 5 * NL + p * L, *E2 = tab + 6 * NL + p * L, *E3 = tab + 7 * NL + p * L;
@@ -134,7 +137,7 @@ __device__ __forceinline__ T yfwd(T (&v)[L], Fetch fetch, const T *tab, int p, i
   T gm = fetch(0), g0 = fetch(1), y = 0;
 #pragma unroll
   for (int r = 0; r < L; ++r) {
-    if (r % YGRP == 0) YFENCE();
+    if (r % G == 0) YFENCE();
     const int j = j0 + r;
     const T gp = fetch(r + 2);
     T x = E1[r] * gm + E2[r] * g0 + E3[r] * gp;
@@ -191,11 +194,11 @@ __device__ __forceinline__ void ybwd(T (&v)[L], T yend, const T *tab, int p, int
 }
 
 // solve for a chunk whose input rows are read from a global column
-template <int L, typename T>
+template <int L, int G = YGRP, typename T>
 __device__ __forceinline__ void ysolve_col(T (&v)[L], const T *col, size_t s, int nin, const T *tab, int p, int lane,
                                            int n, T bot, T top, T *xch) {
   const int j0 = p * L;
-  const T yend = yfwd<L>(
+  const T yend = yfwd<L, G>(
       v, [&](int k) { return col[(size_t)yclamp(j0 - 1 + k, nin) * s]; }, tab, p, n, bot, top, T(0), T(0));
   ybwd<L>(v, yend, tab, p, lane, xch);
 }
@@ -260,7 +263,8 @@ interpyy_y(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict_
   const YBnd<T> bd = interpyy_bnd(loguv, j0, L, ny, inby, g, s);
   __syncthreads(); // table staged
   T v[L];
-  ysolve_col<L>(v, g, s, loguv == 0 ? ny + 1 : ny, tl, q.p, q.lane, ny, bd.bot, bd.top, xch);
+  ysolve_col<L, sizeof(T) == 4 ? L : YGRP>(v, g, s, loguv == 0 ? ny + 1 : ny, tl, q.p, q.lane, ny, bd.bot, bd.top,
+                                           xch);
 #pragma unroll
   for (int r = 0; r < L; ++r)
     if (q.ok && j0 + r < ny) o[(size_t)(j0 + r) * s] = v[r];
@@ -350,7 +354,7 @@ rhsu_loop1_y(const T *__restrict__ uv, const T *__restrict__ uw, T *__restrict__
   const YBnd<T> bd = differzy_bnd(0, j0, L, ny, c.dcby, g, s);
   __syncthreads();
   T v[L];
-  ysolve_col<L>(v, g, s, ny, tl, q.p, q.lane, ny + 1, bd.bot, bd.top, xch);
+  ysolve_col<L, sizeof(T) == 4 ? L : YGRP>(v, g, s, ny, tl, q.p, q.lane, ny + 1, bd.bot, bd.top, xch);
   const T kz = c.kaz[q.kr / 2];
   const T sg = (q.kr & 1) ? T(1) : T(-1);
   const T *w = uw + q.ip * pu + (q.kr ^ 1);
