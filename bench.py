@@ -99,11 +99,12 @@ def profiled_traffic(workload, precision, kname):
 
 def cpu_baseline(cfg_full):
     """The CPU FP64 oracle on a bounded sample of the same workload: one
-    substep of an nx=32 slab with the box's ny, nz (single thread)."""
+    substep of an nx=64 slab with the box's ny, nz (single thread, about
+    15 s)."""
     from dataclasses import replace
     from oracle.oracle import OracleBox, build
     build()
-    slab = replace(cfg_full, nx=32, name=cfg_full.name + "-slab32",
+    slab = replace(cfg_full, nx=64, name=cfg_full.name + "-slab64",
                    kzmax=min(cfg_full.kzmax, 8)).check()
     st = ic.make_state(slab)
     ob = OracleBox(slab, st)
