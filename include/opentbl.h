@@ -43,8 +43,10 @@ typedef struct otbl_params {
                                 flux, coefficient factorisation, mean-flow
                                 (kz = 0) pressure solve and Poisson residuals;
                                 needs the register z engine (nz 256..4096),
-                                a power-of-two Bluestein length (nx-1 >= 129)
-                                and the chunked y solver (ny <= 384).  Fields
+                                a register-engine Bluestein length
+                                (nx-1 >= 129) and the chunked y solver
+                                (ny <= 384).  The mean-flow (kz = 0) u and p
+                                are kept in FP64 on the device.  Fields
                                 cross otbl_set_field / otbl_get_field as FP64
                                 either way; the operator entry points
                                 (otbl_interpxx ...) stay FP64. */
