@@ -714,6 +714,11 @@ __global__ void __launch_bounds__(TPB) dct_rows_kernel(const double *src, double
   }
 }
 
+// threads of the 64 x 64 tile kernels between planes and pencils (the
+// divergence and the transposes back): 16 rows of the tile per pass, so a
+// thread issues the loads of its 4 rows at once
+static constexpr int DIV_TPB = 1024;
+
 // Tiled transposes between the planes layout (planes 2..nx, lines
 // 0..Lp-1, plane stride ps) and x-contiguous pencils P[l*N + x].
 __global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F, double *P, int Lp, int N, size_t ps) {
@@ -735,22 +740,32 @@ __global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F,
 // With P0, the lines of mode 0 (kr = 0, 1) are taken from the FP64 pencils
 // P0 (line (j-1, kr) at P0[(2 (j-1) + kr) N]) instead of P.
 template <typename T>
-__global__ void __launch_bounds__(TPB)
-pencils_to_planes_kernel(const T *P, T *F, int Lp, int N, size_t ps, const double *P0, int nz1) {
+__global__ void __launch_bounds__(DIV_TPB)
+pencils_to_planes_kernel(const T *__restrict__ P, T *__restrict__ F, int Lp, int N, size_t ps,
+                         const double *__restrict__ P0, int nz1) {
+  constexpr int XR = DIV_TPB / 64, NQ = 64 / XR; // lines per pass, passes
   __shared__ T tile[64][65];
   unsigned bx, by;
   xcd_block2(bx, by); // XCD-contiguous tiles: neighbouring rows' tiles share lines
   const int l0 = (int)bx * 64, x0 = (int)by * 64, t = threadIdx.x;
-  for (int ll = t / 64; ll < 64; ll += TPB / 64) {
-    int l = l0 + ll, x = x0 + (t & 63);
+  const int lane = t & 63, xr = t / 64;
+  // the NQ lines of this thread: every load issued before the first use
+  T a[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int l = l0 + xr + XR * q, x = x0 + lane;
     if (l < Lp && x < N)
-      tile[ll][t & 63] = (P0 && l % nz1 < 2) ? (T)P0[(size_t)(2 * (l / nz1) + l % nz1) * N + x]
-                                             : P[(size_t)l * N + x];
+      a[q] = (P0 && l % nz1 < 2) ? (T)P0[(size_t)(2 * (l / nz1) + l % nz1) * N + x] : P[(size_t)l * N + x];
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int l = l0 + xr + XR * q, x = x0 + lane;
+    if (l < Lp && x < N) tile[xr + XR * q][lane] = a[q];
   }
   __syncthreads();
-  for (int xx = t / 64; xx < 64; xx += TPB / 64) {
-    int l = l0 + (t & 63), x = x0 + xx;
-    if (l < Lp && x < N) F[(size_t)(x + 1) * ps + l] = tile[t & 63][xx];
+  for (int xx = xr; xx < 64; xx += XR) {
+    const int l = l0 + lane, x = x0 + xx;
+    if (l < Lp && x < N) F[(size_t)(x + 1) * ps + l] = tile[lane][xx];
   }
 }
 
@@ -828,7 +843,7 @@ void k_cosftx(Box &b, const double *in, double *out, int iopt) {
   if (b.dct_fft) {
     LAUNCH(b, "to_pencils", tgrid(b), TPB, 0, planes_to_pencils_kernel, in, b.rest2, Lp, N, b.pv);
     dct_pencils(b, b.rest2, iopt);
-    LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<double>, b.rest2, out, Lp, N, b.pv,
+    LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, b.rest2, out, Lp, N, b.pv,
            (const double *)nullptr, b.nz1);
   } else {
     dim3 grid(nblk(Lp, 64), nblk(N, 64));
@@ -1498,9 +1513,11 @@ __global__ void poisson_y_kernel(double *rr, double *a2s, DevCoef c, int nx, int
 // divergence (pois.f90:41-54) written straight into x-contiguous pencils
 // through an LDS tile (64 lines x 64 planes)
 template <typename T>
-__global__ void __launch_bounds__(TPB)
-div_pencils_kernel(const T *u, const T *v, const T *w, T *P, DevCoefT<T> c, double idx_d, int nx, int ny,
-                   int nz1, double *P0, DevCoef cd, const double *um) {
+__global__ void __launch_bounds__(DIV_TPB)
+div_pencils_kernel(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w, T *__restrict__ P,
+                   DevCoefT<T> c, double idx_d, int nx, int ny, int nz1, double *__restrict__ P0, DevCoef cd,
+                   const double *__restrict__ um) {
+  constexpr int XR = DIV_TPB / 64, NQ = 64 / XR; // planes per pass, passes
   __shared__ T tile[64][65];
   const T idx = (T)idx_d;
   const int Lp = (ny - 1) * nz1, N = nx - 1;
@@ -1508,32 +1525,56 @@ div_pencils_kernel(const T *u, const T *v, const T *w, T *P, DevCoefT<T> c, doub
   unsigned bx, by;
   xcd_block2(bx, by); // XCD-contiguous tiles: neighbouring rows' tiles share lines
   const int l0 = (int)bx * 64, x0 = (int)by * 64, t = threadIdx.x;
-  for (int xx = t / 64; xx < 64; xx += TPB / 64) {
-    int l = l0 + (t & 63), x = x0 + xx;
-    if (l < Lp && x < N) {
-      int i = x + 2, j = l / nz1 + 1, kr = l % nz1;
-      size_t ow = (size_t)(i - 1) * pu + (size_t)j * s;       // row j+1
-      size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s; // row j
-      if (P0 && kr < 2) {
-        // the mean-flow (kz = 0) lines of FP32 fields, formed in FP64 (with
-        // the FP64 mean-flow u um); line (j-1, kr) at P0[(2 (j-1) + kr) N]
-        const double wp = w[ow + (kr ^ 1)], kz = cd.kaz[0];
-        const double d = (kr ? wp * kz : -wp * kz) + ((double)v[ov + s + kr] - (double)v[ov + kr]) * cd.idy[j];
-        const double du = (um && kr == 0) ? um[(size_t)j * nx + (i - 1)] - um[(size_t)j * nx + (i - 2)]
-                                          : (double)u[ow + kr] - (double)u[ow - pu + kr];
-        P0[(size_t)(2 * (j - 1) + kr) * N + x] = d + idx_d * du;
-      }
-      T kz = c.kaz[kr / 2];
-      T wp = w[ow + (kr ^ 1)];
-      T tz = (kr & 1) ? wp * kz : -wp * kz;
-      T d = tz + (v[ov + s + kr] - v[ov + kr]) * c.idy[j];
-      tile[xx][t & 63] = d + idx * (u[ow + kr] - u[ow - pu + kr]);
+  const int lane = t & 63, xr = t / 64;
+  const int l = l0 + lane;
+  const bool lok = l < Lp;
+  const int j = lok ? l / nz1 + 1 : 1, kr = lok ? l % nz1 : 0;
+  const T kz = c.kaz[kr / 2], iy = c.idy[j];
+  // the NQ planes of this thread: every load issued before the first use
+  T wp[NQ], v1[NQ], v0[NQ], u1[NQ], u0[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int x = x0 + xr + XR * q;
+    if (lok && x < N) {
+      const int i = x + 2;
+      const size_t ow = (size_t)(i - 1) * pu + (size_t)j * s;       // row j+1
+      const size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s; // row j
+      wp[q] = w[ow + (kr ^ 1)];
+      v1[q] = v[ov + s + kr];
+      v0[q] = v[ov + kr];
+      u1[q] = u[ow + kr];
+      u0[q] = u[ow - pu + kr];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int x = x0 + xr + XR * q;
+    if (lok && x < N) {
+      const T tz = (kr & 1) ? wp[q] * kz : -wp[q] * kz;
+      const T d = tz + (v1[q] - v0[q]) * iy;
+      tile[xr + XR * q][lane] = d + idx * (u1[q] - u0[q]);
+    }
+  }
+  if (P0 && lok && kr < 2) {
+    // the mean-flow (kz = 0) lines of FP32 fields, formed in FP64 (with the
+    // FP64 mean-flow u um); line (j-1, kr) at P0[(2 (j-1) + kr) N]
+    for (int q = 0; q < NQ; ++q) {
+      const int x = x0 + xr + XR * q;
+      if (x >= N) break;
+      const int i = x + 2;
+      const size_t ow = (size_t)(i - 1) * pu + (size_t)j * s;
+      const size_t ov = (size_t)(i - 1) * pv + (size_t)(j - 1) * s;
+      const double wq = w[ow + (kr ^ 1)], kz0 = cd.kaz[0];
+      const double d = (kr ? wq * kz0 : -wq * kz0) + ((double)v[ov + s + kr] - (double)v[ov + kr]) * cd.idy[j];
+      const double du = (um && kr == 0) ? um[(size_t)j * nx + (i - 1)] - um[(size_t)j * nx + (i - 2)]
+                                        : (double)u[ow + kr] - (double)u[ow - pu + kr];
+      P0[(size_t)(2 * (j - 1) + kr) * N + x] = d + idx_d * du;
     }
   }
   __syncthreads();
-  for (int ll = t / 64; ll < 64; ll += TPB / 64) {
-    int l = l0 + ll, x = x0 + (t & 63);
-    if (l < Lp && x < N) P[(size_t)l * N + x] = tile[t & 63][ll];
+  for (int ll = xr; ll < 64; ll += XR) {
+    const int lw = l0 + ll, x = x0 + lane;
+    if (lw < Lp && x < N) P[(size_t)lw * N + x] = tile[lane][ll];
   }
 }
 
@@ -1670,7 +1711,7 @@ static void pois_mixed(Box &b, double varstep, bool use_m) {
   const int nx = b.nx, ny = b.ny, nz1 = b.nz1;
   const int Lp = (ny - 1) * nz1, N = nx - 1;
   float *P = fld<float>(b.rest2), *F = fld<float>(b.rest);
-  LAUNCH(b, "div", tgrid(b), TPB, 0, div_pencils_kernel<float>, fld<float>(b.f[OTBL_U]), fld<float>(b.f[OTBL_V]),
+  LAUNCH(b, "div", tgrid(b), DIV_TPB, 0, div_pencils_kernel<float>, fld<float>(b.f[OTBL_U]), fld<float>(b.f[OTBL_V]),
          fld<float>(b.f[OTBL_W]), P, b.dcf, b.hc.idx, nx, ny, nz1, b.P0, b.dc, (const double *)b.um64);
   kz_dct_pencils<float>(b, P, -1, Lp);
   kz_dct_pencils<double>(b, b.P0, -1, 2 * (ny - 1));
@@ -1678,7 +1719,7 @@ static void pois_mixed(Box &b, double varstep, bool use_m) {
   LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.P0, b.S0, b.dc, nx, ny, 2, 2);
   kz_dct_pencils<float>(b, P, 1, Lp);
   kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1));
-  LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pv, b.P0, nz1);
+  LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pv, b.P0, nz1);
   const int pm = prun<float>();
   LAUNCH(b, "project", pgrid(b, pm), TPB, 0, project_kernel<float>, fld<float>(b.f[OTBL_U]),
          fld<float>(b.f[OTBL_V]), fld<float>(b.f[OTBL_W]), fld<float>(b.f[OTBL_P]), F, b.dcf, b.ds, varstep,
@@ -1694,7 +1735,7 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
     return;
   }
   if (b.dct_fft) {
-    LAUNCH(b, "div", tgrid(b), TPB, 0, div_pencils_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest2,
+    LAUNCH(b, "div", tgrid(b), DIV_TPB, 0, div_pencils_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest2,
            b.dc, b.hc.idx, nx, ny, nz1, (double *)nullptr, b.dc, (const double *)nullptr);
     dct_pencils(b, b.rest2, -1);
     if (b.pchunk) {
@@ -1708,7 +1749,7 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
              nz1, nz1);
     }
     dct_pencils(b, b.rest2, 1);
-    LAUNCH(b, "to_planes", tgrid(b), TPB, 0, pencils_to_planes_kernel<double>, b.rest2, b.rest, Lp, N, b.pv,
+    LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, b.rest2, b.rest, Lp, N, b.pv,
            (const double *)nullptr, nz1);
   } else {
     size_t ndiv = (size_t)(ny - 1) * nz1 * (nx - 1);
