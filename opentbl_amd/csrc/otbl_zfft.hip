@@ -165,7 +165,7 @@ rowmax_kernel(const double *__restrict__ rowmax, DevScalars *ds, unsigned long l
 // (w interpolated to v rows is read from ya), vv = r2c(v_u^2) -> yb (v
 // interpolated to u rows is read from yb), ww = r2c(w^2) -> yc.
 template <int N, typename R>
-__global__ void __launch_bounds__(ZTeam<N>::TPB_)
+__global__ void __launch_bounds__(ZTeam<N>::TPB_, sizeof(R) == 4 ? 4 : 1)
 products2_z(const R *__restrict__ resv, const R *__restrict__ resw, R *ya, R *yb, R *yc,
             const typename CT<R>::c *__restrict__ tw, int ny, int nz1, int nz2, int nrows) {
   using C = typename CT<R>::c;
