@@ -76,6 +76,25 @@ int otbl_get_um(otbl_ctx *ctx, double *um);
 int otbl_get_genflu(otbl_ctx *ctx, double *out6);
 /* energies (rhsp.F90:493-509, stotf.F90:395-440) of the last substep 1 */
 int otbl_get_ener(otbl_ctx *ctx, double *ener15);
+
+/* Per-substep diagnostics: info_per_step (genflu.F90:247-295), which the
+ * reference writes as one line of 49 reals to unit 62 at every substep when
+ * built without NOINFOSTEP (genflu.F90:157-160, 287-289).  Here each substep
+ * appends the same 49 values, in the same order, to a ring of records in
+ * device memory, with no host synchronisation: tiempo, rthout, utauout,
+ * gamma(1:2), dout, jtop, ntop(1:2), then x_plane(8), u_inf(8), u_tau(8),
+ * d_out(8), r_thout(8) at the sample planes ((nx-1) i)/8 + 1, i = 1..8 (the
+ * planes the reference samples with one plane per rank). */
+#define OTBL_DIAG_LEN 49
+/* Ring of `capacity` records (0 turns the diagnostics off).  Captured
+ * substep graphs are dropped and captured again on next use. */
+int otbl_diag_enable(otbl_ctx *ctx, int capacity);
+/* Waits for the context's stream, then copies the records written since the
+ * last read, oldest first, at most max_records of them, into out
+ * (max_records x OTBL_DIAG_LEN doubles).  *n_records receives the number
+ * copied; *dropped the number lost to the ring's wrap-around since the last
+ * read (records beyond max_records stay for the next read). */
+int otbl_diag_read(otbl_ctx *ctx, double *out, int max_records, int *n_records, long long *dropped);
 int otbl_set_time(otbl_ctx *ctx, double tiempo, double timeinit, double dt);
 int otbl_get_time(otbl_ctx *ctx, double *tiempo, double *dt);
 /* Coefficient arrays after setup, by their reference names (inxu, dcbx,

@@ -311,9 +311,11 @@ static void free_box(Box *b) {
   double *bufs[] = {b->coefbuf, b->resu, b->resv, b->resw, b->rhsu, b->rhsv, b->rhsw, b->wki1, b->wki2,
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
                     b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
-                    b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax, b->P0, b->S0, b->p0, b->um64, b->rm64};
+                    b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax, b->P0, b->S0, b->p0, b->um64, b->rm64,
+                    b->diag};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
+  if (b->diag_count) (void)hipFree(b->diag_count);
   float *fbufs[] = {b->coefbuf_f, b->ytab_f, b->ztw_f, b->dct_chirp_f, b->dct_bf_f, b->dct_tw4_f, b->dct_ztw_f};
   for (float *p : fbufs)
     if (p) (void)hipFree(p);
@@ -683,6 +685,64 @@ int otbl_get_ener(otbl_ctx *ctx, double *e) {
   DevScalars d;
   get_ds(b, 0, &d, sizeof d);
   for (int q = 0; q < 15; ++q) e[q] = d.ener[q + 1];
+  API_END
+}
+
+// graphs bake kernel arguments in: drop them when the launch set changes
+static void drop_graphs(Box &b) {
+  for (int q = 0; q < 4; ++q)
+    for (int f = 0; f < 2; ++f)
+      if (b.gexec[q][f]) {
+        (void)hipGraphExecDestroy(b.gexec[q][f]);
+        b.gexec[q][f] = nullptr;
+      }
+}
+
+int otbl_diag_enable(otbl_ctx *ctx, int capacity) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(capacity >= 0, "otbl_diag_enable: capacity must be >= 0");
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  drop_graphs(b);
+  if (b.diag) (void)hipFree(b.diag);
+  if (b.diag_count) (void)hipFree(b.diag_count);
+  b.diag = nullptr;
+  b.diag_count = nullptr;
+  b.diag_cap = 0;
+  b.diag_read = 0;
+  if (capacity > 0) {
+    dalloc(b.diag, (size_t)capacity * OTBL_DIAG_LEN);
+    dalloc(b.diag_count, 1);
+    b.diag_cap = capacity;
+  }
+  API_END
+}
+
+int otbl_diag_read(otbl_ctx *ctx, double *out, int max_records, int *n_records, long long *dropped) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(b.diag != nullptr, "otbl_diag_read: diagnostics are off (otbl_diag_enable)");
+  need(max_records >= 0 && (out != nullptr || max_records == 0), "otbl_diag_read: bad output buffer");
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  unsigned long long n = 0;
+  d2h(b, &n, b.diag_count, sizeof n);
+  const unsigned long long cap = (unsigned long long)b.diag_cap;
+  long long lost = 0;
+  if (n - b.diag_read > cap) {
+    lost = (long long)(n - cap - b.diag_read);
+    b.diag_read = n - cap;
+  }
+  const unsigned long long take = std::min<unsigned long long>(n - b.diag_read, (unsigned long long)max_records);
+  std::vector<double> ring((size_t)b.diag_cap * OTBL_DIAG_LEN);
+  if (take) d2h(b, ring.data(), b.diag, ring.size() * sizeof(double));
+  for (unsigned long long r = 0; r < take; ++r) {
+    const size_t slot = (size_t)((b.diag_read + r) % cap);
+    std::copy(ring.begin() + slot * OTBL_DIAG_LEN, ring.begin() + (slot + 1) * OTBL_DIAG_LEN,
+              out + r * OTBL_DIAG_LEN);
+  }
+  b.diag_read += take;
+  if (n_records) *n_records = (int)take;
+  if (dropped) *dropped = lost;
   API_END
 }
 

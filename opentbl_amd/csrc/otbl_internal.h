@@ -55,6 +55,7 @@ struct DevScalars {
   double dt, tiempo, timeinit;
   double var1, var2, var3, var4, rkk, varstep;
   double rthin, din, rthout, dout, utauout, gamma1, gamma2, timec;
+  double jtopout; // top row of the outlet profile (genflu.F90:82-85)
   double sumax;
   unsigned long long umax_bits, wmax_bits; // |u|, |w_x| maxima (IEEE order)
   int dt_valid, pad;
@@ -136,6 +137,13 @@ struct Box {
   int dct_M = 0;
   FFTPlan dct_plan{};
   double *dct_chirp = nullptr, *dct_bf = nullptr, *dct_tw4 = nullptr;
+  // per-substep diagnostics ring (info_per_step records): diag_cap records
+  // of OTBL_DIAG_LEN doubles, the device count of records written, and the
+  // host's read cursor
+  double *diag = nullptr;
+  unsigned long long *diag_count = nullptr;
+  int diag_cap = 0;
+  unsigned long long diag_read = 0;
   // graphs
   bool use_graphs = false;
   // one graph per substep m and buffer role assignment (flip: the velocity

@@ -1101,6 +1101,7 @@ __global__ void genflu_stats_kernel(const T *resu, double *um, int *ibuf, double
     double rthout = sum * re / uinf;
     double g2 = ds->rthin / rthout, g1 = pow(g2, 0.125);
     ds->utauout = utau;
+    ds->jtopout = jtop;
     ds->dout = y[jtop];
     ds->rthout = rthout;
     ds->gamma1 = g1;
@@ -1201,6 +1202,66 @@ __global__ void genflu_stats_kernel(const T *resu, double *um, int *ibuf, double
       }
     }
   }
+}
+
+// info_per_step (genflu.F90:247-295): one record per substep into the
+// diagnostics ring, in the column order of the reference's unit-62 line:
+// tiempo, rthout, utauout, gamma(1:2), dout, jtop, ntop(1:2) of the recycling
+// statistics (genflu_stats of this substep), then per sample plane x_plane,
+// u_inf, u_tau, d_out, r_thout (8 values each).  The reference samples the
+// first plane of ranks ((nummpi-1) i)/8, i = 1..8; with one plane per rank
+// that is plane ((nx-1) i)/8 + 1, the planes sampled here.  The profile is
+// mode 0 (real part) of u saved before the predictor (the reference's ut at
+// the call inside genflu, planes >= 2), or, with FP32 fields, the FP64
+// mean-flow copy rm ([ny+1][nx]).  One wavefront per sample plane: its lanes
+// stage the profile in LDS, lane 0 forms the sums in the reference's order.
+template <typename T>
+__global__ void __launch_bounds__(512)
+diag_kernel(const T *us, const double *rm, const int *ntop, const DevScalars *ds, DevCoef c, double *ring,
+            unsigned long long *count, int cap, int nx, int ny, int nz1, double re) {
+  extern __shared__ double prof[]; // [8][ny+1]
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int ip = ((nx - 1) * (w + 1)) / 8 + 1; // 1-based sample plane
+  double *um = prof + (size_t)w * (ny + 1);
+  for (int j = lane; j <= ny; j += 64)
+    um[j] = rm ? rm[(size_t)j * nx + (ip - 1)] : (double)us[((size_t)(ip - 1) * (ny + 1) + j) * nz1];
+  __syncthreads();
+  const unsigned long long n = *count;
+  double *rec = ring + (size_t)(n % (unsigned long long)cap) * OTBL_DIAG_LEN;
+  if (lane == 0) {
+    const double *y = c.y, *dy = c.dy;
+    // um(j) of the reference is um[j-1]
+    double uinf = 0.0;
+    for (int j = ny + 1 - 20; j <= ny + 1 - 5; ++j) uinf += um[j - 1];
+    uinf /= 16.0;
+    const double u99 = (double)0.99f * uinf; // default-real literal
+    const double utau = sqrt(fabs((um[1] - um[0]) / (y[2] - y[1]) / re));
+    int jtop = 0;
+    for (int j = 1; j <= ny - 2; ++j) {
+      if (um[j - 1] >= u99) break;
+      jtop = j + 1;
+    }
+    double sum = 0.0;
+    for (int j = 2; j <= jtop + 3; ++j) sum += (uinf - um[j - 1]) * um[j - 1] * dy[j - 1];
+    rec[9 + w] = ip;
+    rec[17 + w] = uinf;
+    rec[25 + w] = utau;
+    rec[33 + w] = y[jtop];
+    rec[41 + w] = sum * re / uinf;
+    if (w == 0) {
+      rec[0] = ds->tiempo;
+      rec[1] = ds->rthout;
+      rec[2] = ds->utauout;
+      rec[3] = ds->gamma1;
+      rec[4] = ds->gamma2;
+      rec[5] = ds->dout;
+      rec[6] = ds->jtopout;
+      rec[7] = ntop[0];
+      rec[8] = ntop[1];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *count = n + 1;
 }
 
 // Plane 1 of u (blockIdx.y 0), w (1) and v (2).  For modes k >= 1 the value
@@ -1866,6 +1927,10 @@ template <typename T> static void rhsp_t(Box &b, int m) {
            dbuf, b.dc, b.ds, ny, nz1, b.xout, b.re);
     LAUNCH(b, "genflu", dim3(nblk((size_t)nz1 * (ny + 1)), 3), TPB, 0, genflu_plane_kernel<T>, u, v, w, resu, resv,
            resw, b.u0, b.v0, ibuf, dbuf, b.dc, b.ds, ny, nz1, b.xout, um, nx);
+    if (b.diag)
+      LAUNCH(b, "diag", 1, 512, 8 * (size_t)(ny + 1) * sizeof(double), diag_kernel<T>, resu,
+             F64 ? (const double *)nullptr : (const double *)b.rm64, ibuf + 2 * (ny + 1), b.ds, b.dc, b.diag,
+             b.diag_count, b.diag_cap, nx, ny, nz1, b.re);
   }
   // plane loop 2 (planes 2..nx-1)
   if (nx > 2) {

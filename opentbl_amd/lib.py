@@ -20,6 +20,10 @@ VP = C.c_void_p
 OTBL_FP64, OTBL_MIXED = 0, 1
 OTBL_DT_EVERY_STEP, OTBL_DT_FROZEN = 0, 1
 FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "rhsupa": 4, "rhsvpa": 5, "rhswpa": 6}
+# values per diagnostics record (OTBL_DIAG_LEN) and their names in order
+DIAG_LEN = 49
+DIAG_NAMES = (["tiempo", "rthout", "utauout", "gamma1", "gamma2", "dout", "jtop", "ntop1", "ntop2"]
+              + [f"{n}{i}" for n in ("x_plane", "u_inf", "u_tau", "d_out", "r_thout") for i in range(1, 9)])
 
 
 class Params(C.Structure):
@@ -39,6 +43,8 @@ SIGNATURES = {
     "otbl_get_um": (I, [VP, PD]),
     "otbl_get_genflu": (I, [VP, PD]),
     "otbl_get_ener": (I, [VP, PD]),
+    "otbl_diag_enable": (I, [VP, I]),
+    "otbl_diag_read": (I, [VP, PD, I, C.POINTER(I), C.POINTER(C.c_longlong)]),
     "otbl_set_time": (I, [VP, D, D, D]),
     "otbl_get_time": (I, [VP, PD, PD]),
     "otbl_get_coef": (I, [VP, C.c_char_p, PD, C.POINTER(I), C.POINTER(I)]),
@@ -224,6 +230,19 @@ class GpuBox:
         o = np.zeros(15)
         check(lib().otbl_get_ener(self.h, _pd(o)), "get_ener")
         return o
+
+    def diag_enable(self, capacity):
+        """Per-substep info_per_step records into a device ring of `capacity`
+        records (0: off)."""
+        check(lib().otbl_diag_enable(self.h, int(capacity)), "diag_enable")
+
+    def diag_read(self, max_records=4096):
+        """The records written since the last read, oldest first, as an
+        (n, DIAG_LEN) array, and the count lost to wrap-around."""
+        out = np.zeros((max(1, max_records), DIAG_LEN))
+        n, lost = I(), C.c_longlong()
+        check(lib().otbl_diag_read(self.h, _pd(out), int(max_records), C.byref(n), C.byref(lost)), "diag_read")
+        return out[:n.value].copy(), lost.value
 
     def coef(self, name):
         r, c = I(), I()

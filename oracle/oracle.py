@@ -168,3 +168,39 @@ class OracleBox:
     # operator-level helpers on complex (nplanes, R, Mz) arrays
     def op(self, name, *args):
         return getattr(lib(), "oc_op_" + name)(self.h, *args)
+
+
+def info_per_step_planes(u, y, re):
+    """info_per_step (genflu.F90:247-295) at the eight sample planes, in
+    numpy: rows x_plane, u_inf, u_tau, d_out, r_thout (5 x 8).
+
+    u is a u-type field in the planes layout (nx, ny+1, nz1); the profile of
+    a plane is its mode 0 (real part), um(j) = u[ip-1, j-1, 0].  The sample
+    planes are ((nx-1) i)/8 + 1, i = 1..8: the first planes of ranks
+    ((nummpi-1) i)/8 (genflu.F90:265) when every rank holds one plane.  y is
+    y(0:ny+1); dy(j) = y(j+1) - y(j).  Sums run in the reference's order;
+    0.99 is a default-real literal (genflu.F90:270)."""
+    nx, R = u.shape[0], u.shape[1]
+    ny = R - 1
+    y = np.asarray(y, np.float64)
+    dy = y[1:] - y[:-1]
+    out = np.zeros((5, 8))
+    for i in range(1, 9):
+        ip = ((nx - 1) * i) // 8 + 1
+        um = np.real(np.asarray(u[ip - 1, :, 0])).astype(np.float64)  # um(j) = um[j-1]
+        uinf = 0.0
+        for j in range(ny + 1 - 20, ny + 1 - 5 + 1):
+            uinf += um[j - 1]
+        uinf /= 16.0
+        u99 = float(np.float32(0.99)) * uinf
+        utau = np.sqrt(abs((um[1] - um[0]) / (y[2] - y[1]) / re))
+        jtop = 0
+        for j in range(1, ny - 1):
+            if um[j - 1] >= u99:
+                break
+            jtop = j + 1
+        s = 0.0
+        for j in range(2, jtop + 4):
+            s += (uinf - um[j - 1]) * um[j - 1] * dy[j - 1]
+        out[:, i - 1] = (ip, uinf, utau, y[jtop], s * re / uinf)
+    return out

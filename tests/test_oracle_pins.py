@@ -460,3 +460,44 @@ def test_fp32_transposes_round_at_the_transposes(oracle_lib):
     ua, ub = a.get("u"), b.get("u")
     rel = np.abs(ua - ub).max() / np.abs(ua).max()
     assert 1e-9 < rel < 1e-5
+
+
+def test_info_per_step_pins(oracle_lib):
+    """The numpy info_per_step (the diagnostics records' per-plane columns)
+    against the C oracle's inlet_retheta (bl_1.F90:372-396: the delta_99 and
+    Re_theta formulas of info_per_step, restated separately) on a field whose
+    mean profile is the inflow profile u0 at every plane, and against closed
+    forms: the sample-plane indices, u_inf of a flat top, u_tau of a linear
+    wall profile, delta_99 of a ramp."""
+    from oracle.oracle import info_per_step_planes
+    cfg = config.small()
+    st = ic.make_state(cfg)
+    ob = oracle_lib.OracleBox(cfg, st)
+    nx, ny, y = cfg.nx, cfg.ny, np.asarray(st["y"])
+    u = np.zeros((nx, ny + 1, 2), complex)
+    u[:, :, 0] = st["u0"]
+    out = info_per_step_planes(u, y, st["re"])
+    assert list(out[0]) == [((nx - 1) * i) // 8 + 1 for i in range(1, 9)]
+    assert np.all(out[3] == ob.scalar("din"))
+    assert np.abs(out[4] - ob.scalar("rthin")).max() <= 1e-13 * ob.scalar("rthin")
+    # ramp U min(1, (y - y1)/delta): slope U/delta at the wall, flat top
+    U, delta = 1.3, 0.5 * (y[ny // 2] - y[1])
+    prof = U * np.minimum(1.0, (y[1:ny + 2] - y[1]) / delta)
+    u[:, :, 0] = prof
+    out = info_per_step_planes(u, y, st["re"])
+    assert np.allclose(out[1], U, rtol=1e-15)
+    assert np.allclose(out[2], np.sqrt(U / delta / st["re"]), rtol=1e-12)
+    # the first row at or above 0.99 U (float32 0.99, the reference's literal)
+    j99 = next(j for j in range(1, ny + 1) if prof[j - 1] >= float(np.float32(0.99)) * U)
+    assert np.all(out[3] == y[j99])
+    assert abs(y[j99] - (y[1] + 0.99 * delta)) <= (y[j99] - y[j99 - 1])
+    # momentum thickness of the ramp, U^2 delta / 6: the reference's sum is a
+    # first-order rectangle rule, so on a fine grid (ny = 320) within 2%
+    cf = config.small(ny=320)
+    sf = ic.make_state(cf)
+    yf = np.asarray(sf["y"])
+    df = 0.5 * (yf[cf.ny // 2] - yf[1])
+    uf = np.zeros((cf.nx, cf.ny + 1, 1))
+    uf[:, :, 0] = U * np.minimum(1.0, (yf[1:cf.ny + 2] - yf[1]) / df)
+    theta = info_per_step_planes(uf, yf, sf["re"])[4] / sf["re"] / U
+    assert np.abs(theta - df / 6).max() < 0.02 * df / 6
