@@ -93,6 +93,20 @@ module otbl_gpu
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function otbl_sync
+     integer(c_int) function otbl_diag_enable(ctx, capacity) bind(C, name="otbl_diag_enable")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: capacity
+     end function otbl_diag_enable
+     integer(c_int) function otbl_diag_read(ctx, out, max_records, n_records, dropped) &
+          bind(C, name="otbl_diag_read")
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       real(c_double) :: out(*)
+       integer(c_int), value :: max_records
+       integer(c_int) :: n_records
+       integer(c_long_long) :: dropped
+     end function otbl_diag_read
   end interface
 
 contains
@@ -135,6 +149,24 @@ contains
     real(8), intent(out) :: a(*)
     call otbl_check(otbl_get_field(otbl_box, id, a), 'otbl_get_field')
   end subroutine otbl_fetch
+
+  ! info_per_step (genflu.F90:247-295): the records the substeps appended to
+  ! the device ring since the last call, one line each in the format of the
+  ! reference's unit 62 (genflu.F90:287-288); capacity from otbl_diag_enable
+  subroutine otbl_write_info(unit, capacity)
+    integer, intent(in) :: unit, capacity
+    real(8), allocatable :: rec(:)
+    integer(c_int) :: n
+    integer(c_long_long) :: lost
+    integer :: r
+    allocate(rec(49 * capacity))
+    call otbl_check(otbl_diag_read(otbl_box, rec, int(capacity, c_int), n, lost), 'otbl_diag_read')
+    do r = 0, n - 1
+       write(unit, '(49(d22.14))') rec(49 * r + 1:49 * r + 49)
+    end do
+    call flush(unit)
+    deallocate(rec)
+  end subroutine otbl_write_info
 
   subroutine otbl_close()
     integer(c_int) :: rc
