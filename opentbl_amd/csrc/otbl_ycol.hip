@@ -16,8 +16,10 @@
 // per matrix (once per substep for the dt-dependent implicit operator).  The
 // carries C and D of each chunk follow from the chunk end values by two
 // recurrences over the 16 chunks, exchanged through LDS.  The explicit stencil
-// coefficients of each operator are stored in its system's table, which a
-// workgroup stages in LDS.
+// coefficients of each operator are stored in its system's table.  The
+// kernels read the tables in place: a wave's table indices are uniform, so
+// they are scalar loads into scalar registers (FP32 rk_final, which measured
+// faster that way, stages its two tables in LDS).
 #include "otbl_kcommon.h"
 #include "otbl_ycol.h"
 
@@ -248,13 +250,12 @@ __device__ __forceinline__ YBnd<T> vistyy_bnd(int jee, int j0, int L, int ny, co
 // kernels
 // ---------------------------------------------------------------------------
 template <int L, typename T>
-__global__ void __launch_bounds__(YTPB)
+__global__ void __launch_bounds__(YTPB, sizeof(T) == 8 ? 8 : 1)
 interpyy_y(const T *__restrict__ gen, T *__restrict__ geni, const T *__restrict__ tab,
            const T *__restrict__ inby, int loguv, int ny, int nz1, int i0, int nplanes, size_t ips,
            size_t ops) {
   __shared__ T xch[2 * YNCH * 64];
-  __shared__ T tl[ytab_size(L)];
-  ytab_stage<L>(tl, tab);
+  const T *tl = tab; // read in place (wave-uniform indices: scalar loads)
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, (size_t)(i0 - 1));
   const T *g = gen + q.ip * ips + q.kr;
   T *o = geni + q.ip * ops + q.kr;
@@ -275,8 +276,7 @@ __global__ void __launch_bounds__(YTPB)
 differzy_y(const T *__restrict__ gen, T *__restrict__ res, const T *__restrict__ tab, DevCoefT<T> c,
            int which, int ny, int nz1, int nplanes, size_t ips, size_t ops) {
   __shared__ T xch[2 * YNCH * 64];
-  __shared__ T tl[ytab_size(L)];
-  ytab_stage<L>(tl, tab);
+  const T *tl = tab; // read in place (wave-uniform indices: scalar loads)
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 0);
   const T *g = gen + q.ip * ips + q.kr;
   T *o = res + q.ip * ops + q.kr;
@@ -297,8 +297,7 @@ __global__ void __launch_bounds__(YTPB)
 vistyy_y(const T *__restrict__ u, T *__restrict__ res, const T *__restrict__ tab, DevCoefT<T> c,
          int jee, int ny, int nz1, int nplanes) {
   __shared__ T xch[2 * YNCH * 64];
-  __shared__ T tl[ytab_size(L)];
-  ytab_stage<L>(tl, tab);
+  const T *tl = tab; // read in place (wave-uniform indices: scalar loads)
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 0);
   const size_t s = nz1, off = q.ip * s * jee + q.kr;
   const int j0 = q.p * L;
@@ -318,8 +317,7 @@ __global__ void __launch_bounds__(YTPB)
 implzy_y(T *u, const T *__restrict__ res, const T *__restrict__ tab, const double *__restrict__ f,
          int jee, int nz1, int nplanes) {
   __shared__ T xch[2 * YNCH * 64];
-  __shared__ T tl[ytab_size(L)];
-  ytab_stage<L>(tl, tab);
+  const T *tl = tab; // read in place (wave-uniform indices: scalar loads)
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 0);
   const size_t s = nz1, off = q.ip * s * jee + q.kr;
   const int j0 = q.p * L;
@@ -341,12 +339,11 @@ implzy_y(T *u, const T *__restrict__ res, const T *__restrict__ tab, const doubl
 
 // rhsu = d(uv)/dy + i kz (uw) on all planes (rhsp.F90:164-169)
 template <int L, typename T>
-__global__ void __launch_bounds__(YTPB)
+__global__ void __launch_bounds__(YTPB, sizeof(T) == 8 ? 8 : 1)
 rhsu_loop1_y(const T *__restrict__ uv, const T *__restrict__ uw, T *__restrict__ rhsu,
              const T *__restrict__ tab, DevCoefT<T> c, int ny, int nz1, int nplanes) {
   __shared__ T xch[2 * YNCH * 64];
-  __shared__ T tl[ytab_size(L)];
-  ytab_stage<L>(tl, tab);
+  const T *tl = tab; // read in place (wave-uniform indices: scalar loads)
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 0);
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   const int j0 = q.p * L;
@@ -385,10 +382,7 @@ rhs_loop2_y(T *rhsu, T *rhsv, T *rhsw, const T *__restrict__ resu,
             const T *__restrict__ vv, const T *__restrict__ ww, const T *__restrict__ tabu,
             const T *__restrict__ tabv, DevCoefT<T> c, int ny, int nz1, int nplanes) {
   __shared__ T xch[2 * YNCH * 64];
-  __shared__ T tlu[ytab_size(L)];
-  __shared__ T tlv[ytab_size(L)];
-  ytab_stage<L>(tlu, tabu);
-  ytab_stage<L>(tlv, tabv);
+  const T *tlu = tabu, *tlv = tabv; // read in place (scalar loads)
   const YCol q = ycol((size_t)nz1 * nplanes, nz1, 1);
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   const int j0 = q.p * L;
@@ -486,11 +480,19 @@ rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, const T *
            const T *__restrict__ tabv, const T *__restrict__ tabi, DevCoefT<T> c,
            const double *__restrict__ f, const DevScalars *__restrict__ ds, int jee, int ny, int nz1, int nx) {
   __shared__ T xch[2 * YNCH * 64];
-  __shared__ T tlv[ytab_size(L)];
-  __shared__ T tli[ytab_size(L)];
   __shared__ T hal[2 * YNCH * 64];
-  ytab_stage<L>(tlv, tabv);
-  ytab_stage<L>(tli, tabi);
+  // FP64: the tables are read in place (wave-uniform indices: scalar loads,
+  // which frees the vector registers the LDS copies took); FP32 measured
+  // faster with the tables staged in LDS
+  constexpr bool STAGE = sizeof(T) == 4;
+  __shared__ T tlv_s[STAGE ? ytab_size(L) : 1], tli_s[STAGE ? ytab_size(L) : 1];
+  const T *tlv = tabv, *tli = tabi;
+  if constexpr (STAGE) {
+    ytab_stage<L>(tlv_s, tabv);
+    ytab_stage<L>(tli_s, tabi);
+    tlv = tlv_s;
+    tli = tli_s;
+  }
   const YCol q = ycol((size_t)nz1 * nx, nz1, 0);
   const size_t s = nz1, off = q.ip * s * jee + q.kr;
   const bool act = q.ok && q.ip > 0;
@@ -570,11 +572,9 @@ mean_rk_final_y(double *um, const double *__restrict__ rm, const T *__restrict__
                 const double *__restrict__ tabv, const double *__restrict__ tabi, DevCoef c,
                 const double *__restrict__ f, const DevScalars *__restrict__ ds, int ny, int nz1, int nx) {
   __shared__ double xch[2 * YNCH * 64];
-  __shared__ double tlv[ytab_size(L)];
-  __shared__ double tli[ytab_size(L)];
+  const double *tlv = tabv, *tli = tabi; // read in place (scalar loads)
   __shared__ double hal[2 * YNCH * 64];
-  ytab_stage<L>(tlv, tabv);
-  ytab_stage<L>(tli, tabi);
+
   const int jee = ny + 1;
   const YCol q = ycol((size_t)nx, nx, 0); // column kr = plane index
   const int ip = q.kr;
