@@ -149,3 +149,23 @@ def test_mixed_rejects_unsupported():
     st = ic.make_state(cfg)
     with pytest.raises(Exception, match="OTBL_MIXED"):
         GpuBox(cfg, st, precision=OTBL_MIXED)
+
+
+def test_mixed_long_x_vs_fp64():
+    """A box whose Poisson x transforms take the 3 x 1024 Bluestein length
+    (nx - 1 = 1099) and whose x lines are split into segments: the mixed
+    steps against the FP64 steps on the GPU (the oracle's direct DCT is too
+    slow at this length; its cosftx parity is covered in FP64)."""
+    cfg = config.small(name="mixlong", nx=1100, ny=40, nz=256, Lx=8.0 * 1100 / 32, kzmax=40)
+    st = ic.make_state(cfg)
+    gd = GpuBox(cfg, st)
+    gm = GpuBox(cfg, st, precision=OTBL_MIXED, graphs=True)
+    pm = gm.paths()
+    assert pm["dct"] == "engine3072" and int(pm["xseg"]) > 1, pm
+    gd.step(2)
+    gm.step(2)
+    errs = errors(gm, gd)
+    gd.close()
+    gm.close()
+    print("mixed vs FP64, nx=1100, after 2 steps", errs, flush=True)
+    check(errs)
