@@ -40,6 +40,10 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
   // the interface unknowns F, L
   __shared__ T red[PK][6][PK];
   __shared__ T itf[PK][2][PK];
+  // REF: the interface elimination's matrix factors of each step (VF den,
+  // den, VL, the be entering the step, de), kept for the refinement's
+  // second solve, whose matrix is the same
+  __shared__ T fac[REF ? PK : 1][5][PK];
   // row coefficients of the operator, rows 0 .. PK L: a(1,r) (0 where absent),
   // a(2,r) (1 past ny-1), a(3,r) (0 where absent); FP64 copies for REF
   __shared__ T ca1[PK * L + 2], ca2[PK * L + 2], ca3[PK * L + 2];
@@ -175,6 +179,13 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
         const T den = T(1) / (T(1) + qVF * be);
         const T ga = (qyF - qVF * al) * den, de = -qWF * den;
         const T nal = qyL - qVL * (al + be * ga), nbe = -qVL * be * de - qWL;
+        if constexpr (REF) {
+          fac[q][0][col] = qVF * den;
+          fac[q][1][col] = den;
+          fac[q][2][col] = qVL;
+          fac[q][3][col] = be;
+          fac[q][4][col] = de;
+        }
         red[q][0][col] = ga;
         red[q][1][col] = de;
         red[q][2][col] = nal;
@@ -203,6 +214,47 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
     XF = k < PK - 1 ? itf[k + 1][0][col] : T(0);
     __syncthreads(); // red / itf are reused by a second solve
   };
+  // the same interface system with another right-hand side, through the
+  // factors the first elimination kept (no division on the serial chain)
+  auto interface_rhs = [&](T yF, T yL, T &XL, T &XF) {
+    red[k][0][col] = yF;
+    red[k][1][col] = yL;
+    __syncthreads();
+    if (wave == 0 && lane < PK) {
+      T al = 0;
+      T n0 = red[0][0][col], n1 = red[0][1][col];
+      T f0 = fac[0][0][col], f1 = fac[0][1][col], f2 = fac[0][2][col], f3 = fac[0][3][col];
+#pragma unroll 2
+      for (int q = 0; q < PK; ++q) {
+        const T qyF = n0, qyL = n1, vden = f0, den = f1, qVL = f2, be = f3;
+        if (q + 1 < PK) {
+          n0 = red[q + 1][0][col];
+          n1 = red[q + 1][1][col];
+          f0 = fac[q + 1][0][col];
+          f1 = fac[q + 1][1][col];
+          f2 = fac[q + 1][2][col];
+          f3 = fac[q + 1][3][col];
+        }
+        const T ga = qyF * den - vden * al;
+        const T nal = qyL - qVL * (al + be * ga);
+        red[q][0][col] = ga;
+        red[q][2][col] = nal;
+        al = nal;
+      }
+      T Fn = 0; // F(q+1)
+#pragma unroll 2
+      for (int q = PK - 1; q >= 0; --q) {
+        const T ga = red[q][0][col], nal = red[q][2][col], de = fac[q][4][col];
+        const T nbe = q + 1 < PK ? fac[q + 1][3][col] : T(0); // be leaving step q
+        itf[q][1][col] = nal + nbe * Fn; // L(q)
+        Fn = ga + de * Fn;
+        itf[q][0][col] = Fn; // F(q)
+      }
+    }
+    __syncthreads();
+    XL = k > 0 ? itf[k - 1][1][col] : T(0);
+    XF = k < PK - 1 ? itf[k + 1][0][col] : T(0);
+  };
   T XL, XF;
   interface_solve(xv[0], xv[L - 1], XL, XF);
   combine(xv, XL, XF);
@@ -224,7 +276,7 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
     __builtin_amdgcn_sched_barrier(0);
     local_solve(d);
     T dL, dF;
-    interface_solve(d[0], d[L - 1], dL, dF);
+    interface_rhs(d[0], d[L - 1], dL, dF);
 #pragma unroll
     for (int q = 0; q < L; ++q) xv[q] += d[q];
     combine(xv, dL, dF);
