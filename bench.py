@@ -76,7 +76,7 @@ TRACE_NAMES = {"dct2_rows": "dct_rows_z", "dct3_rows": "dct_rows_z", "div": "div
                "poisson_y": "poisson_y_chunk", "to_planes": "pencils_to_planes_kernel",
                "products1": "products1_z", "products2": "products2_z", "interpxx": "interpxx_w",
                "difvisxx": "difvisxx_w", "interpyy": "interpyy_y", "rhsu_loop1": "rhsu_loop1_y",
-               "rhs_loop2": "rhs_loop2_y", "rk_final": "rk_final_y"}
+               "rhs_loop2": "rhs_loop2_y", "rk_final": ("rk_final_y", "rk_final_yh")}
 
 
 def profiled_traffic(workload, precision, kname):
@@ -88,11 +88,13 @@ def profiled_traffic(workload, precision, kname):
     path = os.path.join(ROOT, "profiles", f"substep_kernels_{workload}_{precision}.md")
     if not os.path.exists(path):
         return None
-    fn = TRACE_NAMES.get(kname, kname + "_kernel")
+    fns = TRACE_NAMES.get(kname, kname + "_kernel")
+    fns = (fns,) if isinstance(fns, str) else fns
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         name = cells[0] if cells else ""
-        if len(cells) >= 7 and (name == fn or name.startswith(fn + "<")) and cells[5] and cells[6]:
+        hit = any(name == fn or name.startswith(fn + "<") for fn in fns)
+        if len(cells) >= 7 and hit and cells[5] and cells[6]:
             return (float(cells[5]) + float(cells[6])) * 1e9
     return None
 

@@ -316,7 +316,7 @@ static void free_box(Box *b) {
   for (double *p : bufs)
     if (p) (void)hipFree(p);
   if (b->diag_count) (void)hipFree(b->diag_count);
-  float *fbufs[] = {b->coefbuf_f, b->ytab_f, b->ztw_f, b->dct_chirp_f, b->dct_bf_f, b->dct_tw4_f, b->dct_ztw_f};
+  float *fbufs[] = {b->coefbuf_f, b->ytab_f, b->ytabh_f, b->ztw_f, b->dct_chirp_f, b->dct_bf_f, b->dct_tw4_f, b->dct_ztw_f};
   for (float *p : fbufs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
@@ -427,6 +427,11 @@ static void setup_mixed(Box &b) {
   for (size_t q = 0; q < sizeof(DevCoef) / sizeof(void *); ++q) dst[q] = b.coefbuf_f + (src[q] - b.coefbuf);
   dalloc(b.ytab_f, YT_COUNT * ytab_size(b.yL));
   k_d2f(b, b.ytab, b.ytab_f, YT_COUNT * ytab_size(b.yL));
+  const char *yh = std::getenv("OTBL_Y_HALF");
+  if (b.yL % 2 == 0 && !(yh && yh[0] == '0')) {
+    dalloc(b.ytabh_f, 4 * ytabh_size(b.yL / 2));
+    k_ytab_halve_fixed(b);
+  }
   dalloc(b.ztw_f, 2 * (size_t)b.nz);
   k_d2f(b, b.ztw, b.ztw_f, 2 * (size_t)b.nz);
   const size_t N = b.nx - 1, M = b.dct_M;
@@ -966,8 +971,8 @@ int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len) {
   Box &b = box_of(ctx);
   need(buf && len > 0, "otbl_get_paths: null buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof tmp, "y:%s%d z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d", b.yL ? "chunk" : "march", b.yL,
-                b.zfast ? "engine" : "stockham", b.nz, b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"),
+  std::snprintf(tmp, sizeof tmp, "y:%s%d%s z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d", b.yL ? "chunk" : "march", b.yL,
+                b.ytabh_f ? "h" : "", b.zfast ? "engine" : "stockham", b.nz, b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"),
                 b.dct_M, b.xwin ? "window" : "march", b.xseg, b.pchunk ? "chunk" : "march", b.pchunk);
   std::snprintf(buf, len, "%s", tmp);
   API_END

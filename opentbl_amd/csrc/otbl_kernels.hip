@@ -357,7 +357,7 @@ void k_vistyy(Box &b, const double *u, double *res, int R, int nplanes) {
 // before they are zeroed.  Dynamic LDS: 3*(ny+2) + 4*YNCH*L doubles.
 __global__ void impl_factor_kernel(double *fu, double *fv, DevCoef c, int ny, const DevScalars *ds,
                                    double rkk_explicit, int use_explicit, double *tu, double *tv, int L, float *tuf,
-                                   float *tvf) {
+                                   float *tvf, float *thu, float *thv) {
   extern __shared__ double shf[];
   const int jee = blockIdx.x == 0 ? ny + 1 : ny;
   const double *icon = blockIdx.x == 0 ? c.vyui : c.vyvi;
@@ -405,6 +405,12 @@ __global__ void impl_factor_kernel(double *fu, double *fv, DevCoef c, int ny, co
     __syncthreads();
     for (int q = threadIdx.x; q < ytab_size(L); q += blockDim.x) tabf[q] = (float)tab[q];
   }
+  // and of its half-wave chunking
+  float *tabh = blockIdx.x == 0 ? thu : thv;
+  if (L && tabh) {
+    __syncthreads();
+    ytab_halve_block(tabh, tab, L);
+  }
 }
 
 static void launch_impl_factor(Box &b, double rkk, int use_explicit) {
@@ -412,9 +418,11 @@ static void launch_impl_factor(Box &b, double rkk, int use_explicit) {
   double *tv = b.yL ? b.ytab + YT_IMPV * ytab_size(b.yL) : nullptr;
   float *tuf = b.ytab_f ? b.ytab_f + YT_IMPU * ytab_size(b.yL) : nullptr;
   float *tvf = b.ytab_f ? b.ytab_f + YT_IMPV * ytab_size(b.yL) : nullptr;
+  float *thu = b.ytabh_f ? b.ytabh_f + 2 * ytabh_size(b.yL / 2) : nullptr;
+  float *thv = b.ytabh_f ? b.ytabh_f + 3 * ytabh_size(b.yL / 2) : nullptr;
   const size_t shm = (3 * (size_t)(b.ny + 2) + 4 * (size_t)YNCH * b.yL) * sizeof(double);
   LAUNCH(b, "impl_factor", 2, TPB, shm, impl_factor_kernel, b.implu, b.implv, b.dc, b.ny, b.ds, rkk,
-         use_explicit, tu, tv, b.yL, tuf, tvf);
+         use_explicit, tu, tv, b.yL, tuf, tvf, thu, thv);
 }
 
 // implzy (lip.f90:528-597): interior rows 2..jee-1 of u from res; rows 1 and

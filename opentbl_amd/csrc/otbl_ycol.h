@@ -129,6 +129,54 @@ __device__ inline void ytab_build_block(double *tab, int n, int L, F row, E sten
   __syncthreads();
 }
 
+// Half-wave chunking of the same systems: YHCH chunks of L/2 rows (the
+// same YNCH*L rows), one per half-wavefront of a workgroup of YTPB threads
+// that takes 32 columns.
+static constexpr int YHCH = 2 * YNCH;
+__host__ __device__ constexpr size_t ytabh_size(int L2) { return (size_t)8 * YHCH * L2 + 3 * YHCH; }
+
+// FP32 table of the half-wave chunking from the FP64 table src of the
+// YNCH-chunk layout with L rows per chunk (L even, at most 24): the row
+// arrays M, A2, B, E1..E3 are the same rows; the chunk responses G, S and
+// PE, G0, S0 are recomputed (in FP64) for chunks of L/2 rows.  Run by a
+// whole workgroup after src is complete; no synchronisation at the end.
+__device__ inline void ytab_halve_block(float *dst, const double *src, int L) {
+  const int NL = YNCH * L, L2 = L / 2;
+  for (int j = threadIdx.x; j < NL; j += blockDim.x) {
+    dst[j] = (float)src[j];
+    dst[NL + j] = (float)src[NL + j];
+    dst[2 * NL + j] = (float)src[2 * NL + j];
+    dst[5 * NL + j] = (float)src[5 * NL + j];
+    dst[6 * NL + j] = (float)src[6 * NL + j];
+    dst[7 * NL + j] = (float)src[7 * NL + j];
+  }
+  if (threadIdx.x < YHCH) {
+    const int q = threadIdx.x;
+    const double *M = src, *A2 = src + NL, *B = src + 2 * NL;
+    float *G = dst + 3 * NL, *S = dst + 4 * NL, *PE = dst + 8 * NL, *G0 = PE + YHCH, *S0 = G0 + YHCH;
+    double pr[12], acc = 1.0; // L2 <= 12: unrolled, in registers
+#pragma unroll
+    for (int r = 0; r < 12; ++r)
+      if (r < L2) {
+        acc *= M[q * L2 + r];
+        pr[r] = acc;
+      }
+    PE[q] = (float)acc;
+    double g = 0.0, sa = 1.0;
+#pragma unroll
+    for (int r = 11; r >= 0; --r)
+      if (r < L2) {
+        const int j = q * L2 + r;
+        g = A2[j] * pr[r] + B[j] * g;
+        sa = B[j] * sa;
+        G[j] = (float)g;
+        S[j] = (float)sa;
+      }
+    G0[q] = (float)g;
+    S0[q] = (float)sa;
+  }
+}
+
 // matrices factored by coefinv (coeft.f90:1338-1353): a(1,j) multiplier,
 // a(2,j) inverse pivot, a(3,j) super-diagonal
 __device__ inline TriRow coefinv_row(const double *a, int j) {
@@ -163,6 +211,10 @@ template <typename T>
 void ky_rhs_loop2(Box &b, T *rhsu, T *rhsv, T *rhsw, const T *resu, const T *resv, const T *resw, const T *vw,
                   const T *vv, const T *ww);
 template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, const T *rhsp, int jee);
+// FP32 tables of the half-wave chunking of the fixed vistyy systems
+// (Box::ytabh_f slots 0 and 1; slots 2 and 3, the implicit systems, are
+// written with each factorisation)
+void k_ytab_halve_fixed(Box &b);
 // rk_final of the FP64 mean-flow u (kz = 0 real part) of the mixed substep
 template <typename T> void ky_mean_rk_final(Box &b, double *um, const double *rm, const T *rhs, const T *rhsp);
 

@@ -62,6 +62,15 @@ void k_ytab_fixed(Box &b) {
   LAUNCH(b, "ytab_fixed", 1, 64, 0, ytab_fixed_kernel, b.ytab, b.dc, b.ny, b.yL);
 }
 
+// half-wave chunked FP32 tables of the vistyy systems: block 0 u, block 1 v
+__global__ void ytab_halve_kernel(float *dst, const double *src, int L) {
+  ytab_halve_block(dst + blockIdx.x * ytabh_size(L / 2), src + blockIdx.x * ytab_size(L), L);
+}
+
+void k_ytab_halve_fixed(Box &b) {
+  LAUNCH(b, "ytab_halve", 2, 256, 0, ytab_halve_kernel, b.ytabh_f, b.ytab + YT_VIYU * ytab_size(b.yL), b.yL);
+}
+
 
 // ---------------------------------------------------------------------------
 // per-wave column helpers
@@ -131,9 +140,9 @@ __device__ __forceinline__ T bnd4(const T *cf, int ld, int row, const T *col, si
 // G: rows per fenced group (a group's input loads are issued together);
 // the FP32 interpolation and rhsu kernels gain from whole-chunk groups, the
 // kernels with more live state lose.
-template <int L, int G = YGRP, typename T, class Fetch>
+template <int L, int G = YGRP, int NCH = YNCH, typename T, class Fetch>
 __device__ __forceinline__ T yfwd(T (&v)[L], Fetch fetch, const T *tab, int p, int n, T bot, T top, T add1, T addm) {
-  constexpr int NL = YNCH * L;
+  constexpr int NL = NCH * L;
   const T *M = tab + p * L, *E1 = tab + 5 * NL + p * L, *E2 = tab + 6 * NL + p * L, *E3 = tab + 7 * NL + p * L;
   const int j0 = p * L;
   T gm = fetch(0), g0 = fetch(1), y = 0;
@@ -158,13 +167,13 @@ __device__ __forceinline__ T yfwd(T (&v)[L], Fetch fetch, const T *tab, int p, i
 
 // Backward local sweep, carry exchange between the chunks of each column
 // and the spike correction: on return v holds the solution of the chunk.
-// xch: LDS, 2*YNCH*64 doubles.
-template <int L, typename T>
+// xch: LDS, 2*NCH*CW values (NCH chunks of CW columns).
+template <int L, int NCH = YNCH, int CW = 64, typename T>
 __device__ __forceinline__ void ybwd(T (&v)[L], T yend, const T *tab, int p, int lane, T *xch) {
-  constexpr int NL = YNCH * L;
+  constexpr int NL = NCH * L;
   const T *A2 = tab + NL + p * L, *B = tab + 2 * NL + p * L;
   const T *G = tab + 3 * NL + p * L, *S = tab + 4 * NL + p * L;
-  const T *PE = tab + 8 * NL, *G0 = PE + YNCH, *S0 = G0 + YNCH;
+  const T *PE = tab + 8 * NL, *G0 = PE + NCH, *S0 = G0 + NCH;
   v[L - 1] = A2[L - 1] * v[L - 1];
 #pragma unroll
   for (int r = L - 2; r >= 0; --r) {
@@ -173,18 +182,18 @@ __device__ __forceinline__ void ybwd(T (&v)[L], T yend, const T *tab, int p, int
   }
   YFENCE();
   __syncthreads(); // earlier readers of xch are done
-  xch[p * 64 + lane] = yend;
-  xch[(YNCH + p) * 64 + lane] = v[0];
+  xch[p * CW + lane] = yend;
+  xch[(NCH + p) * CW + lane] = v[0];
   __syncthreads();
   // forward carry C_p = y(row above chunk p): C_{q+1} = yend_q + PE_q C_q.
   // backward carry D_p = x(row below chunk p) = sum over q > p of
   // (xstart_q + G0_q C_q) * S0_{p+1} ... S0_{q-1}, accumulated forwards.
   T C = 0;
-  for (int q = 0; q < p; ++q) C = xch[q * 64 + lane] + PE[q] * C;
+  for (int q = 0; q < p; ++q) C = xch[q * CW + lane] + PE[q] * C;
   T Cq = C, D = 0, prod = 1;
-  for (int q = p + 1; q < YNCH; ++q) {
-    Cq = xch[(q - 1) * 64 + lane] + PE[q - 1] * Cq;
-    D += prod * (xch[(YNCH + q) * 64 + lane] + G0[q] * Cq);
+  for (int q = p + 1; q < NCH; ++q) {
+    Cq = xch[(q - 1) * CW + lane] + PE[q - 1] * Cq;
+    D += prod * (xch[(NCH + q) * CW + lane] + G0[q] * Cq);
     prod *= S0[q];
   }
 #pragma unroll
@@ -196,13 +205,13 @@ __device__ __forceinline__ void ybwd(T (&v)[L], T yend, const T *tab, int p, int
 }
 
 // solve for a chunk whose input rows are read from a global column
-template <int L, int G = YGRP, typename T>
+template <int L, int G = YGRP, int NCH = YNCH, int CW = 64, typename T>
 __device__ __forceinline__ void ysolve_col(T (&v)[L], const T *col, size_t s, int nin, const T *tab, int p, int lane,
                                            int n, T bot, T top, T *xch) {
   const int j0 = p * L;
-  const T yend = yfwd<L, G>(
+  const T yend = yfwd<L, G, NCH>(
       v, [&](int k) { return col[(size_t)yclamp(j0 - 1 + k, nin) * s]; }, tab, p, n, bot, top, T(0), T(0));
-  ybwd<L>(v, yend, tab, p, lane, xch);
+  ybwd<L, NCH, CW>(v, yend, tab, p, lane, xch);
 }
 
 // one-sided closures of the operators (lip.f90), evaluated by the wave that
@@ -560,6 +569,96 @@ rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, const T *
   }
 }
 
+// FP32 rk_final with half-wave chunks: the same solves as rk_final_y with a
+// workgroup taking 32 columns and each of its YHCH half-wavefronts L rows
+// of them (L is half the rows per chunk of rk_final_y).  A lane holds half
+// the rows, so two workgroups share a CU (64 VGPRs) and one workgroup's
+// loads overlap the other's serial solve.  The two half-waves of a wave
+// hold different chunks: table reads come from LDS (broadcast).
+template <int L>
+__global__ void __launch_bounds__(YTPB, 8)
+rk_final_yh(float *u, const float *__restrict__ res, const float *__restrict__ rhs, const float *__restrict__ rhsp,
+            const float *__restrict__ tabv, const float *__restrict__ tabi, DevCoefT<float> c,
+            const double *__restrict__ f, const DevScalars *__restrict__ ds, int jee, int ny, int nz1, int nx) {
+  using T = float;
+  constexpr int CW = 32;
+  constexpr int TS = (int)ytabh_size(L);
+  __shared__ T xch[2 * YHCH * CW];
+  __shared__ T hal[2 * YHCH * CW];
+  __shared__ T tlv[TS], tli[TS];
+  for (int k = threadIdx.x; k < TS; k += YTPB) {
+    tlv[k] = tabv[k];
+    tli[k] = tabi[k];
+  }
+  const int lane = threadIdx.x & (CW - 1), p = threadIdx.x / CW;
+  const unsigned ncols = (unsigned)nz1 * (unsigned)nx;
+  unsigned cc = xcd_block() * CW + lane;
+  const bool ok = cc < ncols;
+  if (!ok) cc = 0; // a dead lane addresses column 0 of the first plane
+  const unsigned kr = cc % (unsigned)nz1, ip = cc / (unsigned)nz1;
+  // 32-bit element offsets from the uniform field bases (a field of FP32
+  // values stays below 2^30 elements, checked by the launcher): one VGPR
+  // per row address
+  const unsigned s = (unsigned)nz1, off = ip * s * (unsigned)jee + kr;
+  auto at = [](const T *base, unsigned e) -> const T & {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (size_t)(e * 4u));
+  };
+  const bool act = ok && ip > 0;
+  const int j0 = p * L;
+  const YBnd<T> bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, res + off, (size_t)s);
+  const T u1 = at(u, off), uj = at(u, off + (unsigned)(jee - 1) * s);
+  const T a1b = (T)f[3 * (jee + 1)], a3b = (T)f[3 * (jee + 1) + 1];
+  const T var2 = (ip == (unsigned)(nx - 1)) ? T(0) : (T)ds->var2;
+  const T var3 = (T)ds->var3, var4 = (T)ds->var4;
+  T t[L], w[L];
+  {
+    // the explicit part w = u - (var3 rhs + var4 rhs_prev), its 3L rows
+    // issued at once
+    T rh[L], rp[L], uu[L];
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+      const unsigned o = off + (unsigned)yclamp(j0 + r, jee) * s;
+      rh[r] = at(rhs, o);
+      rp[r] = at(rhsp, o);
+      uu[r] = at(u, o);
+    }
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+      w[r] = uu[r] - (var3 * rh[r] + var4 * rp[r]);
+      asm volatile("" : "+v"(w[r])); // formed here, not sunk to its use after the solve
+    }
+  }
+  YFENCE();
+  __syncthreads(); // tables staged
+  {
+    const T yend = yfwd<L, YGRP, YHCH>(
+        t, [&](int k) { return at(res, off + (unsigned)yclamp(j0 - 1 + k, jee) * s); }, tlv, p, jee, bd.bot,
+        bd.top, T(0), T(0));
+    ybwd<L, YHCH, CW>(t, yend, tlv, p, lane, xch);
+  }
+#pragma unroll
+  for (int r = 0; r < L; ++r)
+    if (act && j0 + r < jee) t[r] = w[r] - var2 * t[r];
+  YFENCE();
+  // neighbour rows of the combined right-hand side
+  hal[p * CW + lane] = t[0];
+  hal[(YHCH + p) * CW + lane] = t[L - 1];
+  __syncthreads();
+  const T tm = p > 0 ? hal[(YHCH + p - 1) * CW + lane] : T(0);
+  const T tp = p < YHCH - 1 ? hal[(p + 1) * CW + lane] : T(0);
+  // implzy on t, in place
+  const T yend = yfwd<L, YGRP, YHCH>(
+      t, [&](int k) { return k == 0 ? tm : (k == L + 1 ? tp : t[k - 1]); }, tli, p, jee, u1, uj, -a1b * u1,
+      -a3b * uj);
+  ybwd<L, YHCH, CW>(t, yend, tli, p, lane, xch);
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    const int j = j0 + r;
+    if (act && j >= 1 && j < jee - 1)
+      *reinterpret_cast<T *>(reinterpret_cast<char *>(u) + (size_t)((off + (unsigned)j * s) * 4u)) = t[r];
+  }
+}
+
 // rk_final of the FP64 mean-flow u of the mixed substep (FP32 fields): the
 // explicit RK3 combination with vistyy (rhsp.F90:448-475) and the implicit y
 // solve (rhsp.F90:479-484) of the kz = 0 real part of u, every plane a
@@ -683,6 +782,19 @@ void ky_rhs_loop2(Box &b, T *rhsu, T *rhsv, T *rhsw, const T *resu, const T *res
 template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, const T *rhsp, int jee) {
   const unsigned g = ygrid((size_t)b.nz1 * b.nx);
   const bool uu = jee == b.ny + 1;
+  if constexpr (sizeof(T) == 4) {
+    if (b.ytabh_f) {
+      if ((size_t)b.nz1 * b.nx * (b.ny + 1) >= ((size_t)1 << 30))
+        throw std::invalid_argument("half-wave rk_final: a field of more than 2^30 values");
+      const unsigned gh = (unsigned)(((size_t)b.nz1 * b.nx + 31) / 32);
+      const size_t th = ytabh_size(b.yL / 2);
+      YDISPATCH(b.yL, auto kf = &rk_final_yh<L / 2>;
+                LAUNCH(b, "rk_final", gh, YTPB, 0, kf, u, res, rhs, rhsp, b.ytabh_f + (uu ? 0 : th),
+                       b.ytabh_f + (uu ? 2 : 3) * th, coefs<T>(b), uu ? b.implu : b.implv, b.ds, jee, b.ny,
+                       b.nz1, b.nx));
+      return;
+    }
+  }
   YDISPATCH(b.yL, auto kf = &rk_final_y<L, T>;
             LAUNCH(b, "rk_final", g, YTPB, 0, kf, u, res, rhs, rhsp, ytab<T>(b, uu ? YT_VIYU : YT_VIYV),
                    ytab<T>(b, uu ? YT_IMPU : YT_IMPV), coefs<T>(b), uu ? b.implu : b.implv, b.ds, jee, b.ny, b.nz1,

@@ -169,3 +169,24 @@ def test_mixed_long_x_vs_fp64():
     gm.close()
     print("mixed vs FP64, nx=1100, after 2 steps", errs, flush=True)
     check(errs)
+
+
+@pytest.mark.parametrize("ny,chunk", [(150, 12), (315, 20)], ids=["y150", "y315"])
+def test_mixed_tall_columns_vs_fp64(ny, chunk):
+    """Columns of the production height (ny = 315) and of an intermediate
+    one: the y lines take 20 (12) rows per chunk and the FP32 rk_final its
+    half-wave chunking of 10 (6) rows; mixed steps against the FP64 steps on
+    the GPU."""
+    cfg = config.small(name=f"mixy{ny}", nx=130, ny=ny, nz=256, Lx=8.0 * 130 / 32, kzmax=40)
+    st = ic.make_state(cfg)
+    gd = GpuBox(cfg, st)
+    gm = GpuBox(cfg, st, precision=OTBL_MIXED, graphs=True)
+    pm = gm.paths()
+    assert pm["y"] == f"chunk{chunk}h", pm
+    gd.step(2)
+    gm.step(2)
+    errs = errors(gm, gd)
+    gd.close()
+    gm.close()
+    print(f"mixed vs FP64, ny={ny}, after 2 steps", errs, flush=True)
+    check(errs)
