@@ -429,7 +429,7 @@ static void setup_mixed(Box &b) {
   k_d2f(b, b.ytab, b.ytab_f, YT_COUNT * ytab_size(b.yL));
   const char *yh = std::getenv("OTBL_Y_HALF");
   if (b.yL % 2 == 0 && !(yh && yh[0] == '0')) {
-    dalloc(b.ytabh_f, 4 * ytabh_size(b.yL / 2));
+    dalloc(b.ytabh_f, YT_COUNT * ytabh_size(b.yL / 2));
     k_ytab_halve_fixed(b);
   }
   dalloc(b.ztw_f, 2 * (size_t)b.nz);

@@ -79,8 +79,9 @@ struct Box {
   // factorisation): coefficients, y-line tables, twiddles, Bluestein tables
   DevCoefT<float> dcf{};
   float *coefbuf_f = nullptr, *ytab_f = nullptr, *ztw_f = nullptr;
-  // FP32 tables of the half-wave chunked rk_final (vistyy u, v; implicit
-  // u, v), null when that kernel is off (odd row count per chunk, OTBL_Y_HALF=0)
+  // FP32 tables of the half-wave chunked y kernels of the mixed substep,
+  // one per system in the order of ytab; null when those kernels are off
+  // (odd row count per chunk, OTBL_Y_HALF=0)
   float *ytabh_f = nullptr;
   float *dct_chirp_f = nullptr, *dct_bf_f = nullptr, *dct_tw4_f = nullptr, *dct_ztw_f = nullptr;
   // Poisson y solves of the modes kz >= 1 partitioned in registers: rows

@@ -418,8 +418,8 @@ static void launch_impl_factor(Box &b, double rkk, int use_explicit) {
   double *tv = b.yL ? b.ytab + YT_IMPV * ytab_size(b.yL) : nullptr;
   float *tuf = b.ytab_f ? b.ytab_f + YT_IMPU * ytab_size(b.yL) : nullptr;
   float *tvf = b.ytab_f ? b.ytab_f + YT_IMPV * ytab_size(b.yL) : nullptr;
-  float *thu = b.ytabh_f ? b.ytabh_f + 2 * ytabh_size(b.yL / 2) : nullptr;
-  float *thv = b.ytabh_f ? b.ytabh_f + 3 * ytabh_size(b.yL / 2) : nullptr;
+  float *thu = b.ytabh_f ? b.ytabh_f + YT_IMPU * ytabh_size(b.yL / 2) : nullptr;
+  float *thv = b.ytabh_f ? b.ytabh_f + YT_IMPV * ytabh_size(b.yL / 2) : nullptr;
   const size_t shm = (3 * (size_t)(b.ny + 2) + 4 * (size_t)YNCH * b.yL) * sizeof(double);
   LAUNCH(b, "impl_factor", 2, TPB, shm, impl_factor_kernel, b.implu, b.implv, b.dc, b.ny, b.ds, rkk,
          use_explicit, tu, tv, b.yL, tuf, tvf, thu, thv);

@@ -211,9 +211,8 @@ template <typename T>
 void ky_rhs_loop2(Box &b, T *rhsu, T *rhsv, T *rhsw, const T *resu, const T *resv, const T *resw, const T *vw,
                   const T *vv, const T *ww);
 template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, const T *rhsp, int jee);
-// FP32 tables of the half-wave chunking of the fixed vistyy systems
-// (Box::ytabh_f slots 0 and 1; slots 2 and 3, the implicit systems, are
-// written with each factorisation)
+// FP32 tables of the half-wave chunking of the six fixed systems
+// (Box::ytabh_f; the implicit systems' are written with each factorisation)
 void k_ytab_halve_fixed(Box &b);
 // rk_final of the FP64 mean-flow u (kz = 0 real part) of the mixed substep
 template <typename T> void ky_mean_rk_final(Box &b, double *um, const double *rm, const T *rhs, const T *rhsp);

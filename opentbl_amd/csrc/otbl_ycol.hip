@@ -62,13 +62,14 @@ void k_ytab_fixed(Box &b) {
   LAUNCH(b, "ytab_fixed", 1, 64, 0, ytab_fixed_kernel, b.ytab, b.dc, b.ny, b.yL);
 }
 
-// half-wave chunked FP32 tables of the vistyy systems: block 0 u, block 1 v
+// half-wave chunked FP32 tables of the fixed systems, one per block
 __global__ void ytab_halve_kernel(float *dst, const double *src, int L) {
   ytab_halve_block(dst + blockIdx.x * ytabh_size(L / 2), src + blockIdx.x * ytab_size(L), L);
 }
 
 void k_ytab_halve_fixed(Box &b) {
-  LAUNCH(b, "ytab_halve", 2, 256, 0, ytab_halve_kernel, b.ytabh_f, b.ytab + YT_VIYU * ytab_size(b.yL), b.yL);
+  static_assert(YT_INYU == 0 && YT_VIYV == 5, "the fixed systems lead the table order");
+  LAUNCH(b, "ytab_halve", YT_VIYV + 1, 256, 0, ytab_halve_kernel, b.ytabh_f, b.ytab, b.yL);
 }
 
 
@@ -569,46 +570,71 @@ rk_final_y(T *u, const T *__restrict__ res, const T *__restrict__ rhs, const T *
   }
 }
 
-// FP32 rk_final with half-wave chunks: the same solves as rk_final_y with a
-// workgroup taking 32 columns and each of its YHCH half-wavefronts L rows
-// of them (L is half the rows per chunk of rk_final_y).  A lane holds half
-// the rows, so two workgroups share a CU (64 VGPRs) and one workgroup's
-// loads overlap the other's serial solve.  The two half-waves of a wave
-// hold different chunks: table reads come from LDS (broadcast).
+// ---------------------------------------------------------------------------
+// Half-wave chunked FP32 rk_final of the mixed substep.  A workgroup takes
+// YHC = 32 columns and each of its YHCH = 32 half-wavefronts L rows of them
+// (half the rows per chunk of the kernels above), so a lane holds half the
+// rows, two workgroups share a CU (64 VGPRs) and one workgroup's loads
+// overlap the other's two serial solves.  The two half-waves of a wave hold
+// different chunks, so the tables are staged in LDS (two addresses per
+// wave: broadcast reads).  Field elements are addressed by 32-bit offsets
+// from the uniform field base (one VGPR per row address; the launcher
+// checks that every field stays below 2^30 values).  The one-solve FP32
+// kernels (interpyy, rhsu_loop1) measured slower in this layout, and
+// rhs_loop2's live state does not fit 64 VGPRs: they keep the full-wave
+// chunks.
+// ---------------------------------------------------------------------------
+static constexpr int YHC = 32;
+
+struct YColH {
+  int p, lane;
+  bool ok;
+  unsigned kr, ip; // ip: 0-based plane
+};
+
+__device__ inline YColH ycolh(unsigned ncols, int nz1, unsigned ip0) {
+  YColH y;
+  y.lane = threadIdx.x & (YHC - 1);
+  y.p = threadIdx.x / YHC;
+  unsigned c = xcd_block() * YHC + y.lane;
+  y.ok = c < ncols;
+  if (!y.ok) c = 0; // a dead lane addresses column 0 of the first plane
+  y.kr = c % (unsigned)nz1;
+  y.ip = ip0 + c / (unsigned)nz1;
+  return y;
+}
+
+__device__ __forceinline__ float yat(const float *base, unsigned e) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (size_t)(e * 4u));
+}
+__device__ __forceinline__ float &yref(float *base, unsigned e) {
+  return *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (size_t)(e * 4u));
+}
+
+template <int L> __device__ __forceinline__ void ytabh_stage(float *dst, const float *__restrict__ src) {
+  for (int k = threadIdx.x; k < (int)ytabh_size(L); k += YTPB) dst[k] = src[k];
+}
+
+// rk_final (rhsp.F90:448-484), as rk_final_y
 template <int L>
 __global__ void __launch_bounds__(YTPB, 8)
 rk_final_yh(float *u, const float *__restrict__ res, const float *__restrict__ rhs, const float *__restrict__ rhsp,
             const float *__restrict__ tabv, const float *__restrict__ tabi, DevCoefT<float> c,
             const double *__restrict__ f, const DevScalars *__restrict__ ds, int jee, int ny, int nz1, int nx) {
   using T = float;
-  constexpr int CW = 32;
-  constexpr int TS = (int)ytabh_size(L);
-  __shared__ T xch[2 * YHCH * CW];
-  __shared__ T hal[2 * YHCH * CW];
-  __shared__ T tlv[TS], tli[TS];
-  for (int k = threadIdx.x; k < TS; k += YTPB) {
-    tlv[k] = tabv[k];
-    tli[k] = tabi[k];
-  }
-  const int lane = threadIdx.x & (CW - 1), p = threadIdx.x / CW;
-  const unsigned ncols = (unsigned)nz1 * (unsigned)nx;
-  unsigned cc = xcd_block() * CW + lane;
-  const bool ok = cc < ncols;
-  if (!ok) cc = 0; // a dead lane addresses column 0 of the first plane
-  const unsigned kr = cc % (unsigned)nz1, ip = cc / (unsigned)nz1;
-  // 32-bit element offsets from the uniform field bases (a field of FP32
-  // values stays below 2^30 elements, checked by the launcher): one VGPR
-  // per row address
-  const unsigned s = (unsigned)nz1, off = ip * s * (unsigned)jee + kr;
-  auto at = [](const T *base, unsigned e) -> const T & {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (size_t)(e * 4u));
-  };
-  const bool act = ok && ip > 0;
-  const int j0 = p * L;
+  __shared__ T xch[2 * YHCH * YHC];
+  __shared__ T hal[2 * YHCH * YHC];
+  __shared__ T tlv[ytabh_size(L)], tli[ytabh_size(L)];
+  ytabh_stage<L>(tlv, tabv);
+  ytabh_stage<L>(tli, tabi);
+  const YColH q = ycolh((unsigned)nz1 * nx, nz1, 0);
+  const unsigned s = nz1, off = q.ip * s * (unsigned)jee + q.kr;
+  const bool act = q.ok && q.ip > 0;
+  const int p = q.p, lane = q.lane, j0 = p * L;
   const YBnd<T> bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, res + off, (size_t)s);
-  const T u1 = at(u, off), uj = at(u, off + (unsigned)(jee - 1) * s);
+  const T u1 = yat(u, off), uj = yat(u, off + (unsigned)(jee - 1) * s);
   const T a1b = (T)f[3 * (jee + 1)], a3b = (T)f[3 * (jee + 1) + 1];
-  const T var2 = (ip == (unsigned)(nx - 1)) ? T(0) : (T)ds->var2;
+  const T var2 = (q.ip == (unsigned)(nx - 1)) ? T(0) : (T)ds->var2;
   const T var3 = (T)ds->var3, var4 = (T)ds->var4;
   T t[L], w[L];
   {
@@ -618,9 +644,9 @@ rk_final_yh(float *u, const float *__restrict__ res, const float *__restrict__ r
 #pragma unroll
     for (int r = 0; r < L; ++r) {
       const unsigned o = off + (unsigned)yclamp(j0 + r, jee) * s;
-      rh[r] = at(rhs, o);
-      rp[r] = at(rhsp, o);
-      uu[r] = at(u, o);
+      rh[r] = yat(rhs, o);
+      rp[r] = yat(rhsp, o);
+      uu[r] = yat(u, o);
     }
 #pragma unroll
     for (int r = 0; r < L; ++r) {
@@ -632,30 +658,29 @@ rk_final_yh(float *u, const float *__restrict__ res, const float *__restrict__ r
   __syncthreads(); // tables staged
   {
     const T yend = yfwd<L, YGRP, YHCH>(
-        t, [&](int k) { return at(res, off + (unsigned)yclamp(j0 - 1 + k, jee) * s); }, tlv, p, jee, bd.bot,
+        t, [&](int k) { return yat(res, off + (unsigned)yclamp(j0 - 1 + k, jee) * s); }, tlv, p, jee, bd.bot,
         bd.top, T(0), T(0));
-    ybwd<L, YHCH, CW>(t, yend, tlv, p, lane, xch);
+    ybwd<L, YHCH, YHC>(t, yend, tlv, p, lane, xch);
   }
 #pragma unroll
   for (int r = 0; r < L; ++r)
     if (act && j0 + r < jee) t[r] = w[r] - var2 * t[r];
   YFENCE();
   // neighbour rows of the combined right-hand side
-  hal[p * CW + lane] = t[0];
-  hal[(YHCH + p) * CW + lane] = t[L - 1];
+  hal[p * YHC + lane] = t[0];
+  hal[(YHCH + p) * YHC + lane] = t[L - 1];
   __syncthreads();
-  const T tm = p > 0 ? hal[(YHCH + p - 1) * CW + lane] : T(0);
-  const T tp = p < YHCH - 1 ? hal[(p + 1) * CW + lane] : T(0);
+  const T tm = p > 0 ? hal[(YHCH + p - 1) * YHC + lane] : T(0);
+  const T tp = p < YHCH - 1 ? hal[(p + 1) * YHC + lane] : T(0);
   // implzy on t, in place
   const T yend = yfwd<L, YGRP, YHCH>(
       t, [&](int k) { return k == 0 ? tm : (k == L + 1 ? tp : t[k - 1]); }, tli, p, jee, u1, uj, -a1b * u1,
       -a3b * uj);
-  ybwd<L, YHCH, CW>(t, yend, tli, p, lane, xch);
+  ybwd<L, YHCH, YHC>(t, yend, tli, p, lane, xch);
 #pragma unroll
   for (int r = 0; r < L; ++r) {
     const int j = j0 + r;
-    if (act && j >= 1 && j < jee - 1)
-      *reinterpret_cast<T *>(reinterpret_cast<char *>(u) + (size_t)((off + (unsigned)j * s) * 4u)) = t[r];
+    if (act && j >= 1 && j < jee - 1) yref(u, off + (unsigned)j * s) = t[r];
   }
 }
 
@@ -734,6 +759,15 @@ template <> inline const double *ytab<double>(const Box &b, int k) {
 }
 template <> inline const float *ytab<float>(const Box &b, int k) { return b.ytab_f + (size_t)k * ytab_size(b.yL); }
 
+// the half-wave rk_final (FP32 with half-wave tables): workgroups for
+// ncols columns, table of a system, and the 32-bit offset bound
+static inline unsigned ygridh(size_t ncols) { return (unsigned)((ncols + YHC - 1) / YHC); }
+static inline const float *ytabh(const Box &b, int k) { return b.ytabh_f + (size_t)k * ytabh_size(b.yL / 2); }
+static void yh_check(const Box &b, size_t last) {
+  if (last >= ((size_t)1 << 30))
+    throw std::invalid_argument("half-wave y kernels: a field of more than 2^30 values");
+}
+
 template <typename T>
 void ky_interpyy(Box &b, const T *gen, T *geni, int loguv, int i0, int nplanes, size_t ips, size_t ops) {
   const unsigned g = ygrid((size_t)b.nz1 * nplanes);
@@ -784,14 +818,11 @@ template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs,
   const bool uu = jee == b.ny + 1;
   if constexpr (sizeof(T) == 4) {
     if (b.ytabh_f) {
-      if ((size_t)b.nz1 * b.nx * (b.ny + 1) >= ((size_t)1 << 30))
-        throw std::invalid_argument("half-wave rk_final: a field of more than 2^30 values");
-      const unsigned gh = (unsigned)(((size_t)b.nz1 * b.nx + 31) / 32);
-      const size_t th = ytabh_size(b.yL / 2);
+      yh_check(b, (size_t)b.nz1 * b.nx * (b.ny + 1));
       YDISPATCH(b.yL, auto kf = &rk_final_yh<L / 2>;
-                LAUNCH(b, "rk_final", gh, YTPB, 0, kf, u, res, rhs, rhsp, b.ytabh_f + (uu ? 0 : th),
-                       b.ytabh_f + (uu ? 2 : 3) * th, coefs<T>(b), uu ? b.implu : b.implv, b.ds, jee, b.ny,
-                       b.nz1, b.nx));
+                LAUNCH(b, "rk_final", ygridh((size_t)b.nz1 * b.nx), YTPB, 0, kf, u, res, rhs, rhsp,
+                       ytabh(b, uu ? YT_VIYU : YT_VIYV), ytabh(b, uu ? YT_IMPU : YT_IMPV), coefs<T>(b),
+                       uu ? b.implu : b.implv, b.ds, jee, b.ny, b.nz1, b.nx));
       return;
     }
   }
