@@ -49,8 +49,11 @@ def kernel_model(cfg, s=8):
     m = {}
     m["interpxx"] = ((2 * (Ru + Rv + Ru) * f) / 3, 0.0, 3)
     m["difvisxx"] = (((4 * Ru + 3 * Rv + 3 * Ru) * f) / 3, 0.0, 3)
-    m["products1"] = (X * (Rv * 8 + 5) * z * s, X * (Rv * 5 + 3) * 5 * nz * math.log2(nz) * 2, 1)
-    m["products2"] = ((X - 2) * Rv * 7 * z * s, (X - 2) * Rv * 4 * 5 * nz * math.log2(nz) * 2, 1)
+    # complex transforms of length nz (5 nz log2 nz flops each, two real rows
+    # per transform): 5 per row j <= ny and 3 for row ny+1 in plane loop 1,
+    # 4 per row in plane loop 2
+    m["products1"] = (X * (Rv * 8 + 5) * z * s, X * (Rv * 5 + 3) * 5 * nz * math.log2(nz), 1)
+    m["products2"] = ((X - 2) * Rv * 7 * z * s, (X - 2) * Rv * 4 * 5 * nz * math.log2(nz), 1)
     m["interpyy"] = ((2 * Ru * X + (Ru + Rv) * (X - 2) + (Rv + Ru) * (X - 2)) * z * s / 3, 0.0, 3)
     m["rhsu_loop1"] = ((Rv + 2 * Ru) * f, 0.0, 1)
     m["rhs_loop2"] = ((X - 2) * z * s * (2 * Ru + 2 * Rv + 2 * Ru + Ru + Rv + Ru + Ru + Rv), 0.0, 1)
