@@ -1,6 +1,6 @@
 """Throughput of the RK3 substep on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload zpg1536x315x1024]
                     [--precision both|mixed|fp64]
 
 A step is one RK3 step (3 substeps: rhsp, outflow_correction, pois) of the
@@ -184,7 +184,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(config.CONFIGS))
+    ap.add_argument("--workload", default="zpg1536x315x1024", choices=sorted(config.CONFIGS))
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
@@ -234,7 +234,7 @@ def main():
     head = "mixed" if "mixed" in runs else "fp64"
     h = runs[head]
     steps = args.steps
-    cfgd = {"workload": f"{cfg.name}: {cfg.nx}x{cfg.ny}x{cfg.nz} ZPG boundary layer, "
+    cfgd = {"workload": f"{cfg.nx}x{cfg.ny}x{cfg.nz} ZPG boundary layer, "
                         f"Re_theta_in={cfg.re_theta:g}, {head} substep, one hipGraph per substep",
             "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": head, "graphs": not args.no_graphs,
             "dt": "CFL every step", "paths": h["paths"],

@@ -79,13 +79,13 @@ class BoxConfig:
 
 CONFIGS = {
     # plain FP64 reference method, CPU-runnable
-    "c1": BoxConfig("c1", 256, 96, 64, 60.0, 4.0, 8.0, 300.0, 1e-3, 8, "gauss", 0, 20),
+    "zpg256x96x64": BoxConfig("zpg256x96x64", 256, 96, 64, 60.0, 4.0, 8.0, 300.0, 1e-3, 8, "gauss", 0, 20),
     # FP64 GPU drop-in, production-like single-GPU size
-    "c2": BoxConfig("c2", 1024, 192, 512, 60.0, 4.0, 8.0, 600.0, 0.05, 32, "k53", 1, 50),
+    "zpg1024x192x512": BoxConfig("zpg1024x192x512", 1024, 192, 512, 60.0, 4.0, 8.0, 600.0, 0.05, 32, "k53", 1, 50),
     # mixed precision headline box
-    "c3": BoxConfig("c3", 1536, 315, 1024, 90.0, 4.0, 12.0, 1000.0, 0.1, 64, "k53", 2, 100),
+    "zpg1536x315x1024": BoxConfig("zpg1536x315x1024", 1536, 315, 1024, 90.0, 4.0, 12.0, 1000.0, 0.1, 64, "k53", 2, 100),
     # largest single-GPU box
-    "c5": BoxConfig("c5", 3072, 360, 2048, 180.0, 4.0, 24.0, 1500.0, 0.1, 128, "k53", 3, 200),
+    "zpg3072x360x2048": BoxConfig("zpg3072x360x2048", 3072, 360, 2048, 180.0, 4.0, 24.0, 1500.0, 0.1, 128, "k53", 3, 200),
 }
 
 
