@@ -33,6 +33,16 @@ namespace otbl {
 static constexpr int PK = 32;     // chunks per column = columns per workgroup
 static constexpr int PROWS = 512; // rows the LDS coefficient copy holds (ny - 1 <= PROWS)
 
+// 1 / x: the FP32 solves (followed by a refinement step whose FP64
+// residual absorbs a 1-ulp pivot error) take the hardware reciprocal, off
+// the serial interface chain's critical path; FP64 divides
+template <typename T> __device__ __forceinline__ T prcp(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_rcpf(x);
+  else
+    return T(1) / x;
+}
+
 template <int L, typename T, bool REF>
 __global__ void __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1)
 poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, int nz1) {
@@ -74,26 +84,31 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
   const unsigned rs = (unsigned)(nz1 * N * sizeof(T));
   const unsigned base = (unsigned)(((2 * kz + (col & 1)) * N + x) * sizeof(T));
   const int r0 = k * L + 1;
+  // rows q < nq of the chunk are operator rows (r0 + q <= ny - 1), the rest
+  // identity rows: one count instead of L row indices kept live
+  const int nq = ny1 - r0 + 1;
   char *Pc = reinterpret_cast<char *>(P);
-  auto AT = [&](int q) -> T & { return *reinterpret_cast<T *>(Pc + (base + (unsigned)(r0 + q - 1) * rs)); };
+  // the chunk's first row; row q adds the uniform q rs
+  const unsigned b0 = base + (unsigned)(r0 - 1) * rs;
+  auto AT = [&](int q) -> T & { return *reinterpret_cast<T *>(Pc + (b0 + (unsigned)q * rs)); };
   const T sh = c.kaz2[kz] + c.kmod[x + 1];
   auto A1 = [&](int r) -> T { return ca1[r]; };
   auto A3 = [&](int r) -> T { return ca3[r]; };
-  auto DG = [&](int r) -> T { return r <= ny1 ? ca2[r] - sh : T(1); };
+  auto DG = [&](int q) -> T { return q < nq ? ca2[r0 + q] - sh : T(1); };
 
   // chunk factorisation: inverse pivots
   T ip[L];
 #pragma unroll
   for (int q = 0; q < L; ++q) {
     const int r = r0 + q;
-    T piv = DG(r);
+    T piv = DG(q);
     if (q > 0) piv -= A1(r) * ip[q - 1] * A3(r - 1);
-    ip[q] = T(1) / piv;
+    ip[q] = prcp(piv);
   }
   // right-hand side, read once
   T xv[L];
 #pragma unroll
-  for (int q = 0; q < L; ++q) xv[q] = (live && r0 + q <= ny1) ? AT(q) : T(0);
+  for (int q = 0; q < L; ++q) xv[q] = (live && q < nq) ? AT(q) : T(0);
   // local solve in place
   auto local_solve = [&](T(&v)[L]) {
 #pragma unroll
@@ -176,7 +191,7 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
           n4 = red[q + 1][4][col];
           n5 = red[q + 1][5][col];
         }
-        const T den = T(1) / (T(1) + qVF * be);
+        const T den = prcp(T(1) + qVF * be);
         const T ga = (qyF - qVF * al) * den, de = -qWF * den;
         const T nal = qyL - qVL * (al + be * ga), nbe = -qVL * be * de - qWL;
         if constexpr (REF) {
@@ -267,10 +282,10 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
       // one row at a time: its loads are not hoisted above the previous row
       __builtin_amdgcn_sched_barrier(0);
       const int r = r0 + q;
-      const bool in = live && r <= ny1;
+      const bool in = live && q < nq;
       const double b = in ? (double)AT(q) : 0.0;
       const double pm = q > 0 ? (double)xv[q - 1] : (double)XL, pp = q < L - 1 ? (double)xv[q + 1] : (double)XF;
-      const double dg = r <= ny1 ? da2[r] - shd : 1.0;
+      const double dg = q < nq ? da2[r] - shd : 1.0;
       d[q] = in ? (T)(b - (da1[r] * pm + dg * (double)xv[q] + da3[r] * pp)) : T(0);
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -283,7 +298,7 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
   }
 #pragma unroll
   for (int q = 0; q < L; ++q)
-    if (live && r0 + q <= ny1) AT(q) = xv[q];
+    if (live && q < nq) AT(q) = xv[q];
 }
 
 // rows per chunk for ny-1 rows (0: not instantiated)
