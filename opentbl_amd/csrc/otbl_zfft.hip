@@ -240,7 +240,7 @@ fourxz_z(double *fou, double *phys, int iopt, const double2 *__restrict__ tw, co
 // iopt < 0: DCT-II x 1/N, iopt > 0: DCT-III x 0.5; in place (src == dst)
 // allowed, each team reads its two lines before writing them.
 template <int M, typename R>
-__global__ void __launch_bounds__(ZTeam<M>::TPB_)
+__global__ void __launch_bounds__(ZTeam<M>::TPB_, sizeof(R) == 4 ? 6 : 1)
 dct_rows_z(const R *src, R *dst, int Lp, int N, int iopt, const typename CT<R>::c *__restrict__ chirp,
            const typename CT<R>::c *__restrict__ bf, const typename CT<R>::c *__restrict__ tw4,
            const typename CT<R>::c *__restrict__ tw) {
