@@ -190,3 +190,26 @@ def test_mixed_tall_columns_vs_fp64(ny, chunk):
     gm.close()
     print(f"mixed vs FP64, ny={ny}, after 2 steps", errs, flush=True)
     check(errs)
+
+
+def test_mixed_full_wave_rk_final(mbox, monkeypatch):
+    """OTBL_Y_HALF=0 keeps the full-wave FP32 rk_final (64 columns per
+    workgroup, 16 chunks): the same steps as the half-wave kernel up to FP32
+    rounding (other chunk boundaries), both within the bar of the FP64
+    path."""
+    cfg, st = mbox
+    gh = GpuBox(cfg, st, precision=OTBL_MIXED)
+    monkeypatch.setenv("OTBL_Y_HALF", "0")
+    gf = GpuBox(cfg, st, precision=OTBL_MIXED)
+    monkeypatch.delenv("OTBL_Y_HALF")
+    gd = GpuBox(cfg, st)
+    ph, pf = gh.paths(), gf.paths()
+    assert ph["y"].endswith("h") and not pf["y"].endswith("h"), (ph, pf)
+    for g in (gh, gf, gd):
+        g.step(2)
+    e_hf, e_fd = errors(gh, gf), errors(gf, gd)
+    for g in (gh, gf, gd):
+        g.close()
+    print("half-wave vs full-wave rk_final", e_hf, "full-wave vs FP64", e_fd, flush=True)
+    check(e_hf)
+    check(e_fd)
