@@ -763,7 +763,7 @@ template <> inline const float *ytab<float>(const Box &b, int k) { return b.ytab
 // ncols columns, table of a system, and the 32-bit offset bound
 static inline unsigned ygridh(size_t ncols) { return (unsigned)((ncols + YHC - 1) / YHC); }
 static inline const float *ytabh(const Box &b, int k) { return b.ytabh_f + (size_t)k * ytabh_size(b.yL / 2); }
-static void yh_check(const Box &b, size_t last) {
+static void yh_check(size_t last) {
   if (last >= ((size_t)1 << 30))
     throw std::invalid_argument("half-wave y kernels: a field of more than 2^30 values");
 }
@@ -818,7 +818,7 @@ template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs,
   const bool uu = jee == b.ny + 1;
   if constexpr (sizeof(T) == 4) {
     if (b.ytabh_f) {
-      yh_check(b, (size_t)b.nz1 * b.nx * (b.ny + 1));
+      yh_check((size_t)b.nz1 * b.nx * (b.ny + 1));
       YDISPATCH(b.yL, auto kf = &rk_final_yh<L / 2>;
                 LAUNCH(b, "rk_final", ygridh((size_t)b.nz1 * b.nx), YTPB, 0, kf, u, res, rhs, rhsp,
                        ytabh(b, uu ? YT_VIYU : YT_VIYV), ytabh(b, uu ? YT_IMPU : YT_IMPV), coefs<T>(b),
