@@ -1720,13 +1720,10 @@ poisson_y_pencil_kernel(double *__restrict__ P, double *__restrict__ S, DevCoef 
 // mean-flow pressure and um the FP64 mean-flow u: their differences are
 // taken in FP64 (else all null).
 template <typename T>
-__global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T> c, const DevScalars *ds,
-                               double vstep, int use_m, double idxx_d, int nx, int ny, int nz1,
-                               const double *__restrict__ P0, double *p0, const double *idyy_d, double *um,
-                               int pm) {
-  int kr, j;
-  // rows 2..ny of planes 2..nx change
-  if (!pcol(nz1, ny + 1, kr, j) || j < 2 || j > ny) return;
+__device__ __forceinline__ void project_col(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T> c,
+                                            const DevScalars *ds, double vstep, int use_m, double idxx_d, int nx,
+                                            int ny, int nz1, const double *__restrict__ P0, double *p0,
+                                            const double *idyy_d, double *um, int pm, int kr, int j) {
   const int i0 = max(2, (int)blockIdx.y * pm + 1), i1 = min(nx, (int)blockIdx.y * pm + pm);
   const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
   const double vs = use_m ? ds->varstep : vstep;
@@ -1773,6 +1770,74 @@ __global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T
   }
 }
 
+template <typename T>
+__global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T> c, const DevScalars *ds,
+                               double vstep, int use_m, double idxx_d, int nx, int ny, int nz1,
+                               const double *__restrict__ P0, double *p0, const double *idyy_d, double *um,
+                               int pm) {
+  int kr, j;
+  // rows 2..ny of planes 2..nx change
+  if (!pcol(nz1, ny + 1, kr, j) || j < 2 || j > ny) return;
+  project_col(u, v, w, p, rest, c, ds, vstep, use_m, idxx_d, nx, ny, nz1, P0, p0, idyy_d, um, pm, kr, j);
+}
+
+// The FP32 projection with one thread per mode (Re and Im, kr = 2 kz and
+// 2 kz + 1 adjacent in memory): 8-byte accesses, and the Im (Re) part of
+// phi that the w update of the Re (Im) part needs is the thread's own.
+// Mode 0 (its Re part takes the FP64 mean-flow lines) runs project_col per
+// part.  nz1 is even, so every pair is 8-byte aligned.
+__global__ void project_pair_kernel(float *u, float *v, float *w, float *p, const float *rest, DevCoefT<float> c,
+                                    const DevScalars *ds, double vstep, int use_m, double idxx_d, int nx, int ny,
+                                    int nz1, const double *__restrict__ P0, double *p0, const double *idyy_d,
+                                    double *um, int pm) {
+  int kp, j;
+  if (!pcol(nz1 / 2, ny + 1, kp, j) || j < 2 || j > ny) return;
+  if (kp == 0) {
+    for (int kr = 0; kr < 2; ++kr)
+      project_col(u, v, w, p, rest, c, ds, vstep, use_m, idxx_d, nx, ny, nz1, P0, p0, idyy_d, um, pm, kr, j);
+    return;
+  }
+  const int i0 = max(2, (int)blockIdx.y * pm + 1), i1 = min(nx, (int)blockIdx.y * pm + pm);
+  const size_t s = nz1, pu = s * (ny + 1), pv = s * ny;
+  const float varstep = (float)(use_m ? ds->varstep : vstep), idxx = (float)idxx_d;
+  const float kz = c.kaz[kp];
+  const float iy = j <= ny - 1 ? c.idyy[j] : 0.0f;
+  float2 *u2 = reinterpret_cast<float2 *>(u), *v2 = reinterpret_cast<float2 *>(v);
+  float2 *w2 = reinterpret_cast<float2 *>(w), *p2 = reinterpret_cast<float2 *>(p);
+  const float2 *ph = reinterpret_cast<const float2 *>(rest + (size_t)(i0 - 1) * pv);
+  // float2 indices: element offsets / 2
+  size_t e = ((size_t)(i0 - 1) * pu + (size_t)(j - 1) * s) / 2 + kp;
+  size_t ov = ((size_t)(i0 - 1) * pv + (size_t)(j - 1) * s) / 2 + kp;
+  const size_t o = (size_t)(j - 2) * s / 2 + kp, pv2 = pv / 2, pu2 = pu / 2, s2 = s / 2;
+  float2 phc = i0 <= i1 ? ph[o] : make_float2(0.0f, 0.0f);
+  for (int i = i0; i <= i1; ++i, e += pu2, ov += pv2, ph += pv2) {
+    const float2 phn = i <= nx - 1 ? ph[pv2 + o] : make_float2(0.0f, 0.0f);
+    float2 pp = p2[ov];
+    pp.x = varstep * phc.x + pp.x;
+    pp.y = varstep * phc.y + pp.y;
+    p2[ov] = pp;
+    // w -= i kz phi: Re gets +kz Im phi, Im gets -kz Re phi
+    float2 ww = w2[e];
+    ww.x += kz * phc.y;
+    ww.y -= kz * phc.x;
+    w2[e] = ww;
+    if (j <= ny - 1) {
+      const float2 pn = ph[o + s2];
+      float2 vv = v2[ov];
+      vv.x -= iy * (pn.x - phc.x);
+      vv.y -= iy * (pn.y - phc.y);
+      v2[ov] = vv;
+    }
+    if (i <= nx - 1) {
+      float2 uu = u2[e];
+      uu.x -= idxx * (phn.x - phc.x);
+      uu.y -= idxx * (phn.y - phc.y);
+      u2[e] = uu;
+    }
+    phc = phn;
+  }
+}
+
 // FP32 pressure solve of the mixed path: the kz = 0 lines run in FP64 on
 // their own pencils (divergence, both x transforms and the pinned y solve),
 // the other modes in FP32 with one FP64-residual refinement of the y solve.
@@ -1790,9 +1855,10 @@ static void pois_mixed(Box &b, double varstep, bool use_m) {
   kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1));
   LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pv, b.P0, nz1);
   const int pm = prun<float>();
-  LAUNCH(b, "project", pgrid(b, pm), TPB, 0, project_kernel<float>, fld<float>(b.f[OTBL_U]),
-         fld<float>(b.f[OTBL_V]), fld<float>(b.f[OTBL_W]), fld<float>(b.f[OTBL_P]), F, b.dcf, b.ds, varstep,
-         use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1, (const double *)b.P0, b.p0, b.dc.idyy, b.um64, pm);
+  const dim3 pg(nblk(b.pu / 2), (unsigned)((nx + pm - 1) / pm)); // one thread per mode (Re, Im)
+  LAUNCH(b, "project", pg, TPB, 0, project_pair_kernel, fld<float>(b.f[OTBL_U]), fld<float>(b.f[OTBL_V]),
+         fld<float>(b.f[OTBL_W]), fld<float>(b.f[OTBL_P]), F, b.dcf, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx,
+         ny, nz1, (const double *)b.P0, b.p0, b.dc.idyy, b.um64, pm);
 }
 
 void k_pois(Box &b, int m, double varstep, bool use_m) {
