@@ -958,13 +958,13 @@ __global__ void rhsu_loop1_kernel(const double *__restrict__ uv, const double *_
 // [ny+1][nx] (mode-0 real part), updated in FP64 and saved to rm (else
 // all null).
 template <typename T>
-__global__ void save_predict_kernel(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
-                                    const T *__restrict__ p, T *__restrict__ un, T *__restrict__ vn,
-                                    T *__restrict__ wn, const T *idyy, const T *kaz, const DevScalars *ds,
-                                    double idxx_d, int nx, int ny, int nz1, const double *__restrict__ p0,
-                                    const double *idyy_d, double *um, double *rm, int pm) {
-  int kr, j;
-  if (!pcol(nz1, ny + 1, kr, j)) return;
+__device__ __forceinline__ void save_predict_col(const T *__restrict__ u, const T *__restrict__ v,
+                                                 const T *__restrict__ w, const T *__restrict__ p,
+                                                 T *__restrict__ un, T *__restrict__ vn, T *__restrict__ wn,
+                                                 const T *idyy, const T *kaz, const DevScalars *ds, double idxx_d,
+                                                 int nx, int ny, int nz1, const double *__restrict__ p0,
+                                                 const double *idyy_d, double *um, double *rm, int pm, int kr,
+                                                 int j) {
   const int i0 = (int)blockIdx.y * pm + 1, i1 = min(nx, i0 + pm - 1);
   const size_t pu = (size_t)nz1 * (ny + 1), pv = (size_t)nz1 * ny;
   const T var1 = (T)ds->var1, idxx = (T)idxx_d;
@@ -1013,6 +1013,76 @@ __global__ void save_predict_kernel(const T *__restrict__ u, const T *__restrict
           vo = vv - var1 * idyy[j] * (p[ev + nz1] - pc);
       }
       vn[ev] = vo;
+    }
+    pc = pn;
+  }
+}
+
+template <typename T>
+__global__ void save_predict_kernel(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+                                    const T *__restrict__ p, T *__restrict__ un, T *__restrict__ vn,
+                                    T *__restrict__ wn, const T *idyy, const T *kaz, const DevScalars *ds,
+                                    double idxx_d, int nx, int ny, int nz1, const double *__restrict__ p0,
+                                    const double *idyy_d, double *um, double *rm, int pm) {
+  int kr, j;
+  if (!pcol(nz1, ny + 1, kr, j)) return;
+  save_predict_col(u, v, w, p, un, vn, wn, idyy, kaz, ds, idxx_d, nx, ny, nz1, p0, idyy_d, um, rm, pm, kr, j);
+}
+
+// The FP32 save and predictor with one thread per mode (Re, Im as float2,
+// as project_pair_kernel); mode 0 (its Re part carries the FP64 mean flow)
+// runs save_predict_col per part.
+__global__ void save_predict_pair_kernel(const float *__restrict__ u, const float *__restrict__ v,
+                                         const float *__restrict__ w, const float *__restrict__ p,
+                                         float *__restrict__ un, float *__restrict__ vn, float *__restrict__ wn,
+                                         const float *idyy, const float *kaz, const DevScalars *ds, double idxx_d,
+                                         int nx, int ny, int nz1, const double *__restrict__ p0,
+                                         const double *idyy_d, double *um, double *rm, int pm) {
+  int kp, j;
+  if (!pcol(nz1 / 2, ny + 1, kp, j)) return;
+  if (kp == 0) {
+    for (int kr = 0; kr < 2; ++kr)
+      save_predict_col(u, v, w, p, un, vn, wn, idyy, kaz, ds, idxx_d, nx, ny, nz1, p0, idyy_d, um, rm, pm, kr, j);
+    return;
+  }
+  const int i0 = (int)blockIdx.y * pm + 1, i1 = min(nx, i0 + pm - 1);
+  const size_t pu2 = (size_t)nz1 * (ny + 1) / 2, pv2 = (size_t)nz1 * ny / 2, s2 = (size_t)nz1 / 2;
+  const float var1 = (float)ds->var1, idxx = (float)idxx_d;
+  const bool vrow = j <= ny;
+  const float kz = kaz[kp];
+  const float2 *u2 = reinterpret_cast<const float2 *>(u), *v2 = reinterpret_cast<const float2 *>(v);
+  const float2 *w2 = reinterpret_cast<const float2 *>(w), *p2 = reinterpret_cast<const float2 *>(p);
+  float2 *un2 = reinterpret_cast<float2 *>(un), *vn2 = reinterpret_cast<float2 *>(vn);
+  float2 *wn2 = reinterpret_cast<float2 *>(wn);
+  size_t e = (size_t)(i0 - 1) * pu2 + (size_t)(j - 1) * s2 + kp;
+  size_t ev = (size_t)(i0 - 1) * pv2 + (size_t)(j - 1) * s2 + kp;
+  float2 pc = vrow ? p2[ev] : make_float2(0.0f, 0.0f); // p(i, j), carried from plane to plane
+  for (int i = i0; i <= i1; ++i, e += pu2, ev += pv2) {
+    const float2 pn = (vrow && i < nx) ? p2[ev + pv2] : make_float2(0.0f, 0.0f); // p(i+1, j)
+    const float2 uu = u2[e], ww = w2[e];
+    float2 uo = uu;
+    if (i >= 2 && i <= nx - 1 && j >= 2 && j <= ny) {
+      uo.x = uu.x - var1 * idxx * (pn.x - pc.x);
+      uo.y = uu.y - var1 * idxx * (pn.y - pc.y);
+    }
+    un2[e] = uo;
+    float2 wo = ww;
+    if (i >= 2 && j >= 2 && j <= ny) {
+      // w -= var1 * (i kz p): Re gets -(-kz Im p), Im gets -(kz Re p)
+      const float tx = -kz * pc.y, ty = kz * pc.x;
+      wo.x = ww.x - var1 * tx;
+      wo.y = ww.y - var1 * ty;
+    }
+    wn2[e] = wo;
+    if (vrow) {
+      const float2 vv = v2[ev];
+      float2 vo = vv;
+      if (i >= 2 && j >= 2 && j <= ny - 1) {
+        const float2 pj = p2[ev + s2]; // p(i, j+1)
+        vo.x = vv.x - var1 * idyy[j] * (pj.x - pc.x);
+        vo.y = vv.y - var1 * idyy[j] * (pj.y - pc.y);
+      }
+      vn2[ev] = vo;
     }
     pc = pn;
   }
@@ -1985,8 +2055,14 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   // save velocities, pressure-gradient predictor
   double *um = F64 ? nullptr : b.um64; // the FP64 mean-flow u of the mixed substep
   const int pm = prun<T>();
-  LAUNCH(b, "save_predict", pgrid(b, pm), TPB, 0, save_predict_kernel<T>, u, v, w, p, resu, resv, resw, c.idyy,
-         c.kaz, b.ds, b.hc.idxx, nx, ny, nz1, F64 ? nullptr : (const double *)b.p0, b.dc.idyy, um, b.rm64, pm);
+  if constexpr (sizeof(T) == 4) {
+    const dim3 pg(nblk(b.pu / 2), (unsigned)((nx + pm - 1) / pm)); // one thread per mode (Re, Im)
+    LAUNCH(b, "save_predict", pg, TPB, 0, save_predict_pair_kernel, u, v, w, p, resu, resv, resw, c.idyy, c.kaz,
+           b.ds, b.hc.idxx, nx, ny, nz1, (const double *)b.p0, b.dc.idyy, um, b.rm64, pm);
+  } else {
+    LAUNCH(b, "save_predict", pgrid(b, pm), TPB, 0, save_predict_kernel<T>, u, v, w, p, resu, resv, resw, c.idyy,
+           c.kaz, b.ds, b.hc.idxx, nx, ny, nz1, F64 ? nullptr : (const double *)b.p0, b.dc.idyy, um, b.rm64, pm);
+  }
   // the predicted velocities now live in the former save buffers, and this
   // substep's rhs buffers become the rhs_prev fields
   k_swap_saved(b);
