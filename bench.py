@@ -79,7 +79,9 @@ TRACE_NAMES = {"dct2_rows": "dct_rows_z", "dct3_rows": "dct_rows_z", "div": "div
                "poisson_y": "poisson_y_chunk", "to_planes": "pencils_to_planes_kernel",
                "products1": "products1_z", "products2": "products2_z", "interpxx": "interpxx_w",
                "difvisxx": "difvisxx_w", "interpyy": "interpyy_y", "rhsu_loop1": "rhsu_loop1_y",
-               "rhs_loop2": "rhs_loop2_y", "rk_final": ("rk_final_y", "rk_final_yh")}
+               "rhs_loop2": "rhs_loop2_y", "rk_final": ("rk_final_y", "rk_final_yh"),
+               "project": ("project_kernel", "project_pair_kernel"),
+               "save_predict": ("save_predict_kernel", "save_predict_pair_kernel")}
 
 
 def profiled_traffic(workload, precision, kname):
