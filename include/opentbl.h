@@ -95,6 +95,12 @@ int otbl_diag_enable(otbl_ctx *ctx, int capacity);
  * copied; *dropped the number lost to the ring's wrap-around since the last
  * read (records beyond max_records stay for the next read). */
 int otbl_diag_read(otbl_ctx *ctx, double *out, int max_records, int *n_records, long long *dropped);
+/* Re_theta(x) and Cf(x) at every plane x = 1..nx from the kz = 0 mode of u
+ * (the FP64 mean-flow u in the mixed substep): Re_theta by the sums of
+ * genflu.F90:77-87, which compute-Re_theta.f90:34-49 repeats as a
+ * post-processing tool, and Cf = 2 u_tau^2 with u_tau^2 from genflu.F90:76.
+ * rth and cf receive nx doubles each (host memory). */
+int otbl_retheta_cf(otbl_ctx *ctx, double *rth, double *cf);
 int otbl_set_time(otbl_ctx *ctx, double tiempo, double timeinit, double dt);
 int otbl_get_time(otbl_ctx *ctx, double *tiempo, double *dt);
 /* Coefficient arrays after setup, by their reference names (inxu, dcbx,

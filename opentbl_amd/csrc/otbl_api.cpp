@@ -70,6 +70,10 @@ template <class T> static void dalloc(T *&p, size_t n) {
   OTBL_HIP(hipDeviceSynchronize());
 }
 
+// A field or work field of n elements of the box's element size (8 bytes
+// in FP64, 4 in the mixed substep), held behind a double pointer.
+static void falloc(const Box &b, double *&p, size_t n) { dalloc(p, (n * b.esz + 7) / 8); }
+
 // Host <-> device copies ordered on the box's stream, complete on return.
 // (The stream is non-blocking: a copy on the null stream is not ordered
 // before kernels enqueued on it, and a pageable host-to-device hipMemcpy may
@@ -312,7 +316,7 @@ static void free_box(Box *b) {
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
                     b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
                     b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax, b->P0, b->S0, b->p0, b->um64, b->rm64,
-                    b->diag};
+                    b->diag, b->op64, b->rthcf};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
   if (b->diag_count) (void)hipFree(b->diag_count);
@@ -352,8 +356,8 @@ static void run_substep(Box &b, int m) {
     try {
       enqueue_substep(b, m);
     } catch (...) {
-      hipGraph_t dummy;
-      (void)hipStreamEndCapture(b.s, &dummy);
+      hipGraph_t dummy = nullptr;
+      if (hipStreamEndCapture(b.s, &dummy) == hipSuccess && dummy) (void)hipGraphDestroy(dummy);
       if (b.flip != fl) k_swap_saved(b);
       throw;
     }
@@ -427,8 +431,10 @@ static void setup_mixed(Box &b) {
   for (size_t q = 0; q < sizeof(DevCoef) / sizeof(void *); ++q) dst[q] = b.coefbuf_f + (src[q] - b.coefbuf);
   dalloc(b.ytab_f, YT_COUNT * ytab_size(b.yL));
   k_d2f(b, b.ytab, b.ytab_f, YT_COUNT * ytab_size(b.yL));
+  // the half-wave rk_final addresses a field by 32-bit byte offsets: fields
+  // of 2^30 FP32 values or more take the full-wave kernel
   const char *yh = std::getenv("OTBL_Y_HALF");
-  if (b.yL % 2 == 0 && !(yh && yh[0] == '0')) {
+  if (b.yL % 2 == 0 && !(yh && yh[0] == '0') && b.nu < (size_t(1) << 30)) {
     dalloc(b.ytabh_f, YT_COUNT * ytabh_size(b.yL / 2));
     k_ytab_halve_fixed(b);
   }
@@ -491,12 +497,12 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   build_coef(b->hc, b->nx, b->ny, b->nz, b->re, b->ax, b->az, b->cfl, y);
   upload_coef(*b);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
-    dalloc(b->f[f], (f == OTBL_V || f == OTBL_P || f == OTBL_RHSVPA) ? b->nv : b->nu);
-  dalloc(b->resu, b->nu); dalloc(b->resv, b->nv); dalloc(b->resw, b->nu);
-  dalloc(b->rhsu, b->nu); dalloc(b->rhsv, b->nv); dalloc(b->rhsw, b->nu);
-  dalloc(b->wki1, b->nu); dalloc(b->wki2, b->nv); dalloc(b->wki3, b->nu);
-  dalloc(b->ya, b->nu); dalloc(b->yb, b->nu); dalloc(b->yc, b->nu);
-  dalloc(b->rest, b->nv); dalloc(b->rest2, b->nv);
+    falloc(*b, b->f[f], (f == OTBL_V || f == OTBL_P || f == OTBL_RHSVPA) ? b->nv : b->nu);
+  falloc(*b, b->resu, b->nu); falloc(*b, b->resv, b->nv); falloc(*b, b->resw, b->nu);
+  falloc(*b, b->rhsu, b->nu); falloc(*b, b->rhsv, b->nv); falloc(*b, b->rhsw, b->nu);
+  falloc(*b, b->wki1, b->nu); falloc(*b, b->wki2, b->nv); falloc(*b, b->wki3, b->nu);
+  falloc(*b, b->ya, b->nu); falloc(*b, b->yb, b->nu); falloc(*b, b->yc, b->nu);
+  falloc(*b, b->rest, b->nv); falloc(*b, b->rest2, b->nv);
   dalloc(b->implu, 3 * (size_t)(b->ny + 2) + 8);
   dalloc(b->implv, 3 * (size_t)(b->ny + 2) + 8);
   dalloc(b->um, b->ny + 2); dalloc(b->u0, b->ny + 2); dalloc(b->v0, b->ny + 2);
@@ -568,6 +574,11 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     dalloc(b->ytab, YT_COUNT * ytab_size(b->yL));
     k_ytab_fixed(*b);
   }
+  // launch limits of the partitioned Poisson solve, checked here so that a
+  // step fails only before any state changes
+  if (b->pchunk)
+    need((size_t)(2 * b->pchunk + 1) * b->nz1 * (b->nx - 1) * b->esz < (size_t(1) << 32),
+         "Poisson pencil rows too long for the partitioned y solve's 32-bit offsets");
   if (b->precision == OTBL_MIXED) setup_mixed(*b);
   OTBL_HIP(hipStreamSynchronize(b->s));
   *out = new otbl_ctx{b};
@@ -611,10 +622,15 @@ int otbl_set_field(otbl_ctx *ctx, int id, const double *host) {
   if (b.esz == 8) {
     h2d(b, b.f[id], host, n * sizeof(double));
   } else {
-    // staged through a work field (sized for FP64 elements), rounded to FP32
-    h2d(b, b.ya, host, n * sizeof(double));
-    k_d2f(b, b.ya, fld<float>(b.f[id]), n);
-    OTBL_HIP(hipStreamSynchronize(b.s));
+    // staged through a work field (nu FP32 elements: room for nu / 2
+    // doubles at a time), rounded to FP32
+    const size_t chunk = b.nu / 2;
+    for (size_t o = 0; o < n; o += chunk) {
+      const size_t m = std::min(chunk, n - o);
+      h2d(b, b.ya, host + o, m * sizeof(double));
+      k_d2f(b, b.ya, fld<float>(b.f[id]) + o, m);
+      OTBL_HIP(hipStreamSynchronize(b.s));
+    }
     if (id == OTBL_P || id == OTBL_U) {
       // the mean-flow pressure ([nx][ny]) and u ([ny+1][nx]) (real parts of
       // mode 0) are kept in FP64
@@ -635,9 +651,13 @@ int otbl_get_field(otbl_ctx *ctx, int id, double *host) {
   if (b.esz == 8) {
     d2h(b, host, b.f[id], n * sizeof(double));
   } else {
-    k_f2d(b, fld<float>(b.f[id]), b.ya, n);
-    OTBL_HIP(hipStreamSynchronize(b.s));
-    d2h(b, host, b.ya, n * sizeof(double));
+    const size_t chunk = b.nu / 2;
+    for (size_t o = 0; o < n; o += chunk) {
+      const size_t m = std::min(chunk, n - o);
+      k_f2d(b, fld<float>(b.f[id]) + o, b.ya, m);
+      OTBL_HIP(hipStreamSynchronize(b.s));
+      d2h(b, host + o, b.ya, m * sizeof(double));
+    }
     if (id == OTBL_P || id == OTBL_U) {
       std::vector<double> m(n / b.nz1);
       d2h(b, m.data(), id == OTBL_P ? b.p0 : b.um64, m.size() * sizeof(double));
@@ -748,6 +768,22 @@ int otbl_diag_read(otbl_ctx *ctx, double *out, int max_records, int *n_records, 
   b.diag_read += take;
   if (n_records) *n_records = (int)take;
   if (dropped) *dropped = lost;
+  API_END
+}
+
+int otbl_retheta_cf(otbl_ctx *ctx, double *rth, double *cf) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(rth && cf, "otbl_retheta_cf: null pointer");
+  if (!b.rthcf) {
+    OTBL_HIP(hipStreamSynchronize(b.s));
+    dalloc(b.rthcf, 2 * (size_t)b.nx);
+  }
+  k_retheta_cf(b, b.rthcf);
+  std::vector<double> h(2 * (size_t)b.nx);
+  d2h(b, h.data(), b.rthcf, h.size() * sizeof(double));
+  std::copy(h.begin(), h.begin() + b.nx, rth);
+  std::copy(h.begin() + b.nx, h.end(), cf);
   API_END
 }
 

@@ -65,12 +65,14 @@ struct DevScalars {
 struct Box {
   int nx, ny, nz, nz1, nz2, mz, xout, precision, dt_mode;
   // bytes per field element: 8 (OTBL_FP64) or 4 (OTBL_MIXED, FP32 fields);
-  // buffers are sized for 8-byte elements either way, so the FP64 operator
-  // entry points stay valid on a mixed box
+  // fields and work fields are allocated at that size
   int esz = 8;
+  // FP64 pencils of the cosftx operator entry point on a mixed box (whose
+  // own pencil buffers hold FP32), allocated on first use
+  double *op64 = nullptr;
   double re, ax, az, cfl;
-  size_t nu, nv;     // doubles per u-type / v-type field
-  size_t pu, pv;     // doubles per u-type / v-type plane
+  size_t nu, nv;     // elements per u-type / v-type field
+  size_t pu, pv;     // elements per u-type / v-type plane
   hipStream_t s = nullptr;
   HostCoef hc;
   DevCoef dc{};
@@ -146,6 +148,8 @@ struct Box {
   // host's read cursor
   double *diag = nullptr;
   unsigned long long *diag_count = nullptr;
+  // Re_theta(x) and Cf(x) of the last otbl_retheta_cf call (2 nx)
+  double *rthcf = nullptr;
   int diag_cap = 0;
   unsigned long long diag_read = 0;
   // graphs
@@ -184,6 +188,8 @@ void k_substep_rhsp(Box &b, int m);
 void k_outflow(Box &b);
 void k_pois(Box &b, int m, double varstep, bool use_m);
 void k_end_step(Box &b);
+// Re_theta(x), Cf(x) of the kz = 0 mode of u into out[0..nx), out[nx..2nx)
+void k_retheta_cf(Box &b, double *out);
 void k_swap_saved(Box &b);
 // partitioned Poisson y solves of the modes kz >= 1 on pencils
 // (otbl_poisson.hip); REF: one FP64-residual refinement step

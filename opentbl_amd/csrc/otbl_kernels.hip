@@ -849,9 +849,19 @@ static dim3 tgrid(const Box &b) {
 void k_cosftx(Box &b, const double *in, double *out, int iopt) {
   const int Lp = (b.ny - 1) * b.nz1, N = b.nx - 1;
   if (b.dct_fft) {
-    LAUNCH(b, "to_pencils", tgrid(b), TPB, 0, planes_to_pencils_kernel, in, b.rest2, Lp, N, b.pv);
-    dct_pencils(b, b.rest2, iopt);
-    LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, b.rest2, out, Lp, N, b.pv,
+    // FP64 pencils: the box's own on an FP64 box, else a buffer of the
+    // operator's own (a mixed box's pencil buffers hold FP32)
+    double *P = b.rest2;
+    if (b.esz != 8) {
+      if (!b.op64) {
+        OTBL_HIP(hipStreamSynchronize(b.s));
+        OTBL_HIP(hipMalloc((void **)&b.op64, b.nv * sizeof(double) + 64));
+      }
+      P = b.op64;
+    }
+    LAUNCH(b, "to_pencils", tgrid(b), TPB, 0, planes_to_pencils_kernel, in, P, Lp, N, b.pv);
+    dct_pencils(b, P, iopt);
+    LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, P, out, Lp, N, b.pv,
            (const double *)nullptr, b.nz1);
   } else {
     dim3 grid(nblk(Lp, 64), nblk(N, 64));
@@ -2168,6 +2178,45 @@ void k_f2d(Box &b, const float *a, double *o, size_t n) {
 
 __global__ void end_step_kernel(DevScalars *ds) {
   if (threadIdx.x == 0) ds->tiempo += ds->dt;
+}
+
+// Re_theta(x) and Cf(x) from the kz = 0 mode of u at every plane: the
+// momentum-thickness Reynolds number of genflu.F90:77-87 (the same sums as
+// compute-Re_theta.f90:34-49: U_inf the mean of rows ny-19..ny-4, jtop the
+// first row past which U >= 0.99 U_inf) and Cf = 2 u_tau^2 with
+// u_tau^2 = |U(2) - U(1)| / ((y(2) - y(1)) Re) (genflu.F90:76).  One thread
+// per plane, sums in the reference's row order.  rm: the FP64 mean-flow u
+// ([ny+1][nx]) of the mixed substep, else mode 0 of the field u.
+template <typename T>
+__global__ void retheta_cf_kernel(const T *u, const double *rm, DevCoef c, double *out, int nx, int ny, int nz1,
+                                  double re) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x; // 0-based plane
+  if (i >= nx) return;
+  auto U = [&](int j) -> double { // 1-based row
+    return rm ? rm[(size_t)(j - 1) * nx + i] : (double)u[((size_t)i * (ny + 1) + (j - 1)) * nz1];
+  };
+  double uinf = 0.0;
+  for (int j = ny + 1 - 20; j <= ny + 1 - 5; ++j) uinf += U(j);
+  uinf /= 16.0;
+  const double u99 = (double)0.99f * uinf;
+  int jtop = 0;
+  for (int j = 1; j <= ny - 2; ++j) {
+    if (U(j) >= u99) break;
+    jtop = j + 1;
+  }
+  double sum = 0.0;
+  for (int j = 2; j <= jtop + 3; ++j) sum += (uinf - U(j)) * U(j) * c.dy[j - 1];
+  out[i] = sum * re / uinf;
+  out[nx + i] = 2.0 * fabs((U(2) - U(1)) / (c.y[2] - c.y[1]) / re);
+}
+
+void k_retheta_cf(Box &b, double *out) {
+  if (b.precision == OTBL_MIXED)
+    LAUNCH(b, "retheta_cf", nblk((size_t)b.nx), TPB, 0, retheta_cf_kernel<float>, fld<float>(b.f[OTBL_U]),
+           (const double *)b.um64, b.dc, out, b.nx, b.ny, b.nz1, b.re);
+  else
+    LAUNCH(b, "retheta_cf", nblk((size_t)b.nx), TPB, 0, retheta_cf_kernel<double>, b.f[OTBL_U],
+           (const double *)nullptr, b.dc, out, b.nx, b.ny, b.nz1, b.re);
 }
 
 void k_end_step(Box &b) { LAUNCH(b, "end_step", 1, 64, 0, end_step_kernel, b.ds); }

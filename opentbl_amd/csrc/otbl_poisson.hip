@@ -79,17 +79,19 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
   const size_t t = (size_t)N + (size_t)xcd_block() * (PK / 2) + (col >> 1); // kz >= 1 only
   const bool live = t < (size_t)nzh * N;
   const int kz = live ? (int)(t / N) : 1, x = live ? (int)(t % N) : 0;
-  // 32-bit byte offsets from the (uniform) pencil base: the pencils are
-  // smaller than 4 GiB (checked by the launcher)
+  // 32-bit byte offsets from a wave-uniform base (the wave's first row, a
+  // scalar 64-bit address): the two chunks of a wave span 2 L rows, which
+  // stay below 4 GiB for any pencil set the launcher accepts
   const unsigned rs = (unsigned)(nz1 * N * sizeof(T));
   const unsigned base = (unsigned)(((2 * kz + (col & 1)) * N + x) * sizeof(T));
   const int r0 = k * L + 1;
   // rows q < nq of the chunk are operator rows (r0 + q <= ny - 1), the rest
   // identity rows: one count instead of L row indices kept live
   const int nq = ny1 - r0 + 1;
-  char *Pc = reinterpret_cast<char *>(P);
-  // the chunk's first row; row q adds the uniform q rs
-  const unsigned b0 = base + (unsigned)(r0 - 1) * rs;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  char *Pc = reinterpret_cast<char *>(P) + (size_t)(2 * L * wave_u) * rs;
+  // the chunk's first row relative to the wave's; row q adds the uniform q rs
+  const unsigned b0 = base + (unsigned)((lane >> 5) * L) * rs;
   auto AT = [&](int q) -> T & { return *reinterpret_cast<T *>(Pc + (b0 + (unsigned)q * rs)); };
   const T sh = c.kaz2[kz] + c.kmod[x + 1];
   auto A1 = [&](int r) -> T { return ca1[r]; };
@@ -314,8 +316,9 @@ template <typename T, bool REF> void kp_poisson_y(Box &b, T *P) {
   const int N = b.nx - 1, nzh = b.nz1 / 2;
   const size_t cols = (size_t)(nzh - 1) * N;
   if (cols == 0) return;
-  if ((size_t)(b.ny - 1) * b.nz1 * N * sizeof(T) >= (size_t(1) << 32))
-    throw std::invalid_argument("Poisson pencils of 4 GiB or more");
+  // a wave addresses 2 L rows plus one row of offsets in 32 bits
+  if ((size_t)(2 * kp_chunk_rows(b.ny) + 1) * b.nz1 * N * sizeof(T) >= (size_t(1) << 32))
+    throw std::invalid_argument("Poisson pencil rows too long for 32-bit offsets");
   const unsigned grid = (unsigned)((cols + PK / 2 - 1) / (PK / 2)); // 16 modes (Re, Im) per workgroup
   switch (kp_chunk_rows(b.ny)) {
 #define PCASE(LL)                                                                                          \

@@ -45,6 +45,7 @@ SIGNATURES = {
     "otbl_get_ener": (I, [VP, PD]),
     "otbl_diag_enable": (I, [VP, I]),
     "otbl_diag_read": (I, [VP, PD, I, C.POINTER(I), C.POINTER(C.c_longlong)]),
+    "otbl_retheta_cf": (I, [VP, PD, PD]),
     "otbl_set_time": (I, [VP, D, D, D]),
     "otbl_get_time": (I, [VP, PD, PD]),
     "otbl_get_coef": (I, [VP, C.c_char_p, PD, C.POINTER(I), C.POINTER(I)]),
@@ -225,6 +226,12 @@ class GpuBox:
         o = np.zeros(6)
         check(lib().otbl_get_genflu(self.h, _pd(o)), "get_genflu")
         return dict(zip(("rthin", "din", "rthout", "dout", "gamma1", "gamma2"), o))
+
+    def retheta_cf(self):
+        """Re_theta(x) and Cf(x) at every plane (two arrays of nx)."""
+        rth, cf = np.zeros(self.cfg.nx), np.zeros(self.cfg.nx)
+        check(lib().otbl_retheta_cf(self.h, _pd(rth), _pd(cf)), "retheta_cf")
+        return rth, cf
 
     def ener(self):
         o = np.zeros(15)

@@ -21,6 +21,7 @@
  */
 #include "otbl_oracle.h"
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
@@ -64,11 +65,19 @@ struct oc_box {
   double *pen_u, *pen_v, *pen_w, *pen_a; /* pencils */
   double *resu, *resv, *resw, *rhsu, *rhsv, *rhsw, *wki1, *wki2, *wki3;
   double *rest, *tbuf;
-  double *wkp, *wkpo, *wkf, *wkfo, *wk2r;
   double *vm; /* 1..ny */
   /* FFT */
-  double *fft_tw, *fft_a, *fft_b, *fft_t;
+  double *fft_tw;
+  /* per-thread scratch of the plane loops (OpenMP over planes; every
+   * parallel loop writes disjoint data and the only reductions are maxima,
+   * so results do not depend on the thread count) */
+  int nth;
+  struct oc_tls *tls;
 };
+struct oc_tls {
+  double *wkp, *wkpo, *wkf, *wkfo, *wk2r, *fft_a, *fft_b, *fft_t, *vm;
+};
+#define TLS(b) (&(b)->tls[omp_get_thread_num()])
 
 static double *zalloc(size_t n) {
   double *p = (double *)calloc(n ? n : 1, sizeof(double));
@@ -590,6 +599,19 @@ static void fft_rec(int n, int N, const double *tw, const double *x, int xs,
   }
   int p = small_factor(n), m = n / p, step = N / n;
   for (int r = 0; r < p; r++) fft_rec(m, N, tw, x + 2 * (size_t)r * xs, xs * p, y + 2 * r * m, sgn, tmp);
+  if (p == 2) { /* radix 2: y0[k] +- w^k y1[k] */
+    for (int k = 0; k < m; k++) {
+      double wr = tw[2 * k * step], wi = sgn * tw[2 * k * step + 1];
+      double ar = y[2 * k], ai = y[2 * k + 1], br = y[2 * (m + k)], bi = y[2 * (m + k) + 1];
+      double tr = wr * br - wi * bi, ti = wr * bi + wi * br;
+      tmp[2 * k] = ar + tr;
+      tmp[2 * k + 1] = ai + ti;
+      tmp[2 * (k + m)] = ar - tr;
+      tmp[2 * (k + m) + 1] = ai - ti;
+    }
+    memcpy(y, tmp, 2 * sizeof(double) * n);
+    return;
+  }
   for (int q = 0; q < p; q++)
     for (int k = 0; k < m; k++) {
       int kk = k + m * q;
@@ -607,9 +629,40 @@ static void fft_rec(int n, int N, const double *tw, const double *x, int xs,
   memcpy(y, tmp, 2 * sizeof(double) * n);
 }
 
+/* The same transform (x contiguous, N == n) by the iterative radix-2
+ * decimation in time for powers of two, else by fft_rec. */
+static void fft_any(int n, const double *tw, const double *x, double *y, int sgn, double *tmp) {
+  if (n & (n - 1)) {
+    fft_rec(n, n, tw, x, 1, y, sgn, tmp);
+    return;
+  }
+  int lg = 0;
+  while ((1 << lg) < n) lg++;
+  for (int i = 0; i < n; i++) {
+    int r = 0;
+    for (int q = 0; q < lg; q++) r |= ((i >> q) & 1) << (lg - 1 - q);
+    y[2 * r] = x[2 * i];
+    y[2 * r + 1] = x[2 * i + 1];
+  }
+  for (int len = 2; len <= n; len *= 2) {
+    const int half = len / 2, step = n / len;
+    for (int i0 = 0; i0 < n; i0 += len)
+      for (int k = 0; k < half; k++) {
+        double wr = tw[2 * k * step], wi = sgn * tw[2 * k * step + 1];
+        double *a = y + 2 * (i0 + k), *c = y + 2 * (i0 + k + half);
+        double tr = wr * c[0] - wi * c[1], ti = wr * c[1] + wi * c[0];
+        c[0] = a[0] - tr;
+        c[1] = a[1] - ti;
+        a[0] += tr;
+        a[1] += ti;
+      }
+  }
+}
+
 void oc_c2r(const oc_box *b, const double *fou, double *phys) {
   int nz = b->nz;
-  double *a = b->fft_a, *o = b->fft_b;
+  struct oc_tls *t = TLS(b);
+  double *a = t->fft_a, *o = t->fft_b;
   memset(a, 0, 2 * sizeof(double) * nz);
   a[0] = fou[0]; /* Im of mode 0 does not enter a real inverse transform */
   for (int k = 1; k <= b->nz2; k++) {
@@ -618,18 +671,19 @@ void oc_c2r(const oc_box *b, const double *fou, double *phys) {
     a[2 * (nz - k)] = fou[2 * k];
     a[2 * (nz - k) + 1] = -fou[2 * k + 1];
   }
-  fft_rec(nz, nz, b->fft_tw, a, 1, o, +1, b->fft_t);
+  fft_any(nz, b->fft_tw, a, o, +1, t->fft_t);
   for (int n = 0; n < nz; n++) phys[n] = o[2 * n];
 }
 
 void oc_r2c(const oc_box *b, const double *phys, double *fou) {
   int nz = b->nz;
-  double *a = b->fft_a, *o = b->fft_b;
+  struct oc_tls *t = TLS(b);
+  double *a = t->fft_a, *o = t->fft_b;
   for (int n = 0; n < nz; n++) {
     a[2 * n] = phys[n];
     a[2 * n + 1] = 0.0;
   }
-  fft_rec(nz, nz, b->fft_tw, a, 1, o, -1, b->fft_t);
+  fft_any(nz, b->fft_tw, a, o, -1, t->fft_t);
   double dnf = 1.0 / nz;
   for (int k = 0; k <= b->nz2; k++) {
     fou[2 * k] = dnf * o[2 * k];
@@ -638,31 +692,165 @@ void oc_r2c(const oc_box *b, const double *phys, double *fou) {
   fou[1] = 0.0;
 }
 
-/* cosftx (ffts.f90:147-181): FFTW REDFT10 x 1/n (iopt<0), REDFT01 x 0.5
- * (iopt>0), evaluated from their definitions.  Line l starts at
+/* Complex DFT of any length n, y[k] = sum_j x[j] exp(sgn 2 pi i j k / n)
+ * (interleaved complex): the mixed-radix fft_rec when every prime factor of
+ * n is at most 64, else Bluestein's chirp-z identity
+ *   y[k] = conj(c_k) sum_j (x_j conj(c_j)) c_{k-j},  c_t = exp(-sgn i pi t^2 / n)
+ * (2jk = j^2 + k^2 - (k-j)^2)
+ * evaluated as a cyclic convolution of power-of-two length M >= 2n - 1 with
+ * fft_rec (own code; the reference calls FFTW). */
+typedef struct {
+  int n, M, blue;
+  double *tw, *twM, *chirp, *cf, *a, *o, *t;
+} dft_plan;
+static int max_prime(int n) {
+  int m = 1;
+  for (int p = 2; p * p <= n; p++)
+    while (n % p == 0) {
+      m = p;
+      n /= p;
+    }
+  return n > 1 ? (n > m ? n : m) : m;
+}
+static void dft_init(dft_plan *q, int n, int sgn) {
+  memset(q, 0, sizeof *q);
+  q->n = n;
+  q->blue = max_prime(n) > 64;
+  int L = n;
+  if (q->blue) {
+    q->M = 1;
+    while (q->M < 2 * n - 1) q->M *= 2;
+    L = q->M;
+  }
+  q->tw = zalloc(2 * (size_t)n);
+  for (int t = 0; t < n; t++) {
+    q->tw[2 * t] = cos(2.0 * PI_D * t / n);
+    q->tw[2 * t + 1] = sin(2.0 * PI_D * t / n);
+  }
+  q->a = zalloc(2 * (size_t)L);
+  q->o = zalloc(2 * (size_t)L);
+  q->t = zalloc(2 * (size_t)L);
+  if (!q->blue) return;
+  const int M = q->M;
+  q->twM = zalloc(2 * (size_t)M);
+  for (int t = 0; t < M; t++) {
+    q->twM[2 * t] = cos(2.0 * PI_D * t / M);
+    q->twM[2 * t + 1] = sin(2.0 * PI_D * t / M);
+  }
+  /* c_t = exp(-sgn i pi t^2 / n), t^2 reduced mod 2n */
+  q->chirp = zalloc(2 * (size_t)n);
+  for (int t = 0; t < n; t++) {
+    long r = ((long)t * t) % (2L * n);
+    q->chirp[2 * t] = cos(PI_D * r / n);
+    q->chirp[2 * t + 1] = -sgn * sin(PI_D * r / n);
+  }
+  /* forward transform of the wrapped chirp, divided by M */
+  double *h = zalloc(2 * (size_t)M);
+  for (int t = 0; t < n; t++) {
+    h[2 * t] = q->chirp[2 * t];
+    h[2 * t + 1] = q->chirp[2 * t + 1];
+    if (t) {
+      h[2 * (M - t)] = q->chirp[2 * t];
+      h[2 * (M - t) + 1] = q->chirp[2 * t + 1];
+    }
+  }
+  q->cf = zalloc(2 * (size_t)M);
+  fft_any(M, q->twM, h, q->cf, -1, q->t);
+  for (int t = 0; t < 2 * M; t++) q->cf[t] /= M;
+  free(h);
+}
+static void dft_free(dft_plan *q) {
+  free(q->tw);
+  free(q->twM);
+  free(q->chirp);
+  free(q->cf);
+  free(q->a);
+  free(q->o);
+  free(q->t);
+}
+/* x -> y (both 2n doubles; may alias) */
+static void dft_run(dft_plan *q, const double *x, double *y, int sgn) {
+  const int n = q->n;
+  if (!q->blue) {
+    fft_any(n, q->tw, x, q->o, sgn, q->t);
+    memcpy(y, q->o, 2 * sizeof(double) * n);
+    return;
+  }
+  const int M = q->M;
+  double *a = q->a, *o = q->o;
+  memset(a, 0, 2 * sizeof(double) * M);
+  for (int j = 0; j < n; j++) { /* x_j conj(c_j) */
+    double cr = q->chirp[2 * j], ci = q->chirp[2 * j + 1];
+    a[2 * j] = x[2 * j] * cr + x[2 * j + 1] * ci;
+    a[2 * j + 1] = x[2 * j + 1] * cr - x[2 * j] * ci;
+  }
+  fft_any(M, q->twM, a, o, -1, q->t);
+  for (int m = 0; m < M; m++) {
+    double ar = o[2 * m], ai = o[2 * m + 1], br = q->cf[2 * m], bi = q->cf[2 * m + 1];
+    a[2 * m] = ar * br - ai * bi;
+    a[2 * m + 1] = ar * bi + ai * br;
+  }
+  fft_any(M, q->twM, a, o, +1, q->t);
+  for (int k = 0; k < n; k++) { /* conj(c_k) times the convolution */
+    double cr = q->chirp[2 * k], ci = q->chirp[2 * k + 1], zr = o[2 * k], zi = o[2 * k + 1];
+    y[2 * k] = zr * cr + zi * ci;
+    y[2 * k + 1] = zi * cr - zr * ci;
+  }
+}
+
+/* cosftx (ffts.f90:147-181): FFTW REDFT10 x 1/n (iopt<0) and REDFT01 x 0.5
+ * (iopt>0), with REDFT10 Y_k = 2 sum_j x_j cos(pi k (2j+1) / 2n) and REDFT01
+ * Y_k = x_0 + 2 sum_{j>=1} x_j cos(pi j (2k+1) / 2n) (FFTW's definitions).
+ * Each line is one complex DFT of length n of Makhoul's even/odd reordering
+ * v_j = x_{2j}, v_{n-1-j} = x_{2j+1}:
+ *   sum_j x_j cos(pi k (2j+1) / 2n) = Re(exp(-i pi k / 2n) V_k),
+ * and REDFT01 is the unnormalised inverse of that map.  Line l starts at
  * c + l*stride. */
 static void cosft_lines(int n, int nlines, double *c, size_t stride, int iopt) {
-  double *tab = zalloc(4 * (size_t)n), *tmp = zalloc(n);
-  for (int t = 0; t < 4 * n; t++) tab[t] = cos(PI_D * t / (2.0 * n));
+  double *e = zalloc(2 * (size_t)n);
+  for (int k = 0; k < n; k++) {
+    e[2 * k] = cos(PI_D * k / (2.0 * n));
+    e[2 * k + 1] = sin(PI_D * k / (2.0 * n));
+  }
+#pragma omp parallel
+  {
+  double *z = zalloc(2 * (size_t)n);
+  dft_plan q;
+  dft_init(&q, n, iopt < 0 ? -1 : +1);
+#pragma omp for schedule(static)
   for (int l = 0; l < nlines; l++) {
     double *x = c + (size_t)l * stride;
     if (iopt < 0) {
-      for (int k = 0; k < n; k++) {
-        double s = 0.0;
-        for (int j = 0; j < n; j++) s += x[j] * tab[((long)k * (2 * j + 1)) % (4 * n)];
-        tmp[k] = 2.0 * s / n;
+      for (int j = 0; 2 * j < n; j++) {
+        z[2 * j] = x[2 * j];
+        z[2 * j + 1] = 0.0;
       }
+      for (int j = 0; 2 * j + 1 < n; j++) {
+        z[2 * (n - 1 - j)] = x[2 * j + 1];
+        z[2 * (n - 1 - j) + 1] = 0.0;
+      }
+      dft_run(&q, z, z, -1);
+      /* Re(exp(-i pi k/2n) V_k), times 2 (REDFT10) and 1/n (cosftx) */
+      for (int k = 0; k < n; k++) x[k] = 2.0 / n * (e[2 * k] * z[2 * k] + e[2 * k + 1] * z[2 * k + 1]);
     } else {
-      for (int k = 0; k < n; k++) {
-        double s = x[0];
-        for (int j = 1; j < n; j++) s += 2.0 * x[j] * tab[((long)j * (2 * k + 1)) % (4 * n)];
-        tmp[k] = 0.5 * s;
+      /* V_k = exp(i pi k/2n) (X_k - i X_{n-k}), V_0 = X_0 */
+      z[0] = x[0];
+      z[1] = 0.0;
+      for (int k = 1; k < n; k++) {
+        double ar = x[k], ai = -x[n - k];
+        z[2 * k] = e[2 * k] * ar - e[2 * k + 1] * ai;
+        z[2 * k + 1] = e[2 * k] * ai + e[2 * k + 1] * ar;
       }
+      dft_run(&q, z, z, +1);
+      /* y_{2j} = z_j, y_{2j+1} = z_{n-1-j}, times 0.5 (cosftx) */
+      for (int j = 0; 2 * j < n; j++) x[2 * j] = 0.5 * z[2 * j];
+      for (int j = 0; 2 * j + 1 < n; j++) x[2 * j + 1] = 0.5 * z[2 * (n - 1 - j)];
     }
-    memcpy(x, tmp, sizeof(double) * n);
   }
-  free(tab);
-  free(tmp);
+  dft_free(&q);
+  free(z);
+  }
+  free(e);
 }
 void oc_cosft(int n, int nlines, double *c, int iopt) { cosft_lines(n, nlines, c, (size_t)n, iopt); }
 
@@ -674,6 +862,7 @@ void oc_cosft(int n, int nlines, double *c, int iopt) { cosft_lines(n, nlines, c
  * ---------------------------------------------------------------------- */
 static void chp2x(const oc_box *b, double *pen, const double *plan, int R) {
   size_t P = (size_t)b->nz1 * R;
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < b->nx; i++)
     for (size_t p = 0; p < P; p++) {
       double v = plan[(size_t)i * P + p];
@@ -682,6 +871,7 @@ static void chp2x(const oc_box *b, double *pen, const double *plan, int R) {
 }
 static void chx2p(const oc_box *b, double *plan, const double *pen, int R) {
   size_t P = (size_t)b->nz1 * R;
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < b->nx; i++)
     for (size_t p = 0; p < P; p++) {
       double v = pen[p * b->nx + i];
@@ -697,7 +887,10 @@ static void interpxx(const oc_box *b, const double *gen, double *geni,
                      const double *icon, const double *econ, const double *inbx,
                      int chi, int nlin, int loguv) {
   int nx = b->nx, chi2 = (loguv == 1) ? chi + 1 : chi + 3;
+#pragma omp parallel
+  {
   double *bb = zalloc(nx + 1);
+#pragma omp for schedule(static)
   for (int k = 0; k < nlin; k++) {
     const double *g = gen + (size_t)k * nx - 1; /* g[1..nx] */
     for (int i = 2; i <= nx - 1; i++) bb[i] = C2(econ, 1, i) * g[i] + C2(econ, 2, i) * g[i + 1];
@@ -712,6 +905,7 @@ static void interpxx(const oc_box *b, const double *gen, double *geni,
     for (int i = 1; i <= nx; i++) o[i] = bb[i];
   }
   free(bb);
+  }
 }
 
 /* difvisxx (lip.f90:52-142) on pencils; result may alias upen_int. */
@@ -720,7 +914,10 @@ static void difvisxx(const oc_box *b, const double *upen_int, const double *upen
                      const double *dcbx, const double *econ, const double *econv,
                      const double *cofvbx, int chv, int nlin, int vel, double rex) {
   int nx = b->nx, chv1 = chv + 2, ic = (vel == 0) ? 1 : 3;
+#pragma omp parallel
+  {
   double *bb = zalloc(nx + 1), *cv = zalloc(nx + 1);
+#pragma omp for schedule(static)
   for (int k = 0; k < nlin; k++) {
     const double *g = upen_int + (size_t)k * nx - 1;
     const double *q = upencil + (size_t)k * nx - 1;
@@ -749,6 +946,7 @@ static void difvisxx(const oc_box *b, const double *upen_int, const double *upen
   }
   free(bb);
   free(cv);
+  }
 }
 
 /* Row access for y-operators: element (k, j) of a (stride, rows) block. */
@@ -917,17 +1115,22 @@ oc_box *oc_create(int nx, int ny, int nz, int xout, double re, double ax,
   b->wki1 = zalloc(b->nu); b->wki2 = zalloc(b->nv); b->wki3 = zalloc(b->nu);
   b->rest = zalloc(b->nv); b->tbuf = zalloc(b->nu);
   size_t prow = (size_t)(nz + 2) * (ny + 1);
-  b->wkp = zalloc(prow); b->wkpo = zalloc(prow);
-  b->wkf = zalloc((size_t)nz1 * (ny + 1)); b->wkfo = zalloc((size_t)nz1 * (ny + 1));
-  b->wk2r = zalloc(nz + 2);
   b->vm = zalloc(ny + 1);
   b->fft_tw = zalloc(2 * (size_t)nz);
   for (int t = 0; t < nz; t++) {
     b->fft_tw[2 * t] = cos(2.0 * PI_D * t / nz);
     b->fft_tw[2 * t + 1] = sin(2.0 * PI_D * t / nz);
   }
-  b->fft_a = zalloc(2 * (size_t)nz); b->fft_b = zalloc(2 * (size_t)nz);
-  b->fft_t = zalloc(2 * (size_t)nz);
+  b->nth = omp_get_max_threads();
+  b->tls = (struct oc_tls *)calloc(b->nth, sizeof(struct oc_tls));
+  for (int t = 0; t < b->nth; t++) {
+    struct oc_tls *q = &b->tls[t];
+    q->wkp = zalloc(prow); q->wkpo = zalloc(prow);
+    q->wkf = zalloc((size_t)nz1 * (ny + 1)); q->wkfo = zalloc((size_t)nz1 * (ny + 1));
+    q->wk2r = zalloc(nz + 2); q->vm = zalloc(ny + 1);
+    q->fft_a = zalloc(2 * (size_t)nz); q->fft_b = zalloc(2 * (size_t)nz);
+    q->fft_t = zalloc(2 * (size_t)nz);
+  }
   b->setstep = 1;
   return b;
 }
@@ -943,9 +1146,15 @@ void oc_destroy(oc_box *b) {
                      &b->cofvyv, &b->ayp, &b->vmagic, &b->um, &b->u0, &b->v0, &b->pen_u,
                      &b->pen_v, &b->pen_w, &b->pen_a, &b->resu, &b->resv, &b->resw,
                      &b->rhsu, &b->rhsv, &b->rhsw, &b->wki1, &b->wki2, &b->wki3, &b->rest,
-                     &b->tbuf, &b->wkp, &b->wkpo, &b->wkf, &b->wkfo, &b->wk2r, &b->vm,
-                     &b->fft_tw, &b->fft_a, &b->fft_b, &b->fft_t};
+                     &b->tbuf, &b->vm,
+                     &b->fft_tw};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(*ptrs[i]);
+  for (int t = 0; t < b->nth; t++) {
+    struct oc_tls *q = &b->tls[t];
+    double *tp[] = {q->wkp, q->wkpo, q->wkf, q->wkfo, q->wk2r, q->vm, q->fft_a, q->fft_b, q->fft_t};
+    for (size_t i = 0; i < sizeof tp / sizeof tp[0]; i++) free(tp[i]);
+  }
+  free(b->tls);
   for (int f = 0; f < OC_NFIELDS; f++) free(b->fld[f]);
   free(b);
 }
@@ -1262,7 +1471,6 @@ void oc_rhsp(oc_box *b, int m) {
   double *rhsupa = b->fld[OC_RHSUPA], *rhsvpa = b->fld[OC_RHSVPA], *rhswpa = b->fld[OC_RHSWPA];
   double *rhsu = b->rhsu, *rhsv = b->rhsv, *rhsw = b->rhsw;
   double *wki1 = b->wki1, *wki2 = b->wki2, *wki3 = b->wki3;
-  double *wkp = b->wkp, *wkpo = b->wkpo, *wkf = b->wkf, *wkfo = b->wkfo, *wk2r = b->wk2r;
   int mpu = s * ru, mpv = s * ny;
   const double *kaz = b->kaz;
   int setstep = b->setstep && !(b->freeze_dt && b->dt_valid);
@@ -1288,7 +1496,14 @@ void oc_rhsp(oc_box *b, int m) {
     for (int q = 1; q <= 15; q++) b->ener[q] = uner[q];
   }
 
-  /* plane loop 1 (rhsp.F90:120-171) */
+  /* plane loop 1 (rhsp.F90:120-171), planes over threads */
+  for (int t = 0; t < b->nth; t++)
+    for (int j = 1; j <= ny; j++) b->tls[t].vm[j] = -1e14;
+#pragma omp parallel num_threads(b->nth) reduction(max : um_, wm)
+  {
+  struct oc_tls *tl = TLS(b);
+  double *wkp = tl->wkp, *wkpo = tl->wkpo, *wkf = tl->wkf, *wk2r = tl->wk2r, *vm = tl->vm;
+#pragma omp for schedule(static)
   for (int i = 1; i <= nx; i++) {
     double *ui = PL(b, u, i, ru), *w1i = PL(b, wki1, i, ru), *w2i = PL(b, wki2, i, ny),
            *w3i = PL(b, wki3, i, ru), *rui = PL(b, rhsu, i, ru);
@@ -1307,7 +1522,7 @@ void oc_rhsp(oc_box *b, int m) {
     interpyy(b, wkp, ps, wkpo, ps, b->inyu, b->cofiuy, b->inby, 0, nz);
     for (int j = 1; j <= ny; j++) {
       oc_c2r(b, &RW(w2i, s, 0, j), wk2r);
-      if (setstep) b->vm[j] = cmaxabs(wk2r, nz, b->vm[j]);
+      if (setstep) vm[j] = cmaxabs(wk2r, nz, vm[j]);
       for (int n = 0; n < nz; n++) wk2r[n] = wk2r[n] * RW(wkpo, ps, n, j);
       oc_r2c(b, wk2r, &RW(w2i, s, 0, j));
     }
@@ -1319,6 +1534,10 @@ void oc_rhsp(oc_box *b, int m) {
         RW(rui, s, 2 * k + 1, j) = RW(wkf, s, 2 * k + 1, j) + wr * kaz[k];
       }
   }
+  }
+  for (int t = 0; t < b->nth; t++)
+    for (int j = 1; j <= ny; j++)
+      if (b->tls[t].vm[j] > b->vm[j]) b->vm[j] = b->tls[t].vm[j];
 
   /* x storage: convective + viscous x terms (rhsp.F90:181-193) */
   chp2x(b, b->pen_a, wki1, ru);
@@ -1334,6 +1553,7 @@ void oc_rhsp(oc_box *b, int m) {
            b->cofvbx, 2, mpu, 0, b->rex);
   chx2p(b, wki3, b->pen_a, ru);
   /* complete rhs (rhsp.F90:199-204) */
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < b->nu; q++) rhsu[q] += wki1[q];
   memcpy(rhsv, wki2, sizeof(double) * b->nv);
   memcpy(rhsw, wki3, sizeof(double) * b->nu);
@@ -1358,7 +1578,7 @@ void oc_rhsp(oc_box *b, int m) {
   b->setstep = 0;
   double dt = b->dt;
   double var1 = dt * (b->rkcv[m] + b->rkdv[m]);
-  double var2 = dt * b->rex * b->rkcv[m];
+  const double var2_ = dt * b->rex * b->rkcv[m];
   double var3 = dt * b->rkc[m];
   double var4 = dt * b->rkd[m];
 
@@ -1370,12 +1590,14 @@ void oc_rhsp(oc_box *b, int m) {
 
   /* pressure-gradient predictor (rhsp.F90:291-339).  With one process the
    * last plane receives no p(nx+1) halo and is left unchanged. */
+#pragma omp parallel for schedule(static)
   for (int i = 2; i <= nx - 1; i++) {
     double *ui = PL(b, u, i, ru), *pi = PL(b, p, i, ny), *pn = PL(b, p, i + 1, ny);
     for (int j = 2; j <= ny; j++)
       for (int k = 0; k < s; k++)
         RW(ui, s, k, j) -= var1 * b->idxx * (RW(pn, s, k, j) - RW(pi, s, k, j));
   }
+#pragma omp parallel for schedule(static)
   for (int i = 2; i <= nx; i++) {
     double *vi = PL(b, v, i, ny), *wi = PL(b, w, i, ru), *pi = PL(b, p, i, ny);
     for (int j = 2; j <= ny - 1; j++)
@@ -1389,7 +1611,12 @@ void oc_rhsp(oc_box *b, int m) {
       }
   }
 
-  /* plane loop 2 (rhsp.F90:351-418) */
+  /* plane loop 2 (rhsp.F90:351-418), planes over threads */
+#pragma omp parallel num_threads(b->nth)
+  {
+  struct oc_tls *tl = TLS(b);
+  double *wkp = tl->wkp, *wkpo = tl->wkpo, *wkf = tl->wkf, *wkfo = tl->wkfo, *wk2r = tl->wk2r;
+#pragma omp for schedule(static)
   for (int i = 2; i <= nx - 1; i++) {
     double *rui = PL(b, rhsu, i, ru), *rvi = PL(b, rhsv, i, ny), *rwi = PL(b, rhsw, i, ru);
     double *sui = PL(b, b->resu, i, ru), *svi = PL(b, b->resv, i, ny), *swi = PL(b, b->resw, i, ru);
@@ -1441,6 +1668,7 @@ void oc_rhsp(oc_box *b, int m) {
       }
     }
   }
+  }
 
   boun(b);
   /* RHS boundary rows (rhsp.F90:423-445) */
@@ -1467,8 +1695,12 @@ void oc_rhsp(oc_box *b, int m) {
     }
   }
   /* explicit RK3 combination (rhsp.F90:448-475) */
+#pragma omp parallel num_threads(b->nth)
+  {
+  double *wkf = TLS(b)->wkf;
+#pragma omp for schedule(static)
   for (int i = 2; i <= nx; i++) {
-    if (i == nx) var2 = 0.0;
+    const double var2 = i == nx ? 0.0 : var2_;
     vistyy(b, PL(b, b->resv, i, ny), wkf, b->viyv, b->cofvyv, b->cofvby, ny);
     for (size_t q = 0; q < (size_t)mpv; q++)
       PL(b, wki2, i, ny)[q] = PL(b, v, i, ny)[q] -
@@ -1482,11 +1714,13 @@ void oc_rhsp(oc_box *b, int m) {
       PL(b, wki3, i, ru)[q] = PL(b, w, i, ru)[q] -
                               (var3 * PL(b, rhsw, i, ru)[q] + var2 * wkf[q] + var4 * PL(b, rhswpa, i, ru)[q]);
   }
+  }
   memcpy(rhsupa, rhsu, sizeof(double) * b->nu);
   memcpy(rhsvpa, rhsv, sizeof(double) * b->nv);
   memcpy(rhswpa, rhsw, sizeof(double) * b->nu);
   /* implicit y viscous step (rhsp.F90:479-484) */
   double rkk = b->rex * dt * b->rkdv[m];
+#pragma omp parallel for schedule(static)
   for (int i = 2; i <= nx; i++) {
     implzy(b, PL(b, u, i, ru), PL(b, wki1, i, ru), b->vyui, b->cofvyu, ny + 1, rkk);
     implzy(b, PL(b, v, i, ny), PL(b, wki2, i, ny), b->vyvi, b->cofvyv, ny, rkk);
@@ -1537,6 +1771,7 @@ void oc_pois(oc_box *b, double varstep) {
   double *rest = b->rest;
   const double *kaz = b->kaz;
   /* divergence (pois.f90:41-54) */
+#pragma omp parallel for schedule(static)
   for (int i = 2; i <= nx; i++) {
     double *ri = PL(b, rest, i, ny), *wi = PL(b, w, i, ru), *vi = PL(b, v, i, ny);
     for (int j = 1; j <= ny - 1; j++)
@@ -1546,6 +1781,7 @@ void oc_pois(oc_box *b, double varstep) {
         RW(ri, s, 2 * k + 1, j) = wr * kaz[k] + (RW(vi, s, 2 * k + 1, j + 1) - RW(vi, s, 2 * k + 1, j)) * b->idy[j];
       }
   }
+#pragma omp parallel for schedule(static)
   for (int i = 2; i <= nx; i++) {
     double *ri = PL(b, rest, i, ny), *ui = PL(b, u, i, ru), *um1 = PL(b, u, i - 1, ru);
     for (int j = 1; j <= ny - 1; j++)
@@ -1559,7 +1795,10 @@ void oc_pois(oc_box *b, double varstep) {
   for (int i = 1; i <= nx; i++)
     for (int j = 1; j <= ny; j++) RW(PL(b, rest, i, ny), s, 1, j) = 0.0;
   /* per-mode tridiagonal in y (pois.f90:106-126) */
+#pragma omp parallel
+  {
   double(*a)[3] = (double(*)[3])malloc(sizeof(double) * 3 * (ny + 1));
+#pragma omp for schedule(static)
   for (int i = 2; i <= nx; i++) {
     double *ri = PL(b, rest, i, ny);
     for (int k = 0; k <= nz2; k++) {
@@ -1578,12 +1817,14 @@ void oc_pois(oc_box *b, double varstep) {
     }
   }
   free(a);
+  }
   /* inverse DCT (pois.f90:130-142) */
   chp2x(b, pen, rest, ny);
   cosft_lines(nx - 1, s * ny, pen + 1, (size_t)nx, 1);
   chx2p(b, rest, pen, ny);
   /* projection (pois.f90:144-162); with one process the last plane of u
    * gets no correction (no phi(nx+1) halo) */
+#pragma omp parallel for schedule(static)
   for (int i = 2; i <= nx; i++) {
     double *ri = PL(b, rest, i, ny), *pi = PL(b, p, i, ny), *wi = PL(b, w, i, ru), *vi = PL(b, v, i, ny);
     for (int j = 2; j <= ny; j++)
@@ -1597,6 +1838,7 @@ void oc_pois(oc_box *b, double varstep) {
     for (int j = 2; j <= ny - 1; j++)
       for (int k = 0; k < s; k++) RW(vi, s, k, j) -= b->idyy[j] * (RW(ri, s, k, j) - RW(ri, s, k, j - 1));
   }
+#pragma omp parallel for schedule(static)
   for (int i = 2; i <= nx - 1; i++) {
     double *ui = PL(b, u, i, ru), *ri = PL(b, rest, i, ny), *rn = PL(b, rest, i + 1, ny);
     for (int j = 1; j <= ny - 1; j++)

@@ -30,7 +30,10 @@ VEL_TOL = 1e-6
 P_TOL = 1e-5
 
 
-@pytest.fixture(scope="module", params=[(130, 256), (257, 512)], ids=["x130z256", "x257z512"])
+# nz = 1024 (radix 16-16-4) and 2048 (a two-wave team) are the z lengths of
+# the 1536 x 315 x 1024 and 3072 x 360 x 2048 boxes
+@pytest.fixture(scope="module", params=[(130, 256), (257, 512), (130, 1024), (130, 2048)],
+                ids=["x130z256", "x257z512", "x130z1024", "x130z2048"])
 def mbox(request):
     nx, nz = request.param
     cfg = config.small(name=f"mix{nx}", nx=nx, ny=40, nz=nz, Lx=8.0 * nx / 32, kzmax=min(40, nz // 3 - 1))
@@ -151,23 +154,27 @@ def test_mixed_rejects_unsupported():
         GpuBox(cfg, st, precision=OTBL_MIXED)
 
 
-def test_mixed_long_x_vs_fp64():
+def test_mixed_long_x(oracle_lib):
     """A box whose Poisson x transforms take the 3 x 1024 Bluestein length
-    (nx - 1 = 1099) and whose x lines are split into segments: the mixed
-    steps against the FP64 steps on the GPU (the oracle's direct DCT is too
-    slow at this length; its cosftx parity is covered in FP64)."""
+    (nx - 1 = 1099) and whose x lines are split into segments: one mixed step
+    against the oracle, two against the FP64 steps on the GPU."""
     cfg = config.small(name="mixlong", nx=1100, ny=40, nz=256, Lx=8.0 * 1100 / 32, kzmax=40)
     st = ic.make_state(cfg)
+    ob = oracle_lib.OracleBox(cfg, st)
     gd = GpuBox(cfg, st)
     gm = GpuBox(cfg, st, precision=OTBL_MIXED, graphs=True)
     pm = gm.paths()
     assert pm["dct"] == "engine3072" and int(pm["xseg"]) > 1, pm
+    ob.step()
+    gm.step(1)
+    e_o = errors(gm, ob)
     gd.step(2)
-    gm.step(2)
+    gm.step(1)
     errs = errors(gm, gd)
     gd.close()
     gm.close()
-    print("mixed vs FP64, nx=1100, after 2 steps", errs, flush=True)
+    print("mixed vs oracle, nx=1100, after 1 step", e_o, "vs FP64 after 2 steps", errs, flush=True)
+    check(e_o)
     check(errs)
 
 

@@ -102,7 +102,7 @@ def xbox(request, oracle_lib):
 @pytest.mark.parametrize("iopt", [-1, 1])
 def test_cosftx_long(xbox, oracle_lib, iopt):
     """Poisson x transforms on the engine's Bluestein lengths (powers of two
-    and 3 x 1024) against the oracle's direct DCT sums."""
+    and 3 x 1024) against the oracle's DCT (itself pinned against scipy)."""
     cfg, st = xbox
     gb = GpuBox(cfg, st)
     want = next(m for m in (512, 1024, 2048, 3072, 4096) if m >= 2 * (cfg.nx - 1) - 1)
@@ -118,9 +118,9 @@ def test_cosftx_long(xbox, oracle_lib, iopt):
 
 
 def test_step_long_x(xbox, oracle_lib):
+    """A whole FP64 step of boxes whose Poisson x transforms take each of the
+    engine's Bluestein lengths, against the oracle."""
     cfg, st = xbox
-    if cfg.nx > 300:
-        pytest.skip("the oracle's direct DCT makes this box slow; cosftx covers its length")
     ob = oracle_lib.OracleBox(cfg, st)
     gb = GpuBox(cfg, st)
     ob.step()

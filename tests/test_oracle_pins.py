@@ -311,16 +311,22 @@ def test_fourxz_matches_numpy(box, oracle_lib):
     assert back[1] == 0.0
 
 
-@pytest.mark.parametrize("n", [10, 31, 255])
+@pytest.mark.parametrize("n", [2, 3, 10, 31, 255, 383, 1023, 1535, 1536, 3071])
 def test_cosft_matches_scipy(oracle_lib, n):
     """cosftx (ffts.f90:147-181): FFTW REDFT10 / n and REDFT01 * 0.5 are
-    scipy's unnormalised DCT-II / n and DCT-III * 0.5; round trip = 1."""
+    scipy's unnormalised DCT-II / n and DCT-III * 0.5; round trip = 1.
+    The lengths cover the oracle's mixed-radix path (every prime factor at
+    most 64: 255, 1023, 1536) and its Bluestein path (a prime factor above
+    64: 383, 1535 = 5 * 307, 3071 = 37 * 83)."""
     from oracle.oracle import ptr
     rng = np.random.default_rng(n)
     x = rng.standard_normal((3, n))
     a = x.copy()
     oracle_lib.lib().oc_cosft(n, 3, ptr(a), -1)
     assert np.abs(a - sfft.dct(x, 2, axis=1) / n).max() < 1e-13
+    c = x.copy()
+    oracle_lib.lib().oc_cosft(n, 3, ptr(c), 1)
+    assert np.abs(c - 0.5 * sfft.dct(x, 3, axis=1)).max() < 1e-12 * np.abs(c).max()
     b = a.copy()
     oracle_lib.lib().oc_cosft(n, 3, ptr(b), 1)
     assert np.abs(b - x).max() < 1e-12
