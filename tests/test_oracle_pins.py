@@ -507,3 +507,396 @@ def test_info_per_step_pins(oracle_lib):
     uf[:, :, 0] = U * np.minimum(1.0, (yf[1:cf.ny + 2] - yf[1]) / df)
     theta = info_per_step_planes(uf, yf, sf["re"])[4] / sf["re"] / U
     assert np.abs(theta - df / 6).max() < 0.02 * df / 6
+
+
+# ------------------------------------------------ difvisxx (lip.f90:52-142)
+def _difvis(ob, cfg, prod, vel, which_u):
+    from oracle.oracle import ptr
+    R = cfg.ny + 1
+    g = np.zeros((cfg.nx, R, cfg.nz1))
+    q = np.zeros_like(g)
+    g[:, 0, 0] = prod
+    q[:, 0, 0] = vel
+    out = np.zeros_like(g)
+    ob.op("difvisxx", ptr(g), ptr(q), ptr(out), R, which_u)
+    return out[:, 0, 0]
+
+
+@pytest.mark.parametrize("which_u", [1, 0])
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_difvisxx_convective_exact_on_cubics(box, which_u, deg):
+    """The convective part of difvisxx (the compact staggered first
+    derivative, tridiagonal dcx* with the 2-point right-hand side and the
+    4-point boundary rows dcbx) is exact on cubics at every row but the
+    outflow row: the product, given at the midpoints x(i + 1/2), is
+    differentiated at x(i).  (The x grid is uniform, so the u and v/w
+    coefficient sets coincide up to round-off: both are checked.)"""
+    cfg, st, ob = box
+    x = x_grid(cfg)
+    h = x[1]
+    xf = x[1:cfg.nx + 1]
+    L = x[-1]
+    got = _difvis(ob, cfg, ((xf + h / 2) / L) ** deg, np.zeros(cfg.nx), which_u)
+    want = deg * (xf / L) ** max(deg - 1, 0) / L if deg else np.zeros(cfg.nx)
+    assert np.abs(got[:-1] - want[:-1]).max() < 1e-11
+
+
+@pytest.mark.parametrize("which_u", [1, 0])
+@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4])
+def test_difvisxx_viscous_exact_on_quartics(box, which_u, deg):
+    """The viscous part, rex d2/dx2 of the velocity (tridiagonal vix*, 3-point
+    right-hand side, 4-point boundary rows cofvbx), is exact on quartics at
+    every row but the outflow row, and degree 5 already shows the closures'
+    truncation error."""
+    cfg, st, ob = box
+    x = x_grid(cfg)
+    xf = x[1:cfg.nx + 1]
+    L = x[-1]
+    rex = -1.0 / st["re"]
+    got = _difvis(ob, cfg, np.zeros(cfg.nx), (xf / L) ** deg, which_u)
+    want = rex * deg * (deg - 1) * (xf / L) ** max(deg - 2, 0) / L ** 2
+    # round-off scale: the stencil weights are of order rex / h^2
+    scale = max(np.abs(want).max(), abs(rex) / x[1] ** 2)
+    assert np.abs(got[:-1] - want[:-1]).max() < 1e-11 * scale
+    got5 = _difvis(ob, cfg, np.zeros(cfg.nx), (xf / L) ** 5, which_u)
+    want5 = rex * 20 * (xf / L) ** 3 / L ** 2
+    assert np.abs(got5[:-1] - want5[:-1]).max() > 1e-9 * np.abs(want5).max()
+
+
+@pytest.mark.parametrize("which_u", [1, 0])
+def test_difvisxx_outflow_row(box, which_u):
+    """At the outflow plane the convective derivative is replaced by the
+    convective outflow condition U_inf (q(nx) - q(nx-1)) / dx with U_inf = 1,
+    and the viscous term by zero (lip.f90:105,132), whatever the product."""
+    cfg, st, ob = box
+    rng = np.random.default_rng(5)
+    prod, vel = rng.standard_normal(cfg.nx), rng.standard_normal(cfg.nx)
+    h = x_grid(cfg)[1]
+    got = _difvis(ob, cfg, prod, vel, which_u)
+    assert got[-1] == pytest.approx((vel[-1] - vel[-2]) / h, rel=1e-14)
+
+
+# ---------------------------------------------------- implzy (lip.f90:528-597)
+@pytest.mark.parametrize("uu", [True, False], ids=["u_rows", "v_rows"])
+def test_implzy_dense_solve(box, uu):
+    """implzy solves (A + rkk B) u = A wki on rows 2..jee-1 with A the
+    unfactored compact second-derivative matrix (vyui / vyvi, the copy made
+    before coefinv) and B its explicit stencil (cofvyu / cofvyv, entries for
+    rows j+1, j, j-1), rows 1 and jee of u kept (Dirichlet): against a dense
+    solve of the assembled system."""
+    from oracle.oracle import ptr
+    cfg, st, ob = box
+    jee = cfg.ny + 1 if uu else cfg.ny
+    A = ob.coef("vyui" if uu else "vyvi")      # (3, jee): column j-1 is row j
+    B = ob.coef("cofvyu" if uu else "cofvyv")  # (3, ny)
+    rkk = -0.37
+    rng = np.random.default_rng(9)
+    u0 = rng.standard_normal((1, jee, cfg.nz1))
+    wki = rng.standard_normal((1, jee, cfg.nz1))
+    got = u0.copy()
+    ob.op("implzy", ptr(got), ptr(wki), jee, rkk, 1)
+    M = np.eye(jee)
+    rhs = u0[0].copy()
+    for j in range(2, jee):                   # 1-based rows 2..jee-1
+        r = j - 1
+        M[r, r - 1] = A[0, r] + rkk * B[2, r]
+        M[r, r] = A[1, r] + rkk * B[1, r]
+        M[r, r + 1] = A[2, r] + rkk * B[0, r]
+        rhs[r] = A[0, r] * wki[0, r - 1] + A[1, r] * wki[0, r] + A[2, r] * wki[0, r + 1]
+    ref = np.linalg.solve(M, rhs)
+    assert np.abs(got[0] - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("uu", [True, False], ids=["u_rows", "v_rows"])
+def test_implzy_inverts_the_explicit_viscous_operator(box, uu):
+    """The implicit step is the inverse of I + rkk D_yy with D_yy the
+    explicit compact second derivative vistyy (lip.f90:447-513) applies:
+    for wki = u* + rkk D_yy u* (rows 2..jee-1) and the Dirichlet rows of u*,
+    implzy returns u* (the implicit system is the explicit operator's own
+    matrix pair, coeft.f90:1116-1134)."""
+    from oracle.oracle import ptr
+    cfg, st, ob = box
+    jee = cfg.ny + 1 if uu else cfg.ny
+    rkk = -0.05
+    rng = np.random.default_rng(4)
+    us = rng.standard_normal((1, jee, cfg.nz1))
+    d = np.zeros_like(us)
+    ob.op("vistyy", ptr(us), ptr(d), jee, 1)
+    wki = us + rkk * d
+    got = us.copy()
+    got[0, 1:-1] = 0.0
+    ob.op("implzy", ptr(got), ptr(wki), jee, rkk, 1)
+    assert np.abs(got - us).max() < 1e-12 * np.abs(us).max()
+
+
+# ------------------------------------------- Re_theta(x), Cf(x) diagnostics
+def test_retheta_cf_closed_forms(oracle_lib):
+    """Re_theta(x) and Cf(x) of the kz = 0 u mode (genflu.F90:76-87): equal
+    to the inlet values of inlet_retheta (bl_1.F90:372-396, the same sums
+    restated separately) where the plane's profile is u0; for a ramp
+    U min(1, (y - y1) / delta) Cf = 2 U / (delta Re) exactly and Re_theta =
+    Re U delta / 6 up to the sum's first-order quadrature."""
+    cfg = config.small(ny=320)
+    st = ic.make_state(cfg)
+    ob = oracle_lib.OracleBox(cfg, st)
+    y = np.asarray(st["y"])
+    ny, re = cfg.ny, st["re"]
+    u = ob.get("u")
+    u[:, :, 0] = st["u0"]
+    # planes 5.. take ramps of growing thickness
+    U = 1.1
+    deltas = np.linspace(0.3, 0.6, cfg.nx - 4)
+    yu = 0.5 * (y[0:ny + 1] + y[1:ny + 2])
+    for q, dl in enumerate(deltas):
+        u[4 + q, :, 0] = U * np.minimum(1.0, (yu - y[1]) / dl)
+    ob.set("u", u)
+    rth, cf = ob.retheta_cf()
+    assert np.abs(rth[:4] - ob.scalar("rthin")).max() <= 1e-13 * ob.scalar("rthin")
+    assert np.allclose(cf[4:], 2.0 * U / (deltas * re), rtol=1e-12)
+    assert np.abs(rth[4:] / (re * U * deltas / 6.0) - 1.0).max() < 0.02
+
+
+# ------------------------------------------ genflu rescaling (genflu.F90:88-242)
+def _dobleu_integral(eta):
+    """W(eta_i) = integral from eta_1 to eta_i of the inner/outer weight
+    0.5 (1 + coth(4) tanh(4 (s - 0.2) / (0.6 s + 0.2))) (1 for s >= 1) of
+    dobleu (genflu.F90:175-203), by adaptive quadrature, and its value at
+    eta = 1."""
+    from scipy.integrate import quad
+
+    def wgt(s):
+        return 1.0 if s >= 1.0 else 0.5 * (1.0 + math.tanh(4.0 * (s - 0.2) / (0.6 * s + 0.2)) / math.tanh(4.0))
+    W = np.zeros(len(eta))
+    for i in range(1, len(eta)):
+        W[i] = W[i - 1] + quad(wgt, eta[i - 1], eta[i], points=[1.0] if eta[i - 1] < 1.0 < eta[i] else None)[0]
+    w1 = quad(wgt, eta[0], 1.0, points=[0.2])[0]
+    return W, w1
+
+
+def test_genflu_rescaling_far_from_identity(oracle_lib):
+    """The recycled inflow of genflu with gamma far from 1: the outlet plane
+    xout carries a mean profile twice as thick as the inflow profile u0 (so
+    Re_theta,out ~ 2 Re_theta,in and gamma_2 ~ 0.5), and, in every mode >= 1,
+    a profile that after the tanh blend of genflu.F90:55-60 is linear in the
+    rescaled coordinate ysca, a + b ysca.  Linear interpolation reproduces a
+    linear profile, so the inflow modes must be (a + b wlwa)/gamma_1 with
+    wlwa = ysca/gamma_1 + (1/gamma_2 - 1/gamma_1) W(ysca)/W(1), rows past
+    ntop - 3 copies of row ntop - 3, the u/w ghost row the negative of row 2,
+    and mode 0 the inflow profiles.  gamma_2 = Re_theta,in / Re_theta,out
+    with Re_theta,out the Re_theta of the outlet profile (the running mean um
+    is set to it, so the update of genflu.F90:67-75 keeps it), gamma_1 =
+    gamma_2^(1/8) exactly; W, the weight integral of dobleu, to its midpoint
+    rule's accuracy against quadrature."""
+    cfg = config.small(ny=64, nx=32)
+    st = dict(ic.make_state(cfg))
+    ny, s = cfg.ny, cfg.nz1
+    y = np.asarray(st["y"])
+    yu = 0.5 * (y[0:ny + 1] + y[1:ny + 2])       # u rows 1..ny+1
+    u0 = np.asarray(st["u0"])
+    uout = np.interp(yu / 2.0, yu, u0)          # twice as thick
+    uout[0] = -uout[1]
+    probe = oracle_lib.OracleBox(cfg, st)
+    din, rthin = probe.scalar("din"), probe.scalar("rthin")
+    jy = min(119, ny)
+    fblend = 0.5 * (1.0 - np.tanh(y[1:ny + 2] - y[jy]))
+    ys_u = 0.5 * (y[1:ny + 1] + y[2:ny + 2]) / din    # ysca of u, i = 1..ny (row i+1)
+    ys_v = y[1:ny + 1] / din
+    rng = np.random.default_rng(12)
+    a = rng.standard_normal(s - 2) * 1e-2
+    b = rng.standard_normal(s - 2) * 1e-2
+    u, v, w = (np.array(st[n]) for n in ("u", "v", "w"))
+    xo = cfg.xout - 1
+    u[xo, :, 0] = uout
+    for arr, fac in ((u, 1.0), (w, -0.5)):
+        pr = np.zeros((ny + 1, s - 2))
+        pr[1:] = (a[None, :] + b[None, :] * ys_u[:, None]) * fac / fblend[1:, None]
+        arr.view(np.float64).reshape(cfg.nx, ny + 1, s)[xo, :, 2:] = pr
+    v.view(np.float64).reshape(cfg.nx, ny, s)[xo, :, 2:] = 0.7 * (a[None, :] + b[None, :] * ys_v[:, None])
+    st.update(u=u, v=v, w=w, um=uout.copy())
+    ob = oracle_lib.OracleBox(cfg, st)
+    rth, _ = ob.retheta_cf()
+    ob.rhsp(1)
+    info = ob.genflu_info()
+    g2 = rthin / rth[xo]
+    assert info["gamma2"] == pytest.approx(g2, rel=1e-13)
+    assert 0.4 < g2 < 0.6
+    g1 = info["gamma1"]
+    assert g1 == pytest.approx(g2 ** 0.125, rel=1e-15)
+    uo, vo, wo = (ob.get(n) for n in ("u", "v", "w"))
+    # mode 0: the inflow profiles (rows the boundary conditions leave alone)
+    assert np.abs(uo[0, 1:ny, 0].real - u0[1:ny]).max() == 0.0
+    assert np.abs(vo[0, 1:ny - 1, 0].real - np.asarray(st["v0"])[1:ny - 1]).max() == 0.0
+    for kind, ys, out, fac, first in (("u", ys_u, uo, 1.0, 1), ("w", ys_u, wo, -0.5, 1),
+                                      ("v", ys_v, vo, 0.7, 0)):
+        W, w1 = _dobleu_integral(ys)
+        wl = ys / g1 + (1.0 / g2 - 1.0 / g1) / w1 * W
+        ntop = max(i + 1 for i in range(ny) if wl[i] <= ys[-1])
+        R = out.shape[1]
+        got = out[0].view(np.float64).reshape(R, s)[:, 2:]
+        want = np.zeros((R, s - 2))
+        rows = np.arange(1, ntop + 1)                 # i = 1..ntop
+        vals = fac * (a[None, :] + b[None, :] * wl[rows - 1][:, None]) / g1
+        want[rows - 1 + first] = vals                 # u/w: row i+1, v: row i
+        want[ntop - 3:] = want[ntop - 4]              # rows ntop-2.. copy row ntop-3
+        # rows the boundary conditions rewrite afterwards are left out
+        lo, hi = (1, R - 1) if kind != "v" else (1, R - 1)
+        err = np.abs(got[lo:hi] - want[lo:hi]).max() / np.abs(want).max()
+        assert err < 2e-3, (kind, err)
+        # exact parts: the copied rows past ntop - 3 are row ntop - 3
+        assert np.abs(got[ntop - 3:hi] - got[ntop - 4]).max() == 0.0, kind
+
+
+# ------------------------------ rhsp composition: a manufactured solution
+# Each velocity component is a mean mode and one spanwise mode,
+#   f(x, y, z) = F0(x, y) + F1(x, y) exp(i beta z) + c.c.,
+# with F0, F1 sums of separable terms c X(x) Y(y) whose derivatives are known
+# in closed form; products go to spanwise modes 0, 1, 2 (no aliasing on
+# nz = 24).  A factor is (value, first, second derivative).
+def _sin(a, ph):
+    return (lambda t: np.sin(a * t + ph), lambda t: a * np.cos(a * t + ph), lambda t: -a * a * np.sin(a * t + ph))
+
+
+def _one():
+    return (np.ones_like, np.zeros_like, np.zeros_like)
+
+
+def _tanh(c):
+    return (lambda t: np.tanh(c * t), lambda t: c * (1 - np.tanh(c * t) ** 2),
+            lambda t: -2 * c * c * np.tanh(c * t) * (1 - np.tanh(c * t) ** 2))
+
+
+def _wall(c):
+    # t exp(-c t): zero at the wall, as interpyy's no-slip row 1 assumes
+    return (lambda t: t * np.exp(-c * t), lambda t: np.exp(-c * t) * (1 - c * t),
+            lambda t: np.exp(-c * t) * (c * c * t - 2 * c))
+
+
+def _fac(f, t, d):
+    if isinstance(f[0], tuple):          # product of two factors
+        a, b = f
+        return [a[0](t) * b[0](t), a[1](t) * b[0](t) + a[0](t) * b[1](t),
+                a[2](t) * b[0](t) + 2 * a[1](t) * b[1](t) + a[0](t) * b[2](t)][d]
+    return f[d](t)
+
+
+def _val(M, X, Y, dx=0, dy=0):
+    out = np.zeros(np.broadcast(X, Y).shape, complex)
+    for c, fx, gy in M:
+        out += c * _fac(fx, X, dx) * _fac(gy, Y, dy)
+    return out
+
+
+def _zprod(F, G):
+    """Spanwise modes 0, 1, 2 of the product of F = [F0, F1] and G."""
+    def mul(A, B):
+        return [(ca * cb, (fa, fb), (ga, gb)) for ca, fa, ga in A for cb, fb, gb in B]
+
+    def conj(A):
+        return [(np.conj(c), f, g) for c, f, g in A]
+    return [mul(F[0], G[0]) + mul(F[1], conj(G[1])) + mul(conj(F[1]), G[1]),
+            mul(F[0], G[1]) + mul(F[1], G[0]), mul(F[1], G[1])]
+
+
+def _mms_errors(oracle_lib, nx, ny):
+    """Largest error of the oracle's right-hand side (rhsp.F90:120-418, the
+    rhs fields after rhsp) against d(u_j u)/dx_j - nu (d2/dx2 + d2/dz2) u of
+    the manufactured field, each relative to the largest exact term: planes
+    2..nx-1 (plane 1 is the inflow, the outflow plane takes the convective
+    outflow condition), rows 2..ny of u, w and 2..ny-1 of v (the rows the
+    boundary conditions do not overwrite)."""
+    cfg = config.small(nx=nx, ny=ny, nz=24, kzmax=0)
+    st = dict(ic.make_state(cfg, with_perturbation=False))
+    y = np.asarray(st["y"])
+    h = cfg.ax * math.pi / (cfg.nx + 1)
+    x = np.arange(cfg.nx + 2) * h
+    xu, xv = x[1:cfg.nx + 1], x[1:cfg.nx + 1] - h / 2       # u at x(i); v, w at x(i - 1/2)
+    yu, yv = 0.5 * (y[0:ny + 1] + y[1:ny + 2]), y[1:ny + 1]  # u, w rows; v rows
+    a = 2 * math.pi / x[-1]
+    U = [[(1.0, _sin(0.5 * a, 0.3), _tanh(2.0)), (0.2, _one(), _wall(0.5))],
+         [(0.1 + 0.05j, _sin(a, 1.0), _wall(0.7))]]
+    V = [[(0.05, _sin(a, 0.2), _wall(0.6))], [(0.03 - 0.1j, _sin(0.5 * a, 0.5), _wall(0.8))]]
+    W = [[(0.04, _sin(0.7 * a, 0.1), _wall(0.9))], [(0.06 + 0.02j, _sin(a, 2.0), _wall(0.5))]]
+    # the reference's spanwise wavenumber is k/az rounded to FP32 (bl_1.F90:360)
+    beta = ic.kaz_values(cfg)[1]
+    nu = 1.0 / st["re"]
+
+    def put(F, xs, ys):
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        out = np.zeros((cfg.nx, len(ys), cfg.mz), complex)
+        out[:, :, 0] = _val(F[0], X, Y).real
+        out[:, :, 1] = _val(F[1], X, Y)
+        return out
+
+    st.update(u=put(U, xu, yu), v=put(V, xv, yv), w=put(W, xv, yu))
+    st["p"] = np.zeros_like(st["v"])
+    ob = oracle_lib.OracleBox(cfg, st)
+    ob.rhsp(1)
+
+    def exact(cx, cy, cz, F, xs, ys):
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        out = np.zeros((cfg.nx, len(ys), 3), complex)
+        for k in range(3):
+            t = _val(cx[k], X, Y, dx=1) + _val(cy[k], X, Y, dy=1) + 1j * k * beta * _val(cz[k], X, Y)
+            if k < 2:
+                t -= nu * (_val(F[k], X, Y, dx=2) - (k * beta) ** 2 * _val(F[k], X, Y))
+            out[:, :, k] = t
+        return out
+
+    UU, UV, UW, VV, VW, WW = (_zprod(*p) for p in ((U, U), (U, V), (U, W), (V, V), (V, W), (W, W)))
+    want = {"rhsupa": exact(UU, UV, UW, U, xu, yu), "rhsvpa": exact(UV, VV, VW, V, xv, yv),
+            "rhswpa": exact(UW, VW, WW, W, xv, yu)}
+    errs = {}
+    for n, wv in want.items():
+        rows = slice(1, ny - 1) if n == "rhsvpa" else slice(1, ny)
+        g = ob.get(n)[1:-1, rows, :3]
+        errs[n] = float(np.abs(g - wv[1:-1, rows]).max() / np.abs(wv[1:-1, rows]).max())
+    return errs
+
+
+def test_rhs_manufactured_solution_converges(oracle_lib):
+    """The composition of rhsp: on a manufactured field, the oracle's
+    right-hand side of all three components converges to the analytic
+    d(u_j u_i)/dx_j - nu (d2/dx2 + d2/dz2) u_i under grid refinement (x and y
+    spacing halved twice, the tanh grid nested), at second order or better
+    (the 3-point x outflow closures and the stretched-grid y schemes are
+    second order), boundary-closure rows included.  A wrong term, sign,
+    staggering or buffer would leave an error that does not shrink."""
+    levels = [(32, 40), (65, 79), (131, 157)]
+    errs = [_mms_errors(oracle_lib, nx, ny) for nx, ny in levels]
+    for n in errs[0]:
+        e = [lv[n] for lv in errs]
+        assert e[0] < 2e-3, (n, e)
+        assert e[0] / e[1] > 3.2 and e[1] / e[2] > 3.2, (n, e)
+
+
+def test_rk3_step_amplification(oracle_lib):
+    """The three rhsp substeps of one RK3 step (bl_1.F90:94-125) on one
+    spanwise mode that is constant in x and y: only the explicit z-viscous
+    term acts at the box centre, so u_m = u_{m-1} + dt (gamma_m L u_{m-1} +
+    zeta_m L u_{m-2}) with gamma = (8/15, 5/12, 3/4), zeta = (0, -17/60,
+    -5/12) (coeft.f90:494-508) must multiply the mode by the third-order
+    Taylor polynomial 1 + z + z^2/2 + z^3/6 of z = -nu kz^2 dt.  This pins
+    the RK constants and the rhs / previous-rhs bookkeeping across substeps
+    (rhsp.F90:448-475).  The pressure projection between substeps is left
+    out (its own pins cover it): the no-slip ghost rows and the recycled
+    inflow make this field's divergence non-zero near the walls and the
+    inflow, and its projection would reach the centre."""
+    cfg = config.small(nx=64, kzmax=0)
+    st = dict(ic.make_state(cfg, with_perturbation=False))
+    k = cfg.nz2
+    for n in ("u", "v", "w"):
+        st[n] = np.zeros_like(st[n])
+    eps = 1e-6
+    st["u"][:, :, k] = eps
+    ob = oracle_lib.OracleBox(cfg, st, freeze_dt=True)
+    for m in (1, 2, 3):
+        ob.rhsp(m)
+        ob.outflow_correction()
+    dt = ob.time()[1]
+    z = -(ic.kaz_values(cfg)[k] ** 2) * dt / st["re"]
+    g = 1 + z + z * z / 2 + z ** 3 / 6
+    assert abs(z) > 0.05          # the z^2 and z^3 terms are well above the tolerance
+    u = ob.get("u")
+    i, j = cfg.nx // 2, cfg.ny // 2
+    assert u[i, j, k].real == pytest.approx(g * eps, rel=1e-12)
+    assert abs(u[i, j, k].real - math.exp(z) * eps) > 1e-5 * eps
