@@ -152,6 +152,20 @@ int otbl_fourxz(otbl_ctx *ctx, double *fou, double *phys, int iopt, int nrows);
  * rows 1..ny: iopt < 0 DCT-II x 1/(nx-1), iopt > 0 DCT-III x 0.5 */
 int otbl_cosftx(otbl_ctx *ctx, const double *in, double *out, int iopt);
 
+/* cosftx (ffts.f90:147-181) on nlines contiguous device lines of nx-1
+ * doubles, in place: iopt < 0 FFTW REDFT10 (DCT-II) x 1/(nx-1), iopt > 0
+ * REDFT01 (DCT-III) x 0.5.  The length is the box's: cosfti(n) is called
+ * with n = nx - 1 (ffts.f90:124, bl_1.F90:304). */
+int otbl_cosft_lines(otbl_ctx *ctx, double *lines, int nlines, int iopt);
+/* soltriy (pois.f90:211-239): a is the caller's host array (3, 1:ny1),
+ * LU-factored in place as the reference does; then column ii (0..nz2) of
+ * the complex device plane u(0:nz2, 1:ny1) is solved into the device plane
+ * us (us may be u). */
+int otbl_soltriy(otbl_ctx *ctx, const double *u, double *us, int ii, double *a, int ny1);
+/* coefinv (coeft.f90:1338-1353): in-place LU of the caller's host array
+ * a(3, npter1:neq), computed on the GPU. */
+int otbl_coefinv(otbl_ctx *ctx, double *a, int npter1, int neq);
+
 /* ---- device memory helpers (plain pointers, no framework types) ---- */
 int otbl_dev_alloc(size_t bytes, void **ptr);
 int otbl_dev_free(void *ptr);

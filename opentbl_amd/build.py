@@ -62,6 +62,7 @@ def build(verbose=False):
 
 FORTRAN = os.path.join(HERE, "fortran")
 DRIVER = os.path.join(FORTRAN, "otbl_driver")
+FLIB = os.path.join(FORTRAN, "libotbl_fortran.so")
 
 
 def flang():
@@ -82,9 +83,14 @@ def build_fortran():
     os.makedirs(bdir, exist_ok=True)
     mod_src = os.path.join(FORTRAN, "otbl_gpu.f90")
     obj = os.path.join(bdir, "otbl_gpu.o")
-    cmds = [[fc, "-O2", "-c", mod_src, "-o", obj, "-J", bdir],
+    cmds = [[fc, "-O2", "-fPIC", "-c", mod_src, "-o", obj, "-J", bdir],
             [fc, "-O2", os.path.join(FORTRAN, "main.F90"), obj, "-I", bdir, "-J", bdir,
-             "-L", HERE, "-lopentbl_amd", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER]]
+             "-L", HERE, "-lopentbl_amd", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER],
+            # the binding with its drop-ins as a shared library (loaded by
+            # the tests through its C-callable setup and the drop-ins'
+            # Fortran names)
+            [fc, "-shared", "-fPIC", obj, "-L", HERE, "-lopentbl_amd", "-Wl,-rpath,$ORIGIN/..",
+             "-o", FLIB]]
     for cmd in cmds:
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

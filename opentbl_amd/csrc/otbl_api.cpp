@@ -968,6 +968,56 @@ int otbl_cosftx(otbl_ctx *ctx, const double *in, double *out, int iopt) {
   API_END
 }
 
+int otbl_cosft_lines(otbl_ctx *ctx, double *lines, int nlines, int iopt) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(lines && nlines >= 1, "otbl_cosft_lines: null buffer or no lines");
+  k_cosft_lines(b, lines, nlines, iopt);
+  API_END
+}
+
+int otbl_soltriy(otbl_ctx *ctx, const double *u, double *us, int ii, double *a, int ny1) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(u && us && a && ny1 >= 1, "otbl_soltriy: null argument or ny1 < 1");
+  need(ii >= 0 && ii <= b.nz2, "otbl_soltriy: column outside 0..nz2");
+  double *da = nullptr;
+  const size_t bytes = 3 * (size_t)(ny1 + 1) * sizeof(double);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  OTBL_HIP(hipMalloc((void **)&da, bytes));
+  try {
+    // the host array is (3, 1:ny1), stored from row 1: column j at 3 (j - 1)
+    h2d(b, da + 3, a, 3 * (size_t)ny1 * sizeof(double));
+    k_soltriy(b, u, us, ii, da, ny1, b.nz2 + 1);
+    d2h(b, a, da + 3, 3 * (size_t)ny1 * sizeof(double));
+  } catch (...) {
+    (void)hipFree(da);
+    throw;
+  }
+  OTBL_HIP(hipFree(da));
+  API_END
+}
+
+int otbl_coefinv(otbl_ctx *ctx, double *a, int npter1, int neq) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(a && neq >= npter1, "otbl_coefinv: null array or neq < npter1");
+  const int nrow = neq - npter1 + 1;
+  double *da = nullptr;
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  OTBL_HIP(hipMalloc((void **)&da, 3 * (size_t)(nrow + 1) * sizeof(double)));
+  try {
+    h2d(b, da + 3, a, 3 * (size_t)nrow * sizeof(double));
+    k_coefinv(b, da, nrow);
+    d2h(b, a, da + 3, 3 * (size_t)nrow * sizeof(double));
+  } catch (...) {
+    (void)hipFree(da);
+    throw;
+  }
+  OTBL_HIP(hipFree(da));
+  API_END
+}
+
 int otbl_dev_alloc(size_t bytes, void **ptr) {
   API_BEGIN
   need(ptr != nullptr, "null pointer");

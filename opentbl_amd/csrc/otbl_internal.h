@@ -184,6 +184,9 @@ void k_vistyy(Box &b, const double *u, double *res, int R, int nplanes);
 void k_implzy_op(Box &b, double *u, const double *res, int R, double rkk, int nplanes);
 void k_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows);
 void k_cosftx(Box &b, const double *in, double *out, int iopt);
+void k_cosft_lines(Box &b, double *P, int nlines, int iopt);
+void k_coefinv(Box &b, double *a, int nrow);
+void k_soltriy(Box &b, const double *u, double *us, int ii, double *a, int ny1, int nmode);
 void k_substep_rhsp(Box &b, int m);
 void k_outflow(Box &b);
 void k_pois(Box &b, int m, double varstep, bool use_m);

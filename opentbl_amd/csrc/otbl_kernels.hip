@@ -60,6 +60,56 @@ void k_factor_coef(Box &b, double *) {
   LAUNCH(b, "factor_coef", 1, 64, 0, factor_kernel, job);
 }
 
+// coefinv on one caller array (the drop-in of coeft.f90:1338-1353): the
+// same in-place LU on a (3, npter1:neq) array
+__global__ void coefinv_kernel(double *a, int nrow) {
+  if (threadIdx.x != 0) return;
+  for (int j = 2; j <= nrow; ++j) {
+    double d = 1.0 / C3(a, 2, j - 1);
+    C3(a, 2, j - 1) = d;
+    C3(a, 1, j) = -C3(a, 1, j) * d;
+    C3(a, 2, j) = C3(a, 2, j) + C3(a, 1, j) * C3(a, 3, j - 1);
+  }
+  C3(a, 2, nrow) = 1.0 / C3(a, 2, nrow);
+}
+
+void k_coefinv(Box &b, double *a, int nrow) { LAUNCH(b, "coefinv", 1, 64, 0, coefinv_kernel, a, nrow); }
+
+// soltriy (pois.f90:211-239): LU of the tridiagonal a(3, ny1) in place, then
+// the Re and Im parts of complex column ii of u(0:nmode-1, ny1) solved into
+// us (us may be u).  Lane 0 factors, lanes 0 and 1 sweep one part each.
+__global__ void soltriy_kernel(const double *u, double *us, int ii, double *a, int ny1, int nmode) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    for (int j = 2; j <= ny1; ++j) {
+      const double d = 1.0 / C3(a, 2, j - 1);
+      C3(a, 2, j - 1) = d;
+      C3(a, 1, j) = -C3(a, 1, j) * d;
+      C3(a, 2, j) = C3(a, 2, j) + C3(a, 1, j) * C3(a, 3, j - 1);
+    }
+    C3(a, 2, ny1) = 1.0 / C3(a, 2, ny1);
+  }
+  __syncthreads();
+  if (t >= 2) return;
+  const size_t s = 2 * (size_t)nmode, c = 2 * (size_t)ii + t;
+  double y = u[c];
+  us[c] = y;
+  for (int j = 2; j <= ny1; ++j) {
+    y = u[(size_t)(j - 1) * s + c] + C3(a, 1, j) * y;
+    us[(size_t)(j - 1) * s + c] = y;
+  }
+  double x = y * C3(a, 2, ny1);
+  us[(size_t)(ny1 - 1) * s + c] = x;
+  for (int j = ny1 - 1; j >= 1; --j) {
+    x = (us[(size_t)(j - 1) * s + c] - C3(a, 3, j) * x) * C3(a, 2, j);
+    us[(size_t)(j - 1) * s + c] = x;
+  }
+}
+
+void k_soltriy(Box &b, const double *u, double *us, int ii, double *a, int ny1, int nmode) {
+  LAUNCH(b, "soltriy", 1, 64, 0, soltriy_kernel, u, us, ii, a, ny1, nmode);
+}
+
 // ---------------------------------------------------------------------------
 // x-line operators
 // ---------------------------------------------------------------------------
@@ -843,6 +893,17 @@ static void dct_pencils(Box &b, double *P, int iopt) {
 static dim3 tgrid(const Box &b) {
   const int Lp = (b.ny - 1) * b.nz1, N = b.nx - 1;
   return dim3(nblk(Lp, 64), nblk(N, 64));
+}
+
+// cosftx on nlines contiguous lines of nx-1 doubles, in place
+void k_cosft_lines(Box &b, double *P, int nlines, int iopt) {
+  if (b.dct_zfast) {
+    kz_dct_pencils<double>(b, P, iopt, nlines);
+    return;
+  }
+  if (!b.dct_fft) throw std::invalid_argument("cosft on lines needs an FFT-based Poisson x transform");
+  LAUNCH(b, iopt < 0 ? "dct2_rows" : "dct3_rows", (nlines + 1) / 2, TPB, dct_shm(b), dct_rows_kernel, P, P, nlines,
+         iopt, dct_args(b));
 }
 
 // cosftx on a planes-layout v-type field (rows 1..ny-1 of planes 2..nx).

@@ -65,6 +65,9 @@ SIGNATURES = {
     "otbl_implzy": (I, [VP, VP, VP, I, D, I]),
     "otbl_fourxz": (I, [VP, VP, VP, I, I]),
     "otbl_cosftx": (I, [VP, VP, VP, I]),
+    "otbl_cosft_lines": (I, [VP, VP, I, I]),
+    "otbl_soltriy": (I, [VP, VP, VP, I, PD, I]),
+    "otbl_coefinv": (I, [VP, PD, I, I]),
     "otbl_dev_alloc": (I, [SZ, C.POINTER(VP)]),
     "otbl_dev_free": (I, [VP]),
     "otbl_h2d": (I, [VP, VP, SZ]),

@@ -1,6 +1,7 @@
 """The Fortran driver (opentbl_amd/fortran/main.F90) reaches the GPU kernels
 through the ISO_C_BINDING layer with the reference's rhsp / outflow_correction
-/ pois interfaces; two steps of it match the CPU FP64 oracle."""
+/ pois interfaces; two steps of it match the CPU FP64 oracle, with the state
+resident on the device and with the drop-ins mirroring the caller's arrays."""
 import os
 import subprocess
 
@@ -42,7 +43,11 @@ def test_fortran_driver_builds():
 
 
 @pytest.mark.gpu
-def test_fortran_driver_matches_oracle(tmp_path, oracle_lib):
+@pytest.mark.parametrize("mode", ["resident", "mirror"])
+def test_fortran_driver_matches_oracle(tmp_path, oracle_lib, mode):
+    """resident: the state stays on the device between the drop-in calls;
+    mirror: the drop-ins copy the fields in and out of the driver's arrays
+    at every call, as the reference's callees update them in place."""
     drv = build.DRIVER
     if not os.path.exists(drv):
         pytest.skip("Fortran driver not built (no Fortran compiler where the build ran)")
@@ -50,7 +55,8 @@ def test_fortran_driver_matches_oracle(tmp_path, oracle_lib):
     st = ic.make_state(cfg)
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     write_input(fin, cfg, st)
-    r = subprocess.run([drv, fin, fout, "2"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([drv, fin, fout, "2"] + (["mirror"] if mode == "mirror" else []), capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     t, dt, out = read_output(fout, cfg)
     ob = oracle_lib.OracleBox(cfg, st)
