@@ -59,6 +59,8 @@ def lib():
         L.oc_r2c.argtypes = [C.c_void_p, PD, PD]
         L.oc_cosft.argtypes = [I, I, PD, I]
         L.oc_retheta_cf.argtypes = [C.c_void_p, PD, PD]
+        L.oc_max_threads.restype = I
+        L.oc_max_threads.argtypes = []
         _LIB = L
     return _LIB
 

@@ -13,11 +13,20 @@
  *                                projection; mtr == div(grad) identity
  *   outflow_correction           net flux exactly zero afterwards
  *   boun                         boundary rows satisfy their defining identities
- *   genflu (dobleu, profile)     identity case gamma=1 reproduces the blended
- *                                recycled plane; remaining rescaling paths are
- *                                parity unpinned (no reference fixture exists)
- *   rhsp composition             parity unpinned beyond the operator pins and
- *                                the single-mode viscous-decay check
+ *   genflu (dobleu, profile)     gamma ~ 0.5: a profile linear in the rescaled
+ *                                coordinate comes back as the closed form with
+ *                                the dobleu weight integral by quadrature;
+ *                                gamma_1 = gamma_2^(1/8); mode 0 = u0, v0
+ *   difvisxx                     exact on cubics (convective) and quartics
+ *                                (viscous) with the boundary closures; outflow
+ *                                row U_inf dq/dx
+ *   implzy                       dense solve of (A + rkk B) u = A wki; inverse
+ *                                of I + rkk D_yy applied by vistyy
+ *   rhsp composition             manufactured solution: the rhs converges to
+ *                                d(u_j u_i)/dx_j - nu lap u_i at second order
+ *                                under grid doubling; one RK3 step multiplies
+ *                                a z-viscous mode by 1 + z + z^2/2 + z^3/6
+ *   Re_theta(x), Cf(x)           inlet values of inlet_retheta; ramp closed forms
  */
 #include "otbl_oracle.h"
 #include <math.h>
@@ -1954,3 +1963,5 @@ void oc_retheta_cf(const oc_box *b, double *rth, double *cf) {
   }
   free(U);
 }
+
+int oc_max_threads(void) { return omp_get_max_threads(); }

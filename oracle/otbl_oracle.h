@@ -88,6 +88,8 @@ void oc_r2c(const oc_box *b, const double *phys, double *fou);
 void oc_cosft(int n, int nlines, double *c, int iopt);
 /* Per-x diagnostics from the kz=0 mode of u: Re_theta(x), Cf(x). */
 void oc_retheta_cf(const oc_box *b, double *rth, double *cf);
+/* OpenMP threads the oracle's parallel loops use. */
+int oc_max_threads(void);
 
 #ifdef __cplusplus
 }
