@@ -251,12 +251,72 @@ def roofline(rows, model, workload, precision, hbm_only=False):
     return roof
 
 
+ENSEMBLE = "zpg_ensemble8"
+
+
+def run_ensemble(args):
+    """The uneven eight-box ensemble (nx 256..1024, ny 96..192, nz 64..256;
+    mixed precision unless --precision fp64): K RK3 steps of all boxes
+    through the ensemble (one graph launch per substep for all boxes), then
+    the same K steps with each box stepped alone, one box after another.
+    Returns the JSON record."""
+    from opentbl_amd.lib import Ensemble
+    cfgs = config.ensemble_configs()
+    prec = OTBL_FP64 if args.precision == "fp64" else OTBL_MIXED
+    name = "fp64" if prec == OTBL_FP64 else "mixed"
+    states = [ic.make_state(c) for c in cfgs]
+    boxes = [GpuBox(c, s, precision=prec, dt_mode=OTBL_DT_EVERY_STEP, graphs=True) for c, s in zip(cfgs, states)]
+    del states
+    points = sum(c.points for c in cfgs)
+    ens = Ensemble(boxes)
+    for w in range(args.warmup):
+        ens.step(1)
+        ens.sync()
+        for b in boxes:
+            b.step(1)
+            b.sync()
+        log(f"[bench] ensemble warmup step {w + 1}/{args.warmup} done")
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    ens.sync()
+    wall = time.perf_counter() - t0
+    log(f"[bench] ensemble: {args.steps} steps in {wall:.3f} s")
+    t0 = time.perf_counter()
+    for b in boxes:
+        b.step(args.steps)
+    for b in boxes:
+        b.sync()
+    wall_seq = time.perf_counter() - t0
+    log(f"[bench] boxes one after another: {args.steps} steps in {wall_seq:.3f} s")
+    for b in boxes:
+        f = b.get("u")
+        if not np.isfinite(f).all():
+            raise SystemExit("bench.py: non-finite u in the ensemble")
+    ens.close()
+    for b in boxes:
+        b.close()
+    value = points * 3 * args.steps / wall / 1e6
+    return {
+        "metric": METRIC, "value": value, "unit": "Mpts*substep/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if prec == OTBL_MIXED else "f64",
+        "data": "synthetic",
+        "config": {"workload": "uneven 8-box ZPG ensemble (nx 256..1024, ny 96..192, nz 64..256, "
+                               f"Re_theta_in 300..650), {name} substep, one hipGraph per substep for all boxes",
+                   "boxes": [f"{c.nx}x{c.ny}x{c.nz}" for c in cfgs], "points": points,
+                   "one_after_another_value": points * 3 * args.steps / wall_seq / 1e6,
+                   "ensemble_over_one_after_another": wall_seq / wall},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="zpg1536x315x1024", choices=sorted(config.CONFIGS))
+    ap.add_argument("--workload", default="zpg1536x315x1024", choices=sorted(config.CONFIGS) + [ENSEMBLE])
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
@@ -266,6 +326,9 @@ def main():
         print(f"bench.py: this build runs one box on one GPU; --gpus {args.gpus} is not "
               "supported (the x-slab multi-GPU layout is design only)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == ENSEMBLE:
+        print(json.dumps(run_ensemble(args)), flush=True)
+        return
     cfg = config.CONFIGS[args.workload]
     log(f"[bench] workload {cfg.name} {cfg.nx}x{cfg.ny}x{cfg.nz}: generating the state")
     t0 = time.perf_counter()

@@ -42,10 +42,11 @@ typedef struct otbl_params {
                                 arithmetic with FP64 statistics, CFL, mass
                                 flux, coefficient factorisation, mean-flow
                                 (kz = 0) pressure solve and Poisson residuals;
-                                needs the register z engine (nz 256..4096),
-                                a register-engine Bluestein length
-                                (nx-1 >= 129) and the chunked y solver
-                                (ny <= 384).  The mean-flow (kz = 0) u and p
+                                z transforms on the register engine
+                                (nz 256..4096) or the LDS Stockham kernels
+                                (other nz), needs a register-engine
+                                Bluestein length (nx-1 >= 129) and the
+                                chunked y solver (ny <= 384).  The mean-flow (kz = 0) u and p
                                 are kept in FP64 on the device.  Fields
                                 cross otbl_set_field / otbl_get_field as FP64
                                 either way; the operator entry points
@@ -165,6 +166,25 @@ int otbl_soltriy(otbl_ctx *ctx, const double *u, double *us, int ii, double *a, 
 /* coefinv (coeft.f90:1338-1353): in-place LU of the caller's host array
  * a(3, npter1:neq), computed on the GPU. */
 int otbl_coefinv(otbl_ctx *ctx, double *a, int npter1, int neq);
+
+/* ---- ensemble of independent boxes ----
+ * No counterpart in the reference (one OpenTBL run advances one box): the
+ * uneven-box ensemble whose boxes are advanced together to accumulate
+ * converged statistics.  Each RK3 substep of all the boxes runs as one
+ * captured hipGraph (one graph launch per substep for the whole ensemble)
+ * whose independent branches are the boxes' substeps, forked from and
+ * joined into the ensemble's own stream, so the kernels of every box overlap
+ * on the GPU.  The boxes stay owned by the caller, keep their own state,
+ * precision, dt and time, and give the same results as otbl_step on each of
+ * them; they may still be stepped or read alone between ensemble steps
+ * (their streams are ordered after the ensemble's work).  Destroy the
+ * ensemble before any of its boxes.  At most 64 boxes. */
+typedef struct otbl_ensemble otbl_ensemble;
+int otbl_ensemble_create(otbl_ctx *const *boxes, int nbox, otbl_ensemble **out);
+/* nsteps whole RK3 steps of every box (3 substeps, then tiempo += dt) */
+int otbl_ensemble_step(otbl_ensemble *e, int nsteps);
+int otbl_ensemble_sync(otbl_ensemble *e);
+int otbl_ensemble_destroy(otbl_ensemble *e);
 
 /* ---- device memory helpers (plain pointers, no framework types) ---- */
 int otbl_dev_alloc(size_t bytes, void **ptr);

@@ -415,9 +415,9 @@ static size_t mean_slot(const Box &b, int id, size_t r) {
 // FP32 copies of the factored FP64 coefficients and tables, the Poisson
 // pivots and the FP64 mean-flow pencils of the mixed-precision substep.
 static void setup_mixed(Box &b) {
-  need(b.yL && b.zfast && b.dct_zfast && b.pchunk,
-       "OTBL_MIXED needs the chunked y solver (ny <= 24*16), a z length of 256..4096 and a power-of-two "
-       "Bluestein length of 512..8192 (nx-1)");
+  need(b.yL && b.dct_zfast && b.pchunk,
+       "OTBL_MIXED needs the chunked y solver (ny <= 24*16) and a Poisson x transform on the register engine "
+       "(nx - 1 from 129 to 4096)");
   need(b.xwin, "OTBL_MIXED needs x lines that the windowed sweep solves (nx <= 32 or decaying x systems)");
   size_t total = 0;
   for (const auto &sl : kSlots) total += (b.hc.*(sl.vec)).size() + 8;
@@ -715,6 +715,7 @@ int otbl_get_ener(otbl_ctx *ctx, double *e) {
 
 // graphs bake kernel arguments in: drop them when the launch set changes
 static void drop_graphs(Box &b) {
+  ++b.graph_gen;
   for (int q = 0; q < 4; ++q)
     for (int f = 0; f < 2; ++f)
       if (b.gexec[q][f]) {
@@ -1086,6 +1087,183 @@ int otbl_kprof_report(otbl_ctx *ctx, char *buf, size_t len) {
   need(buf && len > s.size(), "otbl_kprof_report: buffer too small");
   memcpy(buf, s.c_str(), s.size() + 1);
   API_END
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensemble of independent boxes: one captured graph per substep whose
+// branches are the boxes' substeps (forked from and joined into the
+// ensemble's own stream), so the kernels of every box overlap on the GPU.
+// ---------------------------------------------------------------------------
+struct otbl_ensemble {
+  std::vector<Box *> boxes;
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join;
+  // graph per (substep m, buffer-role bits of the boxes), captured with the
+  // boxes' graph generations in gens
+  std::map<std::pair<int, unsigned long long>, hipGraphExec_t> graphs;
+  std::vector<unsigned> gens;
+};
+
+namespace {
+
+void ens_drop(otbl_ensemble &e) {
+  for (auto &kv : e.graphs) (void)hipGraphExecDestroy(kv.second);
+  e.graphs.clear();
+}
+
+void ens_free(otbl_ensemble *e) {
+  if (!e) return;
+  if (e->s) (void)hipStreamSynchronize(e->s);
+  ens_drop(*e);
+  for (hipEvent_t ev : e->join)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->done) (void)hipEventDestroy(e->done);
+  if (e->s) (void)hipStreamDestroy(e->s);
+  delete e;
+}
+
+// the ensemble's stream after every box's earlier work
+void ens_after_boxes(otbl_ensemble &e) {
+  for (size_t q = 0; q < e.boxes.size(); ++q) {
+    OTBL_HIP(hipEventRecord(e.join[q], e.boxes[q]->s));
+    OTBL_HIP(hipStreamWaitEvent(e.s, e.join[q], 0));
+  }
+}
+
+// every box's later work after the ensemble's stream
+void ens_before_boxes(otbl_ensemble &e) {
+  OTBL_HIP(hipEventRecord(e.done, e.s));
+  for (Box *b : e.boxes) OTBL_HIP(hipStreamWaitEvent(b->s, e.done, 0));
+}
+
+// Substep m of every box (with the end-of-step time update after m = 3).
+void ens_substep(otbl_ensemble &e, int m) {
+  bool eager = false;
+  unsigned long long roles = 0;
+  for (size_t q = 0; q < e.boxes.size(); ++q) {
+    eager = eager || e.boxes[q]->kprof;
+    roles |= (unsigned long long)e.boxes[q]->flip << q;
+  }
+  if (eager) { // measurement: kernels one by one on each box's own stream
+    for (Box *b : e.boxes) {
+      enqueue_substep(*b, m);
+      k_swap_saved(*b);
+      if (m == 3) k_end_step(*b);
+    }
+    return;
+  }
+  for (size_t q = 0; q < e.boxes.size(); ++q)
+    if (e.gens[q] != e.boxes[q]->graph_gen) {
+      ens_drop(e);
+      for (size_t r = 0; r < e.boxes.size(); ++r) e.gens[r] = e.boxes[r]->graph_gen;
+      break;
+    }
+  hipGraphExec_t &ge = e.graphs[{m, roles}];
+  if (!ge) {
+    hipGraph_t g = nullptr;
+    size_t done = 0;
+    OTBL_HIP(hipStreamBeginCapture(e.s, hipStreamCaptureModeThreadLocal));
+    try {
+      OTBL_HIP(hipEventRecord(e.fork, e.s));
+      for (; done < e.boxes.size(); ++done) {
+        Box &b = *e.boxes[done];
+        OTBL_HIP(hipStreamWaitEvent(b.s, e.fork, 0));
+        enqueue_substep(b, m);
+        if (m == 3) k_end_step(b);
+        OTBL_HIP(hipEventRecord(e.join[done], b.s));
+        OTBL_HIP(hipStreamWaitEvent(e.s, e.join[done], 0));
+      }
+    } catch (...) {
+      if (hipStreamEndCapture(e.s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      // capture ran the host side only: undo the role swaps already made
+      for (size_t q = 0; q < e.boxes.size(); ++q)
+        if (((roles >> q) & 1ull) != (unsigned long long)e.boxes[q]->flip) k_swap_saved(*e.boxes[q]);
+      e.graphs.erase({m, roles});
+      throw;
+    }
+    OTBL_HIP(hipStreamEndCapture(e.s, &g));
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    for (size_t q = 0; q < e.boxes.size(); ++q)
+      if (((roles >> q) & 1ull) != (unsigned long long)e.boxes[q]->flip) k_swap_saved(*e.boxes[q]);
+    if (ie != hipSuccess) {
+      e.graphs.erase({m, roles});
+      fail_hip(ie, "hipGraphInstantiate (ensemble substep)");
+    }
+  }
+  OTBL_HIP(hipGraphLaunch(ge, e.s));
+  for (Box *b : e.boxes) k_swap_saved(*b);
+}
+
+} // namespace
+
+extern "C" {
+
+int otbl_ensemble_create(otbl_ctx *const *boxes, int nbox, otbl_ensemble **out) {
+  otbl_ensemble *e = nullptr;
+  try {
+  need(out != nullptr && boxes != nullptr && nbox >= 1, "otbl_ensemble_create: bad arguments");
+  need(nbox <= 64, "an ensemble holds at most 64 boxes");
+  *out = nullptr;
+  e = new otbl_ensemble();
+  for (int q = 0; q < nbox; ++q) {
+    Box &b = box_of(boxes[q]);
+    for (Box *o : e->boxes) need(o != &b, "otbl_ensemble_create: a box appears twice");
+    e->boxes.push_back(&b);
+    e->gens.push_back(b.graph_gen);
+  }
+  OTBL_HIP(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
+  OTBL_HIP(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+  OTBL_HIP(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
+  e->join.assign(nbox, nullptr);
+  for (int q = 0; q < nbox; ++q) OTBL_HIP(hipEventCreateWithFlags(&e->join[q], hipEventDisableTiming));
+  *out = e;
+  return OTBL_OK;
+  } catch (const std::invalid_argument &x) {
+    ens_free(e);
+    g_err = x.what();
+    return OTBL_EINVAL;
+  } catch (const std::exception &x) {
+    ens_free(e);
+    g_err = x.what();
+    return OTBL_EHIP;
+  }
+}
+
+int otbl_ensemble_step(otbl_ensemble *e, int nsteps) {
+  API_BEGIN
+  need(e != nullptr, "null otbl_ensemble");
+  need(nsteps >= 0, "nsteps must be >= 0");
+  ens_after_boxes(*e);
+  try {
+    for (int st = 0; st < nsteps; ++st)
+      for (int m = 1; m <= 3; ++m) ens_substep(*e, m);
+  } catch (...) {
+    ens_before_boxes(*e);
+    throw;
+  }
+  ens_before_boxes(*e);
+  API_END
+}
+
+int otbl_ensemble_sync(otbl_ensemble *e) {
+  API_BEGIN
+  need(e != nullptr, "null otbl_ensemble");
+  OTBL_HIP(hipStreamSynchronize(e->s));
+  for (Box *b : e->boxes) {
+    OTBL_HIP(hipStreamSynchronize(b->s));
+    drain_kprof(*b);
+  }
+  API_END
+}
+
+int otbl_ensemble_destroy(otbl_ensemble *e) {
+  ens_free(e);
+  return OTBL_OK;
 }
 
 } // extern "C"

@@ -158,6 +158,9 @@ struct Box {
   // fields and their saved copies swap roles at every substep)
   hipGraphExec_t gexec[4][2] = {};
   int flip = 0;
+  // bumped whenever the captured substep would change (diagnostics on or
+  // off), so that graphs captured elsewhere (an ensemble's) are captured again
+  unsigned graph_gen = 0;
   // measurement
   bool kprof = false;
   struct KStat { int n = 0; double ms = 0.0; };

@@ -537,35 +537,49 @@ static constexpr int TW_SLOTS = 64 + 128;
 // Hermitian-extended spectrum of two real signals packed as z = x + i y,
 // x = c2r(X), y = c2r(Y).  X, Y rows hold modes 0..nz2 (re, im pairs); Im of
 // mode 0 does not enter a real inverse transform.  Every z entry is written.
-__device__ inline void pack2(double2 *z, const double *X, const double *Y, int n, int nz2) {
+template <typename R>
+__device__ inline void pack2(typename CT<R>::c *z, const R *X, const R *Y, int n, int nz2) {
+  using C = typename CT<R>::c;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     int kk = k <= nz2 ? k : (k >= n - nz2 ? n - k : -1);
-    double2 v = make_double2(0.0, 0.0);
+    C v = mkc<C>(0.0, 0.0);
     if (kk >= 0) {
-      double xr = X ? X[2 * kk] : 0.0, xi = X ? X[2 * kk + 1] : 0.0;
-      double yr = Y ? Y[2 * kk] : 0.0, yi = Y ? Y[2 * kk + 1] : 0.0;
-      if (k == 0) v = make_double2(xr, yr);
-      else if (k == kk) v = make_double2(xr - yi, xi + yr);
-      else v = make_double2(xr + yi, -xi + yr);
+      R xr = X ? X[2 * kk] : R(0), xi = X ? X[2 * kk + 1] : R(0);
+      R yr = Y ? Y[2 * kk] : R(0), yi = Y ? Y[2 * kk + 1] : R(0);
+      if (k == 0) v = mkc<C>(xr, yr);
+      else if (k == kk) v = mkc<C>(xr - yi, xi + yr);
+      else v = mkc<C>(xr + yi, -xi + yr);
     }
     z[k] = v;
   }
 }
 
 // Split the transform Z of z = a + i b into the spectra of a and b, modes
-// 0..nz2, scaled by 1/n (the r2c scaling of ffts.f90:51).
-__device__ inline void unpack2(const double2 *Z, double *A, double *B, int n, int nz2) {
-  const double sc = 1.0 / n;
+// 0..nz2, scaled by 1/n (the r2c scaling of ffts.f90:51).  With XA, A also
+// receives the mean-flow terms of a product whose mean was left out of the
+// transform: A_k += ck XA_k for k >= 1, A_0 += c0 XA_0, XA read before A is
+// written (XA may be A itself: each k is read and written by one thread).
+template <typename R>
+__device__ inline void unpack2(const typename CT<R>::c *Z, R *A, R *B, int n, int nz2, const R *XA = nullptr,
+                               R c0 = R(0), R ck = R(0)) {
+  using C = typename CT<R>::c;
+  const R sc = R(1.0 / n);
   for (int k = threadIdx.x; k <= nz2; k += blockDim.x) {
-    double2 z = Z[k];
-    double2 w = Z[k == 0 ? 0 : n - k];
+    C z = Z[k];
+    C w = Z[k == 0 ? 0 : n - k];
     if (A) {
-      A[2 * k] = 0.5 * (z.x + w.x) * sc;
-      A[2 * k + 1] = (k == 0) ? 0.0 : 0.5 * (z.y - w.y) * sc;
+      R ar = R(0.5) * (z.x + w.x) * sc, ai = (k == 0) ? R(0) : R(0.5) * (z.y - w.y) * sc;
+      if (XA) {
+        const R xr = XA[2 * k], xi = XA[2 * k + 1];
+        ar += (k == 0 ? c0 : ck) * xr;
+        ai += (k == 0) ? R(0) : ck * xi;
+      }
+      A[2 * k] = ar;
+      A[2 * k + 1] = ai;
     }
     if (B) {
-      B[2 * k] = 0.5 * (z.y + w.y) * sc;
-      B[2 * k + 1] = (k == 0) ? 0.0 : -0.5 * (z.x - w.x) * sc;
+      B[2 * k] = R(0.5) * (z.y + w.y) * sc;
+      B[2 * k + 1] = (k == 0) ? R(0) : R(-0.5) * (z.x - w.x) * sc;
     }
   }
 }
@@ -588,61 +602,78 @@ __device__ inline void block_atomic_max(double v, unsigned long long *dst) {
 // Plane loop 1 of rhsp (rhsp.F90:120-153) for one (j, i) row: with u, u_x,
 // w_x, v_x and u interpolated to v rows (uy) in spectral space, form
 // r2c(u_x^2) -> wki1, r2c(u w_x) -> wki3, r2c(v_x uy) -> wki2 (rows <= ny),
-// and the maxima |u|, |w_x|, |v_x|(j) for the CFL time step.
+// and the maxima |u|, |w_x|, |v_x|(j) for the CFL time step.  In FP32 the
+// mean flow (mode 0) of u_x and uy stays out of the transforms and its
+// products are added in spectral space (u_x^2 = U^2 + 2 U u_x' + u_x'^2,
+// v_x uy = Uy v_x + v_x uy'), as in the register-resident products1_z.
+template <typename R>
 __global__ void __launch_bounds__(TPB)
-products1_kernel(const double *__restrict__ u, double *wki1, double *wki2, double *wki3,
-                 const double *__restrict__ uy, FFTPlan plan, int ny, int nz1, int nz2,
-                 DevScalars *ds, unsigned long long *vmbits) {
-  extern __shared__ double2 sh[];
+products1_kernel(const R *__restrict__ u, R *wki1, R *wki2, R *wki3, const R *__restrict__ uy, FFTPlan plan, int ny,
+                 int nz1, int nz2, DevScalars *ds, unsigned long long *vmbits) {
+  using C = typename CT<R>::c;
+  constexpr bool SPLIT = sizeof(R) == 4;
+  extern __shared__ double2 sh_raw[];
+  C *sh = reinterpret_cast<C *>(sh_raw);
   const int n = plan.n;
-  double2 *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n;
+  C *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n;
   const int j = blockIdx.x + 1, i = blockIdx.y + 1;
   const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
   const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
   const bool vrow = j <= ny;
+  const R Ux = SPLIT ? wki1[ou] : R(0), Uy = (SPLIT && vrow) ? uy[ou] : R(0);
   fill_twiddles(lo, hi, n);
   pack2(Q0, u + ou, wki1 + ou, n, nz2);
   pack2(Q0 + n, wki3 + ou, vrow ? wki2 + ov : nullptr, n, nz2);
+  if (SPLIT && threadIdx.x == 0) Q0[0].y = R(0); // the mean of u_x (slot 0 was packed by this thread)
   __syncthreads();
   const bool inA = fft_multi<+1>(Q0, Q1, n, 2, plan, lo, hi);
-  double2 *R = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
-  const double2 *UY = nullptr;
+  C *Rb = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
+  const C *UY = nullptr;
   if (vrow) {
-    pack2(F, uy + ou, nullptr, n, nz2);
+    pack2(F, uy + ou, (const R *)nullptr, n, nz2);
+    if (SPLIT && threadIdx.x == 0) F[0].x = R(0); // the mean of uy (slot 0 was packed by this thread)
     __syncthreads();
     UY = fft_multi<+1>(F, F + n, n, 1, plan, lo, hi) ? F : F + n;
   }
   double um = 0.0, wm = 0.0, vm = 0.0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    double2 a = R[q], b2 = R[n + q];
-    double uu = a.x, ux = a.y, wx = b2.x, vx = b2.y;
-    um = fmax(um, fabs(uu));
-    wm = fmax(wm, fabs(wx));
-    R[q] = make_double2(ux * ux, uu * wx);
+    C a = Rb[q], b2 = Rb[n + q];
+    R uu = a.x, ux = a.y, wx = b2.x, vx = b2.y;
+    um = fmax(um, (double)fabs(uu));
+    wm = fmax(wm, (double)fabs(wx));
+    Rb[q] = mkc<C>(ux * ux, uu * wx);
     if (vrow) {
-      vm = fmax(vm, fabs(vx));
-      R[n + q] = make_double2(vx * UY[q].x, 0.0);
+      vm = fmax(vm, (double)fabs(vx));
+      Rb[n + q] = mkc<C>(vx * UY[q].x, R(0));
     }
   }
   __syncthreads();
   block_atomic_max(um, &ds->umax_bits);
   block_atomic_max(wm, &ds->wmax_bits);
   if (vrow) block_atomic_max(vm, &vmbits[j - 1]);
-  const double2 *O = fft_multi<-1>(R, F, n, vrow ? 2 : 1, plan, lo, hi) ? R : F;
-  unpack2(O, wki1 + ou, wki3 + ou, n, nz2);
-  if (vrow) unpack2(O + n, wki2 + ov, nullptr, n, nz2);
+  const C *O = fft_multi<-1>(Rb, F, n, vrow ? 2 : 1, plan, lo, hi) ? Rb : F;
+  if (SPLIT) {
+    unpack2(O, wki1 + ou, wki3 + ou, n, nz2, (const R *)(wki1 + ou), Ux, R(2) * Ux);
+    if (vrow) unpack2(O + n, wki2 + ov, (R *)nullptr, n, nz2, (const R *)(wki2 + ov), Uy, Uy);
+  } else {
+    unpack2(O, wki1 + ou, wki3 + ou, n, nz2);
+    if (vrow) unpack2(O + n, wki2 + ov, (R *)nullptr, n, nz2);
+  }
 }
 
 // Plane loop 2 of rhsp (rhsp.F90:367-415) for one (j, i) row, j <= ny:
 // vw = r2c(v * w_v) -> ya (w interpolated to v rows is read from ya),
 // vv = r2c(v_u^2) -> yb (v interpolated to u rows is read from yb),
 // ww = r2c(w^2) -> yc.
+template <typename R>
 __global__ void __launch_bounds__(TPB)
-products2_kernel(const double *__restrict__ resv, const double *__restrict__ resw, double *ya,
-                 double *yb, double *yc, FFTPlan plan, int ny, int nz1, int nz2) {
-  extern __shared__ double2 sh[];
+products2_kernel(const R *__restrict__ resv, const R *__restrict__ resw, R *ya, R *yb, R *yc, FFTPlan plan, int ny,
+                 int nz1, int nz2) {
+  using C = typename CT<R>::c;
+  extern __shared__ double2 sh_raw[];
+  C *sh = reinterpret_cast<C *>(sh_raw);
   const int n = plan.n;
-  double2 *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n;
+  C *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n;
   const int j = blockIdx.x + 1, i = blockIdx.y + 2;
   const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
   const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
@@ -651,16 +682,16 @@ products2_kernel(const double *__restrict__ resv, const double *__restrict__ res
   pack2(Q0 + n, yb + ou, resw + ou, n, nz2);
   __syncthreads();
   const bool inA = fft_multi<+1>(Q0, Q1, n, 2, plan, lo, hi);
-  double2 *R = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
+  C *Rb = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    double2 a = R[q], b2 = R[n + q];
-    R[q] = make_double2(a.x * a.y, b2.x * b2.x);
-    R[n + q] = make_double2(b2.y * b2.y, 0.0);
+    C a = Rb[q], b2 = Rb[n + q];
+    Rb[q] = mkc<C>(a.x * a.y, b2.x * b2.x);
+    Rb[n + q] = mkc<C>(b2.y * b2.y, R(0));
   }
   __syncthreads();
-  const double2 *O = fft_multi<-1>(R, F, n, 2, plan, lo, hi) ? R : F;
+  const C *O = fft_multi<-1>(Rb, F, n, 2, plan, lo, hi) ? Rb : F;
   unpack2(O, ya + ou, yb + ou, n, nz2);
-  unpack2(O + n, yc + ou, nullptr, n, nz2);
+  unpack2(O + n, yc + ou, (R *)nullptr, n, nz2);
 }
 
 // fourxz (ffts.f90:27-67) on independent rows, for the operator-level API.
@@ -674,7 +705,7 @@ fourxz_kernel(double *fou, double *phys, int iopt, FFTPlan plan, int nz1, int nz
   double *Ph = phys + r * n;
   fill_twiddles(lo, hi, n);
   if (iopt >= 0) {
-    pack2(A, F, nullptr, n, nz2);
+    pack2(A, F, (const double *)nullptr, n, nz2);
     __syncthreads();
     const double2 *R = fft_multi<+1>(A, T, n, 1, plan, lo, hi) ? A : T;
     for (int q = threadIdx.x; q < n; q += blockDim.x) Ph[q] = R[q].x;
@@ -682,7 +713,7 @@ fourxz_kernel(double *fou, double *phys, int iopt, FFTPlan plan, int nz1, int nz
     for (int q = threadIdx.x; q < n; q += blockDim.x) A[q] = make_double2(Ph[q], 0.0);
     __syncthreads();
     const double2 *R = fft_multi<-1>(A, T, n, 1, plan, lo, hi) ? A : T;
-    unpack2(R, F, nullptr, n, nz2);
+    unpack2(R, F, (double *)nullptr, n, nz2);
   }
 }
 
@@ -2102,8 +2133,8 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   // plane loop 1 products
   if (b.zfast)
     kz_products1<T>(b);
-  else if constexpr (F64)
-    LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel, u, wki1, wki2, wki3, yc, plan, ny, nz1,
+  else
+    LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel<T>, u, wki1, wki2, wki3, yc, plan, ny, nz1,
            b.nz2, b.ds, b.vmbits);
   if (b.yL)
     ky_rhsu_loop1<T>(b, wki2, wki3, rhsu);
@@ -2164,8 +2195,8 @@ template <typename T> static void rhsp_t(Box &b, int m) {
     }
     if (b.zfast)
       kz_products2<T>(b);
-    else if constexpr (F64)
-      LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel, resv, resw, ya, yb, yc, plan, ny, nz1,
+    else
+      LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel<T>, resv, resw, ya, yb, yc, plan, ny, nz1,
              b.nz2);
     if (b.yL)
       ky_rhs_loop2<T>(b, rhsu, rhsv, rhsw, resu, resv, resw, ya, yb, yc);
@@ -2207,8 +2238,10 @@ void k_substep_rhsp(Box &b, int m) {
 
 void k_setup_attrs(Box &b) {
   const int shm = (int)zfft_shm(b, 4);
-  OTBL_HIP(hipFuncSetAttribute((const void *)products1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-  OTBL_HIP(hipFuncSetAttribute((const void *)products2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  OTBL_HIP(hipFuncSetAttribute((const void *)products1_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  OTBL_HIP(hipFuncSetAttribute((const void *)products2_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  OTBL_HIP(hipFuncSetAttribute((const void *)products1_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  OTBL_HIP(hipFuncSetAttribute((const void *)products2_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
   OTBL_HIP(hipFuncSetAttribute((const void *)fourxz_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)zfft_shm(b, 2)));
   if (b.dct_fft && !b.dct_zfast)

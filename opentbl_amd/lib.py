@@ -68,6 +68,10 @@ SIGNATURES = {
     "otbl_cosft_lines": (I, [VP, VP, I, I]),
     "otbl_soltriy": (I, [VP, VP, VP, I, PD, I]),
     "otbl_coefinv": (I, [VP, PD, I, I]),
+    "otbl_ensemble_create": (I, [C.POINTER(VP), I, C.POINTER(VP)]),
+    "otbl_ensemble_step": (I, [VP, I]),
+    "otbl_ensemble_sync": (I, [VP]),
+    "otbl_ensemble_destroy": (I, [VP]),
     "otbl_dev_alloc": (I, [SZ, C.POINTER(VP)]),
     "otbl_dev_free": (I, [VP]),
     "otbl_h2d": (I, [VP, VP, SZ]),
@@ -281,3 +285,33 @@ class GpuBox:
 
     def op(self, name, *args):
         check(getattr(lib(), "otbl_" + name)(self.h, *args), name)
+
+
+class Ensemble:
+    """Independent boxes advanced together: each RK3 substep of all of them
+    is one captured graph whose branches are the boxes' substeps.  The boxes
+    (GpuBox objects) stay usable on their own; close the ensemble first."""
+
+    def __init__(self, boxes):
+        self.boxes = list(boxes)
+        arr = (VP * len(self.boxes))(*[b.h for b in self.boxes])
+        h = VP()
+        check(lib().otbl_ensemble_create(arr, len(self.boxes), C.byref(h)), "ensemble_create")
+        self.h = h
+
+    def step(self, n=1):
+        check(lib().otbl_ensemble_step(self.h, n), "ensemble_step")
+
+    def sync(self):
+        check(lib().otbl_ensemble_sync(self.h), "ensemble_sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().otbl_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

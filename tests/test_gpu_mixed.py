@@ -31,9 +31,13 @@ P_TOL = 1e-5
 
 
 # nz = 1024 (radix 16-16-4) and 2048 (a two-wave team) are the z lengths of
-# the 1536 x 315 x 1024 and 3072 x 360 x 2048 boxes
-@pytest.fixture(scope="module", params=[(130, 256), (257, 512), (130, 1024), (130, 2048)],
-                ids=["x130z256", "x257z512", "x130z1024", "x130z2048"])
+# the 1536 x 315 x 1024 and 3072 x 360 x 2048 boxes; nz = 64, 96, 160, 224
+# (factors 2, 3, 5, 7) run on the LDS Stockham transforms, the z lengths of
+# the ensemble's small boxes
+@pytest.fixture(scope="module", params=[(130, 256), (257, 512), (130, 1024), (130, 2048), (130, 64), (130, 96),
+                                        (130, 160), (130, 224)],
+                ids=["x130z256", "x257z512", "x130z1024", "x130z2048", "x130z64", "x130z96", "x130z160",
+                     "x130z224"])
 def mbox(request):
     nx, nz = request.param
     cfg = config.small(name=f"mix{nx}", nx=nx, ny=40, nz=nz, Lx=8.0 * nx / 32, kzmax=min(40, nz // 3 - 1))
@@ -63,7 +67,8 @@ def test_mixed_paths(mbox):
     gm = GpuBox(cfg, st, precision=OTBL_MIXED)
     paths = gm.paths()
     gm.close()
-    assert paths["y"].startswith("chunk") and paths["z"].startswith("engine"), paths
+    zpath = "engine" if cfg.nz >= 256 else "stockham"
+    assert paths["y"].startswith("chunk") and paths["z"].startswith(zpath), paths
     assert paths["dct"].startswith("engine") and paths["x"] == "window", paths
 
 
