@@ -98,6 +98,27 @@ def build_fortran():
     return DRIVER
 
 
+IOLIB = os.path.join(HERE, "libopentbl_io.so")
+HDF5_PREFIX = os.environ.get("OTBL_HDF5_PREFIX", "/opt/conda")
+
+
+def build_io():
+    """The restart-file library (host code over the HDF5 C library of
+    HDF5_PREFIX, with its high-level H5LT interface).  Returns its path, or
+    None where no HDF5 headers are installed."""
+    inc, libdir = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", os.path.join(CSRC, "otbl_io.cpp"),
+           "-I", os.path.join(ROOT, "include"), "-I", inc, "-L", libdir, "-lhdf5_hl", "-lhdf5",
+           f"-Wl,-rpath,{libdir}", "-o", IOLIB]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"io build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return IOLIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_fortran())
+    print(build_io())
