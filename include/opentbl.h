@@ -197,9 +197,24 @@ int otbl_d2d(void *dst, const void *src, size_t bytes);
 
 /* Kernel paths the context chose at setup, as text: "y:chunk<L>|march0
  * z:engine<nz>|stockham<nz> dct:engine<M>|stockham<M>|gemm<M> x:window|march
- * xseg:<threads per x line> pois:chunk<L>|march0"
+ * xseg:<threads per x line> pois:chunk<L>|march0 prun:<planes per thread>
+ * tuned:0|1"
  * (no counterpart in the reference; for tests and benchmark records). */
 int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len);
+
+/* Autotuner (no counterpart in the reference): times, on this box's sizes,
+ * the candidate launch shapes of its kernels and keeps the fastest: the
+ * segments per x line of the windowed x solves (1 up to the most the x
+ * systems' decay allows), the planes per thread of the element-wise
+ * predictor and projection (1, 2, 4, 8), and for the mixed substep the
+ * half-wave or full-wave chunks of rk_final.  Each candidate runs `reps`
+ * times (after one untimed run) on the box's work fields, which hold nothing
+ * between substeps, so the state is unchanged.  The plane run changes no
+ * arithmetic; the x segments and the y chunk shapes change only round-off
+ * (a segment's start error is below 1e-17, checked at setup).  report (may
+ * be null) receives the timings; otbl_get_paths shows the choices
+ * ("tuned:1").  Captured graphs are dropped and captured again. */
+int otbl_autotune(otbl_ctx *ctx, int reps, char *report, size_t len);
 
 /* ---- measurement ----
  * With kprof on, every kernel launch is bracketed by HIP events on the

@@ -565,6 +565,11 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     if (const char *e = std::getenv("OTBL_XSEG"))
       nseg = std::max(1, std::min(std::atoi(e), (b->nx + XLINE_LEAD - 1) / XLINE_LEAD));
     b->xseg = (b->xwin && lead < 1e-17) ? nseg : 1;
+    // the most segments the autotuner may try: each at least four lead-ins
+    // and backward windows long
+    int smax = 1;
+    while (smax < 8 && b->nx / (smax + 1) >= 4 * (XLINE_LEAD + XLINE_WINDOW)) ++smax;
+    b->xseg_max = (b->xwin && lead < 1e-17) ? std::max(smax, b->xseg) : 1;
   }
   // chunked y-line solver unless the columns are longer than its
   // instantiations (or OTBL_Y_MARCH asks for one thread per column)
@@ -1058,10 +1063,24 @@ int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len) {
   Box &b = box_of(ctx);
   need(buf && len > 0, "otbl_get_paths: null buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof tmp, "y:%s%d%s z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d", b.yL ? "chunk" : "march", b.yL,
-                b.ytabh_f ? "h" : "", b.zfast ? "engine" : "stockham", b.nz, b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"),
-                b.dct_M, b.xwin ? "window" : "march", b.xseg, b.pchunk ? "chunk" : "march", b.pchunk);
+  std::snprintf(tmp, sizeof tmp, "y:%s%d%s z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d prun:%d tuned:%d",
+                b.yL ? "chunk" : "march", b.yL, (b.ytabh_f && b.yhalf) ? "h" : "", b.zfast ? "engine" : "stockham", b.nz,
+                b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"), b.dct_M, b.xwin ? "window" : "march", b.xseg,
+                b.pchunk ? "chunk" : "march", b.pchunk, b.prun > 0 ? b.prun : (b.esz == 4 ? 4 : 2), b.tuned ? 1 : 0);
   std::snprintf(buf, len, "%s", tmp);
+  API_END
+}
+
+int otbl_autotune(otbl_ctx *ctx, int reps, char *report, size_t len) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(reps >= 1 && reps <= 1000, "otbl_autotune: reps must be 1..1000");
+  need(!b.kprof, "otbl_autotune: turn kprof off first");
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  const std::string rep = k_autotune(b, reps);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  drop_graphs(b); // captured graphs hold the old launch shapes
+  if (report && len > 0) std::snprintf(report, len, "%s", rep.c_str());
   API_END
 }
 

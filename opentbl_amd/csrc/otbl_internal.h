@@ -119,6 +119,15 @@ struct Box {
   // when every forward multiplier product over XLINE_LEAD planes is below
   // round-off as well
   int xseg = 1;
+  // largest segment count the x systems allow (1: segments off)
+  int xseg_max = 1;
+  // plane run of the element-wise predictor / projection kernels (0: the
+  // default for the element size)
+  int prun = 0;
+  // the FP32 rk_final on half-wave chunks when its tables exist
+  bool yhalf = true;
+  // set by otbl_autotune: the choices above were timed on this box
+  bool tuned = false;
   // chunked y-line solver: rows per chunk (0: one thread marches each
   // column) and the tables of its eight systems
   int yL = 0;
@@ -197,6 +206,9 @@ void k_end_step(Box &b);
 // Re_theta(x), Cf(x) of the kz = 0 mode of u into out[0..nx), out[nx..2nx)
 void k_retheta_cf(Box &b, double *out);
 void k_swap_saved(Box &b);
+// times the candidates of xseg, prun and yhalf on the box's work fields and
+// keeps the fastest (otbl_autotune); returns a one-line report
+std::string k_autotune(Box &b, int reps);
 // partitioned Poisson y solves of the modes kz >= 1 on pencils
 // (otbl_poisson.hip); REF: one FP64-residual refinement step
 int kp_chunk_rows(int ny);

@@ -971,6 +971,8 @@ void k_cosftx(Box &b, const double *in, double *out, int iopt) {
 // measured fastest on a 1536 x 316 x 682 box (runs of 1..32 planes timed):
 // 4 planes for FP32 fields, 2 for FP64.
 template <typename T> static inline int prun() { return sizeof(T) == 4 ? 4 : 2; }
+// the box's plane run: the autotuner's choice, else the default
+template <typename T> static inline int prun(const Box &b) { return b.prun > 0 ? b.prun : prun<T>(); }
 __device__ __forceinline__ bool pcol(int nz1, int R, int &kr, int &j) {
   const unsigned el = blockIdx.x * blockDim.x + threadIdx.x;
   if (el >= (unsigned)nz1 * (unsigned)R) return false;
@@ -2010,6 +2012,42 @@ __global__ void project_pair_kernel(float *u, float *v, float *w, float *p, cons
   }
 }
 
+// The save and pressure-gradient predictor launch (rhsp.F90:281-339): u, v,
+// w, p read, the predicted velocities written to un, vn, wn; in the mixed
+// substep the FP64 mean-flow u um is saved to rm and predicted in place.
+template <typename T>
+static void save_predict_launch(Box &b, const T *u, const T *v, const T *w, const T *p, T *un, T *vn, T *wn,
+                                double *um, double *rm, int pm) {
+  const DevCoefT<T> &c = coefs<T>(b);
+  constexpr bool F64 = sizeof(T) == 8;
+  if constexpr (!F64) {
+    const dim3 pg(nblk(b.pu / 2), (unsigned)((b.nx + pm - 1) / pm)); // one thread per mode (Re, Im)
+    LAUNCH(b, "save_predict", pg, TPB, 0, save_predict_pair_kernel, u, v, w, p, un, vn, wn, c.idyy, c.kaz, b.ds,
+           b.hc.idxx, b.nx, b.ny, b.nz1, (const double *)b.p0, b.dc.idyy, um, rm, pm);
+  } else {
+    LAUNCH(b, "save_predict", pgrid(b, pm), TPB, 0, save_predict_kernel<T>, u, v, w, p, un, vn, wn, c.idyy, c.kaz,
+           b.ds, b.hc.idxx, b.nx, b.ny, b.nz1, (const double *)nullptr, b.dc.idyy, (double *)nullptr,
+           (double *)nullptr, pm);
+  }
+}
+
+// The projection launch of pois (pois.f90:144-181): phi (planes layout)
+// applied to u, v, w and p in place; in the mixed substep the FP64 mean-flow
+// phi pencils P0 update the FP64 mean pressure p0 and mean-flow u um.
+static void project_launch(Box &b, double *u, double *v, double *w, double *p, const double *phi, double varstep,
+                           bool use_m, double *p0, double *um, int pm) {
+  if (b.precision == OTBL_MIXED) {
+    const dim3 pg(nblk(b.pu / 2), (unsigned)((b.nx + pm - 1) / pm)); // one thread per mode (Re, Im)
+    LAUNCH(b, "project", pg, TPB, 0, project_pair_kernel, fld<float>(u), fld<float>(v), fld<float>(w), fld<float>(p),
+           (const float *)phi, b.dcf, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, b.nx, b.ny, b.nz1,
+           (const double *)b.P0, p0, b.dc.idyy, um, pm);
+  } else {
+    LAUNCH(b, "project", pgrid(b, pm), TPB, 0, project_kernel<double>, u, v, w, p, phi, b.dc, b.ds, varstep,
+           use_m ? 1 : 0, b.hc.idxx, b.nx, b.ny, b.nz1, (const double *)nullptr, (double *)nullptr, b.dc.idyy,
+           (double *)nullptr, pm);
+  }
+}
+
 // FP32 pressure solve of the mixed path: the kz = 0 lines run in FP64 on
 // their own pencils (divergence, both x transforms and the pinned y solve),
 // the other modes in FP32 with one FP64-residual refinement of the y solve.
@@ -2026,11 +2064,8 @@ static void pois_mixed(Box &b, double varstep, bool use_m) {
   kz_dct_pencils<float>(b, P, 1, Lp);
   kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1));
   LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pv, b.P0, nz1);
-  const int pm = prun<float>();
-  const dim3 pg(nblk(b.pu / 2), (unsigned)((nx + pm - 1) / pm)); // one thread per mode (Re, Im)
-  LAUNCH(b, "project", pg, TPB, 0, project_pair_kernel, fld<float>(b.f[OTBL_U]), fld<float>(b.f[OTBL_V]),
-         fld<float>(b.f[OTBL_W]), fld<float>(b.f[OTBL_P]), F, b.dcf, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx,
-         ny, nz1, (const double *)b.P0, b.p0, b.dc.idyy, b.um64, pm);
+  project_launch(b, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.f[OTBL_P], (const double *)F, varstep, use_m, b.p0,
+                 b.um64, prun<float>(b));
 }
 
 void k_pois(Box &b, int m, double varstep, bool use_m) {
@@ -2067,10 +2102,8 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
     LAUNCH(b, "poisson_y", nblk(ncol), TPB, 0, poisson_y_kernel, b.rest2, b.rest, b.dc, nx, ny, nz1);
     k_cosftx(b, b.rest2, b.rest, 1);
   }
-  const int pm = prun<double>();
-  LAUNCH(b, "project", pgrid(b, pm), TPB, 0, project_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W],
-         b.f[OTBL_P], b.rest, b.dc, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, nx, ny, nz1, (const double *)nullptr,
-         (double *)nullptr, b.dc.idyy, (double *)nullptr, pm);
+  project_launch(b, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.f[OTBL_P], b.rest, varstep, use_m, nullptr, nullptr,
+                 prun<double>(b));
 }
 
 // ---------------------------------------------------------------------------
@@ -2156,15 +2189,7 @@ template <typename T> static void rhsp_t(Box &b, int m) {
          b.hc.dxmin, b.hc.dtret, b.hc.rex, b.hc.rkc[m], b.hc.rkd[m], b.hc.rkcv[m], b.hc.rkdv[m]);
   // save velocities, pressure-gradient predictor
   double *um = F64 ? nullptr : b.um64; // the FP64 mean-flow u of the mixed substep
-  const int pm = prun<T>();
-  if constexpr (sizeof(T) == 4) {
-    const dim3 pg(nblk(b.pu / 2), (unsigned)((nx + pm - 1) / pm)); // one thread per mode (Re, Im)
-    LAUNCH(b, "save_predict", pg, TPB, 0, save_predict_pair_kernel, u, v, w, p, resu, resv, resw, c.idyy, c.kaz,
-           b.ds, b.hc.idxx, nx, ny, nz1, (const double *)b.p0, b.dc.idyy, um, b.rm64, pm);
-  } else {
-    LAUNCH(b, "save_predict", pgrid(b, pm), TPB, 0, save_predict_kernel<T>, u, v, w, p, resu, resv, resw, c.idyy,
-           c.kaz, b.ds, b.hc.idxx, nx, ny, nz1, F64 ? nullptr : (const double *)b.p0, b.dc.idyy, um, b.rm64, pm);
-  }
+  save_predict_launch<T>(b, u, v, w, p, resu, resv, resw, um, b.rm64, prun<T>(b));
   // the predicted velocities now live in the former save buffers, and this
   // substep's rhs buffers become the rhs_prev fields
   k_swap_saved(b);
@@ -2314,5 +2339,108 @@ void k_retheta_cf(Box &b, double *out) {
 }
 
 void k_end_step(Box &b) { LAUNCH(b, "end_step", 1, 64, 0, end_step_kernel, b.ds); }
+
+// ---------------------------------------------------------------------------
+// Autotuner: the launch shapes chosen per box by timing candidates on the
+// box's own sizes.  Every timed launch reads state fields and writes only
+// work fields (dead between substeps) or scratch, so the state is unchanged.
+// ---------------------------------------------------------------------------
+template <typename F> static double time_launches(Box &b, int reps, F &&launch) {
+  hipEvent_t e0, e1;
+  OTBL_HIP(hipEventCreate(&e0));
+  OTBL_HIP(hipEventCreate(&e1));
+  launch(); // warm
+  OTBL_HIP(hipEventRecord(e0, b.s));
+  for (int r = 0; r < reps; ++r) launch();
+  OTBL_HIP(hipEventRecord(e1, b.s));
+  OTBL_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  OTBL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)ms / reps;
+}
+
+template <typename T> static void x_lines_once(Box &b) {
+  const DevCoefT<T> &c = coefs<T>(b);
+  T *u = fld<T>(b.f[OTBL_U]), *wki1 = fld<T>(b.wki1), *rhsu = fld<T>(b.rhsu);
+  if constexpr (sizeof(T) == 8) {
+    k_interpxx(b, u, wki1, b.ny + 1, 1);
+    k_difvisxx(b, wki1, u, rhsu, b.ny + 1, 1, 0);
+  } else {
+    kx_interpxx<T>(b, u, wki1, c.inxu, c.cofiux, 2, 3, b.pu);
+    kx_difvisxx<T>(b, wki1, u, rhsu, c.dcxu, c.vixu, c.cofcxu, c.cofvxu, 1, 3, b.pu, 0);
+  }
+}
+
+template <typename T> static std::string autotune_t(Box &b, int reps) {
+  char buf[256];
+  std::string rep;
+  // x lines: segments per line
+  {
+    double best = 1e30;
+    int pick = b.xseg;
+    for (int n = 1; n <= b.xseg_max; ++n) {
+      b.xseg = n;
+      const double ms = time_launches(b, reps, [&] { x_lines_once<T>(b); });
+      std::snprintf(buf, sizeof buf, "xseg%d=%.4fms ", n, ms);
+      rep += buf;
+      if (ms < best) best = ms, pick = n;
+    }
+    b.xseg = pick;
+  }
+  // element-wise predictor and projection: planes per thread, on work fields
+  // (the mean-flow arrays of the mixed substep on scratch copies)
+  {
+    double *scr = nullptr;
+    const size_t nm = (size_t)b.nx * (b.ny + 1);
+    OTBL_HIP(hipMalloc((void **)&scr, 3 * nm * sizeof(double)));
+    double *um = b.um64 ? scr : nullptr, *rm = b.um64 ? scr + nm : nullptr, *p0 = b.p0 ? scr + 2 * nm : nullptr;
+    T *u = fld<T>(b.f[OTBL_U]), *v = fld<T>(b.f[OTBL_V]), *w = fld<T>(b.f[OTBL_W]), *p = fld<T>(b.f[OTBL_P]);
+    double best = 1e30;
+    int pick = prun<T>(b);
+    try {
+      for (int pm : {1, 2, 4, 8}) {
+        const double ms = time_launches(b, reps, [&] {
+          save_predict_launch<T>(b, u, v, w, p, fld<T>(b.resu), fld<T>(b.resv), fld<T>(b.resw), um, rm, pm);
+          project_launch(b, b.resu, b.resv, b.resw, b.rhsv, b.rest, 0.0, false, p0, um, pm);
+        });
+        std::snprintf(buf, sizeof buf, "prun%d=%.4fms ", pm, ms);
+        rep += buf;
+        if (ms < best) best = ms, pick = pm;
+      }
+    } catch (...) {
+      (void)hipFree(scr);
+      throw;
+    }
+    OTBL_HIP(hipStreamSynchronize(b.s));
+    OTBL_HIP(hipFree(scr));
+    b.prun = pick;
+  }
+  // FP32 rk_final: half-wave or full-wave chunks (u and v rows, on work fields)
+  if constexpr (sizeof(T) == 4) {
+    if (b.ytabh_f) {
+      double best = 1e30;
+      bool pick = b.yhalf;
+      for (bool half : {true, false}) {
+        b.yhalf = half;
+        const double ms = time_launches(b, reps, [&] {
+          ky_rk_final<T>(b, fld<T>(b.wki2), fld<T>(b.resv), fld<T>(b.rhsv), fld<T>(b.f[OTBL_RHSVPA]), b.ny);
+          ky_rk_final<T>(b, fld<T>(b.wki1), fld<T>(b.resu), fld<T>(b.rhsu), fld<T>(b.f[OTBL_RHSUPA]), b.ny + 1);
+        });
+        std::snprintf(buf, sizeof buf, "yhalf%d=%.4fms ", half ? 1 : 0, ms);
+        rep += buf;
+        if (ms < best) best = ms, pick = half;
+      }
+      b.yhalf = pick;
+    }
+  }
+  b.tuned = true;
+  return rep;
+}
+
+std::string k_autotune(Box &b, int reps) {
+  return b.precision == OTBL_MIXED ? autotune_t<float>(b, reps) : autotune_t<double>(b, reps);
+}
 
 } // namespace otbl

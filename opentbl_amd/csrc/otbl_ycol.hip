@@ -817,7 +817,7 @@ template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs,
   const unsigned g = ygrid((size_t)b.nz1 * b.nx);
   const bool uu = jee == b.ny + 1;
   if constexpr (sizeof(T) == 4) {
-    if (b.ytabh_f) {
+    if (b.ytabh_f && b.yhalf) {
       yh_check((size_t)b.nz1 * b.nx * (b.ny + 1));
       YDISPATCH(b.yL, auto kf = &rk_final_yh<L / 2>;
                 LAUNCH(b, "rk_final", ygridh((size_t)b.nz1 * b.nx), YTPB, 0, kf, u, res, rhs, rhsp,

@@ -78,6 +78,7 @@ SIGNATURES = {
     "otbl_d2h": (I, [VP, VP, SZ]),
     "otbl_d2d": (I, [VP, VP, SZ]),
     "otbl_get_paths": (I, [VP, C.c_char_p, SZ]),
+    "otbl_autotune": (I, [VP, I, C.c_char_p, SZ]),
     "otbl_kprof": (I, [VP, I]),
     "otbl_kprof_report": (I, [VP, C.c_char_p, SZ]),
 }
@@ -270,6 +271,13 @@ class GpuBox:
         buf = C.create_string_buffer(256)
         check(lib().otbl_get_paths(self.h, buf, len(buf)), "get_paths")
         return dict(kv.split(":") for kv in buf.value.decode().split())
+
+    def autotune(self, reps=5):
+        """Time the candidate launch shapes on this box and keep the fastest;
+        returns the timings as text."""
+        buf = C.create_string_buffer(4096)
+        check(lib().otbl_autotune(self.h, int(reps), buf, len(buf)), "autotune")
+        return buf.value.decode()
 
     def kprof(self, on):
         check(lib().otbl_kprof(self.h, int(on)), "kprof")
