@@ -438,8 +438,12 @@ static void setup_mixed(Box &b) {
     dalloc(b.ytabh_f, YT_COUNT * ytabh_size(b.yL / 2));
     k_ytab_halve_fixed(b);
   }
-  dalloc(b.ztw_f, 2 * (size_t)b.nz);
-  k_d2f(b, b.ztw, b.ztw_f, 2 * (size_t)b.nz);
+  // the z twiddle table exists only for the register-resident transforms
+  // (the LDS Stockham kernels build their twiddles in LDS)
+  if (b.zfast) {
+    dalloc(b.ztw_f, 2 * (size_t)b.nz);
+    k_d2f(b, b.ztw, b.ztw_f, 2 * (size_t)b.nz);
+  }
   const size_t N = b.nx - 1, M = b.dct_M;
   dalloc(b.dct_chirp_f, 2 * N);
   dalloc(b.dct_tw4_f, 2 * N);

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <type_traits>
+#include <stdexcept>
 
 namespace otbl {
 
@@ -2288,10 +2289,13 @@ __global__ void f2d_kernel(const float *a, double *o, size_t n) {
 }
 
 static unsigned cgrid(size_t n) { return (unsigned)std::min<size_t>(nblk(n), 8192); }
+// (a null buffer is a setup error caught here, on the host)
 void k_d2f(Box &b, const double *a, float *o, size_t n) {
+  if (n && (!a || !o)) throw std::logic_error("k_d2f: null buffer");
   if (n) LAUNCH(b, "d2f", cgrid(n), TPB, 0, d2f_kernel, a, o, n);
 }
 void k_f2d(Box &b, const float *a, double *o, size_t n) {
+  if (n && (!a || !o)) throw std::logic_error("k_f2d: null buffer");
   if (n) LAUNCH(b, "f2d", cgrid(n), TPB, 0, f2d_kernel, a, o, n);
 }
 

@@ -236,7 +236,7 @@ def test_soltriy_dropin(fbox):
     u = np.ascontiguousarray(rng.standard_normal((ny1, cfg.nz1)))
     u0 = u.copy()
     ii = 2
-    af = np.ascontiguousarray(a)
+    af = a.copy()  # a stays the unfactored input
     L.soltriy_(p(u), p(u), iref(ii), p(af))
     x = np.linalg.solve(A, u0[:, 2 * ii] + 1j * u0[:, 2 * ii + 1])
     assert np.abs(u[:, 2 * ii] + 1j * u[:, 2 * ii + 1] - x).max() < 1e-12 * np.abs(x).max()
