@@ -472,6 +472,7 @@ void build_coef(HostCoef &hc, int nx, int ny, int nz, double re, double ax, doub
     at(hc.ayp, 3, 2, lt) = b2 * acp;
   }
   hc.vmagic.assign(nx + 1, 0.0);
+  build_stats_coef(hc);
 }
 
 void inlet_setup(HostCoef &hc, const double *u0, double *rthin, double *din) {
@@ -504,6 +505,226 @@ void inlet_setup(HostCoef &hc, const double *u0, double *rthin, double *din) {
   hc.vmagic[1] = (h[2] - h[1]) / (xx[2] - xx[1]);
   for (int i = 2; i <= nx - 1; ++i) hc.vmagic[i] = (h[i + 1] - h[i - 1]) / (xx[i + 1] - xx[i - 1]);
   hc.vmagic[nx] = (h[nx] - h[nx - 1]) / (xx[nx] - xx[nx - 1]);
+}
+
+
+// ---------------------------------------------------------------------------
+// Stats-only compact schemes (the reference's coeft.f90:56-285: fd_dvdy,
+// fd_dx, fd_ix, fd_iy, fd_iyp).  Every row is the scheme of maximal order on
+// the reference's stencil: with the left-hand points L (weights a_l; the
+// pinned point has weight 1) and the right-hand points P (weights c_q), the
+// weights make
+//     sum_l a_l g(L_l) = sum_q c_q f(P_q),   g = f' (derivative) or f,
+// exact for every polynomial of degree below the number of free weights.
+// That condition fixes the reference's closed-form weights (the same
+// schemes, derived independently here).  Arrays are (8, 1:n) with column i
+// at offset 8 i: rows 1-5 explicit weights, 6-8 (alpha, diagonal, beta),
+// then LU-factored in place (row 6 the multiplier, row 7 the inverse pivot).
+// ---------------------------------------------------------------------------
+namespace {
+
+// Gaussian elimination with partial pivoting, n <= 8 unknowns.
+void solve_dense(int n, double m[8][8], double r[8]) {
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int q = c + 1; q < n; ++q)
+      if (std::fabs(m[q][c]) > std::fabs(m[p][c])) p = q;
+    if (m[p][c] == 0.0) throw std::runtime_error("singular compact-scheme moment system");
+    if (p != c) {
+      for (int j = 0; j < n; ++j) std::swap(m[p][j], m[c][j]);
+      std::swap(r[p], r[c]);
+    }
+    for (int q = c + 1; q < n; ++q) {
+      const double f = m[q][c] / m[c][c];
+      for (int j = c; j < n; ++j) m[q][j] -= f * m[c][j];
+      r[q] -= f * r[c];
+    }
+  }
+  for (int q = n - 1; q >= 0; --q) {
+    double acc = r[q];
+    for (int j = q + 1; j < n; ++j) acc -= m[q][j] * r[j];
+    r[q] = acc / m[q][q];
+  }
+}
+
+// One row of a compact scheme about t0.  L[0..nl) are the left-hand points,
+// L[pin] carries weight 1 (its g value moves to the right of the moment
+// equations), the other left weights go to a (a[pin] = 1); P[0..np) the
+// right-hand points, weights to c.  Distances are scaled by the stencil's
+// half-width h for conditioning; a derivative weight scales back by 1/h.
+void compact_row(double t0, const double *L, int nl, int pin, const double *P, int np, bool deriv, double *a,
+                 double *c) {
+  double h = 0.0;
+  for (int q = 0; q < np; ++q) h = std::max(h, std::fabs(P[q] - t0));
+  for (int l = 0; l < nl; ++l) h = std::max(h, std::fabs(L[l] - t0));
+  const int n = nl - 1 + np;
+  double m[8][8] = {{0}}, r[8] = {0};
+  auto g = [&](double tau, int k) { // g of the monomial tau^k (scaled units)
+    if (!deriv) return std::pow(tau, k);
+    return k == 0 ? 0.0 : k * std::pow(tau, k - 1);
+  };
+  for (int k = 0; k < n; ++k) {
+    int col = 0;
+    for (int l = 0; l < nl; ++l) {
+      const double tau = (L[l] - t0) / h;
+      if (l == pin)
+        r[k] = -g(tau, k);
+      else
+        m[k][col++] = g(tau, k);
+    }
+    for (int q = 0; q < np; ++q) m[k][col++] = -std::pow((P[q] - t0) / h, k);
+  }
+  solve_dense(n, m, r);
+  int col = 0;
+  for (int l = 0; l < nl; ++l) a[l] = l == pin ? 1.0 : r[col++];
+  for (int q = 0; q < np; ++q) c[q] = deriv ? r[col++] / h : r[col++];
+}
+
+// Fills column i of an (8, n) array from one row: the left points are the
+// rows i-1, i, i+1 present in lrow (offsets -1, 0, +1), the diagonal is the
+// weight of offset 0.
+void put_row(std::vector<double> &A, int i, const int *loff, int nl, const double *a, const double *c, int np) {
+  for (int q = 0; q < np; ++q) A[8 * (size_t)i + q] = c[q];
+  for (int l = 0; l < nl; ++l) A[8 * (size_t)i + 6 + loff[l]] = a[l]; // row 6 + off: 5 alpha, 6 diag, 7 beta
+}
+
+void factor8(std::vector<double> &A, int n, int j0) {
+  auto at8 = [&](int r, int i) -> double & { return A[8 * (size_t)i + r - 1]; };
+  for (int j = j0; j <= n; ++j) {
+    at8(6, j) = at8(6, j) / at8(7, j - 1);
+    at8(7, j) = at8(7, j) - at8(6, j) * at8(8, j - 1);
+  }
+  for (int j = j0 - 1; j <= n; ++j) at8(7, j) = 1.0 / at8(7, j);
+}
+
+// One row: points given by index into a coordinate array.
+struct RowSpec {
+  int nl, pin, np;
+  int loff[3];
+  int rhs[5];
+};
+
+void build_rows(std::vector<double> &A, int n, const std::vector<double> &to, const std::vector<double> &from,
+                bool deriv, const RowSpec *spec, int i) {
+  double L[3], P[5], a[3], c[5];
+  for (int l = 0; l < spec->nl; ++l) L[l] = to[i + spec->loff[l]];
+  for (int q = 0; q < spec->np; ++q) P[q] = from[spec->rhs[q]];
+  compact_row(to[i], L, spec->nl, spec->pin, P, spec->np, deriv, a, c);
+  put_row(A, i, spec->loff, spec->nl, a, c, spec->np);
+  (void)n;
+}
+
+} // namespace
+
+void build_stats_coef(HostCoef &hc) {
+  const int nx = hc.nx, ny = hc.ny;
+  const std::vector<double> &y = hc.y; // y(0:ny+1)
+  hc.fd_dvdy.assign(8 * (size_t)(ny + 1), 0.0);
+  hc.fd_iy.assign(8 * (size_t)(ny + 1), 0.0);
+  hc.fd_iyp.assign(8 * (size_t)(ny + 1), 0.0);
+  hc.fd_dx.assign(8 * (size_t)(nx + 1), 0.0);
+  hc.fd_ix.assign(8 * (size_t)(nx + 1), 0.0);
+  auto span = [](RowSpec &s, int first, int count) {
+    s.np = count;
+    for (int q = 0; q < count; ++q) s.rhs[q] = first + q;
+  };
+  // d/dy at the v rows y(1..ny) from the v rows (diffy_inplace,
+  // lip.f90:666-698): rows 1, 2 on y(1..4), interior on y(j-2..j+2), rows
+  // ny-1, ny on y(ny-3..ny)
+  for (int j = 1; j <= ny; ++j) {
+    RowSpec s{};
+    if (j == 1) { s.nl = 2; s.loff[0] = 0; s.loff[1] = 1; s.pin = 0; span(s, 1, 4); }
+    else if (j == 2) { s.nl = 3; s.loff[0] = -1; s.loff[1] = 0; s.loff[2] = 1; s.pin = 1; span(s, 1, 4); }
+    else if (j <= ny - 2) { s.nl = 3; s.loff[0] = -1; s.loff[1] = 0; s.loff[2] = 1; s.pin = 1; span(s, j - 2, 5); }
+    else if (j == ny - 1) { s.nl = 3; s.loff[0] = -1; s.loff[1] = 0; s.loff[2] = 1; s.pin = 1; span(s, ny - 3, 4); }
+    else { s.nl = 2; s.loff[0] = -1; s.loff[1] = 0; s.pin = 1; span(s, ny - 3, 4); }
+    build_rows(hc.fd_dvdy, ny, y, y, true, &s, j);
+  }
+  factor8(hc.fd_dvdy, ny, 2);
+  // midpoint interpolation from the u rows yp(j) = (y(j-1) + y(j))/2 to the
+  // v rows (interpy_new, lip.f90:771-809); u, w take ny+1 input rows, p
+  // ny.  Row 1 (the wall) is explicit; the tridiagonal starts at row 2,
+  // which has no alpha.  The last row pins its alpha and frees the
+  // diagonal, as the reference's row does (coeft.f90:311-312).
+  std::vector<double> yp(ny + 2, 0.0);
+  for (int j = 1; j <= ny + 1; ++j) yp[j] = 0.5 * (y[j - 1] + y[j]);
+  for (int tipo = 0; tipo < 2; ++tipo) {
+    std::vector<double> &A = tipo ? hc.fd_iyp : hc.fd_iy;
+    const int last = ny + 1 - tipo; // last input row
+    for (int j = 1; j <= ny; ++j) {
+      RowSpec s{};
+      if (j == 1) { s.nl = 1; s.loff[0] = 0; s.pin = 0; span(s, 2, 4); }
+      else if (j == 2) { s.nl = 2; s.loff[0] = 0; s.loff[1] = 1; s.pin = 0; span(s, 2, 5); }
+      else if (j <= ny - 2 || (j == ny - 1 && tipo == 0)) {
+        s.nl = 3; s.loff[0] = -1; s.loff[1] = 0; s.loff[2] = 1; s.pin = 1; span(s, j - 1, 4);
+      } else if (j == ny - 1) { s.nl = 3; s.loff[0] = -1; s.loff[1] = 0; s.loff[2] = 1; s.pin = 1; span(s, ny - 2, 3); }
+      else { s.nl = 2; s.loff[0] = -1; s.loff[1] = 0; s.pin = 0; span(s, ny - 2, last - (ny - 2) + 1); }
+      build_rows(A, ny, y, yp, false, &s, j);
+    }
+    factor8(A, ny, 3);
+  }
+  // the uniform x grid, spacing ax pi / (nx + 1) (coeft.f90:181): d/dx at
+  // the planes (diffx_inplace) with the reference's left-hand weights --
+  // boundary rows f'1 + 4 f'2, (1/8) f'1 + f'2 + (3/4) f'3 and their mirror
+  // images, interior (1/3, 1, 1/3) -- and explicit weights of maximal order
+  const double dx = hc.ax * hc.pi / (nx + 1);
+  std::vector<double> xg(nx + 2);
+  for (int i = 0; i <= nx + 1; ++i) xg[i] = i * dx;
+  {
+    auto row_fixed = [&](int i, const double *lw, const int *loff, int nl, int first, int np) {
+      // explicit weights for fixed left weights: moment conditions of order
+      // 0..np-1 (a square system in the explicit weights only)
+      double m[8][8] = {{0}}, r[8] = {0};
+      for (int k = 0; k < np; ++k) {
+        double gk = 0.0;
+        for (int l = 0; l < nl; ++l) {
+          const double tau = (xg[i + loff[l]] - xg[i]) / dx;
+          gk += lw[l] * (k == 0 ? 0.0 : k * std::pow(tau, k - 1));
+        }
+        for (int q = 0; q < np; ++q) m[k][q] = std::pow((xg[first + q] - xg[i]) / dx, k);
+        r[k] = gk;
+      }
+      solve_dense(np, m, r);
+      for (int q = 0; q < np; ++q) hc.fd_dx[8 * (size_t)i + q] = r[q] / dx;
+      for (int l = 0; l < nl; ++l) hc.fd_dx[8 * (size_t)i + 6 + loff[l]] = lw[l];
+    };
+    const int o2[2] = {0, 1}, o2m[2] = {-1, 0}, o3[3] = {-1, 0, 1};
+    const double w1[2] = {1.0, 4.0}, w2[3] = {1.0 / 8.0, 1.0, 3.0 / 4.0}, wi[3] = {1.0 / 3.0, 1.0, 1.0 / 3.0};
+    const double w2m[3] = {3.0 / 4.0, 1.0, 1.0 / 8.0}, w1m[2] = {4.0, 1.0};
+    row_fixed(1, w1, o2, 2, 1, 5);
+    row_fixed(2, w2, o3, 3, 1, 5);
+    for (int i = 3; i <= nx - 2; ++i) row_fixed(i, wi, o3, 3, i - 2, 5);
+    row_fixed(nx - 1, w2m, o3, 3, nx - 4, 5);
+    row_fixed(nx, w1m, o2m, 2, nx - 4, 5);
+    factor8(hc.fd_dx, nx, 2);
+  }
+  // midpoint interpolation along x from plane i to i + 1/2 (interpx_new):
+  // rows 1 (beta 5/3), 2..nx-2 (3/10, 1, 3/10), nx-1 (5/6, 1, 1/14), nx
+  // (7, 1; the point beyond the last plane), explicit weights of maximal
+  // order on 4 points
+  {
+    auto row_fixed = [&](int i, const double *lw, const int *loff, int nl, int first) {
+      double m[8][8] = {{0}}, r[8] = {0};
+      const double t0 = xg[i] + 0.5 * dx;
+      for (int k = 0; k < 4; ++k) {
+        double gk = 0.0;
+        for (int l = 0; l < nl; ++l) gk += lw[l] * std::pow((xg[i + loff[l]] + 0.5 * dx - t0) / dx, k);
+        for (int q = 0; q < 4; ++q) m[k][q] = std::pow((xg[first + q] - t0) / dx, k);
+        r[k] = gk;
+      }
+      solve_dense(4, m, r);
+      for (int q = 0; q < 4; ++q) hc.fd_ix[8 * (size_t)i + q] = r[q];
+      for (int l = 0; l < nl; ++l) hc.fd_ix[8 * (size_t)i + 6 + loff[l]] = lw[l];
+    };
+    const int o2[2] = {0, 1}, o2m[2] = {-1, 0}, o3[3] = {-1, 0, 1};
+    const double w1[2] = {1.0, 5.0 / 3.0}, wi[3] = {3.0 / 10.0, 1.0, 3.0 / 10.0};
+    const double wm[3] = {5.0 / 6.0, 1.0, 1.0 / 14.0}, wn[2] = {7.0, 1.0};
+    row_fixed(1, w1, o2, 2, 1);
+    for (int i = 2; i <= nx - 2; ++i) row_fixed(i, wi, o3, 3, i - 1);
+    row_fixed(nx - 1, wm, o3, 3, nx - 3);
+    row_fixed(nx, wn, o2m, 2, nx - 3);
+    factor8(hc.fd_ix, nx, 2);
+  }
 }
 
 } // namespace otbl

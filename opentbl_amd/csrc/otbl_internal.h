@@ -30,11 +30,17 @@ struct HostCoef {
   std::vector<double> cofvxu, cofvxv, cofvyu, cofvyv, ayp;
   double inbx[16] = {0}, inby[16] = {0}, dcbx[20] = {0}, dcby[16] = {0}, ccon[8] = {0};
   double cofvbx[16] = {0}, cofvby[16] = {0};
+  // stats-only schemes (8, 1:n), LU-factored: d/dy and midpoint y
+  // interpolation (u rows, p rows) at the v rows, d/dx and midpoint x
+  // interpolation on the planes
+  std::vector<double> fd_dvdy, fd_iy, fd_iyp, fd_dx, fd_ix;
 };
 
 // Builds every coefficient of the RK3 step (coeft.f90:10-1234).
 void build_coef(HostCoef &hc, int nx, int ny, int nz, double re, double ax, double az,
                 double cfl, const double *y);
+// the stats-only schemes (coeft.f90:56-285); called by build_coef
+void build_stats_coef(HostCoef &hc);
 // inlet_retheta (bl_1.F90:372-396) and magicnumber (boun.f90:102-143)
 void inlet_setup(HostCoef &hc, const double *u0, double *rthin, double *din);
 

@@ -61,6 +61,18 @@ def lib():
         L.oc_retheta_cf.argtypes = [C.c_void_p, PD, PD]
         L.oc_max_threads.restype = I
         L.oc_max_threads.argtypes = []
+        L.oc_statsp.argtypes = [C.c_void_p]
+        L.oc_stat_name.restype = C.c_char_p
+        L.oc_stat_name.argtypes = [I]
+        L.oc_stats_get.argtypes = [C.c_void_p, I, PD]
+        L.oc_stats_reset.argtypes = [C.c_void_p]
+        L.oc_stats_ical.restype = I
+        L.oc_stats_ical.argtypes = [C.c_void_p]
+        L.oc_stats_ens.restype = C.POINTER(C.c_float)
+        L.oc_stats_ens.argtypes = [C.c_void_p]
+        L.oc_stats_coef.restype = PD
+        L.oc_stats_coef.argtypes = [C.c_void_p, C.c_char_p]
+        L.oc_stats_op.argtypes = [C.c_void_p, I, PD, PD, I]
         _LIB = L
     return _LIB
 
@@ -70,6 +82,7 @@ def ptr(a):
     return a.ctypes.data_as(PD)
 
 
+NSTAT = 40
 FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "rhsupa": 4, "rhsvpa": 5, "rhswpa": 6}
 
 
@@ -166,6 +179,49 @@ class OracleBox:
         c = np.zeros(self.cfg.nx)
         lib().oc_retheta_cf(self.h, ptr(r), ptr(c))
         return r, c
+
+    # statistics (stotf.F90:51-291)
+    def statsp(self):
+        lib().oc_statsp(self.h)
+
+    def stats(self):
+        """The accumulators by name, each (nx, ny+1) (plane, row)."""
+        L = lib()
+        out = {}
+        for q in range(NSTAT):
+            a = np.zeros(self.cfg.nx * (self.cfg.ny + 1))
+            L.oc_stats_get(self.h, q, ptr(a))
+            out[L.oc_stat_name(q).decode()] = a.reshape(self.cfg.nx, self.cfg.ny + 1)
+        return out
+
+    def stats_reset(self):
+        lib().oc_stats_reset(self.h)
+
+    def stats_ical(self):
+        return lib().oc_stats_ical(self.h)
+
+    def ens(self):
+        """The enstrophy field of the last sample, (nx, ny, nz) float32."""
+        p = lib().oc_stats_ens(self.h)
+        if not p:
+            return None
+        c = self.cfg
+        return np.ctypeslib.as_array(p, shape=(c.nx * c.ny * c.nz,)).reshape(c.nx, c.ny, c.nz).copy()
+
+    def stats_coef(self, name):
+        """fd_dvdy, fd_dx, fd_ix, fd_iy, fd_iyp as (8, n) arrays."""
+        p = lib().oc_stats_coef(self.h, name.encode())
+        n = self.cfg.nx if name in ("fd_dx", "fd_ix") else self.cfg.ny
+        return np.ctypeslib.as_array(p, shape=(8 * n,)).copy().reshape(n, 8).T
+
+    def stats_op(self, op, a, n):
+        """op 0 diffy, 1 interpy (u rows in), 2 interpy (p rows in) on n
+        planes of (R, nz1) doubles; 3 diffx, 4 interpx on n pencils of nx."""
+        a = np.ascontiguousarray(a, np.float64)
+        c = self.cfg
+        out = np.zeros(n * c.ny * c.nz1 if op <= 2 else n * c.nx)
+        lib().oc_stats_op(self.h, op, ptr(a), ptr(out), n)
+        return out
 
     # operator-level helpers on complex (nplanes, R, Mz) arrays
     def op(self, name, *args):

@@ -82,6 +82,15 @@ struct oc_box {
    * so results do not depend on the thread count) */
   int nth;
   struct oc_tls *tls;
+  /* statistics (stotf.F90:51-291): the stats-only compact schemes of
+   * coeft.f90:56-285 as (8, n) arrays (column i at offset 8 i), the
+   * accumulators (OC_NSTAT of (ny+1, nx), Fortran order), the last
+   * enstrophy field ens(nz, ny, nx) in FP32 (escru.F90:159-170 writes it)
+   * and the number of accumulated samples (ical, rhsp.F90:92) */
+  double *fd_dvdy, *fd_dx, *fd_ix, *fd_iy, *fd_iyp;
+  double *st;
+  float *ens;
+  int ical;
 };
 struct oc_tls {
   double *wkp, *wkpo, *wkf, *wkfo, *wk2r, *fft_a, *fft_b, *fft_t, *vm;
@@ -1073,6 +1082,205 @@ static void implzy(const oc_box *b, double *u, const double *res, const double *
 }
 
 /* ------------------------------------------------------------------------
+ * Stats-only compact schemes (coeft.f90:56-285), (8, n) arrays: rows 1-5
+ * the explicit right-hand side, rows 6-8 the tridiagonal (alpha, diagonal,
+ * beta), LU-factored in place as coeft.f90 does (row 6 the multiplier,
+ * row 7 the inverse pivot).
+ * ---------------------------------------------------------------------- */
+#define F8(a, r, i) (a)[8 * (size_t)(i) + (r)-1]
+static double sq(double x) { return x * x; }
+static double cu(double x) { return x * x * x; }
+
+/* in-place LU of rows 6-8 from row j0 (coeft.f90:135-139, 191-195, 325-336) */
+static void f8_factor(double *c, int n, int j0) {
+  for (int j = j0; j <= n; j++) {
+    F8(c, 6, j) = F8(c, 6, j) / F8(c, 7, j - 1);
+    F8(c, 7, j) = F8(c, 7, j) - F8(c, 6, j) * F8(c, 8, j - 1);
+  }
+  for (int j = j0 - 1; j <= n; j++) F8(c, 7, j) = 1.0 / F8(c, 7, j);
+}
+
+static void stats_coef(oc_box *b) {
+  const int nx = b->nx, ny = b->ny;
+  const double *y = b->y;
+  double a, bb, c, d, e, f, g;
+  b->fd_dvdy = zalloc(8 * (size_t)(ny + 1));
+  b->fd_iy = zalloc(8 * (size_t)(ny + 1));
+  b->fd_iyp = zalloc(8 * (size_t)(ny + 1));
+  b->fd_dx = zalloc(8 * (size_t)(nx + 1));
+  b->fd_ix = zalloc(8 * (size_t)(nx + 1));
+  double *h = zalloc(ny + 2), *hi = zalloc(ny + 2), *yp = zalloc(ny + 2);
+
+  /* d/dy at the v rows (coeft.f90:56-139) */
+  double *D = b->fd_dvdy;
+  for (int j = 1; j <= ny + 1; j++) h[j] = y[j] - y[j - 1];
+  a = h[2]; bb = h[2] + h[3]; c = h[2] + h[3] + h[4];
+  F8(D, 1, 1) = -(a * bb + a * c + 2 * bb * c) / bb / a / c;
+  F8(D, 3, 1) = -sq(a) * c / (-c + bb) / bb / sq(-bb + a);
+  F8(D, 4, 1) = sq(a) * bb / c / sq(a - c) / (-c + bb);
+  F8(D, 2, 1) = -(F8(D, 1, 1) + F8(D, 3, 1) + F8(D, 4, 1));
+  F8(D, 7, 1) = 1.0;
+  F8(D, 8, 1) = bb * c / (a - c) / (-bb + a);
+  F8(D, 1, 2) = sq(-bb + a) * (a - c) * (a * bb + 2 * a * c + 2 * bb * c) / a / cu(bb) / sq(c);
+  F8(D, 2, 2) = 1 / a * (-3 * a * bb + 2 * bb * c + 5 * sq(a) - 4 * a * c) / (a - c) / (-bb + a);
+  F8(D, 3, 2) = sq(a) * (a - c) * (-2 * a * c + 3 * a * bb + 4 * bb * c - 5 * sq(bb)) / cu(bb) / sq(-c + bb) / (-bb + a);
+  F8(D, 4, 2) = -(F8(D, 1, 2) + F8(D, 2, 2) + F8(D, 3, 2));
+  F8(D, 6, 2) = -sq(-bb + a) * (a - c) / c / sq(bb);
+  F8(D, 7, 2) = 1.0;
+  F8(D, 8, 2) = sq(a) * (a - c) / (-c + bb) / sq(bb);
+  for (int i = 3; i <= ny - 2; i++) {
+    a = h[i - 1]; bb = h[i]; c = h[i + 1]; d = h[i + 2]; f = c + d; e = a + bb;
+    F8(D, 1, i) = -sq(c) * sq(bb) * f / (f + e) / e / (sq(c) + 2 * c * e + sq(e)) / (sq(e) - 2 * e * bb + sq(bb));
+    F8(D, 2, i) = f / bb * e * sq(c) *
+                  (3 * c * bb * f - 2 * c * f * e + 4 * sq(bb) * c - 3 * bb * e * c + 5 * sq(bb) * f -
+                   4 * bb * e * f + 6 * cu(bb) - 5 * e * sq(bb)) /
+                  (sq(f) + 2 * bb * f + sq(bb)) / (cu(bb) + 3 * bb * sq(c) + cu(c) + 3 * sq(bb) * c) /
+                  (sq(e) - 2 * e * bb + sq(bb));
+    F8(D, 3, i) = (c * bb * f - bb * e * c - 2 * bb * e * f + 2 * c * f * e) / f / bb / e / c;
+    F8(D, 4, i) = -(-3 * c * bb * f - 2 * bb * e * f + 3 * bb * e * c + 4 * bb * sq(c) - 5 * sq(c) * f -
+                    4 * c * f * e + 6 * cu(c) + 5 * sq(c) * e) *
+                  sq(bb) / c * e * f / (sq(f) - 2 * f * c + sq(c)) /
+                  (3 * sq(e) * bb * sq(c) + 6 * e * sq(bb) * sq(c) + 6 * e * bb * cu(c) + sq(e) * cu(bb) +
+                   sq(c) * cu(c) + 2 * sq(sq(c)) * e + cu(c) * sq(e) + 3 * sq(bb) * cu(c) + cu(bb) * sq(c) +
+                   3 * bb * sq(sq(c)) + 2 * e * cu(bb) * c + 3 * sq(e) * sq(bb) * c);
+    F8(D, 5, i) = -(F8(D, 1, i) + F8(D, 2, i) + F8(D, 3, i) + F8(D, 4, i));
+    F8(D, 6, i) = -e * sq(c) * f / (f + bb) / (sq(bb) + 2 * bb * c + sq(c)) / (bb - e);
+    F8(D, 7, i) = 1.0;
+    F8(D, 8, i) = -e * sq(bb) * f / (-f + c) / (sq(bb) + 2 * bb * c + sq(c)) / (c + e);
+  }
+  a = h[ny - 2]; bb = h[ny - 1]; c = h[ny]; f = a + bb + c; e = bb + c;
+  F8(D, 1, ny - 1) = sq(c) * sq(c - e) / sq(f) / sq(f - e) / (-f + c);
+  F8(D, 3, ny - 1) = -(5 * sq(c) - 4 * f * c - 3 * c * e + 2 * f * e) / c / (-f + c) / (c - e);
+  F8(D, 4, ny - 1) = -sq(c - e) * (-f + c) * (2 * f * c + c * e + 2 * f * e) / c / sq(f) / cu(e);
+  F8(D, 2, ny - 1) = -(F8(D, 1, ny - 1) + F8(D, 3, ny - 1) + F8(D, 4, ny - 1));
+  F8(D, 6, ny - 1) = -(-f + c) * sq(c) / sq(e) / (f - e);
+  F8(D, 7, ny - 1) = 1.0;
+  F8(D, 8, ny - 1) = -sq(c - e) * (-f + c) / f / sq(e);
+  F8(D, 1, ny) = sq(c) * e / sq(-f + c) / (f - e) / f;
+  F8(D, 2, ny) = -sq(c) * f / e / sq(c - e) / (f - e);
+  F8(D, 4, ny) = (f * c + c * e + 2 * f * e) / c / f / e;
+  F8(D, 3, ny) = -(F8(D, 1, ny) + F8(D, 2, ny) + F8(D, 4, ny));
+  F8(D, 6, ny) = f * e / (-f + c) / (c - e);
+  F8(D, 7, ny) = 1.0;
+  f8_factor(D, ny, 2);
+
+  /* d/dx on the uniform x grid (coeft.f90:146-195) */
+  double *X = b->fd_dx;
+  const double r1[5] = {-37.0 / 12.0, 2.0 / 3.0, 3.0, -2.0 / 3.0, 1.0 / 12.0};
+  const double r2[5] = {-43.0 / 96.0, -5.0 / 6.0, 9.0 / 8.0, 1.0 / 6.0, -1.0 / 96.0};
+  const double ri[5] = {-1.0 / 36.0, -7.0 / 9.0, 0.0, 7.0 / 9.0, 1.0 / 36.0};
+  for (int r = 1; r <= 5; r++) {
+    F8(X, r, 1) = r1[r - 1];
+    F8(X, r, 2) = r2[r - 1];
+    F8(X, r, nx - 1) = -r2[5 - r];
+    F8(X, r, nx) = -r1[5 - r];
+    for (int i = 3; i <= nx - 2; i++) F8(X, r, i) = ri[r - 1];
+  }
+  F8(X, 7, 1) = 1.0; F8(X, 8, 1) = 4.0;
+  F8(X, 6, 2) = 1.0 / 8.0; F8(X, 7, 2) = 1.0; F8(X, 8, 2) = 3.0 / 4.0;
+  for (int i = 3; i <= nx - 2; i++) {
+    F8(X, 6, i) = 1.0 / 3.0; F8(X, 7, i) = 1.0; F8(X, 8, i) = 1.0 / 3.0;
+  }
+  F8(X, 6, nx - 1) = 3.0 / 4.0; F8(X, 7, nx - 1) = 1.0; F8(X, 8, nx - 1) = 1.0 / 8.0;
+  F8(X, 6, nx) = 4.0; F8(X, 7, nx) = 1.0;
+  const double point0 = b->ax * b->pi / (nx + 1);
+  for (int i = 1; i <= nx; i++)
+    for (int r = 1; r <= 5; r++) F8(X, r, i) /= point0;
+  f8_factor(X, nx, 2);
+
+  /* midpoint interpolation along x (coeft.f90:203-243) */
+  double *I = b->fd_ix;
+  const double q1[4] = {5.0 / 24.0, 15.0 / 8.0, 5.0 / 8.0, -1.0 / 24.0};
+  const double qm[4] = {-1.0 / 84.0, 1.0 / 4.0, 5.0 / 4.0, 5.0 / 12.0};
+  const double qn[4] = {1.0 / 8.0, -7.0 / 8.0, 35.0 / 8.0, 35.0 / 8.0};
+  const double qi[4] = {1.0 / 20.0, 3.0 / 4.0, 3.0 / 4.0, 1.0 / 20.0};
+  for (int r = 1; r <= 4; r++) {
+    F8(I, r, 1) = q1[r - 1];
+    F8(I, r, nx - 1) = qm[r - 1];
+    F8(I, r, nx) = qn[r - 1];
+    for (int i = 2; i <= nx - 2; i++) F8(I, r, i) = qi[r - 1];
+  }
+  F8(I, 7, 1) = 1.0; F8(I, 8, 1) = 5.0 / 3.0;
+  for (int i = 2; i <= nx - 2; i++) {
+    F8(I, 6, i) = 3.0 / 10.0; F8(I, 7, i) = 1.0; F8(I, 8, i) = 3.0 / 10.0;
+  }
+  F8(I, 6, nx - 1) = 5.0 / 6.0; F8(I, 7, nx - 1) = 1.0; F8(I, 8, nx - 1) = 1.0 / 14.0;
+  F8(I, 6, nx) = 7.0; F8(I, 7, nx) = 1.0;
+  f8_factor(I, nx, 2);
+
+  /* midpoint interpolation along y from the u rows to the v rows
+   * (coeft.f90:249-336): fd_iy for u, w (ny+1 rows in), fd_iyp for p */
+  double *Y = b->fd_iy, *P = b->fd_iyp;
+  for (int j = 1; j <= ny + 1; j++) {
+    yp[j] = (y[j - 1] + y[j]) * 0.5;
+    hi[j] = y[j] - y[j - 1];
+  }
+  for (int j = 2; j <= ny + 1; j++) h[j] = yp[j] - yp[j - 1];
+  a = hi[2] / 2; bb = a + h[3]; c = bb + h[4]; d = c + h[5];
+  F8(Y, 1, 1) = -c * bb * d / (-d + a) / (-c + a) / (-bb + a);
+  F8(Y, 2, 1) = c * a * d / (-d + bb) / (-c + bb) / (-bb + a);
+  F8(Y, 3, 1) = -bb * a * d / (-d + c) / (sq(c) - a * c - bb * c + bb * a);
+  F8(Y, 4, 1) = 1.0 - (F8(Y, 1, 1) + F8(Y, 2, 1) + F8(Y, 3, 1));
+  a = hi[3] / 2; bb = a + h[4]; c = bb + h[5]; d = c + h[6]; e = hi[2] / 2;
+  F8(Y, 1, 2) = 2 * c * bb * sq(a) * d / (e + d) / (e + c) / (bb + e) / (2 * sq(a) + sq(e) + 3 * e * a);
+  F8(Y, 2, 2) = -2 * c * bb * e * d / (a - d) / (a - c) / (a - bb) / (a + e);
+  F8(Y, 3, 2) = 2 * c * e * sq(a) * d / (-d + bb) / (-c + bb) /
+                (2 * sq(a) * e + 2 * sq(a) * bb - 3 * a * sq(bb) - 3 * a * e * bb + cu(bb) + e * sq(bb));
+  F8(Y, 4, 2) = -2 * sq(a) * bb * e * d / (-d + c) /
+                (3 * e * a * sq(c) - 2 * e * sq(a) * c - 3 * bb * e * a * c - 2 * sq(a) * sq(c) +
+                 2 * sq(a) * e * bb + 3 * a * cu(c) + bb * cu(c) - e * cu(c) + 2 * bb * sq(a) * c -
+                 3 * bb * a * sq(c) + bb * e * sq(c) - sq(sq(c)));
+  F8(Y, 7, 2) = 1.0;
+  F8(Y, 8, 2) = -c * bb * e * d / (2 * a + e) / (2 * a - d) / (2 * a - c) / (2 * a - bb);
+  F8(Y, 5, 2) = (F8(Y, 7, 2) + F8(Y, 8, 2)) - (F8(Y, 1, 2) + F8(Y, 2, 2) + F8(Y, 3, 2) + F8(Y, 4, 2));
+  for (int j = 3; j <= ny - 1; j++) {
+    a = hi[j + 1] / 2; bb = a + h[j + 2]; e = hi[j] / 2; f = e + h[j];
+    F8(Y, 1, j) = 4 * sq(e) * sq(a) * bb / (bb + f) /
+                  (2 * sq(a) * sq(f) + sq(sq(f)) + 4 * sq(e) * sq(a) - 6 * sq(a) * e * f - 9 * e * a * sq(f) +
+                   6 * sq(e) * a * f + 2 * sq(e) * sq(f) + 3 * a * cu(f) - 3 * e * cu(f));
+    F8(Y, 2, j) = -4 * f * sq(a) * bb / (bb + e) / (a + e) / (e - f) / (2 * a + e);
+    F8(Y, 3, j) = -4 * f * sq(e) * bb / (a - bb) / (a + f) / (2 * sq(e) + sq(a) + 3 * e * a);
+    F8(Y, 6, j) = -f * bb * sq(a) / (a + e) / (-f + 2 * e) / (bb + 2 * e) / (a + 2 * e);
+    F8(Y, 7, j) = 1.0;
+    F8(Y, 8, j) = -f * sq(e) * bb / (2 * a - bb) / (2 * a + f) / (2 * sq(a) + sq(e) + 3 * e * a);
+    F8(Y, 4, j) = (F8(Y, 6, j) + F8(Y, 7, j) + F8(Y, 8, j)) - (F8(Y, 1, j) + F8(Y, 2, j) + F8(Y, 3, j));
+  }
+  a = hi[ny + 1] / 2; e = hi[ny] / 2; f = e + h[ny]; g = f + h[ny - 1];
+  F8(Y, 1, ny) = 2 * (4 * sq(e) + 2 * e * a - a * f - 2 * f * e) * sq(e) / (-g + e) / (f - g) / g / (g + a);
+  F8(Y, 2, ny) = -2 * (2 * e * a + 4 * sq(e) - a * g - 2 * g * e) * sq(e) / (e - f) / f / (sq(f) + a * f - g * f - a * g);
+  F8(Y, 3, ny) = 2 * (8 * cu(e) - 4 * g * sq(e) - 2 * f * a * e - 2 * g * e * a - 4 * f * sq(e) + g * a * f +
+                      4 * a * sq(e) + 2 * f * g * e) /
+                 (cu(e) + a * sq(e) - f * sq(e) - g * sq(e) + g * a * f - g * e * a + f * g * e - f * a * e);
+  F8(Y, 4, ny) = 2 * (-2 * f * e - 2 * g * e + g * f + 4 * sq(e)) * sq(e) / (g + a) / (a + f) / (a + e) / a;
+  F8(Y, 6, ny) = 1.0;
+  F8(Y, 7, ny) = (8 * cu(e) - 4 * g * sq(e) - 2 * f * a * e - 2 * g * e * a - 4 * f * sq(e) + g * a * f +
+                  4 * a * sq(e) + 2 * f * g * e) / g / a / f;
+  memcpy(P, Y, 8 * (size_t)(ny + 1) * sizeof(double));
+  {
+    int j = ny - 1;
+    a = hi[j + 1] / 2; e = hi[j] / 2; f = e + h[j];
+    F8(P, 1, j) = 4 * sq(e) * sq(a) / (2 * a + f) / (a + f) / (2 * sq(e) + sq(f) - 3 * e * f);
+    F8(P, 2, j) = -4 * f * sq(a) / (3 * e * a + sq(e) + 2 * sq(a)) / (e - f);
+    F8(P, 3, j) = 4 * sq(e) * f / (cu(a) + 2 * sq(e) * a + 3 * e * sq(a) + 2 * sq(e) * f + f * sq(a) + 3 * f * e * a);
+    F8(P, 6, j) = -f * sq(a) / (a + e) / (2 * e - f) / (2 * e + a);
+    F8(P, 7, j) = 1.0;
+    F8(P, 8, j) = sq(e) * f / (4 * cu(a) + 2 * sq(e) * a + 6 * e * sq(a) + 3 * f * e * a + sq(e) * f + 2 * f * sq(a));
+    j = ny;
+    e = hi[j] / 2; f = e + h[j]; g = f + h[j - 1];
+    F8(P, 1, j) = 2 * sq(e) * (2 * e - f) / (e - g) / (f - g) / g;
+    F8(P, 2, j) = -2 * (-g + 2 * e) * sq(e) / (e - f) / f / (f - g);
+    F8(P, 3, j) = 2 * (4 * sq(e) + g * f - 2 * e * f - 2 * g * e) / (sq(e) - e * f + g * f - g * e);
+    F8(P, 6, j) = 1.0;
+    F8(P, 7, j) = (4 * sq(e) + g * f - 2 * e * f - 2 * g * e) / g / f;
+  }
+  f8_factor(P, ny, 3);
+  f8_factor(Y, ny, 3);
+  free(h);
+  free(hi);
+  free(yp);
+}
+
+/* ------------------------------------------------------------------------
  * Box
  * ---------------------------------------------------------------------- */
 #define PL(b, a, i, R) ((a) + (size_t)((i)-1) * (size_t)(b)->nz1 * (R))
@@ -1112,6 +1320,7 @@ oc_box *oc_create(int nx, int ny, int nz, int xout, double re, double ax,
   b->ayp = zalloc(n3y); b->vmagic = zalloc(nx + 1);
   b->um = zalloc(ny + 2); b->u0 = zalloc(ny + 2); b->v0 = zalloc(ny + 1);
   coef(b);
+  stats_coef(b);
 
   b->nu = (size_t)nz1 * (ny + 1) * nx;
   b->nv = (size_t)nz1 * ny * nx;
@@ -1165,6 +1374,8 @@ void oc_destroy(oc_box *b) {
   }
   free(b->tls);
   for (int f = 0; f < OC_NFIELDS; f++) free(b->fld[f]);
+  free(b->fd_dvdy); free(b->fd_dx); free(b->fd_ix); free(b->fd_iy); free(b->fd_iyp);
+  free(b->st); free(b->ens);
   free(b);
 }
 
@@ -1965,3 +2176,317 @@ void oc_retheta_cf(const oc_box *b, double *rth, double *cf) {
 }
 
 int oc_max_threads(void) { return omp_get_max_threads(); }
+
+/* ========================================================================
+ * Statistics: statsp (stotf.F90:51-291) as the production build compiles it
+ * (makefile-OMP-vul:11: -DNOCORR -DNOSPECTRA, dissipation and triple
+ * products on), with the prologue of rhsp.F90:72-102 that forms its x
+ * interpolation and x derivatives.
+ * ====================================================================== */
+static const char *const kStatNames[OC_NSTAT] = {
+    "us", "vs", "ws", "uv", "ua", "va", "wa", "pp", "pm", "vortx", "vorty", "vortz", "vortxa", "vortya",
+    "vortza", "uw", "vw", "dispu", "dispv", "dispw", "dispuv", "pup", "pvp", "pdudx", "pdvdy", "pdudy",
+    "pdvdx", "pdwdz", "v3", "u2v", "v2u", "w2v", "dudx0", "dudy0", "dvdx0", "dvdy0", "dwdx0", "dwdy0",
+    "u3", "w2u"};
+
+const char *oc_stat_name(int id) { return (id >= 0 && id < OC_NSTAT) ? kStatNames[id] : NULL; }
+
+/* the tridiagonal back end shared by the four operators on the line u
+ * (row j at u[(j-1) s]): forward from row j0 (multipliers in row 6), last
+ * row times its inverse pivot, backward to row j1 (lip.f90:690-697,
+ * 725-731, 759-765, 801-807) */
+static void f8_solve(const double *c, double *u, size_t s, int n, int j0, int j1) {
+  for (int j = j0; j <= n; j++) u[(size_t)(j - 1) * s] -= F8(c, 6, j) * u[(size_t)(j - 2) * s];
+  u[(size_t)(n - 1) * s] *= F8(c, 7, n);
+  for (int j = n - 1; j >= j1; j--)
+    u[(size_t)(j - 1) * s] = (u[(size_t)(j - 1) * s] - F8(c, 8, j) * u[(size_t)j * s]) * F8(c, 7, j);
+}
+
+/* diffx_inplace (lip.f90:704-732) on pencils (nx, nlin) */
+static void diffx_pen(const oc_box *b, const double *in, double *out, int nlin) {
+  const int nx = b->nx;
+  const double *c = b->fd_dx;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nlin; k++) {
+    const double *g = in + (size_t)k * nx - 1; /* g[1..nx] */
+    double *o = out + (size_t)k * nx - 1;
+    for (int r = 0; r < 2; r++) { /* rows 1, 2 and nx-1, nx */
+      int i = 1 + r, ie = nx - 1 + r;
+      o[i] = o[ie] = 0.0;
+      for (int q = 1; q <= 5; q++) {
+        o[i] += F8(c, q, i) * g[q];
+        o[ie] += F8(c, q, ie) * g[nx - 5 + q];
+      }
+    }
+    for (int i = 3; i <= nx - 2; i++) {
+      double t = 0.0;
+      for (int q = 1; q <= 5; q++) t += F8(c, q, 3) * g[i - 3 + q];
+      o[i] = t;
+    }
+    f8_solve(c, o + 1, 1, nx, 2, 1);
+  }
+}
+
+/* interpx_new (lip.f90:737-766) on pencils */
+static void interpx_pen(const oc_box *b, const double *in, double *out, int nlin) {
+  const int nx = b->nx;
+  const double *c = b->fd_ix;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nlin; k++) {
+    const double *g = in + (size_t)k * nx - 1;
+    double *o = out + (size_t)k * nx - 1;
+    o[1] = o[nx - 1] = o[nx] = 0.0;
+    for (int q = 1; q <= 4; q++) {
+      o[1] += F8(c, q, 1) * g[q];
+      o[nx - 1] += F8(c, q, nx - 1) * g[nx - 4 + q];
+      o[nx] += F8(c, q, nx) * g[nx - 4 + q];
+    }
+    for (int i = 2; i <= nx - 2; i++) {
+      double t = 0.0;
+      for (int q = 1; q <= 4; q++) t += F8(c, q, 3) * g[i - 2 + q];
+      o[i] = t;
+    }
+    f8_solve(c, o + 1, 1, nx, 2, 1);
+  }
+}
+
+/* diffy_inplace (lip.f90:666-698) on the ny rows of one plane (stride s) */
+static void diffy_plane(const oc_box *b, const double *in, double *out, int s) {
+  const int ny = b->ny;
+  const double *c = b->fd_dvdy;
+  for (int k = 0; k < s; k++) {
+    RW(out, s, k, 1) = RW(out, s, k, 2) = RW(out, s, k, ny - 1) = RW(out, s, k, ny) = 0.0;
+    for (int q = 1; q <= 4; q++) {
+      RW(out, s, k, 1) += F8(c, q, 1) * RW(in, s, k, q);
+      RW(out, s, k, 2) += F8(c, q, 2) * RW(in, s, k, q);
+      RW(out, s, k, ny - 1) += F8(c, q, ny - 1) * RW(in, s, k, ny - 4 + q);
+      RW(out, s, k, ny) += F8(c, q, ny) * RW(in, s, k, ny - 4 + q);
+    }
+    for (int j = 3; j <= ny - 2; j++) {
+      double t = 0.0;
+      for (int q = 1; q <= 5; q++) t += F8(c, q, j) * RW(in, s, k, j - 3 + q);
+      RW(out, s, k, j) = t;
+    }
+    f8_solve(c, &RW(out, s, k, 1), s, ny, 2, 1);
+  }
+}
+
+/* interpy_new (lip.f90:771-809) on one plane: tipo 0 takes ny+1 rows (u, w
+ * and their x derivatives; the wall value is 0), tipo 1 the ny rows of p */
+static void interpy_plane(const oc_box *b, const double *in, double *out, int s, int tipo) {
+  const int ny = b->ny, ch = tipo == 1 ? 0 : 1;
+  const double *c = tipo == 1 ? b->fd_iyp : b->fd_iy;
+  for (int k = 0; k < s; k++) {
+    double r1 = 0.0, r2 = 0.0, rm = 0.0, rn = 0.0;
+    for (int q = 1; q <= 4; q++) r1 += F8(c, q, 1) * RW(in, s, k, 1 + q);
+    for (int q = 1; q <= 5; q++) r2 += F8(c, q, 2) * RW(in, s, k, 1 + q);
+    RW(out, s, k, 1) = r1;
+    RW(out, s, k, 2) = r2;
+    for (int j = 3; j <= ny - 2; j++) {
+      double t = 0.0;
+      for (int q = 1; q <= 4; q++) t += F8(c, q, j) * RW(in, s, k, j - 2 + q);
+      RW(out, s, k, j) = t;
+    }
+    if (tipo == 0) RW(out, s, k, 1) = 0.0;
+    for (int q = 1; q <= 3 + ch; q++) {
+      rm += F8(c, q, ny - 1) * RW(in, s, k, ny - 3 + q);
+      rn += F8(c, q, ny) * RW(in, s, k, ny - 3 + q);
+    }
+    RW(out, s, k, ny - 1) = rm;
+    RW(out, s, k, ny) = rn;
+    f8_solve(c, &RW(out, s, k, 1), s, ny, 3, 2); /* rows 2..ny (lip.f90:801-807) */
+  }
+}
+
+#define ST(b, id, i, j) (b)->st[((size_t)(id) * (b)->nx + (size_t)((i)-1)) * ((b)->ny + 1) + (size_t)((j)-1)]
+
+void oc_statsp(oc_box *b) {
+  const int nx = b->nx, ny = b->ny, nz = b->nz, s = b->nz1, ru = ny + 1, nz2 = b->nz2;
+  const int mpu = s * ru, mpv = s * ny;
+  const double *u = b->fld[OC_U], *v = b->fld[OC_V], *w = b->fld[OC_W], *p = b->fld[OC_P];
+  const double *kaz = b->kaz;
+  if (!b->st) b->st = zalloc((size_t)OC_NSTAT * nx * (ny + 1));
+  if (!b->ens) {
+    b->ens = (float *)calloc((size_t)nz * ny * nx, sizeof(float));
+    if (!b->ens) abort();
+  }
+  /* prologue (rhsp.F90:67-69, 72-90): u_x = interpx_new(u) moved one plane
+   * up (u_x(1) keeps the plane-1 value), dudx = diffx(u_x), dvdx = diffx(v),
+   * dwdx = diffx(w), each back to planes */
+  double *ux = zalloc(b->nu), *dudx = zalloc(b->nu), *dvdx = zalloc(b->nv), *dwdx = zalloc(b->nu);
+  double *pa = zalloc(b->nu), *pb = zalloc(b->nu);
+  chp2x(b, b->pen_u, u, ru);
+  chp2x(b, b->pen_v, v, ny);
+  chp2x(b, b->pen_w, w, ru);
+  interpx_pen(b, b->pen_u, pa, mpu);
+  for (int l = 0; l < mpu; l++) {
+    double *q = pa + (size_t)l * nx - 1;
+    for (int i = nx - 1; i >= 1; i--) q[i + 1] = q[i];
+  }
+  diffx_pen(b, pa, pb, mpu);
+  chx2p(b, ux, pa, ru);
+  chx2p(b, dudx, pb, ru);
+  diffx_pen(b, b->pen_v, pa, mpv);
+  chx2p(b, dvdx, pa, ny);
+  diffx_pen(b, b->pen_w, pa, mpu);
+  chx2p(b, dwdx, pa, ru);
+  free(pa);
+  free(pb);
+
+#pragma omp parallel
+  {
+    /* buf1 = (u_x)_y, buf2 = w_y, buf3 = (dwdx)_y, buf4 = (dudx)_y, buf5 = p_y,
+     * buf6 = dudy, buf7 = dwdy, buf8 = dvdy, all at the v rows (stotf.F90:105-116) */
+    double *bf[9];
+    for (int q = 1; q <= 8; q++) bf[q] = zalloc((size_t)s * ny);
+    double *fo = zalloc(s), *ph[9];
+    for (int q = 0; q < 9; q++) ph[q] = zalloc(nz);
+#pragma omp for schedule(static)
+    for (int i = 1; i <= nx; i++) {
+      const double *vi = PL(b, v, i, ny), *dvi = PL(b, dvdx, i, ny);
+      interpy_plane(b, PL(b, ux, i, ru), bf[1], s, 0);
+      interpy_plane(b, PL(b, w, i, ru), bf[2], s, 0);
+      interpy_plane(b, PL(b, dwdx, i, ru), bf[3], s, 0);
+      interpy_plane(b, PL(b, dudx, i, ru), bf[4], s, 0);
+      interpy_plane(b, PL(b, p, i, ny), bf[5], s, 1);
+      diffy_plane(b, bf[1], bf[6], s);
+      diffy_plane(b, bf[2], bf[7], s);
+      diffy_plane(b, vi, bf[8], s);
+      for (int j = 1; j <= ny; j++) {
+#define CR(a, k) RW(a, s, 2 * (k), j)
+#define CI(a, k) RW(a, s, 2 * (k) + 1, j)
+        const double *B1 = bf[1], *B2 = bf[2], *B3 = bf[3], *B4 = bf[4], *B5 = bf[5], *B6 = bf[6], *B7 = bf[7],
+                     *B8 = bf[8];
+        /* k = 0: means (stotf.F90:128-144) */
+        ST(b, 4, i, j) += CR(B1, 0);
+        ST(b, 6, i, j) += CR(B2, 0);
+        ST(b, 5, i, j) += CR(vi, 0);
+        ST(b, 8, i, j) += CR(B5, 0);
+        ST(b, 32, i, j) += CR(B4, 0);
+        ST(b, 34, i, j) += CR(dvi, 0);
+        ST(b, 36, i, j) += CR(B3, 0);
+        ST(b, 33, i, j) += CR(B6, 0);
+        ST(b, 35, i, j) += CR(B8, 0);
+        ST(b, 37, i, j) += CR(B7, 0);
+        ST(b, 13, i, j) -= CR(B3, 0);
+        ST(b, 12, i, j) += CR(B7, 0);
+        ST(b, 14, i, j) += CR(dvi, 0) - CR(B6, 0);
+        /* the nine velocity gradients in physical space, enstrophy
+         * (stotf.F90:148-154); d/dz multiplies by kaz = i k/az */
+        const double *grad[9] = {B4, B6, B1, dvi, B8, vi, B3, B7, B2}; /* dudx dudy dudz dvdx dvdy dvdz dwdx dwdy dwdz */
+        for (int q = 0; q < 9; q++) {
+          const int dz = (q % 3) == 2;
+          for (int k = 0; k <= nz2; k++) {
+            double re = CR(grad[q], k), im = CI(grad[q], k);
+            fo[2 * k] = dz ? -kaz[k] * im : re;
+            fo[2 * k + 1] = dz ? kaz[k] * re : im;
+          }
+          oc_c2r(b, fo, ph[q]);
+        }
+        float *ens = b->ens + ((size_t)(i - 1) * ny + (j - 1)) * nz;
+        for (int z = 0; z < nz; z++)
+          ens[z] = (float)sqrt(sq(ph[7][z] - ph[5][z]) + sq(ph[2][z] - ph[6][z]) + sq(ph[3][z] - ph[1][z]));
+        /* sums over the modes (stotf.F90:156-218) */
+        for (int k = 0; k <= nz2; k++) {
+          const double cte = k == 0 ? 1.0 : 2.0, kz = kaz[k];
+          const double b1r = CR(B1, k), b1i = CI(B1, k), b2r = CR(B2, k), b2i = CI(B2, k);
+          const double b3r = CR(B3, k), b3i = CI(B3, k), b4r = CR(B4, k), b4i = CI(B4, k);
+          const double b5r = CR(B5, k), b5i = CI(B5, k), b6r = CR(B6, k), b6i = CI(B6, k);
+          const double b7r = CR(B7, k), b7i = CI(B7, k), b8r = CR(B8, k), b8i = CI(B8, k);
+          const double vr = CR(vi, k), vim = CI(vi, k), dr = CR(dvi, k), di = CI(dvi, k);
+          /* z derivatives: a kaz = (-kz a_i, kz a_r) */
+          const double uzr = -kz * b1i, uzi = kz * b1r, vzr = -kz * vim, vzi = kz * vr;
+          const double wzr = -kz * b2i, wzi = kz * b2r;
+#define RE(ar, ai, br, bi) ((ar) * (br) + (ai) * (bi)) /* Re(a conj(b)) */
+          ST(b, 0, i, j) += cte * RE(b1r, b1i, b1r, b1i);
+          ST(b, 2, i, j) += cte * RE(b2r, b2i, b2r, b2i);
+          ST(b, 1, i, j) += cte * RE(vr, vim, vr, vim);
+          ST(b, 3, i, j) += cte * RE(b1r, b1i, vr, vim);
+          ST(b, 15, i, j) += cte * RE(b1r, b1i, b2r, b2i);
+          ST(b, 16, i, j) += cte * RE(vr, vim, b2r, b2i);
+          ST(b, 21, i, j) += cte * RE(b5r, b5i, b1r, b1i);
+          ST(b, 22, i, j) += cte * RE(b5r, b5i, vr, vim);
+          ST(b, 7, i, j) += cte * RE(b5r, b5i, b5r, b5i);
+          ST(b, 27, i, j) += cte * RE(b5r, b5i, wzr, wzi);
+          ST(b, 23, i, j) += cte * RE(b5r, b5i, b4r, b4i);
+          ST(b, 26, i, j) += cte * RE(b5r, b5i, dr, di);
+          ST(b, 24, i, j) += cte * RE(b5r, b5i, b8r, b8i);
+          ST(b, 25, i, j) += cte * RE(b5r, b5i, b6r, b6i);
+          double ar = b7r - vzr, ai = b7i - vzi;
+          ST(b, 9, i, j) += cte * (ar * ar + ai * ai);
+          ar = uzr - b3r; ai = uzi - b3i;
+          ST(b, 10, i, j) += cte * (ar * ar + ai * ai);
+          ar = dr - b6r; ai = di - b6i;
+          ST(b, 11, i, j) += cte * (ar * ar + ai * ai);
+          ST(b, 17, i, j) += cte * (RE(b4r, b4i, b4r, b4i) + RE(uzr, uzi, uzr, uzi) + RE(b6r, b6i, b6r, b6i));
+          ST(b, 18, i, j) += cte * (RE(dr, di, dr, di) + RE(vzr, vzi, vzr, vzi) + RE(b8r, b8i, b8r, b8i));
+          ST(b, 19, i, j) += cte * (RE(wzr, wzi, wzr, wzi) + RE(b3r, b3i, b3r, b3i) + RE(b7r, b7i, b7r, b7i));
+          ST(b, 20, i, j) += cte * (RE(uzr, uzi, vzr, vzi) + RE(b4r, b4i, dr, di) + RE(b6r, b6i, b8r, b8i));
+#undef RE
+        }
+        /* triple products of u_y, v, w_y in physical space (stotf.F90:256-283) */
+        oc_c2r(b, &RW(B1, s, 0, j), ph[0]);
+        oc_c2r(b, &RW(B2, s, 0, j), ph[1]);
+        oc_c2r(b, &RW(vi, s, 0, j), ph[2]);
+        for (int z = 0; z < nz; z++) {
+          const double U = ph[0][z], W = ph[1][z], V = ph[2][z];
+          ST(b, 38, i, j) += U * U * U;
+          ST(b, 39, i, j) += U * W * W;
+          ST(b, 28, i, j) += V * V * V;
+          ST(b, 31, i, j) += V * W * W;
+          ST(b, 30, i, j) += U * V * V;
+          ST(b, 29, i, j) += V * U * U;
+        }
+#undef CR
+#undef CI
+      }
+    }
+    for (int q = 1; q <= 8; q++) free(bf[q]);
+    for (int q = 0; q < 9; q++) free(ph[q]);
+    free(fo);
+  }
+  b->ical += 1;
+  free(ux);
+  free(dudx);
+  free(dvdx);
+  free(dwdx);
+}
+
+void oc_stats_get(const oc_box *b, int id, double *out) {
+  const size_t n = (size_t)b->nx * (b->ny + 1);
+  if (b->st)
+    memcpy(out, b->st + (size_t)id * n, n * sizeof(double));
+  else
+    memset(out, 0, n * sizeof(double));
+}
+void oc_stats_reset(oc_box *b) {
+  if (b->st) memset(b->st, 0, (size_t)OC_NSTAT * b->nx * (b->ny + 1) * sizeof(double));
+  b->ical = 0;
+}
+int oc_stats_ical(const oc_box *b) { return b->ical; }
+const float *oc_stats_ens(const oc_box *b) { return b->ens; }
+const double *oc_stats_coef(const oc_box *b, const char *name) {
+  if (!strcmp(name, "fd_dvdy")) return b->fd_dvdy + 8;
+  if (!strcmp(name, "fd_dx")) return b->fd_dx + 8;
+  if (!strcmp(name, "fd_ix")) return b->fd_ix + 8;
+  if (!strcmp(name, "fd_iy")) return b->fd_iy + 8;
+  if (!strcmp(name, "fd_iyp")) return b->fd_iyp + 8;
+  return NULL;
+}
+/* the four operators on one plane / pencil set, for the tests:
+ * op 0 diffy (ny rows), 1 interpy u-type (ny+1 rows in), 2 interpy p (ny
+ * rows in) on nplanes planes of s = nz1 columns; op 3 diffx, 4 interpx on
+ * nlin pencils of nx points */
+void oc_stats_op(const oc_box *b, int op, const double *in, double *out, int n) {
+  const int s = b->nz1, ny = b->ny;
+  if (op <= 2) {
+    for (int i = 0; i < n; i++) {
+      if (op == 0) diffy_plane(b, in + (size_t)i * s * ny, out + (size_t)i * s * ny, s);
+      else interpy_plane(b, in + (size_t)i * s * (op == 1 ? ny + 1 : ny), out + (size_t)i * s * ny, s, op == 1 ? 0 : 1);
+    }
+  } else if (op == 3) {
+    diffx_pen(b, in, out, n);
+  } else {
+    interpx_pen(b, in, out, n);
+  }
+}

@@ -88,6 +88,28 @@ void oc_r2c(const oc_box *b, const double *phys, double *fou);
 void oc_cosft(int n, int nlines, double *c, int iopt);
 /* Per-x diagnostics from the kz=0 mode of u: Re_theta(x), Cf(x). */
 void oc_retheta_cf(const oc_box *b, double *rth, double *cf);
+/* Statistics (stotf.F90:51-291 built with -DNOCORR -DNOSPECTRA,
+ * makefile-OMP-vul:11; the x prologue of rhsp.F90:72-102).  oc_statsp adds
+ * one sample of the current u, v, w, p to the OC_NSTAT accumulators, each
+ * (ny+1, nx) in Fortran order (row ny+1 and the rows the reference leaves
+ * unset stay 0), writes the enstrophy field ens(nz, ny, nx) (FP32) and
+ * counts the sample (ical).  Ids follow kStatNames: us vs ws uv ua va wa pp
+ * pm vortx vorty vortz vortxa vortya vortza uw vw dispu dispv dispw dispuv
+ * pup pvp pdudx pdvdy pdudy pdvdx pdwdz v3 u2v v2u w2v dudx0 dudy0 dvdx0
+ * dvdy0 dwdx0 dwdy0 u3 w2u. */
+#define OC_NSTAT 40
+void oc_statsp(oc_box *b);
+const char *oc_stat_name(int id);
+void oc_stats_get(const oc_box *b, int id, double *out);
+void oc_stats_reset(oc_box *b);
+int oc_stats_ical(const oc_box *b);
+const float *oc_stats_ens(const oc_box *b);
+/* fd_dvdy, fd_dx, fd_ix, fd_iy, fd_iyp as (8, n) Fortran arrays from column 1 */
+const double *oc_stats_coef(const oc_box *b, const char *name);
+/* op 0 diffy_inplace, 1 interpy_new (tipo 0), 2 interpy_new (tipo 1) on n
+ * planes; 3 diffx_inplace, 4 interpx_new on n pencils of nx points */
+void oc_stats_op(const oc_box *b, int op, const double *in, double *out, int n);
+
 /* OpenMP threads the oracle's parallel loops use. */
 int oc_max_threads(void);
 

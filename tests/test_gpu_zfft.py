@@ -127,4 +127,4 @@ def test_step_long_x(xbox, oracle_lib):
     gb.step(1)
     errs = state_errors(gb, ob)
     gb.close()
-    assert_state(errs)
+    assert_state(errs, nx=cfg.nx)
