@@ -368,7 +368,7 @@ def main():
             # the mixed substep's specification: velocity within 1e-5 of the
             # FP64 run; each velocity component is judged against its own
             # maximum as well as against the free stream
-            runs[name]["within_target"] = {n: err[n] <= VEL_TARGET for n in ("vel", "u", "v", "w")}
+            runs[name]["within_target"] = {n: bool(err[n] <= VEL_TARGET) for n in ("vel", "u", "v", "w")}
         del vel
     del st, ref
     head = "mixed" if "mixed" in runs else "fp64"
@@ -406,7 +406,16 @@ def main():
         out["cpu_baseline"] = cpu_baseline(cfg)
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=json_default), flush=True)
+
+
+def json_default(o):
+    """numpy scalars and arrays in the output line as plain JSON values."""
+    if isinstance(o, np.generic):
+        return o.item()
+    if isinstance(o, np.ndarray):
+        return o.tolist()
+    raise TypeError(f"{type(o).__name__} is not JSON serializable")
 
 
 if __name__ == "__main__":

@@ -105,7 +105,9 @@ int otbl_retheta_cf(otbl_ctx *ctx, double *rth, double *cf);
 int otbl_set_time(otbl_ctx *ctx, double tiempo, double timeinit, double dt);
 int otbl_get_time(otbl_ctx *ctx, double *tiempo, double *dt);
 /* Coefficient arrays after setup, by their reference names (inxu, dcbx,
- * vyui, ayp, kmod, ...), in Fortran order; rows/cols give the shape. */
+ * vyui, ayp, kmod, ..., and the stats-only schemes fd_dvdy, fd_iy, fd_iyp,
+ * fd_dx, fd_ix of coeft.f90:56-336, LU-factored), in Fortran order;
+ * rows/cols give the shape. */
 int otbl_get_coef(otbl_ctx *ctx, const char *name, double *out, int *rows, int *cols);
 
 /* ---- the RK3 substep (bl_1.F90:94-125) ---- */
@@ -166,6 +168,47 @@ int otbl_soltriy(otbl_ctx *ctx, const double *u, double *us, int ii, double *a, 
 /* coefinv (coeft.f90:1338-1353): in-place LU of the caller's host array
  * a(3, npter1:neq), computed on the GPU. */
 int otbl_coefinv(otbl_ctx *ctx, double *a, int npter1, int neq);
+
+/* ---- statistics (stotf.F90:51-291) ----
+ * The reference's statsp as its production build compiles it
+ * (makefile-OMP-vul:11: -DNOCORR -DNOSPECTRA, so no correlations and no
+ * z-spectra; dissipation and triple products on), with the x interpolation
+ * and x derivatives of rhsp.F90:72-102 that feed it.  A sample is taken
+ * from the state at the start of substep 1, in the same captured graph as
+ * the substep.  The 40 accumulators are (ny+1, nx) Fortran arrays (row ny+1
+ * is never written, as in the reference, and rows the reference sizes ny
+ * are returned with a zero row ny+1), named and ordered as the reference's
+ * statistics files write them (escru.F90:318-341, 490-516):
+ *   us vs ws uv ua va wa pp pm vortx vorty vortz vortxa vortya vortza uw vw
+ *   dispu dispv dispw dispuv pup pvp pdudx pdvdy pdudy pdvdx pdwdz v3 u2v
+ *   v2u w2v dudx0 dudy0 dvdx0 dvdy0 dwdx0 dwdy0 u3 w2u
+ * The enstrophy field of the last sample, ens(nz, ny, nx) in 4-byte reals
+ * (stotf.F90:150, written by escru.F90:159-170), is kept beside them.  FP64
+ * arithmetic; a mixed box samples its FP32 fields. */
+#define OTBL_NSTAT 40
+const char *otbl_stats_name(int id); /* null outside 0..OTBL_NSTAT-1 */
+/* allocates (on) or frees (off) the accumulators, cleared; nz <= 4096 */
+int otbl_stats_enable(otbl_ctx *ctx, int on);
+/* the sampling rule of bl_1.F90:92: a sample at substep 1 of every step
+ * istep with mod(istep, avcrt) == 0 and istep > stest, istep counting the
+ * steps of otbl_step / otbl_ensemble_step from 1 after this call
+ * (avcrt = 0: no scheduled samples) */
+int otbl_stats_schedule(otbl_ctx *ctx, int avcrt, int stest);
+/* dostat = .TRUE. (bl_1.F90:92): the next substep 1 (otbl_rhsp with m = 1,
+ * otbl_substep, otbl_step) takes a sample first */
+int otbl_stats_request(otbl_ctx *ctx);
+/* one sample of the current state now (statsp, with the x prologue of
+ * rhsp.F90:72-90; uses the box's work fields, so not during a substep) */
+int otbl_statsp(otbl_ctx *ctx);
+/* accumulator id as (ny+1) x nx doubles, Fortran order (host memory) */
+int otbl_stats_get(otbl_ctx *ctx, int id, double *out);
+/* samples accumulated since the last reset (ical, rhsp.F90:92) */
+int otbl_stats_ical(otbl_ctx *ctx, int *ical);
+/* clears the accumulators and ical (escru.F90:412-414, 579-585, after the
+ * reference writes its statistics files) */
+int otbl_stats_reset(otbl_ctx *ctx);
+/* the enstrophy field of the last sample: nz * ny * nx floats (host) */
+int otbl_stats_get_ens(otbl_ctx *ctx, float *out);
 
 /* ---- ensemble of independent boxes ----
  * No counterpart in the reference (one OpenTBL run advances one box): the

@@ -52,6 +52,34 @@ int otbl_io_read_header(const char *path, otbl_io_header *h, char *variable, siz
  * rank). */
 int otbl_io_read_field(const char *path, double *field, int nx, int R, int nz1);
 
+/* The enstrophy file of escribezy (escru.F90:159-170): "value" the
+ * enstrophy field ens(nz, ny, nx) of the last statistics sample (4-byte
+ * reals, C order [nx][ny][nz]) with the writeheader header, Variable
+ * "enstrophy". */
+int otbl_io_write_ens(const char *path, const otbl_io_header *h, const double *y, const double *um, const float *ens,
+                      int nz);
+
+/* The statistics files of escrst (escru.F90:263-341, 461-516), Fortran
+ * sequential unformatted records, big-endian:
+ *   <st_path>  (the reference's .st): record 1 tiempo, cfl, Re, ax, ay, az
+ *              (8-byte reals), nx, ny, nz2, ical (4-byte integers); record
+ *              2 y(0:ny+1); record 3 one 8-byte 0; then one record per
+ *              plane i: us(1:ny+1), vs(1:ny), ws(1:ny+1), uv(1:ny),
+ *              ua(1:ny+1), va(1:ny), wa(1:ny+1), pp, pm, vortx, vorty,
+ *              vortz, vortxa, vortya, vortza, uw, vw (1:ny each);
+ *   <budget_path> (.budget): records 1-2 as above, then per plane dispu,
+ *              dispv, dispw, dispuv, pup, pvp, pdudx, pdvdy, pdudy, pdvdx,
+ *              pdwdz, v3, u2v, v2u, w2v, dudx0, dudy0, dvdx0, dvdy0, dwdx0,
+ *              dwdy0 (1:ny each), u3(1:ny+1), w2u(1:ny+1).
+ * stats holds the 40 accumulators of otbl_stats_get in the library's order
+ * (include/opentbl.h), each (ny+1, nx).  Either path may be null. */
+typedef struct otbl_io_stats_header {
+  double tiempo, cfl, re, ax, ay, az; /* ax, ay, az as bl_1 passes them (lengths / pi, / 2 pi) */
+  int nx, ny, nz2, ical;
+} otbl_io_stats_header;
+int otbl_io_write_stats(const char *st_path, const char *budget_path, const otbl_io_stats_header *h,
+                        const double *y, const double *stats);
+
 const char *otbl_io_last_error(void);
 
 #ifdef __cplusplus

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libopentbl_amd.so")
 SOURCES = ["otbl_api.cpp", "otbl_coef.cpp", "otbl_kernels.hip", "otbl_ycol.hip", "otbl_zfft.hip", "otbl_xline.hip",
-           "otbl_poisson.hip"]
+           "otbl_poisson.hip", "otbl_stats.hip"]
 ARCH = os.environ.get("OTBL_ARCH", "gfx950")
 
 

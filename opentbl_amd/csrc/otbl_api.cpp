@@ -316,13 +316,14 @@ static void free_box(Box *b) {
                     b->wki3, b->ya, b->yb, b->yc, b->rest, b->rest2, b->dct2, b->dct3, b->implu,
                     b->implv, b->um, b->u0, b->v0, b->gwork, b->dct_chirp, b->dct_bf, b->dct_tw4,
                     b->ytab, b->ztw, b->zrow, b->dct_ztw, b->rowmax, b->P0, b->S0, b->p0, b->um64, b->rm64,
-                    b->diag, b->op64, b->rthcf};
+                    b->diag, b->op64, b->rthcf, b->st, b->stcoef, b->stmean};
   for (double *p : bufs)
     if (p) (void)hipFree(p);
   if (b->diag_count) (void)hipFree(b->diag_count);
   float *fbufs[] = {b->coefbuf_f, b->ytab_f, b->ytabh_f, b->ztw_f, b->dct_chirp_f, b->dct_bf_f, b->dct_tw4_f, b->dct_ztw_f};
   for (float *p : fbufs)
     if (p) (void)hipFree(p);
+  if (b->ens) (void)hipFree(b->ens);
   for (int f = 0; f < OTBL_NFIELDS; ++f)
     if (b->f[f]) (void)hipFree(b->f[f]);
   if (b->ds) (void)hipFree(b->ds);
@@ -335,26 +336,45 @@ static void free_box(Box *b) {
   delete b;
 }
 
-static void enqueue_substep(Box &b, int m) {
+// With stats, substep 1 first adds a statistics sample of the state it
+// starts from (rhsp.F90:72-102 at dostat, which bl_1.F90:92 sets before
+// substep 1 and rhsp.F90:488 clears).
+static void enqueue_substep(Box &b, int m, bool stats) {
+  if (stats) k_statsp(b);
   k_substep_rhsp(b, m);
   k_outflow(b);
   k_pois(b, m, 0.0, true);
 }
 
+// the sampling rule of bl_1.F90:92, at the start of every step
+static void begin_step(Box &b) {
+  ++b.istep;
+  if (b.st && b.avcrt > 0 && b.istep % b.avcrt == 0 && b.istep > b.stest) b.dostat = true;
+}
+
+// a sample requested for this substep (m = 1 only), then counted
+static bool take_stats(Box &b, int m) { return m == 1 && b.dostat && b.st; }
+static void stats_taken(Box &b) {
+  b.dostat = false;
+  ++b.ical; // rhsp.F90:92
+}
+
 static void run_substep(Box &b, int m) {
+  const bool stats = take_stats(b, m);
   if (!b.use_graphs || b.kprof) {
-    enqueue_substep(b, m);
+    enqueue_substep(b, m, stats);
+    if (stats) stats_taken(b);
     return;
   }
   // the substep swaps the velocity fields with their saved copies: one graph
   // per role assignment, and the host-side swap done again on replay
   const int fl = b.flip;
-  hipGraphExec_t &ge = b.gexec[m][fl];
+  hipGraphExec_t &ge = b.gexec[stats ? 0 : m][fl];
   if (!ge) {
     hipGraph_t g;
     OTBL_HIP(hipStreamBeginCapture(b.s, hipStreamCaptureModeThreadLocal));
     try {
-      enqueue_substep(b, m);
+      enqueue_substep(b, m, stats);
     } catch (...) {
       hipGraph_t dummy = nullptr;
       if (hipStreamEndCapture(b.s, &dummy) == hipSuccess && dummy) (void)hipGraphDestroy(dummy);
@@ -368,6 +388,7 @@ static void run_substep(Box &b, int m) {
   }
   OTBL_HIP(hipGraphLaunch(ge, b.s));
   k_swap_saved(b);
+  if (stats) stats_taken(b);
 }
 
 } // namespace otbl
@@ -836,6 +857,19 @@ int otbl_get_coef(otbl_ctx *ctx, const char *name, double *out, int *rows, int *
       for (int q = 0; q < r * n; ++q) out[q] = dv[(size_t)r * first + q];
     return OTBL_OK;
   }
+  // the stats-only schemes (8, 1:n), LU-factored, from the host set
+  const std::vector<double> *fd = !strcmp(name, "fd_dvdy") ? &b.hc.fd_dvdy
+                                  : !strcmp(name, "fd_iy")  ? &b.hc.fd_iy
+                                  : !strcmp(name, "fd_iyp") ? &b.hc.fd_iyp
+                                  : !strcmp(name, "fd_dx")  ? &b.hc.fd_dx
+                                  : !strcmp(name, "fd_ix")  ? &b.hc.fd_ix
+                                                            : nullptr;
+  if (fd) {
+    *rows = 8;
+    *cols = (int)(fd->size() / 8) - 1;
+    if (out) std::copy(fd->begin() + 8, fd->end(), out);
+    return OTBL_OK;
+  }
   const size_t n = fixed_len(name);
   const double *src = nullptr;
   int r = 4, c = 4;
@@ -857,7 +891,10 @@ int otbl_rhsp(otbl_ctx *ctx, int m) {
   API_BEGIN
   Box &b = box_of(ctx);
   need(m >= 1 && m <= 3, "substep m must be 1..3");
+  const bool stats = take_stats(b, m);
+  if (stats) k_statsp(b);
   k_substep_rhsp(b, m);
+  if (stats) stats_taken(b);
   API_END
 }
 
@@ -892,6 +929,7 @@ int otbl_step(otbl_ctx *ctx, int nsteps) {
   Box &b = box_of(ctx);
   need(nsteps >= 0, "nsteps must be >= 0");
   for (int s = 0; s < nsteps; ++s) {
+    begin_step(b);
     for (int m = 1; m <= 3; ++m) run_substep(b, m);
     k_end_step(b);
   }
@@ -1112,6 +1150,112 @@ int otbl_kprof_report(otbl_ctx *ctx, char *buf, size_t len) {
   API_END
 }
 
+// ---- statistics (stotf.F90:51-291) ----
+static const char *const kStatNames[OTBL_NSTAT] = {
+    "us", "vs", "ws", "uv", "ua", "va", "wa", "pp", "pm", "vortx", "vorty", "vortz", "vortxa", "vortya",
+    "vortza", "uw", "vw", "dispu", "dispv", "dispw", "dispuv", "pup", "pvp", "pdudx", "pdvdy", "pdudy",
+    "pdvdx", "pdwdz", "v3", "u2v", "v2u", "w2v", "dudx0", "dudy0", "dvdx0", "dvdy0", "dwdx0", "dwdy0",
+    "u3", "w2u"};
+
+const char *otbl_stats_name(int id) { return (id >= 0 && id < OTBL_NSTAT) ? kStatNames[id] : nullptr; }
+
+int otbl_stats_enable(otbl_ctx *ctx, int on) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  drop_graphs(b); // captured samples hold the old buffers
+  for (double **p : {&b.st, &b.stcoef, &b.stmean})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  if (b.ens) (void)hipFree(b.ens);
+  b.ens = nullptr;
+  b.ical = 0;
+  b.dostat = false;
+  if (on) {
+    need(k_stats_shm_bytes(b.nz) <= 160 * 1024 && b.nz <= 4096,
+         "otbl_stats_enable: nz above 4096 (the statistics row transforms run in LDS)");
+    dalloc(b.st, (size_t)OTBL_NSTAT * b.nx * (b.ny + 1));
+    const HostCoef &h = b.hc;
+    std::vector<double> c;
+    for (const std::vector<double> *v : {&h.fd_dvdy, &h.fd_iy, &h.fd_iyp, &h.fd_dx, &h.fd_ix})
+      c.insert(c.end(), v->begin(), v->end());
+    dalloc(b.stcoef, c.size());
+    h2d(b, b.stcoef, c.data(), c.size() * sizeof(double));
+    dalloc(b.ens, (size_t)b.nz * b.ny * b.nx);
+    if (b.precision == OTBL_MIXED) dalloc(b.stmean, 5 * (size_t)b.nx * (b.ny + 1));
+    k_stats_attrs(b);
+  }
+  API_END
+}
+
+int otbl_stats_schedule(otbl_ctx *ctx, int avcrt, int stest) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(b.st != nullptr, "otbl_stats_schedule: statistics are off (otbl_stats_enable)");
+  need(avcrt >= 0, "otbl_stats_schedule: avcrt must be >= 0");
+  b.avcrt = avcrt;
+  b.stest = stest;
+  b.istep = 0;
+  API_END
+}
+
+int otbl_stats_request(otbl_ctx *ctx) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(b.st != nullptr, "otbl_stats_request: statistics are off (otbl_stats_enable)");
+  b.dostat = true;
+  API_END
+}
+
+int otbl_statsp(otbl_ctx *ctx) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(b.st != nullptr, "otbl_statsp: statistics are off (otbl_stats_enable)");
+  k_statsp(b);
+  ++b.ical;
+  API_END
+}
+
+int otbl_stats_get(otbl_ctx *ctx, int id, double *out) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(b.st != nullptr, "otbl_stats_get: statistics are off (otbl_stats_enable)");
+  need(id >= 0 && id < OTBL_NSTAT && out, "otbl_stats_get: bad id or null pointer");
+  const size_t n = (size_t)b.nx * (b.ny + 1);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  d2h(b, out, b.st + (size_t)id * n, n * sizeof(double));
+  API_END
+}
+
+int otbl_stats_ical(otbl_ctx *ctx, int *ical) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(ical != nullptr, "otbl_stats_ical: null pointer");
+  *ical = b.ical;
+  API_END
+}
+
+int otbl_stats_reset(otbl_ctx *ctx) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(b.st != nullptr, "otbl_stats_reset: statistics are off (otbl_stats_enable)");
+  k_stats_zero(b);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  b.ical = 0;
+  API_END
+}
+
+int otbl_stats_get_ens(otbl_ctx *ctx, float *out) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(b.ens != nullptr && out, "otbl_stats_get_ens: statistics are off or null pointer");
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  d2h(b, out, b.ens, (size_t)b.nz * b.ny * b.nx * sizeof(float));
+  API_END
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -1124,9 +1268,11 @@ struct otbl_ensemble {
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, done = nullptr;
   std::vector<hipEvent_t> join;
-  // graph per (substep m, buffer-role bits of the boxes), captured with the
-  // boxes' graph generations in gens
-  std::map<std::pair<int, unsigned long long>, hipGraphExec_t> graphs;
+  // graph per (substep m, buffer-role bits of the boxes) and the bits of the
+  // boxes that take a statistics sample, captured with the boxes' graph
+  // generations in gens
+  std::map<std::pair<std::pair<int, unsigned long long>, std::pair<int, unsigned long long>>, hipGraphExec_t>
+      graphs;
   std::vector<unsigned> gens;
 };
 
@@ -1171,11 +1317,16 @@ void ens_substep(otbl_ensemble &e, int m) {
     eager = eager || e.boxes[q]->kprof;
     roles |= (unsigned long long)e.boxes[q]->flip << q;
   }
+  unsigned long long smask = 0; // boxes that take a statistics sample
+  for (size_t q = 0; q < e.boxes.size(); ++q)
+    if (take_stats(*e.boxes[q], m)) smask |= 1ull << q;
   if (eager) { // measurement: kernels one by one on each box's own stream
-    for (Box *b : e.boxes) {
-      enqueue_substep(*b, m);
+    for (size_t q = 0; q < e.boxes.size(); ++q) {
+      Box *b = e.boxes[q];
+      enqueue_substep(*b, m, (smask >> q) & 1ull);
       k_swap_saved(*b);
       if (m == 3) k_end_step(*b);
+      if ((smask >> q) & 1ull) stats_taken(*b);
     }
     return;
   }
@@ -1185,7 +1336,8 @@ void ens_substep(otbl_ensemble &e, int m) {
       for (size_t r = 0; r < e.boxes.size(); ++r) e.gens[r] = e.boxes[r]->graph_gen;
       break;
     }
-  hipGraphExec_t &ge = e.graphs[{m, roles}];
+  const std::pair<int, unsigned long long> key{m, roles}, keyst{0, smask};
+  hipGraphExec_t &ge = e.graphs[{key, keyst}];
   if (!ge) {
     hipGraph_t g = nullptr;
     size_t done = 0;
@@ -1195,7 +1347,7 @@ void ens_substep(otbl_ensemble &e, int m) {
       for (; done < e.boxes.size(); ++done) {
         Box &b = *e.boxes[done];
         OTBL_HIP(hipStreamWaitEvent(b.s, e.fork, 0));
-        enqueue_substep(b, m);
+        enqueue_substep(b, m, (smask >> done) & 1ull);
         if (m == 3) k_end_step(b);
         OTBL_HIP(hipEventRecord(e.join[done], b.s));
         OTBL_HIP(hipStreamWaitEvent(e.s, e.join[done], 0));
@@ -1205,7 +1357,7 @@ void ens_substep(otbl_ensemble &e, int m) {
       // capture ran the host side only: undo the role swaps already made
       for (size_t q = 0; q < e.boxes.size(); ++q)
         if (((roles >> q) & 1ull) != (unsigned long long)e.boxes[q]->flip) k_swap_saved(*e.boxes[q]);
-      e.graphs.erase({m, roles});
+      e.graphs.erase({key, keyst});
       throw;
     }
     OTBL_HIP(hipStreamEndCapture(e.s, &g));
@@ -1214,12 +1366,15 @@ void ens_substep(otbl_ensemble &e, int m) {
     for (size_t q = 0; q < e.boxes.size(); ++q)
       if (((roles >> q) & 1ull) != (unsigned long long)e.boxes[q]->flip) k_swap_saved(*e.boxes[q]);
     if (ie != hipSuccess) {
-      e.graphs.erase({m, roles});
+      e.graphs.erase({key, keyst});
       fail_hip(ie, "hipGraphInstantiate (ensemble substep)");
     }
   }
   OTBL_HIP(hipGraphLaunch(ge, e.s));
-  for (Box *b : e.boxes) k_swap_saved(*b);
+  for (size_t q = 0; q < e.boxes.size(); ++q) {
+    k_swap_saved(*e.boxes[q]);
+    if ((smask >> q) & 1ull) stats_taken(*e.boxes[q]);
+  }
 }
 
 } // namespace
@@ -1263,8 +1418,10 @@ int otbl_ensemble_step(otbl_ensemble *e, int nsteps) {
   need(nsteps >= 0, "nsteps must be >= 0");
   ens_after_boxes(*e);
   try {
-    for (int st = 0; st < nsteps; ++st)
+    for (int st = 0; st < nsteps; ++st) {
+      for (Box *b : e->boxes) begin_step(*b);
       for (int m = 1; m <= 3; ++m) ens_substep(*e, m);
+    }
   } catch (...) {
     ens_before_boxes(*e);
     throw;

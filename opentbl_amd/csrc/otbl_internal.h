@@ -167,10 +167,25 @@ struct Box {
   double *rthcf = nullptr;
   int diag_cap = 0;
   unsigned long long diag_read = 0;
+  // statistics (otbl_stats_*): the accumulators [OTBL_NSTAT][nx][ny+1], the
+  // enstrophy field of the last sample [nx][ny][nz] (FP32), the stats-only
+  // schemes on the device (fd_dvdy, fd_iy, fd_iyp, fd_dx, fd_ix), the
+  // samples taken (ical), the step counter and the sampling rule of
+  // bl_1.F90:92 (every avcrt steps after step stest), and the request for
+  // a sample at the next substep 1
+  double *st = nullptr, *stcoef = nullptr;
+  // mixed box: FP64 mean-flow lines of the statistics (u_x, du/dx as
+  // [ny+1][nx]; B1, B4, B6 as [ny][nx] in slots of [ny+1][nx])
+  double *stmean = nullptr;
+  float *ens = nullptr;
+  int ical = 0, avcrt = 0, stest = 0;
+  long long istep = 0;
+  bool dostat = false;
   // graphs
   bool use_graphs = false;
   // one graph per substep m and buffer role assignment (flip: the velocity
-  // fields and their saved copies swap roles at every substep)
+  // fields and their saved copies swap roles at every substep); slot 0 is
+  // substep 1 with a statistics sample
   hipGraphExec_t gexec[4][2] = {};
   int flip = 0;
   // bumped whenever the captured substep would change (diagnostics on or
@@ -244,6 +259,14 @@ template <typename R> void kz_products1(Box &b);
 template <typename R> void kz_products2(Box &b);
 void kz_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows);
 size_t k_zfft_shm_bytes(int nz, int nbuf);
+
+// statistics (otbl_stats.hip): one sample of the current state added to
+// Box::st (statsp, stotf.F90:51-291); clearing the accumulators; launch
+// attributes; LDS bytes of the row kernel for length nz
+void k_statsp(Box &b);
+void k_stats_zero(Box &b);
+void k_stats_attrs(Box &b);
+size_t k_stats_shm_bytes(int nz);
 
 // launch bookkeeping (otbl_api.cpp)
 void kbegin(Box &b, const char *name);

@@ -8,6 +8,7 @@
 #include <hdf5_hl.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -89,6 +90,57 @@ std::vector<hsize_t> shape(hid_t f, const char *name) {
   return d;
 }
 
+// writeheader (escru.F90:950-997)
+void write_header(hid_t f, const char *variable, const otbl_io_header *h, const double *y, const double *um) {
+  put_string(f, "Variable", variable);
+  put1(f, "tiempo", h->tiempo);
+  put1(f, "cfl", h->cfl);
+  put1(f, "Re", h->re);
+  put1(f, "lx", h->lx);
+  put1(f, "ly", h->ly);
+  put1(f, "lz", h->lz);
+  put1(f, "nx", h->nx);
+  put1(f, "ny", h->ny);
+  put1(f, "nz2", h->nz2);
+  put1(f, "xout", h->xout);
+  put1(f, "procs", h->procs);
+  put1(f, "timeinit", h->timeinit);
+  put1(f, "dt", h->dt);
+  const hsize_t dy = (hsize_t)h->ny + 2, du = (hsize_t)h->ny + 1;
+  h5ok(H5LTmake_dataset_double(f, "y", 1, &dy, y), "write y");
+  h5ok(H5LTmake_dataset_double(f, "um", 1, &du, um), "write um");
+}
+
+// Fortran sequential unformatted records, big-endian (open(...,
+// form='unformatted', convert='Big_endian'), escru.F90:300, 466): each record
+// is its byte count (4-byte integer), the data, the byte count again.
+struct Records {
+  FILE *f = nullptr;
+  std::vector<unsigned char> buf;
+  explicit Records(const char *path) {
+    f = std::fopen(path, "wb");
+    if (!f) throw std::runtime_error(std::string("cannot create ") + path);
+  }
+  ~Records() {
+    if (f) std::fclose(f);
+  }
+  void put_be(const void *p, size_t n) { // n-byte value, host (little-endian) order in
+    const unsigned char *c = (const unsigned char *)p;
+    for (size_t q = 0; q < n; ++q) buf.push_back(c[n - 1 - q]);
+  }
+  void d(double v) { put_be(&v, 8); }
+  void i(int v) { put_be(&v, 4); }
+  void end() {
+    const int n = (int)buf.size();
+    unsigned char m[4];
+    for (int q = 0; q < 4; ++q) m[q] = (unsigned char)((unsigned)n >> (8 * (3 - q)));
+    if (std::fwrite(m, 1, 4, f) != 4 || std::fwrite(buf.data(), 1, buf.size(), f) != buf.size() ||
+        std::fwrite(m, 1, 4, f) != 4)
+      throw std::runtime_error("short write");
+    buf.clear();
+  }
+};
+
 int fail(const std::exception &e, int code) {
   g_err = e.what();
   return code;
@@ -114,23 +166,7 @@ int otbl_io_write_field(const char *path, const char *variable, const otbl_io_he
     File f(path, true);
     const hsize_t dims[3] = {(hsize_t)h->nx, (hsize_t)R, (hsize_t)nz1};
     h5ok(H5LTmake_dataset_float(f.id, "value", 3, dims, r4.data()), "write value");
-    put_string(f.id, "Variable", variable);
-    put1(f.id, "tiempo", h->tiempo);
-    put1(f.id, "cfl", h->cfl);
-    put1(f.id, "Re", h->re);
-    put1(f.id, "lx", h->lx);
-    put1(f.id, "ly", h->ly);
-    put1(f.id, "lz", h->lz);
-    put1(f.id, "nx", h->nx);
-    put1(f.id, "ny", h->ny);
-    put1(f.id, "nz2", h->nz2);
-    put1(f.id, "xout", h->xout);
-    put1(f.id, "procs", h->procs);
-    put1(f.id, "timeinit", h->timeinit);
-    put1(f.id, "dt", h->dt);
-    const hsize_t dy = (hsize_t)h->ny + 2, du = (hsize_t)h->ny + 1;
-    h5ok(H5LTmake_dataset_double(f.id, "y", 1, &dy, y), "write y");
-    h5ok(H5LTmake_dataset_double(f.id, "um", 1, &du, um), "write um");
+    write_header(f.id, variable, h, y, um);
     return 0;
   } catch (const std::invalid_argument &e) {
     return fail(e, 1);
@@ -211,6 +247,78 @@ int otbl_io_read_field(const char *path, double *field, int nx, int R, int nz1) 
       const float *s = r4.data() + r * fz;
       for (size_t k = 0; k < nzz; ++k) o[k] = (double)s[k];
       for (size_t k = nzz; k < (size_t)nz1; ++k) o[k] = 0.0;
+    }
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    return fail(e, 1);
+  } catch (const std::exception &e) {
+    return fail(e, 2);
+  }
+}
+
+int otbl_io_write_ens(const char *path, const otbl_io_header *h, const double *y, const double *um, const float *ens,
+                      int nz) {
+  try {
+    need(path && h && y && um && ens && nz >= 1, "otbl_io_write_ens: null argument or nz < 1");
+    need(h->nx >= 1 && h->ny >= 1, "otbl_io_write_ens: bad sizes in the header");
+    File f(path, true);
+    const hsize_t dims[3] = {(hsize_t)h->nx, (hsize_t)h->ny, (hsize_t)nz};
+    h5ok(H5LTmake_dataset_float(f.id, "value", 3, dims, ens), "write value");
+    write_header(f.id, "enstrophy", h, y, um);
+    return 0;
+  } catch (const std::invalid_argument &e) {
+    return fail(e, 1);
+  } catch (const std::exception &e) {
+    return fail(e, 2);
+  }
+}
+
+int otbl_io_write_stats(const char *st_path, const char *budget_path, const otbl_io_stats_header *h,
+                        const double *y, const double *stats) {
+  try {
+    need(h && y && stats, "otbl_io_write_stats: null argument");
+    need(h->nx >= 1 && h->ny >= 4, "otbl_io_write_stats: bad sizes in the header");
+    const int nx = h->nx, ny = h->ny;
+    const size_t R = (size_t)ny + 1, S = (size_t)nx * R;
+    // accumulator id, plane i (1-based), row j (1-based)
+    auto at = [&](int id, int i, int j) { return stats[(size_t)id * S + (size_t)(i - 1) * R + (size_t)(j - 1)]; };
+    auto head = [&](Records &r) {
+      for (double v : {h->tiempo, h->cfl, h->re, h->ax, h->ay, h->az}) r.d(v);
+      for (int v : {nx, ny, h->nz2, h->ical}) r.i(v);
+      r.end();
+      for (int j = 0; j <= ny + 1; ++j) r.d(y[j]);
+      r.end();
+    };
+    auto rows = [&](Records &r, int id, int i, int n) {
+      for (int j = 1; j <= n; ++j) r.d(at(id, i, j));
+    };
+    if (st_path) {
+      // escru.F90:300-341: header, y, a record holding 0d0, then one record
+      // per plane: us(ny+1) vs uv... as listed (ua, ws, us, wa: ny+1 rows)
+      Records r(st_path);
+      head(r);
+      r.d(0.0);
+      r.end();
+      // (id, rows) in record order; ids as in OTBL stats order
+      static const int kSt[][2] = {{0, 1}, {1, 0}, {2, 1}, {3, 0}, {4, 1}, {5, 0}, {6, 1}, {7, 0}, {8, 0},
+                                   {9, 0}, {10, 0}, {11, 0}, {12, 0}, {13, 0}, {14, 0}, {15, 0}, {16, 0}};
+      for (int i = 1; i <= nx; ++i) {
+        for (const auto &e : kSt) rows(r, e[0], i, ny + e[1]);
+        r.end();
+      }
+    }
+    if (budget_path) {
+      // escru.F90:466-516: header, y, then per plane buf_bud(ny, 21) and
+      // buf_bud2(ny+1, 2)
+      Records r(budget_path);
+      head(r);
+      static const int kBud[21] = {17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37};
+      for (int i = 1; i <= nx; ++i) {
+        for (int id : kBud) rows(r, id, i, ny);
+        rows(r, 38, i, ny + 1);
+        rows(r, 39, i, ny + 1);
+        r.end();
+      }
     }
     return 0;
   } catch (const std::invalid_argument &e) {

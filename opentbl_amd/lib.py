@@ -81,7 +81,19 @@ SIGNATURES = {
     "otbl_autotune": (I, [VP, I, C.c_char_p, SZ]),
     "otbl_kprof": (I, [VP, I]),
     "otbl_kprof_report": (I, [VP, C.c_char_p, SZ]),
+    "otbl_stats_name": (C.c_char_p, [I]),
+    "otbl_stats_enable": (I, [VP, I]),
+    "otbl_stats_schedule": (I, [VP, I, I]),
+    "otbl_stats_request": (I, [VP]),
+    "otbl_statsp": (I, [VP]),
+    "otbl_stats_get": (I, [VP, I, PD]),
+    "otbl_stats_ical": (I, [VP, C.POINTER(I)]),
+    "otbl_stats_reset": (I, [VP]),
+    "otbl_stats_get_ens": (I, [VP, C.POINTER(C.c_float)]),
 }
+
+# the accumulators of otbl_stats_* (OTBL_NSTAT), in the library's order
+NSTAT = 40
 
 
 class OtblError(RuntimeError):
@@ -293,6 +305,45 @@ class GpuBox:
 
     def op(self, name, *args):
         check(getattr(lib(), "otbl_" + name)(self.h, *args), name)
+
+    # statistics (stotf.F90:51-291)
+    def stats_enable(self, on=True):
+        check(lib().otbl_stats_enable(self.h, int(on)), "stats_enable")
+
+    def stats_schedule(self, avcrt, stest=0):
+        """A sample at substep 1 of every step istep with istep % avcrt == 0
+        and istep > stest (bl_1.F90:92)."""
+        check(lib().otbl_stats_schedule(self.h, int(avcrt), int(stest)), "stats_schedule")
+
+    def stats_request(self):
+        check(lib().otbl_stats_request(self.h), "stats_request")
+
+    def statsp(self):
+        check(lib().otbl_statsp(self.h), "statsp")
+
+    def stats(self):
+        """The accumulators by name, each (nx, ny+1) (plane, row)."""
+        out = {}
+        for q in range(NSTAT):
+            a = np.zeros(self.cfg.nx * (self.cfg.ny + 1))
+            check(lib().otbl_stats_get(self.h, q, _pd(a)), "stats_get")
+            out[lib().otbl_stats_name(q).decode()] = a.reshape(self.cfg.nx, self.cfg.ny + 1)
+        return out
+
+    def stats_ical(self):
+        n = I()
+        check(lib().otbl_stats_ical(self.h, C.byref(n)), "stats_ical")
+        return n.value
+
+    def stats_reset(self):
+        check(lib().otbl_stats_reset(self.h), "stats_reset")
+
+    def ens(self):
+        """The enstrophy field of the last sample, (nx, ny, nz) float32."""
+        c = self.cfg
+        a = np.zeros(c.nx * c.ny * c.nz, np.float32)
+        check(lib().otbl_stats_get_ens(self.h, a.ctypes.data_as(C.POINTER(C.c_float))), "stats_get_ens")
+        return a.reshape(c.nx, c.ny, c.nz)
 
 
 class Ensemble:

@@ -34,6 +34,12 @@ class Header(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class StatsHeader(C.Structure):
+    _fields_ = [("tiempo", C.c_double), ("cfl", C.c_double), ("re", C.c_double), ("ax", C.c_double),
+                ("ay", C.c_double), ("az", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz2", C.c_int),
+                ("ical", C.c_int)]
+
+
 class RestartError(RuntimeError):
     pass
 
@@ -52,8 +58,11 @@ def lib():
         L.otbl_io_write_field.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Header), PD, PD, PD, C.c_int]
         L.otbl_io_read_header.argtypes = [C.c_char_p, C.POINTER(Header), C.c_char_p, C.c_size_t, PD, PD]
         L.otbl_io_read_field.argtypes = [C.c_char_p, PD, C.c_int, C.c_int, C.c_int]
+        L.otbl_io_write_ens.argtypes = [C.c_char_p, C.POINTER(Header), PD, PD, C.POINTER(C.c_float), C.c_int]
+        L.otbl_io_write_stats.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(StatsHeader), PD, PD]
         L.otbl_io_last_error.restype = C.c_char_p
-        for f in (L.otbl_io_write_field, L.otbl_io_read_header, L.otbl_io_read_field):
+        for f in (L.otbl_io_write_field, L.otbl_io_read_header, L.otbl_io_read_field, L.otbl_io_write_ens,
+                  L.otbl_io_write_stats):
             f.restype = C.c_int
         _LIB = L
     return _LIB
@@ -152,3 +161,78 @@ def load_state(base, cfg):
               u0=st["u"][0, :, 0].real.copy(), v0=st["v"][0, :, 0].real.copy(), ax=cfg.ax, az=cfg.az,
               xout=cfg.xout)
     return st
+
+
+# ---- statistics files (escrst, escru.F90:263-516) ----
+# accumulator names in the library's order (otbl_stats_get ids)
+STAT_NAMES = ("us", "vs", "ws", "uv", "ua", "va", "wa", "pp", "pm", "vortx", "vorty", "vortz", "vortxa",
+              "vortya", "vortza", "uw", "vw", "dispu", "dispv", "dispw", "dispuv", "pup", "pvp", "pdudx",
+              "pdvdy", "pdudy", "pdvdx", "pdwdz", "v3", "u2v", "v2u", "w2v", "dudx0", "dudy0", "dvdx0", "dvdy0",
+              "dwdx0", "dwdy0", "u3", "w2u")
+# per-plane record of the .st file: (name, rows beyond ny) (escru.F90:318-336)
+ST_RECORD = (("us", 1), ("vs", 0), ("ws", 1), ("uv", 0), ("ua", 1), ("va", 0), ("wa", 1), ("pp", 0), ("pm", 0),
+             ("vortx", 0), ("vorty", 0), ("vortz", 0), ("vortxa", 0), ("vortya", 0), ("vortza", 0), ("uw", 0),
+             ("vw", 0))
+# per-plane record of the .budget file (escru.F90:490-516)
+BUDGET_RECORD = tuple((n, 0) for n in ("dispu", "dispv", "dispw", "dispuv", "pup", "pvp", "pdudx", "pdvdy", "pdudy",
+                                       "pdvdx", "pdwdz", "v3", "u2v", "v2u", "w2v", "dudx0", "dudy0", "dvdx0",
+                                       "dvdy0", "dwdx0", "dwdy0")) + (("u3", 1), ("w2u", 1))
+
+
+def save_stats(st_path, budget_path, cfg, re, tiempo, ical, y, stats):
+    """Write the .st and .budget statistics files (either path may be None)
+    from a dict of accumulators, each (nx, ny+1) (plane, row) as
+    GpuBox.stats() returns them.  The header carries ax, ay, az as bl_1
+    passes them to escrst (domain lengths / pi, / pi, / 2 pi)."""
+    arr = np.ascontiguousarray(np.stack([np.asarray(stats[n], np.float64) for n in STAT_NAMES]))
+    if arr.shape != (len(STAT_NAMES), cfg.nx, cfg.ny + 1):
+        raise RestartError(f"statistics of shape {arr.shape[1:]} for a {cfg.nx}x{cfg.ny} box")
+    y = np.ascontiguousarray(y, np.float64)
+    h = StatsHeader(tiempo=tiempo, cfl=cfg.cfl, re=re, ax=cfg.ax, ay=cfg.Ly / math.pi, az=cfg.az, nx=cfg.nx,
+                    ny=cfg.ny, nz2=cfg.nz2, ical=int(ical))
+    enc = lambda p: p.encode() if p else None  # noqa: E731
+    _check(lib().otbl_io_write_stats(enc(st_path), enc(budget_path), C.byref(h), _pd(y), _pd(arr)), "write stats")
+
+
+def _records(path):
+    """The records of a big-endian Fortran sequential unformatted file."""
+    raw = open(path, "rb").read()
+    out, o = [], 0
+    while o < len(raw):
+        n = int.from_bytes(raw[o:o + 4], "big")
+        out.append(raw[o + 4:o + 4 + n])
+        if int.from_bytes(raw[o + 4 + n:o + 8 + n], "big") != n:
+            raise RestartError(f"{path}: record markers differ at byte {o}")
+        o += 8 + n
+    return out
+
+
+def read_stats(path, kind):
+    """Read a .st (kind "st") or .budget (kind "budget") file: the header
+    (tiempo, cfl, Re, ax, ay, az, nx, ny, nz2, ical), y(0:ny+1) and the
+    accumulators it holds, each (nx, ny+1) with row ny+1 zero where the
+    file stores ny rows."""
+    rec = _records(path)
+    hd = np.frombuffer(rec[0][:48], ">f8")
+    nx, ny, nz2, ical = (int(v) for v in np.frombuffer(rec[0][48:64], ">i4"))
+    head = dict(zip(("tiempo", "cfl", "re", "ax", "ay", "az"), hd.tolist()), nx=nx, ny=ny, nz2=nz2, ical=ical)
+    y = np.frombuffer(rec[1], ">f8").astype(np.float64)
+    layout, first = (ST_RECORD, 3) if kind == "st" else (BUDGET_RECORD, 2)
+    out = {n: np.zeros((nx, ny + 1)) for n, _ in layout}
+    for i in range(nx):
+        vals = np.frombuffer(rec[first + i], ">f8")
+        o = 0
+        for n, extra in layout:
+            out[n][i, :ny + extra] = vals[o:o + ny + extra]
+            o += ny + extra
+    return head, y, out
+
+
+def save_ens(fname, hdr, y, um, ens):
+    """The enstrophy file of escribezy (escru.F90:159-170): ens (nx, ny, nz)
+    4-byte reals with the writeheader header, Variable "enstrophy"."""
+    e = np.ascontiguousarray(ens, np.float32)
+    y = np.ascontiguousarray(y, np.float64)
+    um = np.ascontiguousarray(um, np.float64)
+    _check(lib().otbl_io_write_ens(fname.encode(), C.byref(hdr), _pd(y), _pd(um),
+                                   e.ctypes.data_as(C.POINTER(C.c_float)), e.shape[2]), "write ens")

@@ -60,3 +60,13 @@ def test_profiled_traffic_finds_the_kernel(kname, prec):
     element-wise steps); each lookup must find its row."""
     t = bench.profiled_traffic(CFG.name, prec, kname)
     assert t is not None and t > 1e9, (kname, prec, t)
+
+
+def test_output_line_serialises_numpy_values():
+    """The JSON line holds numpy scalars from the error and roofline sums
+    (np.float64, np.bool_): they print as plain JSON values."""
+    import json
+    import numpy as np
+    line = {"a": np.float64(1.5), "b": np.bool_(True), "c": {"d": np.int64(3)}, "e": np.zeros(2)}
+    assert json.loads(json.dumps(line, default=bench.json_default)) == {"a": 1.5, "b": True, "c": {"d": 3},
+                                                                       "e": [0.0, 0.0]}

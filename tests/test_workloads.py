@@ -113,7 +113,9 @@ def test_zpg1536x315x1024_mixed_vs_fp64():
         f"Cf {rel(cm, cd):.2e}; divergence FP64 {div_d:.2e}, mixed {div_m:.2e}; t {tm[0]:.6f} vs {td[0]:.6f}")
     assert all(np.isfinite(a).all() for a in fm.values())
     assert errs["vel"] < 1e-5, errs
-    assert div_d < 1e-11 and div_m < 1e-5
+    # FP64 round-off of the last projection: 7e-12 at 1024x192x512, 6.7e-11
+    # here (longer x transforms, 315 stretched rows)
+    assert div_d < 1e-10 and div_m < 1e-5
     assert rel(rm, rd) < 1e-5 and rel(cm, cd) < 1e-3
     assert abs(tm[0] - td[0]) <= 1e-5 * td[0]
 
