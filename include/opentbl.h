@@ -165,6 +165,15 @@ int otbl_cosft_lines(otbl_ctx *ctx, double *lines, int nlines, int iopt);
  * the complex device plane u(0:nz2, 1:ny1) is solved into the device plane
  * us (us may be u). */
 int otbl_soltriy(otbl_ctx *ctx, const double *u, double *us, int ii, double *a, int ny1);
+/* otbl_soltriy without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * graph capture, per-mode solve loops of a serving path, harnesses that
+ * forbid hipMalloc).  Same arguments, results and error behaviour as
+ * otbl_soltriy; the tridiagonal is staged in a fixed device scratch of the
+ * library (one per GPU, held until the process ends), so ny1 <= 8192 (a
+ * larger ny1 is rejected with OTBL_EINVAL) and calls that overlap on one
+ * GPU (from several threads) race on that scratch. */
+int otbl_soltriy_noalloc(otbl_ctx *ctx, const double *u, double *us, int ii, double *a, int ny1);
 /* coefinv (coeft.f90:1338-1353): in-place LU of the caller's host array
  * a(3, npter1:neq), computed on the GPU. */
 int otbl_coefinv(otbl_ctx *ctx, double *a, int npter1, int neq);

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <mutex>
 #include <string>
 
 #include "otbl_internal.h"
@@ -16,6 +17,32 @@
 namespace otbl {
 
 static thread_local std::string g_err;
+
+// Scratch of otbl_soltriy_noalloc: the caller's (3, 1:ny1) tridiagonal of
+// one call, at most kNoAllocRows rows.  A file-scope device array: each GPU
+// the library runs on holds its own copy until the process ends, and every
+// call on that GPU shares it.  Host code reaches it only through the
+// per-device address hipGetSymbolAddress returns (the array's name is a
+// host-side stand-in), looked up once per device at otbl_create.
+constexpr int kNoAllocRows = 8192;
+static __device__ __align__(256) double scratch[3 * (kNoAllocRows + 1)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices] = {};
+
+// the address of this device's scratch, set up on first use with the device
+// current (called by otbl_create)
+static double *scratch_of(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw std::invalid_argument("device index above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    OTBL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("hipGetSymbolAddress returned a null scratch address");
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  return g_scratch[dev];
+}
 
 void fail_hip(hipError_t e, const char *what) {
   throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
@@ -518,6 +545,8 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   b->nu = b->pu * b->nx;
   b->nv = b->pv * b->nx;
   need(b->nu < (size_t(1) << 32), "a field must have fewer than 2^32 elements");
+  OTBL_HIP(hipGetDevice(&b->dev));
+  (void)scratch_of(b->dev); // the no-allocation entry point's scratch on this device
   OTBL_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
   build_coef(b->hc, b->nx, b->ny, b->nz, b->re, b->ax, b->az, b->cfl, y);
   upload_coef(*b);
@@ -1043,6 +1072,22 @@ int otbl_soltriy(otbl_ctx *ctx, const double *u, double *us, int ii, double *a, 
     throw;
   }
   OTBL_HIP(hipFree(da));
+  API_END
+}
+
+int otbl_soltriy_noalloc(otbl_ctx *ctx, const double *u, double *us, int ii, double *a, int ny1) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(u && us && a && ny1 >= 1, "otbl_soltriy_noalloc: null argument or ny1 < 1");
+  need(ny1 <= kNoAllocRows, "otbl_soltriy_noalloc: ny1 above 8192, the size of its scratch");
+  need(ii >= 0 && ii <= b.nz2, "otbl_soltriy_noalloc: column outside 0..nz2");
+  // every element the kernel reads (rows 1..3 of columns 1..ny1) is written
+  // by the copy below, so the scratch needs no clearing between calls
+  double *da = scratch_of(b.dev);
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  h2d(b, da + 3, a, 3 * (size_t)ny1 * sizeof(double));
+  k_soltriy(b, u, us, ii, da, ny1, b.nz2 + 1);
+  d2h(b, a, da + 3, 3 * (size_t)ny1 * sizeof(double));
   API_END
 }
 

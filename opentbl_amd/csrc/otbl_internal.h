@@ -70,6 +70,7 @@ struct DevScalars {
 
 struct Box {
   int nx, ny, nz, nz1, nz2, mz, xout, precision, dt_mode;
+  int dev = 0; // the HIP device the box lives on (current at otbl_create)
   // bytes per field element: 8 (OTBL_FP64) or 4 (OTBL_MIXED, FP32 fields);
   // fields and work fields are allocated at that size
   int esz = 8;

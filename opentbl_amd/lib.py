@@ -67,6 +67,7 @@ SIGNATURES = {
     "otbl_cosftx": (I, [VP, VP, VP, I]),
     "otbl_cosft_lines": (I, [VP, VP, I, I]),
     "otbl_soltriy": (I, [VP, VP, VP, I, PD, I]),
+    "otbl_soltriy_noalloc": (I, [VP, VP, VP, I, PD, I]),
     "otbl_coefinv": (I, [VP, PD, I, I]),
     "otbl_ensemble_create": (I, [C.POINTER(VP), I, C.POINTER(VP)]),
     "otbl_ensemble_step": (I, [VP, I]),
