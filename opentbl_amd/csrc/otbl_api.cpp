@@ -582,6 +582,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   // register-resident z transforms for the lengths they are instantiated
   // for (OTBL_Z_STOCKHAM keeps the LDS Stockham kernels)
   b->zfast = zfft_supported(b->nz) && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
+  b->zpair = std::getenv("OTBL_Z_NOPAIR") ? 0 : (std::getenv("OTBL_ZP_OCC2") ? 2 : 1);
   if (b->zfast) {
     dalloc(b->ztw, 2 * (size_t)b->nz);
     dalloc(b->zrow, (size_t)b->nz1);

@@ -147,6 +147,10 @@ struct Box {
   // twiddle table ztw (nz complex), else the LDS Stockham kernels
   FFTPlan zplan{};
   bool zfast = false;
+  // FP32 z products with two rows per team (0: one row, OTBL_Z_NOPAIR; 1:
+  // plane loop 1's pair kernel held to three waves per SIMD; 2: to two,
+  // OTBL_ZP_OCC2)
+  int zpair = 1;
   double *ztw = nullptr;
   double *zrow = nullptr; // nz1 zeros: the absent partner of a packed transform
   double *rowmax = nullptr; // per-row CFL maxima of products1, 3 (ny+1) nx

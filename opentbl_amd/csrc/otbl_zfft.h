@@ -353,4 +353,45 @@ __device__ __forceinline__ void zstore_spectral(const typename CT<R>::c (&v)[N /
   zsync<W>();
 }
 
+// zstore_spectral with the mean-flow terms on both outputs: A_k += cka XA_k
+// (k >= 1), A_0 += c0a XA_0, and the same for B with XB, c0b, ckb (XA and XB
+// read before A and B are written; either may be the output itself; null:
+// no such term).  A or B null: that output is not written.
+template <int N, typename R>
+__device__ __forceinline__ void zstore_spectral2(const typename CT<R>::c (&v)[N / (64 * ZPlan<N>::W)], int t,
+                                                 typename CT<R>::c *lds, R *A, R *B, int nz2, const R *XA, R c0a,
+                                                 R cka, const R *XB, R c0b, R ckb) {
+  using C = typename CT<R>::c;
+  constexpr int W = ZPlan<N>::W, G = 64 * W, RD = ZPlan<N>::rad(0), NB = N / (RD * G);
+#pragma unroll
+  for (int m = 0; m < NB; ++m)
+#pragma unroll
+    for (int q = 0; q < RD; ++q) lds[zpad(t + G * m + q * (N / RD))] = v[m * RD + q];
+  zsync<W>();
+  const R sc = R(1.0 / N);
+  for (int k = t; k <= nz2; k += G) {
+    const C z = lds[zpad(k)];
+    const C w = lds[zpad(k == 0 ? 0 : N - k)];
+    if (A) {
+      R ar = R(0.5) * (z.x + w.x) * sc, ai = (k == 0) ? R(0) : R(0.5) * (z.y - w.y) * sc;
+      if (XA) {
+        ar += (k == 0 ? c0a : cka) * XA[2 * k];
+        ai += (k == 0) ? R(0) : cka * XA[2 * k + 1];
+      }
+      A[2 * k] = ar;
+      A[2 * k + 1] = ai;
+    }
+    if (B) {
+      R br = R(0.5) * (z.y + w.y) * sc, bi = (k == 0) ? R(0) : R(-0.5) * (z.x - w.x) * sc;
+      if (XB) {
+        br += (k == 0 ? c0b : ckb) * XB[2 * k];
+        bi += (k == 0) ? R(0) : ckb * XB[2 * k + 1];
+      }
+      B[2 * k] = br;
+      B[2 * k + 1] = bi;
+    }
+  }
+  zsync<W>();
+}
+
 } // namespace otbl

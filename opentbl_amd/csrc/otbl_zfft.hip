@@ -5,6 +5,8 @@
 #include "otbl_kcommon.h"
 #include "otbl_zfft.h"
 
+#include <type_traits>
+
 namespace otbl {
 
 // exp(2 pi i m / n), m = 0..n-1
@@ -123,6 +125,134 @@ products1_z(const R *__restrict__ u, R *wki1, R *wki2, R *wki3, const R *__restr
     rowmax[row] = um;
     rowmax[nrows + row] = wm;
     rowmax[2 * (size_t)nrows + row] = vrow ? (double)vm : 0.0;
+  }
+}
+
+// Plane loop 1 in FP32 for two rows at once (rows a = 2 pr, b = 2 pr + 1 of
+// the numbering above): the FFTs that products1_z fills with one real
+// signal (v_x u_y inverted alone, u_x forward with an empty partner) carry
+// the second row's signal instead, 8 transforms per two rows against 10.
+// Each product's two rows share the inverse transform: (u w_x of a, of b),
+// (u_x^2 of a, of b), (v_x u_y of a, of b), so at most one row's product is
+// held across a transform.  The mean flow of u_x and u_y stays out of the
+// transforms as in products1_z.
+template <int N, int MINB>
+__global__ void __launch_bounds__(ZTeam<N>::TPB_, MINB)
+products1_zp(const float *__restrict__ u, float *wki1, float *wki2, float *wki3, const float *__restrict__ uy,
+             const float2 *__restrict__ tw, const float *__restrict__ zrow, int ny, int nz1, int nz2, int nrows,
+             double *rowmax) {
+  using C = float2;
+  using Z = ZTeam<N>;
+  __shared__ C lds_all[Z::TEAMS * zlds_elems<N>()];
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
+  C *lds = lds_all + team * zlds_elems<N>();
+  const int pr = blockIdx.x * Z::TEAMS + team;
+  const int ra = 2 * pr, rb = ra + 1;
+  if (ra >= nrows) return; // a whole team
+  const bool hasb = rb < nrows;
+  const int ja = ra % (ny + 1) + 1, ia = ra / (ny + 1) + 1;
+  const int jb = hasb ? rb % (ny + 1) + 1 : ny + 1, ib = hasb ? rb / (ny + 1) + 1 : ia;
+  const size_t oua = (size_t)ra * nz1, oub = (size_t)(hasb ? rb : ra) * nz1;
+  const size_t ova = ((size_t)(ia - 1) * ny + (ja - 1)) * nz1, ovb = ((size_t)(ib - 1) * ny + (jb - 1)) * nz1;
+  const bool va = ja <= ny, vb = hasb && jb <= ny;
+  float uma = 0.f, wma = 0.f, vma = 0.f, umb = 0.f, wmb = 0.f, vmb = 0.f;
+  // ---- u w_x of both rows ----
+  {
+    float sa[Z::E];
+    {
+      C A[Z::E];
+      zload_spectral<N>(A, t, u + oua, wki3 + oua);
+      zrun<N, +1, false>(A, t, tw, lds);
+#pragma unroll
+      for (int e = 0; e < Z::E; ++e) {
+        uma = fmaxf(uma, fabsf(A[e].x));
+        wma = fmaxf(wma, fabsf(A[e].y));
+        sa[e] = A[e].x * A[e].y;
+      }
+    }
+    C S[Z::E];
+    zload_spectral<N>(S, t, u + oub, hasb ? wki3 + oub : zrow);
+    zrun<N, +1, false>(S, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) {
+      umb = fmaxf(umb, fabsf(S[e].x));
+      wmb = fmaxf(wmb, fabsf(S[e].y));
+      S[e] = make_float2(sa[e], S[e].x * S[e].y);
+    }
+    zrun<N, -1, true>(S, t, tw, lds);
+    zstore_spectral2<N, float>(S, t, lds, wki3 + oua, hasb ? wki3 + oub : nullptr, nz2, nullptr, 0.f, 0.f, nullptr,
+                               0.f, 0.f);
+  }
+  // ---- u_x^2 of both rows: one forward transform of (u_x a, u_x b) ----
+  {
+    C B[Z::E];
+    zload_spectral<N>(B, t, wki1 + oua, hasb ? wki1 + oub : zrow);
+    const float Uxa = wki1[oua], Uxb = hasb ? wki1[oub] : 0.f;
+    if (t == 0) B[0] = make_float2(0.f, 0.f); // the means of both u_x (k = 0: lane 0, slot 0)
+    zrun<N, +1, false>(B, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) B[e] = make_float2(B[e].x * B[e].x, B[e].y * B[e].y);
+    zrun<N, -1, true>(B, t, tw, lds);
+    zstore_spectral2<N, float>(B, t, lds, wki1 + oua, hasb ? wki1 + oub : nullptr, nz2, wki1 + oua, Uxa, 2.f * Uxa,
+                               hasb ? wki1 + oub : nullptr, Uxb, 2.f * Uxb);
+  }
+  // ---- v_x u_y of both rows ----
+  if (va || vb) {
+    float pa[Z::E];
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) pa[e] = 0.f;
+    if (va) {
+      C Q[Z::E];
+      zload_spectral<N>(Q, t, wki2 + ova, uy + oua);
+      if (t == 0) Q[0].y = 0.f; // the mean of u_y
+      zrun<N, +1, false>(Q, t, tw, lds);
+#pragma unroll
+      for (int e = 0; e < Z::E; ++e) {
+        vma = fmaxf(vma, fabsf(Q[e].x));
+        pa[e] = Q[e].x * Q[e].y;
+      }
+    }
+    C P[Z::E];
+    if (vb) {
+      zload_spectral<N>(P, t, wki2 + ovb, uy + oub);
+      if (t == 0) P[0].y = 0.f;
+      zrun<N, +1, false>(P, t, tw, lds);
+#pragma unroll
+      for (int e = 0; e < Z::E; ++e) {
+        vmb = fmaxf(vmb, fabsf(P[e].x));
+        P[e] = make_float2(pa[e], P[e].x * P[e].y);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < Z::E; ++e) P[e] = make_float2(pa[e], 0.f);
+    }
+    zrun<N, -1, true>(P, t, tw, lds);
+    const float Uya = va ? uy[oua] : 0.f, Uyb = vb ? uy[oub] : 0.f;
+    zstore_spectral2<N, float>(P, t, lds, va ? wki2 + ova : nullptr, vb ? wki2 + ovb : nullptr, nz2,
+                               va ? wki2 + ova : nullptr, Uya, Uya, vb ? wki2 + ovb : nullptr, Uyb, Uyb);
+  }
+  // per-row maxima (reduced by rowmax_kernel)
+  float mx[6] = {uma, wma, vma, umb, wmb, vmb};
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    for (int o = 32; o > 0; o >>= 1) mx[q] = fmaxf(mx[q], __shfl_down(mx[q], o, 64));
+  if constexpr (Z::W > 1) {
+    if ((t & 63) == 0)
+      for (int q = 0; q < 6; ++q) lds[q * Z::W + t / 64] = make_float2(mx[q], 0.f);
+    zsync<Z::W>();
+    if (t == 0)
+      for (int q = 0; q < 6; ++q)
+        for (int w = 1; w < Z::W; ++w) mx[q] = fmaxf(mx[q], lds[q * Z::W + w].x);
+  }
+  if (t == 0) {
+    rowmax[ra] = mx[0];
+    rowmax[nrows + ra] = mx[1];
+    rowmax[2 * (size_t)nrows + ra] = va ? (double)mx[2] : 0.0;
+    if (hasb) {
+      rowmax[rb] = mx[3];
+      rowmax[nrows + rb] = mx[4];
+      rowmax[2 * (size_t)nrows + rb] = vb ? (double)mx[5] : 0.0;
+    }
   }
 }
 
@@ -345,10 +475,26 @@ template <typename R> void kz_products1(Box &b) {
   zcheck(b);
   const int nrows = (b.ny + 1) * b.nx;
   const R *zrow = fld<R>(b.zrow);
-  ZDISPATCH(b.nz, auto kf = &products1_z<N, R>;
-            LAUNCH(b, "products1", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, kf, fld<R>(b.f[OTBL_U]), fld<R>(b.wki1),
-                   fld<R>(b.wki2), fld<R>(b.wki3), fld<R>(b.yc), ztw<R>(b), zrow, b.ny, b.nz1, b.nz2, nrows,
-                   b.rowmax));
+  if (std::is_same<R, float>::value && b.zpair) {
+    // FP32: two rows per team (the registers two rows need are there in FP32)
+    const float *zr = (const float *)b.zrow;
+    if (b.zpair == 1) {
+      ZDISPATCH(b.nz, auto kf = &products1_zp<N, 3>;
+                LAUNCH(b, "products1", zgrid<N>((nrows + 1) / 2), ZTeam<N>::TPB_, 0, kf, fld<float>(b.f[OTBL_U]),
+                       fld<float>(b.wki1), fld<float>(b.wki2), fld<float>(b.wki3), fld<float>(b.yc),
+                       (const float2 *)b.ztw_f, zr, b.ny, b.nz1, b.nz2, nrows, b.rowmax));
+    } else {
+      ZDISPATCH(b.nz, auto kf = &products1_zp<N, 1>;
+                LAUNCH(b, "products1", zgrid<N>((nrows + 1) / 2), ZTeam<N>::TPB_, 0, kf, fld<float>(b.f[OTBL_U]),
+                       fld<float>(b.wki1), fld<float>(b.wki2), fld<float>(b.wki3), fld<float>(b.yc),
+                       (const float2 *)b.ztw_f, zr, b.ny, b.nz1, b.nz2, nrows, b.rowmax));
+    }
+  } else {
+    ZDISPATCH(b.nz, auto kf = &products1_z<N, R>;
+              LAUNCH(b, "products1", zgrid<N>(nrows), ZTeam<N>::TPB_, 0, kf, fld<R>(b.f[OTBL_U]), fld<R>(b.wki1),
+                     fld<R>(b.wki2), fld<R>(b.wki3), fld<R>(b.yc), ztw<R>(b), zrow, b.ny, b.nz1, b.nz2, nrows,
+                     b.rowmax));
+  }
   LAUNCH(b, "rowmax", b.ny + 1, 256, 0, rowmax_kernel, b.rowmax, b.ds, b.vmbits, b.ny, b.nx);
 }
 

@@ -197,6 +197,39 @@ module otbl_gpu
        type(c_ptr), value :: src
        integer(c_size_t), value :: bytes
      end function otbl_d2h
+     integer(c_int) function otbl_stats_enable(ctx, on) bind(C, name="otbl_stats_enable")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: on
+     end function otbl_stats_enable
+     integer(c_int) function otbl_stats_schedule(ctx, avcrt, stest) bind(C, name="otbl_stats_schedule")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: avcrt, stest
+     end function otbl_stats_schedule
+     integer(c_int) function otbl_stats_request(ctx) bind(C, name="otbl_stats_request")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function otbl_stats_request
+     integer(c_int) function otbl_statsp(ctx) bind(C, name="otbl_statsp")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function otbl_statsp
+     integer(c_int) function otbl_stats_get(ctx, id, out) bind(C, name="otbl_stats_get")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: id
+       real(c_double) :: out(*)
+     end function otbl_stats_get
+     integer(c_int) function otbl_stats_ical(ctx, ical) bind(C, name="otbl_stats_ical")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: ical
+     end function otbl_stats_ical
+     integer(c_int) function otbl_stats_reset(ctx) bind(C, name="otbl_stats_reset")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function otbl_stats_reset
      integer(c_int) function otbl_diag_read(ctx, out, max_records, n_records, dropped) &
           bind(C, name="otbl_diag_read")
        import :: c_int, c_ptr, c_double, c_long_long
@@ -367,6 +400,23 @@ end module otbl_gpu
 
 
 ! ---- drop-in subroutines with the reference's interfaces ----
+
+! stotf.F90:51-60: one statistics sample of the box's state (the u, v, w, p
+! the next rhsp starts from), its x interpolation and x derivatives formed
+! on the GPU (rhsp.F90:72-90); the reference's intermediate and buffer
+! arguments are not read.  Statistics are switched on with
+! otbl_stats_enable (a sample before that stops with the library's
+! message); otbl_stats_request sets dostat for the next substep 1 instead.
+subroutine statsp(u_x,v,w,p,dudx,dvdx,dwdx,buf1,buf2,buf3,buf4,buf5,buf6,buf7,buf8, &
+     buf_cor,buf_corp,buf_big,bphy1,bphy2,bphy3,mpiid,communicator)
+  use otbl_gpu
+  implicit none
+  real(8) :: u_x(*),v(*),w(*),p(*),dudx(*),dvdx(*),dwdx(*),buf1(*),buf2(*),buf3(*),buf4(*)
+  real(8) :: buf5(*),buf6(*),buf7(*),buf8(*),buf_cor(*),buf_corp(*),buf_big(*),bphy1(*),bphy2(*),bphy3(*)
+  integer :: mpiid, communicator
+  call otbl_need_box('statsp')
+  call otbl_check(otbl_statsp(otbl_box), 'statsp')
+end subroutine statsp
 
 ! rhsp.F90:22-27.  dt comes back from the GPU (computed at substep 1).
 subroutine rhsp(ut,vt,wt,pt,rhsupat,rhsvpat,rhswpat, resu,resut,resv,resvt,resw,reswt, &

@@ -18,7 +18,7 @@ from opentbl_amd import build, config, ic
 
 DROPINS = ["rhsp_", "outflow_correction_", "pois_", "chp2x_", "chx2p_", "interpxx_", "difvisxx_", "interpyy_",
            "differzy_", "vistyy_", "implzy_", "fourxz_", "rfti_", "cosfti_", "cosftx_", "soltriy_", "coefinv_",
-           "coef_", "mtr_", "otbl_f_open", "otbl_f_close", "otbl_f_mirror", "otbl_f_box"]
+           "coef_", "mtr_", "statsp_", "otbl_f_open", "otbl_f_close", "otbl_f_mirror", "otbl_f_box"]
 
 OP_TOL = 1e-12
 
@@ -273,3 +273,34 @@ def test_chp2x_chx2p_dropins(fbox):
     back = pen.copy()
     L.chx2p_(p(back), p(pen), p(buf), iref(0), iref(R), iref(0))
     assert np.array_equal(back.reshape(-1), g.reshape(-1))
+
+
+def test_statsp_dropin(fbox, oracle_lib):
+    """statsp (stotf.F90:51-60) called with the reference's argument list:
+    one sample of the box's state, against the oracle's statsp on the same
+    state (the box's fields are set through the C ABI)."""
+    from opentbl_amd.lib import FIELDS, NSTAT, lib
+    L, cfg, st, _ = fbox
+    L.otbl_f_box.restype = C.c_void_p
+    box = C.c_void_p(L.otbl_f_box())
+    ob = oracle_lib.OracleBox(cfg, st)
+    for n in ("u", "v", "w", "p"):
+        a = np.ascontiguousarray(st[n], np.complex128).view(np.float64)
+        assert lib().otbl_set_field(box, FIELDS[n], a.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    assert lib().otbl_stats_enable(box, 1) == 0
+    dummy = np.zeros(8)
+    args = [p(dummy)] * 21 + [iref(0), iref(0)]
+    L.statsp_(*args)
+    L.statsp_(*args)
+    ob.statsp()
+    ob.statsp()
+    want = ob.stats()
+    ical = C.c_int()
+    assert lib().otbl_stats_ical(box, C.byref(ical)) == 0 and ical.value == 2
+    for q in range(NSTAT):
+        name = lib().otbl_stats_name(q).decode()
+        g = np.zeros(cfg.nx * (cfg.ny + 1))
+        assert lib().otbl_stats_get(box, q, g.ctypes.data_as(C.POINTER(C.c_double))) == 0
+        w = want[name]
+        assert np.abs(g.reshape(w.shape) - w).max() <= 1e-10 * max(np.abs(w).max(), 1e-300), name
+    assert lib().otbl_stats_enable(box, 0) == 0
