@@ -80,7 +80,7 @@ def kernel_model(cfg, s=8):
 # kernel-event names -> HIP kernel function names in rocprofv3 output
 TRACE_NAMES = {"dct2_rows": "dct_rows_z", "dct3_rows": "dct_rows_z", "div": "div_pencils_kernel",
                "poisson_y": "poisson_y_chunk", "to_planes": "pencils_to_planes_kernel",
-               "products1": "products1_z", "products2": "products2_z", "interpxx": "interpxx_w",
+               "products1": ("products1_zp", "products1_z"), "products2": "products2_z", "interpxx": "interpxx_w",
                "difvisxx": "difvisxx_w", "interpyy": "interpyy_y", "rhsu_loop1": "rhsu_loop1_y",
                "rhs_loop2": "rhs_loop2_y", "rk_final": ("rk_final_y", "rk_final_yh"),
                "project": ("project_kernel", "project_pair_kernel"),
