@@ -87,15 +87,25 @@ template <int W> __device__ __forceinline__ void zsync() {
   }
 }
 
+// 4-point DFT of (t0, t1, 0, 0)
+template <int S, typename C> __device__ __forceinline__ void bfly4h(C *t) {
+  const C a = t[0], b = t[1], ib = mul_si<S>(b);
+  t[0] = cadd(a, b);
+  t[2] = csub(a, b);
+  t[1] = cadd(a, ib);
+  t[3] = csub(a, ib);
+}
+
 // 16-point DFT as 4 x 4: inner DFTs over stride-4 inputs, twiddles W16^{bq},
-// outer DFTs; X[q + 4p]
-template <int S, typename C> __device__ __forceinline__ void bfly16(C *v) {
+// outer DFTs; X[q + 4p].  H: inputs 8..15 are zero (not read).
+template <int S, bool H = false, typename C> __device__ __forceinline__ void bfly16(C *v) {
   using T = decltype(v[0].x);
   C y[4][4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    C t[4] = {v[b], v[b + 4], v[b + 8], v[b + 12]};
-    bfly4<S>(t);
+    C t[4] = {v[b], v[b + 4], H ? v[b] : v[b + 8], H ? v[b] : v[b + 12]};
+    if constexpr (H) bfly4h<S>(t);
+    else bfly4<S>(t);
 #pragma unroll
     for (int q = 0; q < 4; ++q) y[b][q] = t[q];
   }
@@ -129,14 +139,16 @@ template <int S, typename C> __device__ __forceinline__ void bfly16(C *v) {
 }
 
 // 12-point DFT as 3 x 4: inner 4-point DFTs over stride-3 inputs
-// v[b + 3a], twiddles W12^{bq}, outer 3-point DFTs over b; X[q + 4p]
-template <int S, typename C> __device__ __forceinline__ void bfly12(C *v) {
+// v[b + 3a], twiddles W12^{bq}, outer 3-point DFTs over b; X[q + 4p].
+// H: inputs 6..11 are zero (not read).
+template <int S, bool H = false, typename C> __device__ __forceinline__ void bfly12(C *v) {
   using T = decltype(v[0].x);
   C y[3][4];
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
-    C t[4] = {v[b], v[b + 3], v[b + 6], v[b + 9]};
-    bfly4<S>(t);
+    C t[4] = {v[b], v[b + 3], H ? v[b] : v[b + 6], H ? v[b] : v[b + 9]};
+    if constexpr (H) bfly4h<S>(t);
+    else bfly4<S>(t);
 #pragma unroll
     for (int q = 0; q < 4; ++q) y[b][q] = t[q];
   }
@@ -162,15 +174,65 @@ template <int S, typename C> __device__ __forceinline__ void bfly12(C *v) {
   }
 }
 
-template <int R, int S, typename C> __device__ __forceinline__ void zbfly(C *v) {
-  if constexpr (R == 16) bfly16<S>(v);
-  else if constexpr (R == 12) bfly12<S>(v);
-  else bfly<R, S>(v);
+// 8-point DFT of (v0..v3, 0, 0, 0, 0): bfly8's even/odd split with half
+// inputs
+template <int S, typename C> __device__ __forceinline__ void bfly8h(C *v) {
+  using T = decltype(v[0].x);
+  const T h = (T)0.70710678118654752440;
+  C e[4] = {v[0], v[2], v[0], v[0]}, o[4] = {v[1], v[3], v[1], v[1]};
+  bfly4h<S>(e);
+  bfly4h<S>(o);
+  C o1 = cadd(o[1], mul_si<S>(o[1]));
+  o1.x *= h;
+  o1.y *= h;
+  C o2 = mul_si<S>(o[2]);
+  C o3 = csub(mul_si<S>(o[3]), o[3]);
+  o3.x *= h;
+  o3.y *= h;
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
 }
 
-// Twiddles and butterflies of one pass on the lane's registers.
-template <int N, int G, int R, int Ns, int S, typename C>
-__device__ __forceinline__ void zpass_compute(C (&v)[N / G], int t, const C *__restrict__ tw) {
+// radix-R butterfly; H: the upper half of the inputs is zero
+template <int R, int S, bool H = false, typename C> __device__ __forceinline__ void zbfly(C *v) {
+  if constexpr (R == 16) bfly16<S, H>(v);
+  else if constexpr (R == 12) bfly12<S, H>(v);
+  else if constexpr (H && R == 8) bfly8h<S>(v);
+  else if constexpr (H && R == 4) bfly4h<S>(v);
+  else {
+    static_assert(!H, "no half-input butterfly of this radix");
+    bfly<R, S>(v);
+  }
+}
+
+// Twiddle table entries of one pass: for butterfly m of the lane, W^(k 2^q)
+// (k = (t + G m) mod Ns, q with 2^q < R) in units of N / (Ns R) of the
+// table.  Loaded before the LDS exchange that precedes the pass, so their
+// latency overlaps the exchange.
+template <int N, int G, int R, int Ns, typename C>
+__device__ __forceinline__ void zpass_twload(C (&base)[N / (R * G)][4], int t, const C *__restrict__ tw) {
+  if constexpr (Ns > 1) {
+    constexpr int NB = N / (R * G), st = N / (Ns * R);
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      const int k = (t + G * m) % Ns;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((1 << q) < R) base[m][q] = tw[k * (1 << q) * st];
+    }
+  }
+}
+
+// Twiddles and butterflies of one pass on the lane's registers (H: the
+// upper half of every butterfly's inputs is zero, first pass only).
+template <int N, int G, int R, int Ns, int S, bool H, typename C>
+__device__ __forceinline__ void zpass_compute(C (&v)[N / G], const C (&base_in)[N / (R * G)][4]) {
   constexpr int NB = N / (R * G);
   static_assert(NB * R * G == N, "radix does not tile the lanes");
 #pragma unroll
@@ -180,13 +242,11 @@ __device__ __forceinline__ void zpass_compute(C (&v)[N / G], int t, const C *__r
       // W^r: the powers 1, 2, 4, 8 from the table, every other power as the
       // product of the table powers of its set bits (at most three
       // multiplications deep), formed where it is applied
-      const int k = (t + G * m) % Ns;
-      constexpr int st = N / (Ns * R);
       C base[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if ((1 << q) < R) {
-          base[q] = tw[k * (1 << q) * st];
+          base[q] = base_in[m][q];
           if (S < 0) base[q].y = -base[q].y;
         }
       }
@@ -203,8 +263,23 @@ __device__ __forceinline__ void zpass_compute(C (&v)[N / G], int t, const C *__r
         b[r] = cmulc(b[r], w);
       }
     }
-    zbfly<R, S>(b);
+    zbfly<R, S, H>(b);
   }
+}
+
+// padded index of a + d for a compile-time offset d that keeps the 16-element
+// group structure (d a multiple of 16, or a below a group boundary that a + d
+// does not cross): zpad(a) + d + d / 16
+__device__ __forceinline__ int zpad_off(int pa, int d) { return pa + d + (d >> 4); }
+
+// The store index base + q Ns of pass (R, Ns), base = (j / Ns) Ns R + j mod Ns,
+// is zpad(base) + q Ns + (q Ns) / 16 for every lane when Ns is a multiple of
+// 16, or when Ns divides 16 and Ns R is a multiple of 16 (the group
+// boundary carry is then (q Ns) / 16), or when Ns R divides 16 (no lane's
+// q range crosses a group boundary); one address per butterfly then serves
+// all R stores through the instruction offset.
+template <int R, int Ns> constexpr bool zstore_linear() {
+  return Ns % 16 == 0 || (16 % Ns == 0 && (Ns * R) % 16 == 0) || 16 % (Ns * R) == 0;
 }
 
 // outputs of a pass (radix R, Ns) to LDS
@@ -215,42 +290,64 @@ __device__ __forceinline__ void zpass_store(const C (&v)[N / G], int t, C *lds) 
   for (int m = 0; m < NB; ++m) {
     const int j = t + G * m;
     const int base = (j / Ns) * Ns * R + (j % Ns);
+    if constexpr (zstore_linear<R, Ns>()) {
+      const int pb = zpad(base);
 #pragma unroll
-    for (int q = 0; q < R; ++q) lds[zpad(base + q * Ns)] = v[m * R + q];
+      for (int q = 0; q < R; ++q) lds[zpad_off(pb, q * Ns)] = v[m * R + q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < R; ++q) lds[zpad(base + q * Ns)] = v[m * R + q];
+    }
   }
 }
 
-// inputs of a pass of radix R from LDS: x[j + r N / R]
+// inputs of a pass of radix R from LDS: x[j + r N / R] (N / R and G are
+// multiples of 16, so every index is zpad(t) plus a compile-time offset)
 template <int N, int G, int R, typename C>
 __device__ __forceinline__ void zpass_load(C (&v)[N / G], int t, const C *lds) {
   constexpr int NB = N / (R * G);
+  static_assert((N / R) % 16 == 0 && G % 16 == 0, "load offsets break the padding groups");
+  const int pt = zpad(t);
 #pragma unroll
-  for (int m = 0; m < NB; ++m) {
-    const int j = t + G * m;
+  for (int m = 0; m < NB; ++m)
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[m * R + r] = lds[zpad(j + r * (N / R))];
+    for (int r = 0; r < R; ++r) v[m * R + r] = lds[zpad_off(pt, G * m + r * (N / R))];
+}
+
+// Passes P..NP-1 of the forward (Inv = false: radices in plan order) or the
+// inverse (Inv = true: reversed order) transform; v holds the inputs of pass
+// P and base its twiddle entries.  On return v holds the outputs of the last
+// pass.
+template <int N, int S, bool Inv, int P, int Ns, bool H, typename C, int NBP>
+__device__ __forceinline__ void zrun_from(C (&v)[N / (64 * ZPlan<N>::W)], int t, const C *__restrict__ tw, C *lds,
+                                          const C (&base)[NBP][4]) {
+  constexpr int W = ZPlan<N>::W, G = 64 * W, NP = ZPlan<N>::NP;
+  constexpr int R = ZPlan<N>::rad(Inv ? NP - 1 - P : P);
+  static_assert(NBP == N / (R * G), "twiddle entries of another pass");
+  static_assert(!H || (P == 0 && Ns == 1), "half inputs in the first pass only");
+  zpass_compute<N, G, R, Ns, S, H>(v, base);
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (P + 1 < NP) {
+    constexpr int R2 = ZPlan<N>::rad(Inv ? NP - 2 - P : P + 1);
+    C base2[N / (R2 * G)][4];
+    zpass_twload<N, G, R2, Ns * R>(base2, t, tw);
+    zpass_store<N, G, R, Ns>(v, t, lds);
+    zsync<W>();
+    zpass_load<N, G, R2>(v, t, lds);
+    zsync<W>();
+    zrun_from<N, S, Inv, P + 1, Ns * R, false>(v, t, tw, lds, base2);
   }
 }
 
-// Passes from..NP-1 of the forward (Dir = +1: radices in plan order) or the
-// inverse (Dir = -1: reversed order) transform; v holds the inputs of pass
-// `from`.  On return v holds the outputs of the last pass.
-template <int N, int S, bool Inv, int P = 0, int Ns = 1, typename C>
+// The whole forward (Inv = false) or inverse (Inv = true) transform of the
+// team's row in v, sign S.  H: the inputs of the first pass's upper half
+// (register slots m R + r, r >= R / 2) are zero and not read.
+template <int N, int S, bool Inv, bool H = false, typename C>
 __device__ __forceinline__ void zrun(C (&v)[N / (64 * ZPlan<N>::W)], int t, const C *__restrict__ tw, C *lds) {
   constexpr int W = ZPlan<N>::W, G = 64 * W, NP = ZPlan<N>::NP;
-  if constexpr (P < NP) {
-    constexpr int R = ZPlan<N>::rad(Inv ? NP - 1 - P : P);
-    zpass_compute<N, G, R, Ns, S>(v, t, tw);
-    __builtin_amdgcn_sched_barrier(0);
-    if constexpr (P + 1 < NP) {
-      constexpr int R2 = ZPlan<N>::rad(Inv ? NP - 2 - P : P + 1);
-      zpass_store<N, G, R, Ns>(v, t, lds);
-      zsync<W>();
-      zpass_load<N, G, R2>(v, t, lds);
-      zsync<W>();
-      zrun<N, S, Inv, P + 1, Ns * R>(v, t, tw, lds);
-    }
-  }
+  constexpr int R = ZPlan<N>::rad(Inv ? NP - 1 : 0);
+  C base[N / (R * G)][4]; // the first pass has no twiddles (Ns = 1)
+  zrun_from<N, S, Inv, 0, 1, H>(v, t, tw, lds, base);
 }
 
 // The physical index held in register slot e after the last pass of a
@@ -316,82 +413,72 @@ __device__ __forceinline__ void zload_spectral(typename CT<R>::c (&v)[N / (64 * 
 
 // After the last inverse pass (natural order, R = first forward radix,
 // outputs y[j + q N / R]): through LDS, split the transform Z of a + i b
-// into the spectra of a and b, modes 0..nz2, scaled by 1/N (ffts.f90:51).
-// With XA, A also receives the mean-flow terms of a product whose mean was
-// left out of the transform: A_k += ck XA_k for k >= 1 and A_0 += c0 XA_0,
-// XA read before A is written (XA may be A itself).
-template <int N, typename R>
-__device__ __forceinline__ void zstore_spectral(const typename CT<R>::c (&v)[N / (64 * ZPlan<N>::W)], int t,
-                                                typename CT<R>::c *lds, R *A, R *B, int nz2,
-                                                const R *XA = nullptr, R c0 = R(0), R ck = R(0)) {
-  using C = typename CT<R>::c;
-  constexpr int W = ZPlan<N>::W, G = 64 * W, RD = ZPlan<N>::rad(0), NB = N / (RD * G);
-#pragma unroll
-  for (int m = 0; m < NB; ++m)
-#pragma unroll
-    for (int q = 0; q < RD; ++q) lds[zpad(t + G * m + q * (N / RD))] = v[m * RD + q];
-  zsync<W>();
-  const R sc = R(1.0 / N);
-  for (int k = t; k <= nz2; k += G) {
-    const C z = lds[zpad(k)];
-    const C w = lds[zpad(k == 0 ? 0 : N - k)];
-    if (A) {
-      R ar = R(0.5) * (z.x + w.x) * sc, ai = (k == 0) ? R(0) : R(0.5) * (z.y - w.y) * sc;
-      if (XA) {
-        const R xr = XA[2 * k], xi = XA[2 * k + 1];
-        ar += (k == 0 ? c0 : ck) * xr;
-        ai += (k == 0) ? R(0) : ck * xi;
-      }
-      A[2 * k] = ar;
-      A[2 * k + 1] = ai;
-    }
-    if (B) {
-      B[2 * k] = R(0.5) * (z.y + w.y) * sc;
-      B[2 * k + 1] = (k == 0) ? R(0) : R(-0.5) * (z.x - w.x) * sc;
-    }
-  }
-  zsync<W>();
-}
-
-// zstore_spectral with the mean-flow terms on both outputs: A_k += cka XA_k
-// (k >= 1), A_0 += c0a XA_0, and the same for B with XB, c0b, ckb (XA and XB
-// read before A and B are written; either may be the output itself; null:
-// no such term).  A or B null: that output is not written.
+// into the spectra of a and b, modes 0..nz2, scaled by 1/N (ffts.f90:51),
+// and add the mean-flow terms of products whose mean was left out of the
+// transform: A_k += cka XA_k for k >= 1, A_0 += c0a XA_0, the same for B
+// with XB, c0b, ckb.  XA and XB are read before A and B are written (either
+// may be the output itself); a null XA or XB adds no term, a null A or B is
+// not written.  nz2 = N / 3 - 1, the engine's dealiasing: the mode loop is
+// unrolled to those N / 3 modes, and the mean-flow rows are loaded in one
+// batch ahead of the arithmetic.
 template <int N, typename R>
 __device__ __forceinline__ void zstore_spectral2(const typename CT<R>::c (&v)[N / (64 * ZPlan<N>::W)], int t,
                                                  typename CT<R>::c *lds, R *A, R *B, int nz2, const R *XA, R c0a,
                                                  R cka, const R *XB, R c0b, R ckb) {
   using C = typename CT<R>::c;
   constexpr int W = ZPlan<N>::W, G = 64 * W, RD = ZPlan<N>::rad(0), NB = N / (RD * G);
+  constexpr int NI = (N / 3 + G - 1) / G; // mode loop trips (nz2 + 1 = N / 3 modes)
+  const int pt = zpad(t);
+  // slots whose indices all fall in the zero band nz2 < k < N - nz2 are never
+  // read back, and the last pass does not form them
+  constexpr int NZ2 = N / 3 - 1;
 #pragma unroll
   for (int m = 0; m < NB; ++m)
 #pragma unroll
-    for (int q = 0; q < RD; ++q) lds[zpad(t + G * m + q * (N / RD))] = v[m * RD + q];
+    for (int q = 0; q < RD; ++q) {
+      const int b = G * m + q * (N / RD);
+      if (!(b > NZ2 && b + G - 1 < N - NZ2)) lds[zpad_off(pt, b)] = v[m * RD + q];
+    }
+  C xa[NI], xb[NI];
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const int k = min(t + G * it, nz2);
+    xa[it] = XA ? ((const C *)XA)[k] : mkc<C>(0.0, 0.0);
+    xb[it] = XB ? ((const C *)XB)[k] : mkc<C>(0.0, 0.0);
+  }
   zsync<W>();
   const R sc = R(1.0 / N);
-  for (int k = t; k <= nz2; k += G) {
-    const C z = lds[zpad(k)];
-    const C w = lds[zpad(k == 0 ? 0 : N - k)];
-    if (A) {
-      R ar = R(0.5) * (z.x + w.x) * sc, ai = (k == 0) ? R(0) : R(0.5) * (z.y - w.y) * sc;
-      if (XA) {
-        ar += (k == 0 ? c0a : cka) * XA[2 * k];
-        ai += (k == 0) ? R(0) : cka * XA[2 * k + 1];
+  // N - k for k = t + G it is zpad(N - t) - G it - G it / 16 (t > 0)
+  const int pw = zpad(N - t);
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const int k = t + G * it;
+    if (k <= nz2) {
+      const C z = lds[zpad_off(pt, G * it)];
+      const C w = k == 0 ? lds[0] : lds[pw - G * it - ((G * it) >> 4)];
+      if (A) {
+        R ar = R(0.5) * (z.x + w.x) * sc, ai = (k == 0) ? R(0) : R(0.5) * (z.y - w.y) * sc;
+        ar += (k == 0 ? c0a : cka) * xa[it].x;
+        ai += (k == 0) ? R(0) : cka * xa[it].y;
+        ((C *)A)[k] = mkc<C>(ar, ai);
       }
-      A[2 * k] = ar;
-      A[2 * k + 1] = ai;
-    }
-    if (B) {
-      R br = R(0.5) * (z.y + w.y) * sc, bi = (k == 0) ? R(0) : R(-0.5) * (z.x - w.x) * sc;
-      if (XB) {
-        br += (k == 0 ? c0b : ckb) * XB[2 * k];
-        bi += (k == 0) ? R(0) : ckb * XB[2 * k + 1];
+      if (B) {
+        R br = R(0.5) * (z.y + w.y) * sc, bi = (k == 0) ? R(0) : R(-0.5) * (z.x - w.x) * sc;
+        br += (k == 0 ? c0b : ckb) * xb[it].x;
+        bi += (k == 0) ? R(0) : ckb * xb[it].y;
+        ((C *)B)[k] = mkc<C>(br, bi);
       }
-      B[2 * k] = br;
-      B[2 * k + 1] = bi;
     }
   }
   zsync<W>();
+}
+
+// zstore_spectral2 with at most one mean-flow term (on A)
+template <int N, typename R>
+__device__ __forceinline__ void zstore_spectral(const typename CT<R>::c (&v)[N / (64 * ZPlan<N>::W)], int t,
+                                                typename CT<R>::c *lds, R *A, R *B, int nz2,
+                                                const R *XA = nullptr, R c0 = R(0), R ck = R(0)) {
+  zstore_spectral2<N, R>(v, t, lds, A, B, nz2, XA, c0, ck, nullptr, R(0), R(0));
 }
 
 } // namespace otbl

@@ -363,21 +363,35 @@ fourxz_z(double *fou, double *phys, int iopt, const double2 *__restrict__ tw, co
 // N = nx-1) on x-contiguous pencils, two real lines per complex transform:
 // Makhoul's reordering turns each DCT into a complex DFT of length N, and
 // Bluestein's chirp-z evaluates that DFT as a circular convolution of
-// length M (a power of two >= 2N-1) on the register engine: forward
+// length M (>= 2N-1, so N <= M/2) on the register engine: forward
 // transform (sign -1), product with the transformed conjugate chirp bf at
 // each register's physical index, inverse transform (sign +1), then the
 // chirp and the Makhoul untwisting through the team's LDS buffer.
-// iopt < 0: DCT-II x 1/N, iopt > 0: DCT-III x 0.5; in place (src == dst)
+// IOPT < 0: DCT-II x 1/N, IOPT > 0: DCT-III x 0.5; in place (src == dst)
 // allowed, each team reads its two lines before writing them.
-template <int M, typename R>
+// Register slot m R0 + r of the first pass holds index t + G m + r M/R0;
+// slots with G m + r M/R0 >= M/2 are zero on input (the forward pass skips
+// them) and unused on output (the inverse's last pass never forms them).
+// Every global load of a phase is issued before its arithmetic, with
+// clamped indices and selects in place of branches.
+// base[idx] through a 32-bit byte offset (idx sizeof(T) < 2^32): the load
+// takes a scalar base and one offset register instead of a 64-bit address
+template <typename T> __device__ __forceinline__ T ld32(const T *base, unsigned idx) {
+  return *(const T *)((const char *)base + idx * (unsigned)sizeof(T));
+}
+
+template <int M, typename R, int IOPT>
 __global__ void __launch_bounds__(ZTeam<M>::TPB_, sizeof(R) == 4 ? 6 : 1)
-dct_rows_z(const R *src, R *dst, int Lp, int N, int iopt, const typename CT<R>::c *__restrict__ chirp,
+dct_rows_z(const R *src, R *dst, int Lp, int N, const typename CT<R>::c *__restrict__ chirp,
            const typename CT<R>::c *__restrict__ bf, const typename CT<R>::c *__restrict__ tw4,
            const typename CT<R>::c *__restrict__ tw) {
   using C = typename CT<R>::c;
   using Z = ZTeam<M>;
+  constexpr int R0 = ZPlan<M>::rad(0), NB0 = M / (R0 * Z::G), NO = (M / 2 + Z::G - 1) / Z::G;
+  static_assert(NB0 == 1, "the half-input first pass assumes one butterfly per lane");
   __shared__ C lds_all[Z::TEAMS * zlds_elems<M>()];
-  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G);
+  int t = threadIdx.x % Z::G;
   C *lds = lds_all + team * zlds_elems<M>();
   const int pair = blockIdx.x * Z::TEAMS + team;
   const int l0 = 2 * pair, l1 = l0 + 1;
@@ -385,61 +399,96 @@ dct_rows_z(const R *src, R *dst, int Lp, int N, int iopt, const typename CT<R>::
   const bool has1 = l1 < Lp;
   const R *x0 = src + (size_t)l0 * N, *x1 = src + (size_t)(has1 ? l1 : l0) * N;
   const int half = (N + 1) / 2;
-  constexpr int R0 = ZPlan<M>::rad(0), NB0 = M / (R0 * Z::G);
+  constexpr int RH = R0 / 2; // live input slots r < RH: index t + r M/R0 < M/2
   C v[Z::E];
+  {
+    R a0[RH], a1[RH], b0[RH], b1[RH];
+    C ch[RH], tt[RH];
 #pragma unroll
-  for (int m = 0; m < NB0; ++m)
-#pragma unroll
-    for (int r = 0; r < R0; ++r) {
-      const int k = t + Z::G * m + r * (M / R0);
-      C a = mkc<C>(R(0), R(0));
-      if (k < N) {
-        C z;
-        if (iopt < 0) { // Makhoul reordering of the two real lines
-          const int sidx = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
-          z = mkc<C>(x0[sidx], has1 ? x1[sidx] : R(0));
-        } else { // V = (X_k - i X_{N-k}) exp(+i pi k / 2N), Z = V0 + i V1, conj(Z)
-          C tt = tw4[k];
-          tt.y = -tt.y;
-          const R a0 = x0[k], b0 = k ? x0[N - k] : R(0);
-          const R a1 = has1 ? x1[k] : R(0), b1 = (has1 && k) ? x1[N - k] : R(0);
-          const C v0 = cmulc(mkc<C>(a0, -b0), tt), v1 = cmulc(mkc<C>(a1, -b1), tt);
-          z = mkc<C>(v0.x - v1.y, -(v0.y + v1.x));
-        }
-        a = cmulc(z, chirp[k]);
+    for (int r = 0; r < RH; ++r) {
+      const unsigned k = min(t + r * (M / R0), N - 1);
+      if constexpr (IOPT < 0) { // Makhoul reordering of the two real lines
+        const unsigned sidx = (int)k < half ? 2 * k : 2 * (N - 1 - k) + 1;
+        a0[r] = ld32(x0, sidx);
+        a1[r] = ld32(x1, sidx);
+      } else {
+        const unsigned nk = k ? N - k : 0;
+        a0[r] = ld32(x0, k);
+        a1[r] = ld32(x1, k);
+        b0[r] = ld32(x0, nk);
+        b1[r] = ld32(x1, nk);
+        tt[r] = ld32(tw4, k);
       }
-      v[m * R0 + r] = a;
+      ch[r] = ld32(chirp, k);
     }
-  zrun<M, -1, false>(v, t, tw, lds);
 #pragma unroll
-  for (int e = 0; e < Z::E; ++e) v[e] = cmulc(v[e], bf[zphys_index<M>(t, e)]);
+    for (int r = 0; r < RH; ++r) {
+      const int k = t + r * (M / R0);
+      C z;
+      if constexpr (IOPT < 0) {
+        z = mkc<C>(a0[r], has1 ? a1[r] : R(0));
+      } else { // V = (X_k - i X_{N-k}) exp(+i pi k / 2N), Z = V0 + i V1, conj(Z)
+        C w = tt[r];
+        w.y = -w.y;
+        const R c0 = k ? b0[r] : R(0), c1 = (has1 && k) ? b1[r] : R(0), d1 = has1 ? a1[r] : R(0);
+        const C v0 = cmulc(mkc<C>(a0[r], -c0), w), v1 = cmulc(mkc<C>(d1, -c1), w);
+        z = mkc<C>(v0.x - v1.y, -(v0.y + v1.x));
+      }
+      const C a = cmulc(z, ch[r]);
+      v[r] = k < N ? a : mkc<C>(R(0), R(0));
+    }
+  }
+  zrun<M, -1, false, true>(v, t, tw, lds);
+  {
+    C w[Z::E];
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) w[e] = ld32(bf, (unsigned)zphys_index<M>(t, e));
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) v[e] = cmulc(v[e], w[e]);
+  }
   zrun<M, +1, true>(v, t, tw, lds);
-  // natural-order outputs t + G m + q M / R0; keep k < N times the chirp
+  // natural-order outputs t + q M / R0 (q < R0 / 2 hold every k < N); keep
+  // k < N times the chirp.  The load offsets are formed again here rather
+  // than kept from the input phase (kept, they spill).
+  asm volatile("" : "+v"(t));
+  {
+    C ch[RH];
 #pragma unroll
-  for (int m = 0; m < NB0; ++m)
+    for (int q = 0; q < RH; ++q) ch[q] = ld32(chirp, (unsigned)min(t + q * (M / R0), N - 1));
 #pragma unroll
-    for (int q = 0; q < R0; ++q) {
-      const int k = t + Z::G * m + q * (M / R0);
-      if (k < N) lds[zpad(k)] = cmulc(chirp[k], v[m * R0 + q]);
+    for (int q = 0; q < RH; ++q) {
+      const int k = t + q * (M / R0);
+      if (k < N) lds[zpad(k)] = cmulc(ch[q], v[q]);
     }
+  }
   zsync<Z::W>();
   R *y0 = dst + (size_t)l0 * N, *y1 = dst + (size_t)l1 * N;
-  if (iopt < 0) {
+  if constexpr (IOPT < 0) {
     const R sc = R(2.0 / N);
-    for (int k = t; k < N; k += Z::G) {
-      const C z = lds[zpad(k)], w = lds[zpad(k ? N - k : 0)];
-      const C v0 = mkc<C>(R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y));
-      const C v1 = mkc<C>(R(0.5) * (z.y + w.y), R(-0.5) * (z.x - w.x));
-      const C tt = tw4[k];
-      y0[k] = sc * (tt.x * v0.x - tt.y * v0.y);
-      if (has1) y1[k] = sc * (tt.x * v1.x - tt.y * v1.y);
+    C tt[NO];
+#pragma unroll
+    for (int it = 0; it < NO; ++it) tt[it] = ld32(tw4, (unsigned)min(t + Z::G * it, N - 1));
+#pragma unroll
+    for (int it = 0; it < NO; ++it) {
+      const int k = t + Z::G * it;
+      if (k < N) {
+        const C z = lds[zpad(k)], w = lds[zpad(k ? N - k : 0)];
+        const C v0 = mkc<C>(R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y));
+        const C v1 = mkc<C>(R(0.5) * (z.y + w.y), R(-0.5) * (z.x - w.x));
+        y0[k] = sc * (tt[it].x * v0.x - tt[it].y * v0.y);
+        if (has1) y1[k] = sc * (tt[it].x * v1.x - tt[it].y * v1.y);
+      }
     }
   } else {
-    for (int k = t; k < N; k += Z::G) {
-      const C zc = lds[zpad(k)]; // conj of sum_k Z_k exp(+2 pi i k n / N)
-      const int o = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
-      y0[o] = R(0.5) * zc.x;
-      if (has1) y1[o] = R(-0.5) * zc.y;
+#pragma unroll
+    for (int it = 0; it < NO; ++it) {
+      const int k = t + Z::G * it;
+      if (k < N) {
+        const C zc = lds[zpad(k)]; // conj of sum_k Z_k exp(+2 pi i k n / N)
+        const int o = k < half ? 2 * k : 2 * (N - 1 - k) + 1;
+        y0[o] = R(0.5) * zc.x;
+        if (has1) y1[o] = R(-0.5) * zc.y;
+      }
     }
   }
 }
@@ -517,9 +566,9 @@ template <typename R> static void dct_launch(Box &b, R *P, int iopt, int Lp, con
   switch (b.dct_M) {
 #define ZDCT(MM)                                                                                              \
   case MM: {                                                                                                   \
-    auto kf = &dct_rows_z<MM, R>;                                                                              \
-    LAUNCH(b, name, zgrid<MM>(pairs), ZTeam<MM>::TPB_, 0, kf, P, P, Lp, N, iopt, (const C *)chirp,             \
-           (const C *)bf, (const C *)tw4, (const C *)tw);                                                      \
+    auto kf = iopt < 0 ? &dct_rows_z<MM, R, -1> : &dct_rows_z<MM, R, 1>;                                       \
+    LAUNCH(b, name, zgrid<MM>(pairs), ZTeam<MM>::TPB_, 0, kf, P, P, Lp, N, (const C *)chirp, (const C *)bf,    \
+           (const C *)tw4, (const C *)tw);                                                                     \
   } break;
     ZDCT(512)
     ZDCT(1024)
