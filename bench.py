@@ -57,7 +57,8 @@ def kernel_model(cfg, s=8):
     m["products2"] = ((X - 2) * Rv * 7 * z * s, (X - 2) * Rv * 4 * 5 * nz * math.log2(nz), 1)
     m["interpyy"] = ((2 * Ru * X + (Ru + Rv) * (X - 2) + (Rv + Ru) * (X - 2)) * z * s / 3, 0.0, 3)
     m["rhsu_loop1"] = ((Rv + 2 * Ru) * f, 0.0, 1)
-    m["rhs_loop2"] = ((X - 2) * z * s * (2 * Ru + 2 * Rv + 2 * Ru + Ru + Rv + Ru + Ru + Rv), 0.0, 1)
+    # rhsu, rhsv, rhsw read and written; resu, resv, resw, vw, vv, ww read
+    m["rhs_loop2"] = ((X - 2) * z * s * (2 * Ru + 2 * Rv + 2 * Ru + Ru + Rv + Ru + Ru + Rv + Rv), 0.0, 1)
     # res, rhs, rhs_prev, u read; u written (rhs becomes rhs_prev by a role swap)
     m["rk_final"] = ((5 * Ru + 5 * Rv + 5 * Ru) * f / 3, 0.0, 3)
     m["save_predict"] = ((4 * Ru + 3 * Rv) * f, 0.0, 1)  # u, v, w, p read; predicted u, v, w written

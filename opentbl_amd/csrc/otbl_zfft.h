@@ -240,8 +240,11 @@ __device__ __forceinline__ void zpass_compute(C (&v)[N / G], const C (&base_in)[
     C *b = v + m * R;
     if constexpr (Ns > 1) {
       // W^r: the powers 1, 2, 4, 8 from the table, every other power as the
-      // product of the table powers of its set bits (at most three
-      // multiplications deep), formed where it is applied
+      // product of a smaller power already formed and the table power of
+      // its top bit (one multiplication each, at most three deep); r and
+      // r + HB are taken together so that few powers stay live
+      constexpr int HB = R > 8 ? 8 : (R > 4 ? 4 : (R > 2 ? 2 : 1));
+      constexpr int QB = HB == 8 ? 3 : (HB == 4 ? 2 : (HB == 2 ? 1 : 0));
       C base[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -250,18 +253,15 @@ __device__ __forceinline__ void zpass_compute(C (&v)[N / G], const C (&base_in)[
           if (S < 0) base[q].y = -base[q].y;
         }
       }
+      C w[HB];
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        C w = mkc<C>(1.0, 0.0);
-        bool first = true;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (r & (1 << q)) {
-            w = first ? base[q] : cmulc(w, base[q]);
-            first = false;
-          }
-        b[r] = cmulc(b[r], w);
+      for (int r = 1; r < HB; ++r) {
+        const int hb = r >= 4 ? 4 : (r >= 2 ? 2 : 1), qb = r >= 4 ? 2 : (r >= 2 ? 1 : 0);
+        w[r] = r == hb ? base[qb] : cmulc(w[r - hb], base[qb]);
+        b[r] = cmulc(b[r], w[r]);
+        if (r + HB < R) b[r + HB] = cmulc(b[r + HB], cmulc(w[r], base[QB]));
       }
+      if (HB < R) b[HB] = cmulc(b[HB], base[QB]);
     }
     zbfly<R, S, H>(b);
   }
