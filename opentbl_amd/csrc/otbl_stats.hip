@@ -22,6 +22,7 @@
 // its intermediate fields are held in FP32.
 #include "otbl_kcommon.h"
 #include "otbl_fft.h"
+#include "otbl_zfft.h"
 
 #include <type_traits>
 
@@ -444,6 +445,201 @@ __global__ void __launch_bounds__(TPB) stats_z_kernel(StatsZ a, FFTPlan plan, in
   acc[S_VORTZA * S] += dv0 - b6;
 }
 
+// ---------------------------------------------------------------------------
+// The same row sums on the register-resident z engine (nz of the engine,
+// nz2 = nz/3 - 1): one (j, i) row per team of W wavefronts, rows numbered
+// (i-1) ny + (j-1).  The three packed c2r transforms run in the fields'
+// precision in registers (their inputs formed from the spectral rows as
+// they are loaded), the physical products and every sum in FP64.
+// ---------------------------------------------------------------------------
+template <typename C> struct CPair {
+  C x, y;
+};
+
+// Complex input of the c2r transform of two real signals whose spectra
+// spec(k) gives for 0 <= k <= nz2: the Hermitian-extended spectrum of
+// x + i y in the register slots of the engine's first pass (Im of mode 0
+// does not enter a real signal; zero between nz2 and N - nz2).
+template <int N, typename C, class F>
+__device__ __forceinline__ void zload_pair(C (&v)[N / (64 * ZPlan<N>::W)], int t, F spec) {
+  constexpr int G = 64 * ZPlan<N>::W, RD = ZPlan<N>::rad(0), NB = N / (RD * G), nz2 = N / 3 - 1;
+#pragma unroll
+  for (int m = 0; m < NB; ++m)
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int base = G * m + r * (N / RD), k = base + t;
+      C z = mkc<C>(0.0, 0.0);
+      if (base + G - 1 <= nz2) { // all lanes below the band
+        const CPair<C> s = spec(k);
+        z = k == 0 ? mkc<C>(s.x.x, s.y.x) : mkc<C>(s.x.x - s.y.y, s.x.y + s.y.x);
+      } else if (base >= N - nz2) { // all above: conj(X[N-k]) + i conj(Y[N-k])
+        const CPair<C> s = spec(N - k);
+        z = mkc<C>(s.x.x + s.y.y, -s.x.y + s.y.x);
+      } else if (base <= nz2 || base + G - 1 >= N - nz2) { // straddles an edge
+        const bool lo = k <= nz2, hi = k >= N - nz2;
+        const CPair<C> s = spec(lo ? k : (hi ? N - k : 0));
+        C r2 = lo ? mkc<C>(s.x.x - s.y.y, s.x.y + s.y.x) : mkc<C>(s.x.x + s.y.y, -s.x.y + s.y.x);
+        r2 = k == 0 ? mkc<C>(s.x.x, s.y.x) : r2;
+        z = (lo || hi) ? r2 : mkc<C>(0.0, 0.0);
+      }
+      v[m * RD + r] = z;
+    }
+}
+
+template <int N, typename T>
+__global__ void __launch_bounds__(ZTeam<N>::TPB_)
+stats_zr(StatsZ a, const typename CT<T>::c *__restrict__ tw, int nx, int ny, int nz1, int nrows) {
+  using C = typename CT<T>::c;
+  using Z = ZTeam<N>;
+  constexpr int nz2 = N / 3 - 1, NK = (nz2 + 1 + Z::G - 1) / Z::G;
+  __shared__ C lds_all[Z::TEAMS * zlds_elems<N>()];
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G), t = threadIdx.x % Z::G;
+  C *lds = lds_all + team * zlds_elems<N>();
+  const int row = blockIdx.x * Z::TEAMS + team;
+  if (row >= nrows) return; // a whole team
+  const int j = row % ny + 1, i = row / ny + 1;
+  const size_t ov = (size_t)row * nz1;
+  const C *Bq[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) Bq[q] = (const C *)((const T *)a.B[q] + ov);
+  const C *V = (const C *)((const T *)a.v + ov), *DV = (const C *)((const T *)a.dvdx + ov);
+  const size_t om = (size_t)(j - 1) * nx + (i - 1); // (i, j) of the [ny][nx] mean-flow rows
+  // mode 0 of B1 (u_x at the v rows) with the FP64 mean flow of a mixed box
+  const T m1 = a.m1 ? (T)a.m1[om] : T(0);
+  auto b1at = [&](int k) {
+    C b = Bq[0][k];
+    if (a.m1 && k == 0) b.x = m1;
+    return b;
+  };
+  double sum[NSUM];
+#pragma unroll
+  for (int q = 0; q < NSUM; ++q) sum[q] = 0.0;
+  // ---- physical space (stotf.F90:148-154, 256-283) ----
+  T s2[Z::E], U[Z::E];
+  {
+    C v[Z::E]; // omega_x + i omega_y
+    zload_pair<N>(v, t, [&](int k) {
+      const T kz = (T)a.kaz[k];
+      const C b7 = Bq[6][k], vv = V[k], b1 = b1at(k), b3 = Bq[2][k];
+      return CPair<C>{mkc<C>(b7.x + kz * vv.y, b7.y - kz * vv.x), mkc<C>(-kz * b1.y - b3.x, kz * b1.x - b3.y)};
+    });
+    zrun<N, +1, false>(v, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) s2[e] = v[e].x * v[e].x + v[e].y * v[e].y;
+  }
+  {
+    C v[Z::E]; // omega_z + i u
+    zload_pair<N>(v, t, [&](int k) {
+      const C dv = DV[k], b6 = Bq[5][k];
+      return CPair<C>{mkc<C>(dv.x - b6.x, dv.y - b6.y), b1at(k)};
+    });
+    zrun<N, +1, false>(v, t, tw, lds);
+    float *ens = a.ens + (size_t)row * N;
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) {
+      ens[zphys_index<N>(t, e)] = (float)sqrt((double)s2[e] + (double)v[e].x * (double)v[e].x);
+      U[e] = v[e].y;
+    }
+  }
+  {
+    C v[Z::E]; // v + i w
+    zload_pair<N>(v, t, [&](int k) { return CPair<C>{V[k], Bq[1][k]}; });
+    zrun<N, +1, false>(v, t, tw, lds);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) {
+      const double u = U[e], vv = v[e].x, w = v[e].y;
+      sum[21] += u * u * u;   // u3
+      sum[22] += u * w * w;   // w2u
+      sum[23] += vv * vv * vv; // v3
+      sum[24] += vv * w * w;  // w2v
+      sum[25] += u * vv * vv; // v2u
+      sum[26] += vv * u * u;  // u2v
+    }
+  }
+  // ---- sums over the modes (stotf.F90:156-218), modes t + G it ----
+  auto re = [](double2 x, double2 y) { return x.x * y.x + x.y * y.y; }; // Re(x conj(y))
+  auto wd = [](C x) { return make_double2((double)x.x, (double)x.y); };
+  // one mode per lane at a time: the 10 loads and 21 FP64 sums of a mode
+  // fill the registers the transforms no longer need
+#pragma unroll 1
+  for (int it = 0; it < NK; ++it) {
+    const int k0 = t + Z::G * it, k = min(k0, nz2);
+    const double cte = k0 > nz2 ? 0.0 : (k0 == 0 ? 1.0 : 2.0), kz = a.kaz[k];
+    double2 b1 = wd(Bq[0][k]), b4 = wd(Bq[3][k]), b6 = wd(Bq[5][k]);
+    if (a.m1 && k0 == 0) {
+      b1.x = a.m1[om];
+      b4.x = a.m4[om];
+      b6.x = a.m6[om];
+    }
+    const double2 b2 = wd(Bq[1][k]), b3 = wd(Bq[2][k]), b5 = wd(Bq[4][k]), b7 = wd(Bq[6][k]), b8 = wd(Bq[7][k]);
+    const double2 v = wd(V[k]), dv = wd(DV[k]);
+    const double2 uz = make_double2(-kz * b1.y, kz * b1.x), vz = make_double2(-kz * v.y, kz * v.x);
+    const double2 wz = make_double2(-kz * b2.y, kz * b2.x);
+    const double2 ox = make_double2(b7.x - vz.x, b7.y - vz.y), oy = make_double2(uz.x - b3.x, uz.y - b3.y);
+    const double2 oz = make_double2(dv.x - b6.x, dv.y - b6.y);
+    sum[0] += cte * re(b1, b1);
+    sum[1] += cte * re(v, v);
+    sum[2] += cte * re(b2, b2);
+    sum[3] += cte * re(b1, v);
+    sum[4] += cte * re(b5, b5);
+    sum[5] += cte * re(ox, ox);
+    sum[6] += cte * re(oy, oy);
+    sum[7] += cte * re(oz, oz);
+    sum[8] += cte * re(b1, b2);
+    sum[9] += cte * re(v, b2);
+    sum[10] += cte * (re(b4, b4) + re(uz, uz) + re(b6, b6));
+    sum[11] += cte * (re(dv, dv) + re(vz, vz) + re(b8, b8));
+    sum[12] += cte * (re(wz, wz) + re(b3, b3) + re(b7, b7));
+    sum[13] += cte * (re(uz, vz) + re(b4, dv) + re(b6, b8));
+    sum[14] += cte * re(b5, b1);
+    sum[15] += cte * re(b5, v);
+    sum[16] += cte * re(b5, b4);
+    sum[17] += cte * re(b5, b8);
+    sum[18] += cte * re(b5, b6);
+    sum[19] += cte * re(b5, dv);
+    sum[20] += cte * re(b5, wz);
+  }
+  // ---- reduce across the team ----
+#pragma unroll
+  for (int q = 0; q < NSUM; ++q)
+    for (int o = 32; o > 0; o >>= 1) sum[q] += __shfl_down(sum[q], o, 64);
+  if constexpr (Z::W > 1) { // combine the team's wavefronts through its LDS buffer (8 bytes an element... as doubles)
+    double *red = reinterpret_cast<double *>(lds);
+    if ((t & 63) == 0)
+      for (int q = 0; q < NSUM; ++q) red[q * Z::W + t / 64] = sum[q];
+    zsync<Z::W>();
+    if (t == 0)
+      for (int q = 0; q < NSUM; ++q)
+        for (int w = 1; w < Z::W; ++w) sum[q] += red[q * Z::W + w];
+  }
+  if (t != 0) return;
+  const size_t NR = (size_t)(ny + 1), at0 = (size_t)(i - 1) * NR + (j - 1), S = (size_t)nx * NR;
+  double *acc = a.acc + at0;
+  constexpr int ID[NSUM] = {S_US,   S_VS,    S_WS,    S_UV,    S_PP,    S_VORTX, S_VORTY, S_VORTZ, S_UW,
+                            S_VW,   S_DISPU, S_DISPV, S_DISPW, S_DISPUV, S_PUP,  S_PVP,   S_PDUDX, S_PDVDY,
+                            S_PDUDY, S_PDVDX, S_PDWDZ, S_U3,   S_W2U,   S_V3,    S_W2V,   S_V2U,   S_U2V};
+#pragma unroll
+  for (int q = 0; q < NSUM; ++q) acc[ID[q] * S] += sum[q];
+  // k = 0 means (stotf.F90:128-144): real parts
+  const double b1 = a.m1 ? a.m1[om] : (double)Bq[0][0].x, b4 = a.m1 ? a.m4[om] : (double)Bq[3][0].x;
+  const double b6 = a.m1 ? a.m6[om] : (double)Bq[5][0].x;
+  const double b2 = Bq[1][0].x, b3 = Bq[2][0].x, b5 = Bq[4][0].x, b7 = Bq[6][0].x, b8 = Bq[7][0].x;
+  const double v0 = V[0].x, dv0 = DV[0].x;
+  acc[S_UA * S] += b1;
+  acc[S_WA * S] += b2;
+  acc[S_VA * S] += v0;
+  acc[S_PM * S] += b5;
+  acc[S_DUDX0 * S] += b4;
+  acc[S_DVDX0 * S] += dv0;
+  acc[S_DWDX0 * S] += b3;
+  acc[S_DUDY0 * S] += b6;
+  acc[S_DVDY0 * S] += b8;
+  acc[S_DWDY0 * S] += b7;
+  acc[S_VORTYA * S] -= b3;
+  acc[S_VORTXA * S] += b7;
+  acc[S_VORTZA * S] += dv0 - b6;
+}
+
 __global__ void stats_zero_kernel(double *a, size_t n) {
   for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < n; e += (size_t)gridDim.x * TPB) a[e] = 0.0;
 }
@@ -495,6 +691,26 @@ template <typename T> static void statsp_t(Box &b) {
   az.kaz = b.dc.kaz;
   az.acc = b.st;
   az.ens = b.ens;
+  if (b.zfast) {
+    const int nrows = ny * nx;
+    const typename CT<T>::c *tw = std::is_same<T, double>::value ? (const typename CT<T>::c *)b.ztw
+                                                                  : (const typename CT<T>::c *)b.ztw_f;
+#define SZR(NN)                                                                                                    \
+  case NN:                                                                                                         \
+    LAUNCH(b, "stats_z", (unsigned)((nrows + ZTeam<NN>::TEAMS - 1) / ZTeam<NN>::TEAMS), ZTeam<NN>::TPB_, 0,       \
+           (stats_zr<NN, T>), az, tw, nx, ny, nz1, nrows);                                                      \
+    break;
+    switch (b.nz) {
+      SZR(256)
+      SZR(512)
+      SZR(1024)
+      SZR(2048)
+      SZR(4096)
+#undef SZR
+    default: throw std::invalid_argument("stats: z length not on the register engine");
+    }
+    return;
+  }
   LAUNCH(b, "stats_z", dim3(ny, nx), TPB, k_stats_shm_bytes(b.nz), stats_z_kernel<T>, az, b.zplan, nx, ny, nz1,
          b.nz2);
 }

@@ -71,6 +71,14 @@ inline int zdct_length(int n) {
 }
 inline int zfft_waves(int n) { return n <= 1024 ? 1 : n / 1024; }
 
+// Teams of a launch: one row per team of W wavefronts (G lanes, E elements
+// per lane); single-wavefront teams run four to a workgroup.
+template <int N> struct ZTeam {
+  static constexpr int W = ZPlan<N>::W, G = 64 * W, E = N / G;
+  static constexpr int TEAMS = W == 1 ? 4 : 1; // teams per workgroup
+  static constexpr int TPB_ = TEAMS * G;
+};
+
 // padded LDS index of element a
 __device__ __forceinline__ int zpad(int a) { return a + (a >> 4); }
 template <int N> constexpr int zlds_elems() { return N + N / 16; }

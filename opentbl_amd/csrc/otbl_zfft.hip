@@ -20,12 +20,6 @@ __global__ void ztw_kernel(double2 *tw, int n) {
 
 void kz_twiddles(Box &b, double *tw, int n) { LAUNCH(b, "ztw", nblk(n), TPB, 0, ztw_kernel, (double2 *)tw, n); }
 
-template <int N> struct ZTeam {
-  static constexpr int W = ZPlan<N>::W, G = 64 * W, E = N / G;
-  static constexpr int TEAMS = W == 1 ? 4 : 1; // teams per workgroup
-  static constexpr int TPB_ = TEAMS * G;
-};
-
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
   return v;

@@ -6,7 +6,8 @@ pinned by (tests/test_stats_oracle.py).
   conditions) equal the oracle's closed forms (coeft.f90:56-336);
 * samples taken by otbl_statsp and by the sampling rule inside captured
   substep graphs (bl_1.F90:92) give the oracle's accumulators at 1e-10 and
-  its enstrophy field at FP32 rounding;
+  its enstrophy field at FP32 rounding, with the z sums on the LDS
+  transforms (nz 24) and on the register-resident engine (nz 256);
 * x-independent cubic profiles give the closed forms;
 * a mixed box's statistics stay within 1e-5 of the FP64 box's;
 * an ensemble whose boxes sample on the rule gives each box what the box
@@ -40,9 +41,12 @@ def stat_errors(g, o, ny):
     return out
 
 
-@pytest.fixture(scope="module")
-def scfg():
-    cfg = config.small(name="stg", nx=48, ny=40, nz=24)
+@pytest.fixture(scope="module", params=[24, 256], ids=["nz24-lds", "nz256-engine"])
+def scfg(request):
+    """nz 24 runs the z sums on the LDS transforms, nz 256 on the
+    register-resident engine."""
+    nz = request.param
+    cfg = config.small(name=f"stg{nz}", nx=48, ny=40, nz=nz)
     return cfg, ic.make_state(cfg)
 
 
@@ -97,8 +101,9 @@ def test_stats_vs_oracle(scfg, oracle_lib):
     gr.close()
 
 
-def test_stats_closed_form_gpu():
-    cfg = config.small(name="stc", nx=24, ny=40, nz=24)
+@pytest.mark.parametrize("nz", [24, 256])
+def test_stats_closed_form_gpu(nz):
+    cfg = config.small(name=f"stc{nz}", nx=24, ny=40, nz=nz)
     st, prof, dprof = closed_form_state(cfg, ic.make_state(cfg))
     gb = GpuBox(cfg, st)
     gb.stats_enable()
