@@ -42,8 +42,8 @@ template <> struct ZPlan<2048> {
 };
 // 3 x 1024: the Bluestein length of the x transforms for N up to 1536
 template <> struct ZPlan<3072> {
-  static constexpr int W = 4, NP = 5;
-  static constexpr int rad(int p) { return p == 0 ? 12 : 4; }
+  static constexpr int W = 2, NP = 4;
+  static constexpr int rad(int p) { return p == 0 ? 12 : (p < 3 ? 8 : 4); }
 };
 template <> struct ZPlan<4096> {
   static constexpr int W = 4, NP = 3;

@@ -353,6 +353,12 @@ fourxz_z(double *fou, double *phys, int iopt, const double2 *__restrict__ tw, co
   }
 }
 
+// base[idx] through a 32-bit byte offset (idx sizeof(T) < 2^32): the load
+// takes a scalar base and one offset register instead of a 64-bit address
+template <typename T> __device__ __forceinline__ T ld32(const T *base, unsigned idx) {
+  return *(const T *)((const char *)base + idx * (unsigned)sizeof(T));
+}
+
 // Poisson x transforms (ffts.f90:124-181: FFTW REDFT10 / REDFT01 of length
 // N = nx-1) on x-contiguous pencils, two real lines per complex transform:
 // Makhoul's reordering turns each DCT into a complex DFT of length N, and
@@ -368,12 +374,6 @@ fourxz_z(double *fou, double *phys, int iopt, const double2 *__restrict__ tw, co
 // them) and unused on output (the inverse's last pass never forms them).
 // Every global load of a phase is issued before its arithmetic, with
 // clamped indices and selects in place of branches.
-// base[idx] through a 32-bit byte offset (idx sizeof(T) < 2^32): the load
-// takes a scalar base and one offset register instead of a 64-bit address
-template <typename T> __device__ __forceinline__ T ld32(const T *base, unsigned idx) {
-  return *(const T *)((const char *)base + idx * (unsigned)sizeof(T));
-}
-
 template <int M, typename R, int IOPT>
 __global__ void __launch_bounds__(ZTeam<M>::TPB_, sizeof(R) == 4 ? 6 : 1)
 dct_rows_z(const R *src, R *dst, int Lp, int N, const typename CT<R>::c *__restrict__ chirp,
@@ -382,7 +382,6 @@ dct_rows_z(const R *src, R *dst, int Lp, int N, const typename CT<R>::c *__restr
   using C = typename CT<R>::c;
   using Z = ZTeam<M>;
   constexpr int R0 = ZPlan<M>::rad(0), NB0 = M / (R0 * Z::G), NO = (M / 2 + Z::G - 1) / Z::G;
-  static_assert(NB0 == 1, "the half-input first pass assumes one butterfly per lane");
   __shared__ C lds_all[Z::TEAMS * zlds_elems<M>()];
   const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::G);
   int t = threadIdx.x % Z::G;
@@ -393,43 +392,46 @@ dct_rows_z(const R *src, R *dst, int Lp, int N, const typename CT<R>::c *__restr
   const bool has1 = l1 < Lp;
   const R *x0 = src + (size_t)l0 * N, *x1 = src + (size_t)(has1 ? l1 : l0) * N;
   const int half = (N + 1) / 2;
-  constexpr int RH = R0 / 2; // live input slots r < RH: index t + r M/R0 < M/2
+  // live slots of butterfly m: r < RH (G m + r M/R0 < M/2 exactly then,
+  // as G NB0 = M / R0)
+  constexpr int RH = R0 / 2, NL = NB0 * RH;
+  auto kof = [&](int s) { return t + Z::G * (s / RH) + (s % RH) * (M / R0); }; // live slot s -> index
   C v[Z::E];
   {
-    R a0[RH], a1[RH], b0[RH], b1[RH];
-    C ch[RH], tt[RH];
+    R a0[NL], a1[NL], b0[NL], b1[NL];
+    C ch[NL], tt[NL];
 #pragma unroll
-    for (int r = 0; r < RH; ++r) {
-      const unsigned k = min(t + r * (M / R0), N - 1);
+    for (int s = 0; s < NL; ++s) {
+      const unsigned k = min(kof(s), N - 1);
       if constexpr (IOPT < 0) { // Makhoul reordering of the two real lines
         const unsigned sidx = (int)k < half ? 2 * k : 2 * (N - 1 - k) + 1;
-        a0[r] = ld32(x0, sidx);
-        a1[r] = ld32(x1, sidx);
+        a0[s] = ld32(x0, sidx);
+        a1[s] = ld32(x1, sidx);
       } else {
         const unsigned nk = k ? N - k : 0;
-        a0[r] = ld32(x0, k);
-        a1[r] = ld32(x1, k);
-        b0[r] = ld32(x0, nk);
-        b1[r] = ld32(x1, nk);
-        tt[r] = ld32(tw4, k);
+        a0[s] = ld32(x0, k);
+        a1[s] = ld32(x1, k);
+        b0[s] = ld32(x0, nk);
+        b1[s] = ld32(x1, nk);
+        tt[s] = ld32(tw4, k);
       }
-      ch[r] = ld32(chirp, k);
+      ch[s] = ld32(chirp, k);
     }
 #pragma unroll
-    for (int r = 0; r < RH; ++r) {
-      const int k = t + r * (M / R0);
+    for (int s = 0; s < NL; ++s) {
+      const int k = kof(s);
       C z;
       if constexpr (IOPT < 0) {
-        z = mkc<C>(a0[r], has1 ? a1[r] : R(0));
+        z = mkc<C>(a0[s], has1 ? a1[s] : R(0));
       } else { // V = (X_k - i X_{N-k}) exp(+i pi k / 2N), Z = V0 + i V1, conj(Z)
-        C w = tt[r];
+        C w = tt[s];
         w.y = -w.y;
-        const R c0 = k ? b0[r] : R(0), c1 = (has1 && k) ? b1[r] : R(0), d1 = has1 ? a1[r] : R(0);
-        const C v0 = cmulc(mkc<C>(a0[r], -c0), w), v1 = cmulc(mkc<C>(d1, -c1), w);
+        const R c0 = k ? b0[s] : R(0), c1 = (has1 && k) ? b1[s] : R(0), d1 = has1 ? a1[s] : R(0);
+        const C v0 = cmulc(mkc<C>(a0[s], -c0), w), v1 = cmulc(mkc<C>(d1, -c1), w);
         z = mkc<C>(v0.x - v1.y, -(v0.y + v1.x));
       }
-      const C a = cmulc(z, ch[r]);
-      v[r] = k < N ? a : mkc<C>(R(0), R(0));
+      const C a = cmulc(z, ch[s]);
+      v[(s / RH) * R0 + s % RH] = k < N ? a : mkc<C>(R(0), R(0));
     }
   }
   zrun<M, -1, false, true>(v, t, tw, lds);
@@ -441,18 +443,18 @@ dct_rows_z(const R *src, R *dst, int Lp, int N, const typename CT<R>::c *__restr
     for (int e = 0; e < Z::E; ++e) v[e] = cmulc(v[e], w[e]);
   }
   zrun<M, +1, true>(v, t, tw, lds);
-  // natural-order outputs t + q M / R0 (q < R0 / 2 hold every k < N); keep
-  // k < N times the chirp.  The load offsets are formed again here rather
-  // than kept from the input phase (kept, they spill).
+  // natural-order outputs t + G m + q M / R0 (q < R0 / 2 hold every k < N);
+  // keep k < N times the chirp.  The load offsets are formed again here
+  // rather than kept from the input phase (kept, they spill).
   asm volatile("" : "+v"(t));
   {
-    C ch[RH];
+    C ch[NL];
 #pragma unroll
-    for (int q = 0; q < RH; ++q) ch[q] = ld32(chirp, (unsigned)min(t + q * (M / R0), N - 1));
+    for (int s = 0; s < NL; ++s) ch[s] = ld32(chirp, (unsigned)min(kof(s), N - 1));
 #pragma unroll
-    for (int q = 0; q < RH; ++q) {
-      const int k = t + q * (M / R0);
-      if (k < N) lds[zpad(k)] = cmulc(ch[q], v[q]);
+    for (int s = 0; s < NL; ++s) {
+      const int k = kof(s);
+      if (k < N) lds[zpad(k)] = cmulc(ch[s], v[(s / RH) * R0 + s % RH]);
     }
   }
   zsync<Z::W>();
