@@ -177,6 +177,21 @@ int otbl_soltriy_noalloc(otbl_ctx *ctx, const double *u, double *us, int ii, dou
 /* coefinv (coeft.f90:1338-1353): in-place LU of the caller's host array
  * a(3, npter1:neq), computed on the GPU. */
 int otbl_coefinv(otbl_ctx *ctx, double *a, int npter1, int neq);
+/* soltrix (coeft.f90:1926-1967): the tridiagonal a(3, npter1:nfin),
+ * nfin = npter1 + neq - 1, of the caller's host array a(3, *) LU-factored
+ * in place when *iflag is 0 (then *iflag = 1, as coefinv factors), and the
+ * columns npter2 .. npter2 + nlen - 1 of the host array u(nstr, *), rows
+ * npter1 .. nfin, solved into the same rows of us (us may be u; its other
+ * rows are untouched); on the GPU. */
+int otbl_soltrix(otbl_ctx *ctx, const double *u, int nstr, int nlen, int neq, double *us, int npter1, int npter2,
+                 double *a, int *iflag);
+/* coefb (coeft.f90:1360-1789): the n weights bcc(1:n) of a boundary
+ * closure from the spacings a, b, c, d by its Taylor-moment system, for the
+ * flags the reference calls it with (flagv 0: n = 3; 1: n = 5; 3 to 7:
+ * n = 4; coeft.f90:589-1023); other flags, or another n, return
+ * OTBL_EINVAL.  Host arithmetic, as the box's own coefficient setup; no
+ * context needed. */
+int otbl_coefb(int n, int flagv, double a, double b, double c, double d, double *bcc);
 
 /* ---- statistics (stotf.F90:51-291) ----
  * The reference's statsp as its production build compiles it

@@ -1112,6 +1112,54 @@ int otbl_coefinv(otbl_ctx *ctx, double *a, int npter1, int neq) {
   API_END
 }
 
+int otbl_soltrix(otbl_ctx *ctx, const double *u, int nstr, int nlen, int neq, double *us, int npter1, int npter2,
+                 double *a, int *iflag) {
+  API_BEGIN
+  Box &b = box_of(ctx);
+  need(u && us && a && iflag, "otbl_soltrix: null argument");
+  need(neq >= 1 && nlen >= 0 && npter1 >= 1 && npter2 >= 1 && npter1 + neq - 1 <= nstr,
+       "otbl_soltrix: rows npter1 .. npter1 + neq - 1 must lie in 1 .. nstr, neq >= 1, nlen >= 0");
+  // a(1:3, npter1:nfin) at a + 3 (npter1 - 1); columns npter2 .. of u, us
+  double *ap = a + 3 * (size_t)(npter1 - 1);
+  const size_t c0 = (size_t)(npter2 - 1) * nstr + (size_t)(npter1 - 1), ncol = (size_t)nlen * nstr;
+  double *da = nullptr, *du = nullptr;
+  OTBL_HIP(hipStreamSynchronize(b.s));
+  OTBL_HIP(hipMalloc((void **)&da, 3 * (size_t)(neq + 1) * sizeof(double)));
+  try {
+    if (nlen > 0) OTBL_HIP(hipMalloc((void **)&du, 2 * ncol * sizeof(double)));
+    h2d(b, da + 3, ap, 3 * (size_t)neq * sizeof(double));
+    if (*iflag == 0) k_coefinv(b, da, neq);
+    if (nlen > 0) {
+      // rows npter1 .. nfin of each column: u's into du, us's into du + ncol
+      // (us keeps its other rows)
+      const size_t pitch = (size_t)nstr * sizeof(double), w = (size_t)neq * sizeof(double);
+      OTBL_HIP(hipMemcpy2DAsync(du, pitch, u + c0, pitch, w, nlen, hipMemcpyHostToDevice, b.s));
+      k_soltrix(b, du, du + ncol, da + 3, nstr, nlen, neq);
+      OTBL_HIP(hipMemcpy2DAsync(us + c0, pitch, du + ncol, pitch, w, nlen, hipMemcpyDeviceToHost, b.s));
+    }
+    if (*iflag == 0) d2h(b, ap, da + 3, 3 * (size_t)neq * sizeof(double));
+    OTBL_HIP(hipStreamSynchronize(b.s));
+  } catch (...) {
+    (void)hipStreamSynchronize(b.s);
+    (void)hipFree(da);
+    if (du) (void)hipFree(du);
+    throw;
+  }
+  OTBL_HIP(hipFree(da));
+  if (du) OTBL_HIP(hipFree(du));
+  if (*iflag == 0) *iflag = 1;
+  API_END
+}
+
+int otbl_coefb(int n, int flagv, double a, double b, double c, double d, double *bcc) {
+  API_BEGIN
+  need(bcc != nullptr, "otbl_coefb: null bcc");
+  double w[5];
+  coefb_weights(n, flagv, a, b, c, d, w);
+  for (int q = 0; q < n; ++q) bcc[q] = w[q];
+  API_END
+}
+
 int otbl_dev_alloc(size_t bytes, void **ptr) {
   API_BEGIN
   need(ptr != nullptr, "null pointer");

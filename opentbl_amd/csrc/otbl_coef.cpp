@@ -197,6 +197,31 @@ void set_identity_ends(std::vector<double> &t, int n) {
 
 } // namespace
 
+// coefb (coeft.f90:1360-1789) for the flags the reference calls it with
+// (coeft.f90:589-1023): the n closure weights; another flag or another n
+// than the flag's is rejected.
+void coefb_weights(int n, int flagv, double a, double b, double c, double d, double bcc[5]) {
+  ClosureKind k;
+  int want;
+  switch (flagv) {
+  case 0: k = MIDPOINT3; want = 3; break;
+  case 1: k = SECOND5; want = 5; break;
+  case 3: k = FIRST_OFF; want = 4; break;
+  case 4: k = MIDPOINT4; want = 4; break;
+  case 5: k = FIRST_MIX; want = 4; break;
+  case 6: k = ZEROVORT6; want = 4; break;
+  case 7: k = ZEROVORT7; want = 4; break;
+  default: throw std::invalid_argument("coefb: flagv not one the reference calls (0, 1, 3..7)");
+  }
+  if (n != want) throw std::invalid_argument("coefb: n does not match the flag's closure (0: 3, 1: 5, 3..7: 4)");
+  try {
+    closure(k, a, b, c, d, bcc);
+  } catch (const std::runtime_error &e) {
+    throw std::invalid_argument(e.what()); // coincident points
+  }
+}
+
+
 void build_coef(HostCoef &hc, int nx, int ny, int nz, double re, double ax, double az,
                 double cfl, const double *yin) {
   hc.nx = nx; hc.ny = ny; hc.nz = nz;

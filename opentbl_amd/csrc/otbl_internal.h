@@ -41,6 +41,9 @@ void build_coef(HostCoef &hc, int nx, int ny, int nz, double re, double ax, doub
                 double cfl, const double *y);
 // the stats-only schemes (coeft.f90:56-285); called by build_coef
 void build_stats_coef(HostCoef &hc);
+// coefb (coeft.f90:1360-1789): boundary-closure weights for flag flagv
+// (0, 1, 3..7), n of them; throws std::invalid_argument otherwise
+void coefb_weights(int n, int flagv, double a, double b, double c, double d, double bcc[5]);
 // inlet_retheta (bl_1.F90:372-396) and magicnumber (boun.f90:102-143)
 void inlet_setup(HostCoef &hc, const double *u0, double *rthin, double *din);
 
@@ -224,6 +227,7 @@ void k_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows);
 void k_cosftx(Box &b, const double *in, double *out, int iopt);
 void k_cosft_lines(Box &b, double *P, int nlines, int iopt);
 void k_coefinv(Box &b, double *a, int nrow);
+void k_soltrix(Box &b, const double *u, double *us, const double *a, int nstr, int nlen, int nrow);
 void k_soltriy(Box &b, const double *u, double *us, int ii, double *a, int ny1, int nmode);
 void k_substep_rhsp(Box &b, int m);
 void k_outflow(Box &b);

@@ -76,6 +76,34 @@ __global__ void coefinv_kernel(double *a, int nrow) {
 
 void k_coefinv(Box &b, double *a, int nrow) { LAUNCH(b, "coefinv", 1, 64, 0, coefinv_kernel, a, nrow); }
 
+// soltrix's sweeps (coeft.f90:1952-1964) over nlen columns of nstr doubles,
+// one thread per column: rows 0..nrow-1 of each column of u solved into the
+// same rows of us with the factored a (row i at a[3 i]); in place allowed
+// (each thread reads and writes only its own column)
+__global__ void soltrix_kernel(const double *u, double *us, const double *__restrict__ a, int nstr, int nlen,
+                               int nrow) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlen) return;
+  const double *uc = u + (size_t)j * nstr;
+  double *sc = us + (size_t)j * nstr;
+  double y = uc[0];
+  sc[0] = y;
+  for (int i = 1; i < nrow; ++i) {
+    y = uc[i] + a[3 * i] * y;
+    sc[i] = y;
+  }
+  double x = y * a[3 * (nrow - 1) + 1];
+  sc[nrow - 1] = x;
+  for (int i = nrow - 2; i >= 0; --i) {
+    x = (sc[i] - a[3 * i + 2] * x) * a[3 * i + 1];
+    sc[i] = x;
+  }
+}
+
+void k_soltrix(Box &b, const double *u, double *us, const double *a, int nstr, int nlen, int nrow) {
+  LAUNCH(b, "soltrix", nblk(nlen, 64), 64, 0, soltrix_kernel, u, us, a, nstr, nlen, nrow);
+}
+
 // soltriy (pois.f90:211-239): LU of the tridiagonal a(3, ny1) in place, then
 // the Re and Im parts of complex column ii of u(0:nmode-1, ny1) solved into
 // us (us may be u).  Lane 0 factors, lanes 0 and 1 sweep one part each.

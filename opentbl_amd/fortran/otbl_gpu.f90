@@ -169,6 +169,20 @@ module otbl_gpu
        real(c_double) :: a(*)
        integer(c_int), value :: npter1, neq
      end function otbl_coefinv
+     integer(c_int) function otbl_soltrix(ctx, u, nstr, nlen, neq, us, npter1, npter2, a, iflag) &
+          bind(C, name="otbl_soltrix")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: u(*), us(*), a(*)
+       integer(c_int), value :: nstr, nlen, neq, npter1, npter2
+       integer(c_int) :: iflag
+     end function otbl_soltrix
+     integer(c_int) function otbl_coefb(n, flagv, a, b, c, d, bcc) bind(C, name="otbl_coefb")
+       import :: c_int, c_double
+       integer(c_int), value :: n, flagv
+       real(c_double), value :: a, b, c, d
+       real(c_double) :: bcc(*)
+     end function otbl_coefb
      integer(c_int) function otbl_get_coef(ctx, name, out, rows, cols) bind(C, name="otbl_get_coef")
        import :: c_int, c_ptr, c_double, c_char
        type(c_ptr), value :: ctx
@@ -812,3 +826,58 @@ subroutine mtr(ayp,cyclx,cycly,yy)
   if (any(yy /= otbl_y)) call otbl_fail('mtr: yy is not the grid of the box')
   call otbl_coef('ayp', ayp)
 end subroutine mtr
+
+! coeft.f90:1926-1967: columns npter2 .. npter2+nlen-1 of u(nstr, *), rows
+! npter1 .. npter1+neq-1, solved with the tridiagonal a(3, *) into us (us may
+! be u); a is LU-factored in place first when iflag is 0, which then
+! becomes 1 (on the GPU)
+subroutine soltrix(u,nstr,nlen,neq,us,npter1,npter2,a,iflag)
+  use otbl_gpu
+  implicit none
+  integer :: nstr, nlen, neq, npter1, npter2, iflag
+  real(8) :: u(nstr, *), us(nstr, *), a(3, *)
+  integer(c_int) :: fl
+  call otbl_need_box('soltrix')
+  fl = int(iflag, c_int)
+  call otbl_check(otbl_soltrix(otbl_box, u, int(nstr, c_int), int(nlen, c_int), int(neq, c_int), us, &
+       int(npter1, c_int), int(npter2, c_int), a, fl), 'soltrix')
+  iflag = fl
+end subroutine soltrix
+
+! coeft.f90:1360-1789: the closure weights bcc(1:n) for the spacings a, b,
+! c, d, for the flags the reference calls coefb with (0, 1, 3 to 7); np is
+! the reference's declared size of its work matrix and is not needed
+subroutine coefb(n,np,a,b,c,d,bcc,flagv)
+  use otbl_gpu
+  implicit none
+  integer :: n, np, flagv
+  real(8) :: a, b, c, d, bcc(5)
+  call otbl_check(otbl_coefb(int(n, c_int), int(flagv, c_int), a, b, c, d, bcc), 'coefb')
+end subroutine coefb
+
+! ffts.f90:188-213: each OpenMP thread's range of y rows and pencils for the
+! FFT loops.  The GPU grids cover every row and line and the host side is
+! one thread, so there is nothing to set.
+subroutine pointersfft(mpiid)
+  implicit none
+  integer :: mpiid
+end subroutine pointersfft
+
+! change.f90:150-222: the transpose bookkeeping of the x-slab decomposition
+! (comm_setup for the u and v line sets).  One process holds the whole box
+! and the kernels read the planes layout directly (chp2x / chx2p above), so
+! there is nothing to set.
+subroutine pointers_p2p(rank)
+  implicit none
+  integer :: rank
+end subroutine pointers_p2p
+
+! change.f90:29-148: the per-rank block sizes of the MPI transposes; node
+! and domain are the reference's shared_mem derived types, received here as
+! untyped storage and not touched (one process, no transposes between
+! ranks).
+subroutine comm_setup(nx,ny,nz,rank,size,node,domain,pblock)
+  implicit none
+  integer :: nx, ny, nz, rank, size, pblock
+  integer :: node(*), domain(*)
+end subroutine comm_setup

@@ -69,6 +69,8 @@ SIGNATURES = {
     "otbl_soltriy": (I, [VP, VP, VP, I, PD, I]),
     "otbl_soltriy_noalloc": (I, [VP, VP, VP, I, PD, I]),
     "otbl_coefinv": (I, [VP, PD, I, I]),
+    "otbl_soltrix": (I, [VP, PD, I, I, I, PD, I, I, PD, C.POINTER(I)]),
+    "otbl_coefb": (I, [I, I, C.c_double, C.c_double, C.c_double, C.c_double, PD]),
     "otbl_ensemble_create": (I, [C.POINTER(VP), I, C.POINTER(VP)]),
     "otbl_ensemble_step": (I, [VP, I]),
     "otbl_ensemble_sync": (I, [VP]),

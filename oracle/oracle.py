@@ -61,6 +61,7 @@ def lib():
         L.oc_retheta_cf.argtypes = [C.c_void_p, PD, PD]
         L.oc_max_threads.restype = I
         L.oc_max_threads.argtypes = []
+        L.oc_coefb.argtypes = [I, I, D, D, D, D, PD]
         L.oc_statsp.argtypes = [C.c_void_p]
         L.oc_stat_name.restype = C.c_char_p
         L.oc_stat_name.argtypes = [I]

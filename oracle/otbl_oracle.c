@@ -228,6 +228,11 @@ static void coefb(int n, double a, double b, double c, double d, double bcc[5],
   dense_solve(n, m, bcc);
 }
 
+/* coefb for the tests of the library's closure weights */
+void oc_coefb(int n, int flagv, double a, double b, double c, double d, double *bcc) {
+  coefb(n, a, b, c, d, bcc, flagv);
+}
+
 /* coefinv (coeft.f90:1338-1353) with npter1 = 1: in-place LU of a
  * tridiagonal stored as (3, 1:neq). */
 static void coefinv(double *a, int neq) {

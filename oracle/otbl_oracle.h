@@ -112,6 +112,8 @@ void oc_stats_op(const oc_box *b, int op, const double *in, double *out, int n);
 
 /* OpenMP threads the oracle's parallel loops use. */
 int oc_max_threads(void);
+/* coefb (coeft.f90:1360-1789), flags 0, 1, 3..7: n closure weights */
+void oc_coefb(int n, int flagv, double a, double b, double c, double d, double *bcc);
 
 #ifdef __cplusplus
 }

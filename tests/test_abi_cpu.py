@@ -62,3 +62,38 @@ def test_create_rejects_bad_sizes_without_gpu(libpath):
     prm = lib.Params(64, 40, 24, 64, 1000.0, 1.0, 1.0, 0.5, 0, 0)   # xout = nx
     rc = L.otbl_create(C.byref(prm), y.ctypes.data_as(lib.PD), C.byref(h))
     assert rc == 1 and b"xout" in L.otbl_last_error()
+
+
+# the flags the reference calls coefb with and their weight counts
+# (coeft.f90:589-1023)
+COEFB_FLAGS = {0: 3, 1: 5, 3: 4, 4: 4, 5: 4, 6: 4, 7: 4}
+
+
+def test_coefb_entry_matches_oracle(libpath, oracle_lib):
+    """otbl_coefb (host setup arithmetic, no device) against the oracle's
+    coefb, which the schemes' polynomial-exactness pins cover, on random
+    spacings of both signs (the reference passes negative spacings for the
+    closures at the far end of a line)."""
+    import ctypes as C
+    import numpy as np
+    from opentbl_amd import lib
+    L, O = lib.lib(), oracle_lib.lib()
+    rng = np.random.default_rng(4)
+    for flag, n in COEFB_FLAGS.items():
+        for _ in range(20):
+            sp = rng.uniform(0.2, 2.0, 4) * rng.choice([-1.0, 1.0])
+            got, want = np.zeros(5), np.zeros(5)
+            assert L.otbl_coefb(n, flag, *sp, got.ctypes.data_as(lib.PD)) == 0, L.otbl_last_error()
+            O.oc_coefb(n, flag, *sp, want.ctypes.data_as(C.POINTER(C.c_double)))
+            assert np.abs(got[:n] - want[:n]).max() <= 1e-12 * np.abs(want[:n]).max(), (flag, sp)
+
+
+def test_coefb_entry_rejects_flags_off_the_path(libpath):
+    from opentbl_amd import lib
+    import numpy as np
+    L = lib.lib()
+    w = np.zeros(5)
+    assert L.otbl_coefb(4, 2, 1.0, 1.0, 1.0, 1.0, w.ctypes.data_as(lib.PD)) == 1
+    assert b"flagv" in L.otbl_last_error()
+    assert L.otbl_coefb(4, 0, 1.0, 1.0, 1.0, 1.0, w.ctypes.data_as(lib.PD)) == 1   # flag 0 has 3 weights
+    assert b"n does not match" in L.otbl_last_error()

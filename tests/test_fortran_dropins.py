@@ -1,6 +1,7 @@
 """The Fortran drop-ins of opentbl_amd/fortran/otbl_gpu.f90, called with the
 reference's argument lists and array layouts (lip.f90, ffts.f90, change.f90,
-pois.f90:211, coeft.f90:1244,1338), against the CPU FP64 oracle.
+pois.f90:211, coeft.f90:1244,1338,1360,1926), against the CPU FP64 oracle
+or dense solves.
 
 The binding is loaded as libotbl_fortran.so: the box is opened through its
 C-callable otbl_f_open, the drop-ins are called by their Fortran names
@@ -18,7 +19,8 @@ from opentbl_amd import build, config, ic
 
 DROPINS = ["rhsp_", "outflow_correction_", "pois_", "chp2x_", "chx2p_", "interpxx_", "difvisxx_", "interpyy_",
            "differzy_", "vistyy_", "implzy_", "fourxz_", "rfti_", "cosfti_", "cosftx_", "soltriy_", "coefinv_",
-           "coef_", "mtr_", "statsp_", "otbl_f_open", "otbl_f_close", "otbl_f_mirror", "otbl_f_box"]
+           "coef_", "mtr_", "statsp_", "soltrix_", "coefb_", "pointersfft_", "pointers_p2p_", "comm_setup_",
+           "otbl_f_open", "otbl_f_close", "otbl_f_mirror", "otbl_f_box"]
 
 OP_TOL = 1e-12
 
@@ -304,3 +306,69 @@ def test_statsp_dropin(fbox, oracle_lib):
         w = want[name]
         assert np.abs(g.reshape(w.shape) - w).max() <= 1e-10 * max(np.abs(w).max(), 1e-300), name
     assert lib().otbl_stats_enable(box, 0) == 0
+
+
+@pytest.mark.parametrize("inplace", [True, False])
+def test_soltrix_dropin(fbox, inplace):
+    """soltrix(u, nstr, nlen, neq, us, npter1, npter2, a, iflag)
+    (coeft.f90:1926-1967): columns npter2.. of u(nstr, *), rows npter1.., a
+    dense solve each; a comes back as coefinv factors it and iflag = 1; the
+    rows of us outside the system are untouched.  Then a second call with
+    iflag = 1 reuses the factors.  (The reference calls it in place with
+    one column, coeft.f90:1208,1225.)"""
+    L, cfg, st, ob = fbox
+    rng = np.random.default_rng(17)
+    nstr, ncols, neq, npter1, npter2, nlen = 30, 6, 24, 4, 2, 4
+    a = np.zeros((nstr, 3))                       # Fortran a(3, nstr) as C (nstr, 3)
+    a[:, 0] = rng.uniform(0.3, 1.0, nstr)
+    a[:, 2] = rng.uniform(0.3, 1.0, nstr)
+    a[:, 1] = -(a[:, 0] + a[:, 2] + 0.5)
+    r = slice(npter1 - 1, npter1 - 1 + neq)
+    A = np.diag(a[r, 1]) + np.diag(a[r, 0][1:], -1) + np.diag(a[r, 2][:-1], 1)
+    u = np.ascontiguousarray(rng.standard_normal((ncols, nstr)))   # Fortran u(nstr, ncols)
+    us = u if inplace else np.ascontiguousarray(rng.standard_normal((ncols, nstr)))
+    u0, us0 = u.copy(), us.copy()
+    af = a.copy()
+    fl = C.c_int(0)
+    L.soltrix_(p(u), iref(nstr), iref(nlen), iref(neq), p(us), iref(npter1), iref(npter2), p(af), C.byref(fl))
+    assert fl.value == 1
+    for j in range(ncols):
+        if npter2 - 1 <= j < npter2 - 1 + nlen:
+            x = np.linalg.solve(A, u0[j, r])
+            assert np.abs(us[j, r] - x).max() <= 1e-12 * np.abs(x).max(), j
+            outside = np.r_[0:npter1 - 1, npter1 - 1 + neq:nstr]
+            assert np.array_equal(us[j, outside], us0[j, outside]), j
+        else:
+            assert np.array_equal(us[j], us0[j]), j
+    # the factors are coefinv's (the same loop, coeft.f90:1340-1352)
+    ref = np.ascontiguousarray(a[r].copy())
+    L.coefinv_(p(ref), iref(neq), iref(1))
+    assert np.array_equal(af[r], ref)
+    # iflag = 1: the factored a is used as it is
+    v = np.ascontiguousarray(rng.standard_normal((ncols, nstr)))
+    v0 = v.copy()
+    L.soltrix_(p(v), iref(nstr), iref(1), iref(neq), p(v), iref(npter1), iref(1), p(af), C.byref(fl))
+    x = np.linalg.solve(A, v0[0, r])
+    assert np.abs(v[0, r] - x).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_coefb_and_bookkeeping_dropins(fbox, oracle_lib):
+    """coefb (coeft.f90:1360-1789) gives the oracle's closure weights for
+    every flag the reference uses; pointersfft, pointers_p2p and comm_setup
+    (OpenMP / MPI bookkeeping, ffts.f90:188, change.f90:29,150) are
+    accepted and leave their arguments alone."""
+    L, cfg, st, ob = fbox
+    O = oracle_lib.lib()
+    for flag, n in {0: 3, 1: 5, 3: 4, 4: 4, 5: 4, 6: 4, 7: 4}.items():
+        sp = [0.7, 0.9, 1.1, 1.3]
+        got, want = np.zeros(5), np.zeros(5)
+        L.coefb_(iref(n), iref(5), dref(sp[0]), dref(sp[1]), dref(sp[2]), dref(sp[3]), p(got), iref(flag))
+        O.oc_coefb(n, flag, *sp, p(want))
+        assert np.abs(got[:n] - want[:n]).max() <= 1e-12 * np.abs(want[:n]).max(), flag
+    rank = C.c_int(0)
+    L.pointersfft_(C.byref(rank))
+    L.pointers_p2p_(C.byref(rank))
+    node, domain = np.zeros(8, np.int32), np.zeros(8, np.int32)
+    L.comm_setup_(iref(cfg.nx), iref(cfg.ny), iref(cfg.nz1), C.byref(rank), iref(1),
+                  node.ctypes.data_as(C.POINTER(C.c_int)), domain.ctypes.data_as(C.POINTER(C.c_int)), iref(1))
+    assert rank.value == 0 and not node.any() and not domain.any()
