@@ -265,7 +265,8 @@ int otbl_d2d(void *dst, const void *src, size_t bytes);
 /* Kernel paths the context chose at setup, as text: "y:chunk<L>|march0
  * z:engine<nz>|stockham<nz> dct:engine<M>|stockham<M>|gemm<M> x:window|march
  * xseg:<threads per x line> pois:chunk<L>|march0 prun:<planes per thread>
- * tuned:0|1"
+ * tuned:0|1 streams:1|2" (streams:2: rhsp's plane loop 2 on a second stream
+ * beside plane loop 1)
  * (no counterpart in the reference; for tests and benchmark records). */
 int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len);
 

@@ -359,6 +359,9 @@ static void free_box(Box *b) {
     (void)hipEventDestroy(p.second.first);
     (void)hipEventDestroy(p.second.second);
   }
+  if (b->s2) (void)hipStreamDestroy(b->s2);
+  for (hipEvent_t e : {b->ev_fork, b->ev_p1, b->ev_join})
+    if (e) (void)hipEventDestroy(e);
   if (b->s) (void)hipStreamDestroy(b->s);
   delete b;
 }
@@ -640,6 +643,15 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     need((size_t)(2 * b->pchunk + 1) * b->nz1 * (b->nx - 1) * b->esz < (size_t(1) << 32),
          "Poisson pencil rows too long for the partitioned y solve's 32-bit offsets");
   if (b->precision == OTBL_MIXED) setup_mixed(*b);
+  // the second stream of rhsp's plane loop 2, on the paths that keep ya and
+  // yb free of scratch use (windowed x sweep, chunked y solver, register z
+  // engine)
+  if (b->xwin && b->yL && b->zfast && b->nx > 2 && std::getenv("OTBL_SERIAL_LOOPS") == nullptr) {
+    OTBL_HIP(hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking));
+    OTBL_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    OTBL_HIP(hipEventCreateWithFlags(&b->ev_p1, hipEventDisableTiming));
+    OTBL_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+  }
   OTBL_HIP(hipStreamSynchronize(b->s));
   *out = new otbl_ctx{b};
   b = nullptr;
@@ -1199,10 +1211,11 @@ int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len) {
   Box &b = box_of(ctx);
   need(buf && len > 0, "otbl_get_paths: null buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof tmp, "y:%s%d%s z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d prun:%d tuned:%d",
+  std::snprintf(tmp, sizeof tmp, "y:%s%d%s z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d prun:%d tuned:%d streams:%d",
                 b.yL ? "chunk" : "march", b.yL, (b.ytabh_f && b.yhalf) ? "h" : "", b.zfast ? "engine" : "stockham", b.nz,
                 b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"), b.dct_M, b.xwin ? "window" : "march", b.xseg,
-                b.pchunk ? "chunk" : "march", b.pchunk, b.prun > 0 ? b.prun : (b.esz == 4 ? 4 : 2), b.tuned ? 1 : 0);
+                b.pchunk ? "chunk" : "march", b.pchunk, b.prun > 0 ? b.prun : (b.esz == 4 ? 4 : 2), b.tuned ? 1 : 0,
+                b.s2 ? 2 : 1);
   std::snprintf(buf, len, "%s", tmp);
   API_END
 }

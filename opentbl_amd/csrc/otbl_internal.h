@@ -84,6 +84,13 @@ struct Box {
   size_t nu, nv;     // elements per u-type / v-type field
   size_t pu, pv;     // elements per u-type / v-type plane
   hipStream_t s = nullptr;
+  // plane loop 2 of rhsp on a stream of its own beside plane loop 1 (its y
+  // interpolations need only the substep's initial v, w; its products wait
+  // for plane loop 1's products, whose input yc they overwrite); joined
+  // back into s before rhs_loop2.  s2 null (OTBL_SERIAL_LOOPS, or paths
+  // that use ya / yb as scratch): everything on s.
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_p1 = nullptr, ev_join = nullptr;
   HostCoef hc;
   DevCoef dc{};
   double *coefbuf = nullptr;

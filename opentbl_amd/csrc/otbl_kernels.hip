@@ -2174,6 +2174,35 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   const size_t shm = zfft_shm(b, 4);
 
   LAUNCH(b, "reset_max", 1, TPB, 0, reset_max_kernel, b.ds, b.vmbits, ny);
+  // plane loop 2 beside plane loop 1 (per-kernel timing runs them in order):
+  // its y interpolations (rhsp.F90:367,393) read the substep's initial v, w
+  // on planes 2..nx-1, which nothing before the loop changes (the predictor
+  // writes the other buffer of each pair, genflu plane 1 only); ya, yb are
+  // untouched until then
+  const bool side = b.s2 && !b.kprof && nx > 2;
+  auto on_side = [&](auto &&f) {
+    std::swap(b.s, b.s2);
+    try {
+      f();
+    } catch (...) {
+      std::swap(b.s, b.s2);
+      throw;
+    }
+    std::swap(b.s, b.s2);
+  };
+  if (side) {
+    OTBL_HIP(hipEventRecord(b.ev_fork, b.s));
+    OTBL_HIP(hipStreamWaitEvent(b.s2, b.ev_fork, 0));
+    on_side([&] {
+      if constexpr (F64) {
+        k_interpyy(b, w, ya, 0, 2, nx - 2, b.pu, b.pu);
+        k_interpyy(b, v, yb, 1, 2, nx - 2, b.pv, b.pu);
+      } else {
+        ky_interpyy<T>(b, w, ya, 0, 2, nx - 2, b.pu, b.pu);
+        ky_interpyy<T>(b, v, yb, 1, 2, nx - 2, b.pv, b.pu);
+      }
+    });
+  }
   // x interpolation (rhsp.F90:105-107)
   if constexpr (F64) {
     k_interpxx(b, u, wki1, ru, 1);
@@ -2198,6 +2227,7 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   else
     LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel<T>, u, wki1, wki2, wki3, yc, plan, ny, nz1,
            b.nz2, b.ds, b.vmbits);
+  if (side) OTBL_HIP(hipEventRecord(b.ev_p1, b.s)); // yc read: plane loop 2's products may overwrite it
   if (b.yL)
     ky_rhsu_loop1<T>(b, wki2, wki3, rhsu);
   else if constexpr (F64)
@@ -2239,7 +2269,15 @@ template <typename T> static void rhsp_t(Box &b, int m) {
              b.diag_count, b.diag_cap, nx, ny, nz1, b.re);
   }
   // plane loop 2 (planes 2..nx-1)
-  if (nx > 2) {
+  if (side) {
+    // its products on the side stream once plane loop 1's have read yc,
+    // joined back before rhs_loop2 (which also needs this loop's rhs)
+    OTBL_HIP(hipStreamWaitEvent(b.s2, b.ev_p1, 0));
+    on_side([&] { kz_products2<T>(b); });
+    OTBL_HIP(hipEventRecord(b.ev_join, b.s2));
+    OTBL_HIP(hipStreamWaitEvent(b.s, b.ev_join, 0));
+    ky_rhs_loop2<T>(b, rhsu, rhsv, rhsw, resu, resv, resw, ya, yb, yc);
+  } else if (nx > 2) {
     if constexpr (F64) {
       k_interpyy(b, resw, ya, 0, 2, nx - 2, b.pu, b.pu);
       k_interpyy(b, resv, yb, 1, 2, nx - 2, b.pv, b.pu);

@@ -287,3 +287,33 @@ def test_divergence_after_pois(tiny):
     scale = np.abs(u).max() * idx
     assert np.abs(div).max() / scale < 1e-11
     gb.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
+def test_side_stream_matches_serial(mixed, monkeypatch):
+    """rhsp's plane loop 2 on its own stream beside plane loop 1 (the
+    default where ya / yb hold no scratch) gives the fields the serial
+    substep gives, bit for bit, in captured graphs replayed with host work
+    between them and eagerly: the same kernels on the same inputs, so any
+    missed dependency between the streams would show as a difference."""
+    from opentbl_amd.lib import OTBL_MIXED
+    cfg = config.small(name="side", nx=130, ny=40, nz=256, Lx=8.0 * 130 / 32, kzmax=8)
+    st = ic.make_state(cfg)
+    prec = {"precision": OTBL_MIXED} if mixed else {}
+    two_g = GpuBox(cfg, st, graphs=True, **prec)
+    two_e = GpuBox(cfg, st, **prec)
+    monkeypatch.setenv("OTBL_SERIAL_LOOPS", "1")
+    one = GpuBox(cfg, st, graphs=True, **prec)
+    assert two_g.paths()["streams"] == "2" and one.paths()["streams"] == "1"
+    for _ in range(3):
+        two_g.step(1)
+        two_g.sync()
+        _ = two_g.time()
+    two_e.step(3)
+    one.step(3)
+    for n in ("u", "v", "w", "p"):
+        a, b, c = two_g.get(n), two_e.get(n), one.get(n)
+        assert np.array_equal(a, c), n
+        assert np.array_equal(b, c), n
+    for b in (two_g, two_e, one):
+        b.close()
