@@ -6,9 +6,10 @@
 // radices) lane t owns the butterflies j = t + G m (m < N / (R G)); its
 // registers hold the inputs x[j + r N / R] and, after the pass, the outputs
 // y[(j / Ns) Ns R + (j mod Ns) + q Ns].  Between passes the outputs go
-// through one LDS buffer per team, indexed a + a / 16 (16-byte elements), so
-// the strided writes of the early passes and the unit-stride reads hit
-// distinct banks.
+// through one LDS buffer per team, indexed a + a / 16, which spreads the
+// strided writes of the early passes over the banks (the bank-conflict
+// counters read about 0.2 conflict cycles per LDS instruction for the FP32
+// products and 2-3 for FP64 elements).
 //
 // The last pass of a transform leaves output index j + q N / R_last in the
 // registers of the lane owning butterfly j; an inverse transform whose first
