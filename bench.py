@@ -25,7 +25,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from opentbl_amd import config, ic  # noqa: E402
+from opentbl_amd import config, ic
+from opentbl_amd.fields import divergence_rel  # noqa: E402
 from opentbl_amd.lib import GpuBox, OTBL_DT_EVERY_STEP, OTBL_FP64, OTBL_MIXED  # noqa: E402
 
 METRIC = "Mpts*RK3-substep/s, mixed vs FP64 on 1 GPU at <=1e-5 rel err; % HBM peak"
@@ -343,7 +344,7 @@ def main():
     _a.free()
     _b.free()
     runs = {}
-    ref = None
+    ref, ref_rc = None, None
     order = {"both": [OTBL_FP64, OTBL_MIXED], "mixed": [OTBL_MIXED], "fp64": [OTBL_FP64]}[args.precision]
     for prec in order:
         name = "mixed" if prec == OTBL_MIXED else "fp64"
@@ -351,27 +352,34 @@ def main():
         vel = {n: box.get(n) for n in ("u", "v", "w", "p")}
         if not all(np.isfinite(a).all() for a in vel.values()):
             raise SystemExit(f"bench.py: non-finite velocity after the {name} steps")
+        rth, cf = box.retheta_cf()
         box.close()
         model = kernel_model(cfg, 4 if prec == OTBL_MIXED else 8)
         rows = report(name, rep, model)
         runs[name] = {"wall": wall, "value": cfg.points * 3 * args.steps / wall / 1e6, "paths": paths,
                       "roofline": roofline(rows, model, cfg.name, name),
-                      "hbm_roofline": roofline(rows, model, cfg.name, name, hbm_only=True)}
+                      "hbm_roofline": roofline(rows, model, cfg.name, name, hbm_only=True),
+                      # discrete divergence of pois.f90:41-54 after the run, relative to max|u|/dx
+                      "divergence": divergence_rel(cfg, st["y"], vel["u"], vel["v"], vel["w"])}
         if prec == OTBL_FP64:
             ref = vel
+            ref_rc = (rth, cf)
         elif ref is not None:
             # velocity error relative to the largest velocity component (the
             # free stream); each field also relative to its own maximum
             scale = max(np.abs(ref[n]).max() for n in ("u", "v", "w"))
             err = {"vel": max(float(np.abs(vel[n] - ref[n]).max()) for n in ("u", "v", "w")) / scale}
             err.update({n: float(np.abs(vel[n] - ref[n]).max() / np.abs(ref[n]).max()) for n in vel})
+            # Re_theta(x) and Cf(x) (genflu.F90:76-87) against the FP64 run's
+            err["Re_theta"] = float(np.abs(rth - ref_rc[0]).max() / np.abs(ref_rc[0]).max())
+            err["Cf"] = float(np.abs(cf - ref_rc[1]).max() / np.abs(ref_rc[1]).max())
             runs[name]["rel_err_vs_fp64"] = err
             # the mixed substep's specification: velocity within 1e-5 of the
             # FP64 run; each velocity component is judged against its own
             # maximum as well as against the free stream
             runs[name]["within_target"] = {n: bool(err[n] <= VEL_TARGET) for n in ("vel", "u", "v", "w")}
         del vel
-    del st, ref
+    del st, ref, ref_rc
     head = "mixed" if "mixed" in runs else "fp64"
     h = runs[head]
     steps = args.steps
@@ -384,6 +392,7 @@ def main():
     for name, r in runs.items():
         cfgd[f"{name}_value"] = r["value"]
         cfgd[f"{name}_ms_per_step"] = 1e3 * r["wall"] / steps
+        cfgd[f"{name}_divergence"] = r["divergence"]
     if "mixed" in runs and "fp64" in runs:
         cfgd["mixed_over_fp64"] = runs["mixed"]["value"] / runs["fp64"]["value"]
         cfgd["mixed_rel_err_vs_fp64"] = runs["mixed"]["rel_err_vs_fp64"]
