@@ -586,10 +586,10 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   // for (OTBL_Z_STOCKHAM keeps the LDS Stockham kernels)
   b->zfast = zfft_supported(b->nz) && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
   b->zpair = std::getenv("OTBL_Z_NOPAIR") ? 0 : (std::getenv("OTBL_ZP_OCC2") ? 2 : 1);
+  dalloc(b->rowmax, 3 * (size_t)(b->ny + 1) * b->nx);
   if (b->zfast) {
     dalloc(b->ztw, 2 * (size_t)b->nz);
     dalloc(b->zrow, (size_t)b->nz1);
-    dalloc(b->rowmax, 3 * (size_t)(b->ny + 1) * b->nx);
     OTBL_HIP(hipMemset(b->zrow, 0, (size_t)b->nz1 * sizeof(double)));
     OTBL_HIP(hipDeviceSynchronize());
     kz_twiddles(*b, b->ztw, b->nz);

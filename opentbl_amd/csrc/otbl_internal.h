@@ -272,6 +272,8 @@ void kx_difvisxx(Box &b, const T *prod, const T *vel, T *out, const T *icon, con
 // or the 2(ny-1) mean-flow lines of the mixed path)
 template <typename R> void kz_dct_pencils(Box &b, R *P, int iopt, int Lp);
 template <typename R> void kz_products1(Box &b);
+// the CFL maxima from the per-row maxima products1 left in Box::rowmax
+void kz_rowmax(Box &b);
 template <typename R> void kz_products2(Box &b);
 void kz_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows);
 size_t k_zfft_shm_bytes(int nz, int nbuf);

@@ -613,17 +613,32 @@ __device__ inline void unpack2(const typename CT<R>::c *Z, R *A, R *B, int n, in
   }
 }
 
-__device__ inline void block_atomic_max(double v, unsigned long long *dst) {
-  // v >= 0: IEEE bit patterns of non-negative doubles order like integers
-  __shared__ double red[TPB / 64];
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
-  int w = threadIdx.x / 64;
-  if ((threadIdx.x & 63) == 0) red[w] = v;
+// The block's maxima of three non-negative values, written for row `row`
+// of the per-row maxima (three arrays of nrows; rowmax_kernel reduces them,
+// so no block contends for one address with every other).
+__device__ inline void block_rowmax3(double a, double b, double c, double *rowmax, size_t nrows, size_t row) {
+  __shared__ double red[3][TPB / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmax(a, __shfl_down(a, o, 64));
+    b = fmax(b, __shfl_down(b, o, 64));
+    c = fmax(c, __shfl_down(c, o, 64));
+  }
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+    red[2][w] = c;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int q = 1; q < (int)(blockDim.x / 64); ++q) m = fmax(m, red[q]);
-    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+    for (int q = 1; q < (int)(blockDim.x / 64); ++q) {
+      a = fmax(a, red[0][q]);
+      b = fmax(b, red[1][q]);
+      c = fmax(c, red[2][q]);
+    }
+    rowmax[row] = a;
+    rowmax[nrows + row] = b;
+    rowmax[2 * nrows + row] = c;
   }
   __syncthreads();
 }
@@ -638,7 +653,7 @@ __device__ inline void block_atomic_max(double v, unsigned long long *dst) {
 template <typename R>
 __global__ void __launch_bounds__(TPB)
 products1_kernel(const R *__restrict__ u, R *wki1, R *wki2, R *wki3, const R *__restrict__ uy, FFTPlan plan, int ny,
-                 int nz1, int nz2, DevScalars *ds, unsigned long long *vmbits) {
+                 int nz1, int nz2, double *rowmax, int nx) {
   using C = typename CT<R>::c;
   constexpr bool SPLIT = sizeof(R) == 4;
   extern __shared__ double2 sh_raw[];
@@ -677,9 +692,8 @@ products1_kernel(const R *__restrict__ u, R *wki1, R *wki2, R *wki3, const R *__
     }
   }
   __syncthreads();
-  block_atomic_max(um, &ds->umax_bits);
-  block_atomic_max(wm, &ds->wmax_bits);
-  if (vrow) block_atomic_max(vm, &vmbits[j - 1]);
+  // rows numbered (i-1)(ny+1) + (j-1), as rowmax_kernel reads them
+  block_rowmax3(um, wm, vrow ? vm : 0.0, rowmax, (size_t)(ny + 1) * nx, (size_t)(i - 1) * (ny + 1) + (j - 1));
   const C *O = fft_multi<-1>(Rb, F, n, vrow ? 2 : 1, plan, lo, hi) ? Rb : F;
   if (SPLIT) {
     unpack2(O, wki1 + ou, wki3 + ou, n, nz2, (const R *)(wki1 + ou), Ux, R(2) * Ux);
@@ -2222,11 +2236,13 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   else
     ky_interpyy<T>(b, u, yc, 0, 1, nx, b.pu, b.pu);
   // plane loop 1 products
-  if (b.zfast)
+  if (b.zfast) {
     kz_products1<T>(b);
-  else
+  } else {
     LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel<T>, u, wki1, wki2, wki3, yc, plan, ny, nz1,
-           b.nz2, b.ds, b.vmbits);
+           b.nz2, b.rowmax, nx);
+    kz_rowmax(b);
+  }
   if (side) OTBL_HIP(hipEventRecord(b.ev_p1, b.s)); // yc read: plane loop 2's products may overwrite it
   if (b.yL)
     ky_rhsu_loop1<T>(b, wki2, wki3, rhsu);

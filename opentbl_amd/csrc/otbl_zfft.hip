@@ -540,6 +540,10 @@ template <typename R> void kz_products1(Box &b) {
                      fld<R>(b.wki2), fld<R>(b.wki3), fld<R>(b.yc), ztw<R>(b), zrow, b.ny, b.nz1, b.nz2, nrows,
                      b.rowmax));
   }
+  kz_rowmax(b);
+}
+
+void kz_rowmax(Box &b) {
   LAUNCH(b, "rowmax", b.ny + 1, 256, 0, rowmax_kernel, b.rowmax, b.ds, b.vmbits, b.ny, b.nx);
 }
 
