@@ -258,11 +258,18 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
         red[q][2][col] = nal;
         al = nal;
       }
+      // the reads of step q-1 are issued before step q's arithmetic
       T Fn = 0; // F(q+1)
+      T m0 = red[PK - 1][0][col], m2 = red[PK - 1][2][col], m4 = fac[PK - 1][4][col], m3 = T(0);
 #pragma unroll 2
       for (int q = PK - 1; q >= 0; --q) {
-        const T ga = red[q][0][col], nal = red[q][2][col], de = fac[q][4][col];
-        const T nbe = q + 1 < PK ? fac[q + 1][3][col] : T(0); // be leaving step q
+        const T ga = m0, nal = m2, de = m4, nbe = m3; // nbe: the be leaving step q
+        if (q > 0) {
+          m0 = red[q - 1][0][col];
+          m2 = red[q - 1][2][col];
+          m4 = fac[q - 1][4][col];
+          m3 = fac[q][3][col];
+        }
         itf[q][1][col] = nal + nbe * Fn; // L(q)
         Fn = ga + de * Fn;
         itf[q][0][col] = Fn; // F(q)
