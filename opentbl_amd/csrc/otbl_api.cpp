@@ -559,7 +559,13 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   falloc(*b, b->rhsu, b->nu); falloc(*b, b->rhsv, b->nv); falloc(*b, b->rhsw, b->nu);
   falloc(*b, b->wki1, b->nu); falloc(*b, b->wki2, b->nv); falloc(*b, b->wki3, b->nu);
   falloc(*b, b->ya, b->nu); falloc(*b, b->yb, b->nu); falloc(*b, b->yc, b->nu);
-  falloc(*b, b->rest, b->nv); falloc(*b, b->rest2, b->nv);
+  b->pld = (b->nx - 1 + 31) / 32 * 32;
+  {
+    // the pencils (and the FP64 solve's scratch in the same layout) live in
+    // rest2 / rest: (ny-1) nz1 lines of pld values, or a v-type field
+    const size_t pen = std::max(b->nv, (size_t)(b->ny - 1) * b->nz1 * b->pld);
+    falloc(*b, b->rest, pen); falloc(*b, b->rest2, pen);
+  }
   dalloc(b->implu, 3 * (size_t)(b->ny + 2) + 8);
   dalloc(b->implv, 3 * (size_t)(b->ny + 2) + 8);
   dalloc(b->um, b->ny + 2); dalloc(b->u0, b->ny + 2); dalloc(b->v0, b->ny + 2);

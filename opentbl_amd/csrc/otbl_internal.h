@@ -83,6 +83,10 @@ struct Box {
   double re, ax, az, cfl;
   size_t nu, nv;     // elements per u-type / v-type field
   size_t pu, pv;     // elements per u-type / v-type plane
+  // line stride of the Poisson pencils (lines of nx-1 points): nx-1 rounded
+  // up to 32 values, so every line starts on a 128-byte boundary; the
+  // values past nx-1 are never read or written
+  int pld = 0;
   hipStream_t s = nullptr;
   // plane loop 2 of rhsp on a stream of its own beside plane loop 1 (its y
   // interpolations need only the substep's initial v, w; its products wait
@@ -269,8 +273,8 @@ void kx_difvisxx(Box &b, const T *prod, const T *vel, T *out, const T *icon, con
 // register-resident z transforms (otbl_zfft.hip), valid when Box::zfast /
 // Box::dct_zfast
 // Lp lines of the Poisson x transforms on pencils (all (ny-1)*nz1 lines,
-// or the 2(ny-1) mean-flow lines of the mixed path)
-template <typename R> void kz_dct_pencils(Box &b, R *P, int iopt, int Lp);
+// or the 2(ny-1) mean-flow lines of the mixed path), line l at P + l ld
+template <typename R> void kz_dct_pencils(Box &b, R *P, int iopt, int Lp, int ld);
 template <typename R> void kz_products1(Box &b);
 // the CFL maxima from the per-row maxima products1 left in Box::rowmax
 void kz_rowmax(Box &b);

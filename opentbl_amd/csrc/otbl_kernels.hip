@@ -779,7 +779,7 @@ void k_fourxz(Box &b, double *fou, double *phys, int iopt, int nrows) {
 // j = 1..ny-1) holds planes 2..nx at P[l*N + x], x = i-2.
 // ---------------------------------------------------------------------------
 struct DctArgs {
-  int N, M;
+  int N, M, ld; // points per line, Bluestein length, line stride
   FFTPlan fp;
   const double2 *chirp; // exp(-i pi n^2 / N), n < N
   const double2 *bf;    // FFT_M of the conjugate chirp, / M
@@ -795,7 +795,7 @@ __global__ void __launch_bounds__(TPB) dct_rows_kernel(const double *src, double
   fill_twiddles(lo, hi, M);
   const int l0 = 2 * blockIdx.x, l1 = l0 + 1;
   const bool has1 = l1 < Lp;
-  const double *x0 = src + (size_t)l0 * N, *x1 = src + (size_t)l1 * N;
+  const double *x0 = src + (size_t)l0 * d.ld, *x1 = src + (size_t)l1 * d.ld;
   const int half = (N + 1) / 2;
   for (int k = threadIdx.x; k < M; k += blockDim.x) {
     double2 a = make_double2(0.0, 0.0);
@@ -825,7 +825,7 @@ __global__ void __launch_bounds__(TPB) dct_rows_kernel(const double *src, double
   F = (Y == A) ? B : A;
   for (int k = threadIdx.x; k < N; k += blockDim.x) F[k] = cmulc(d.chirp[k], Y[k]);
   __syncthreads();
-  double *y0 = dst + (size_t)l0 * N, *y1 = dst + (size_t)l1 * N;
+  double *y0 = dst + (size_t)l0 * d.ld, *y1 = dst + (size_t)l1 * d.ld;
   if (iopt < 0) {
     const double sc = 2.0 / N;
     for (int k = threadIdx.x; k < N; k += blockDim.x) {
@@ -852,8 +852,9 @@ __global__ void __launch_bounds__(TPB) dct_rows_kernel(const double *src, double
 static constexpr int DIV_TPB = 1024;
 
 // Tiled transposes between the planes layout (planes 2..nx, lines
-// 0..Lp-1, plane stride ps) and x-contiguous pencils P[l*N + x].
-__global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F, double *P, int Lp, int N, size_t ps) {
+// 0..Lp-1, plane stride ps) and x-contiguous pencils P[l*ld + x], x < N.
+__global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F, double *P, int Lp, int N, size_t ps,
+                                                                int ld) {
   __shared__ double tile[64][65];
   unsigned bx, by;
   xcd_block2(bx, by); // XCD-contiguous tiles: neighbouring rows' tiles share lines
@@ -865,7 +866,7 @@ __global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F,
   __syncthreads();
   for (int ll = t / 64; ll < 64; ll += TPB / 64) {
     int l = l0 + ll, x = x0 + (t & 63);
-    if (l < Lp && x < N) P[(size_t)l * N + x] = tile[t & 63][ll];
+    if (l < Lp && x < N) P[(size_t)l * ld + x] = tile[t & 63][ll];
   }
 }
 
@@ -873,7 +874,7 @@ __global__ void __launch_bounds__(TPB) planes_to_pencils_kernel(const double *F,
 // P0 (line (j-1, kr) at P0[(2 (j-1) + kr) N]) instead of P.
 template <typename T>
 __global__ void __launch_bounds__(DIV_TPB)
-pencils_to_planes_kernel(const T *__restrict__ P, T *__restrict__ F, int Lp, int N, size_t ps,
+pencils_to_planes_kernel(const T *__restrict__ P, T *__restrict__ F, int Lp, int N, int ld, size_t ps,
                          const double *__restrict__ P0, int nz1) {
   constexpr int XR = DIV_TPB / 64, NQ = 64 / XR; // lines per pass, passes
   __shared__ T tile[64][65];
@@ -887,7 +888,7 @@ pencils_to_planes_kernel(const T *__restrict__ P, T *__restrict__ F, int Lp, int
   for (int q = 0; q < NQ; ++q) {
     const int l = l0 + xr + XR * q, x = x0 + lane;
     if (l < Lp && x < N)
-      a[q] = (P0 && l % nz1 < 2) ? (T)P0[(size_t)(2 * (l / nz1) + l % nz1) * N + x] : P[(size_t)l * N + x];
+      a[q] = (P0 && l % nz1 < 2) ? (T)P0[(size_t)(2 * (l / nz1) + l % nz1) * N + x] : P[(size_t)l * ld + x];
   }
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -940,9 +941,10 @@ __global__ void dct_gemm_kernel(const double *__restrict__ M, const double *__re
   }
 }
 
-static DctArgs dct_args(const Box &b) {
+static DctArgs dct_args(const Box &b, int ld) {
   DctArgs d;
   d.N = b.nx - 1;
+  d.ld = ld;
   d.M = b.dct_M;
   d.fp = b.dct_plan;
   d.chirp = (const double2 *)b.dct_chirp;
@@ -953,15 +955,15 @@ static DctArgs dct_args(const Box &b) {
 
 static size_t dct_shm(const Box &b) { return (TW_SLOTS + 2 * (size_t)b.dct_M) * sizeof(double2); }
 
-// Row transforms on pencils (in place).
+// Row transforms on pencils (in place, line stride Box::pld).
 static void dct_pencils(Box &b, double *P, int iopt) {
   if (b.dct_zfast) {
-    kz_dct_pencils<double>(b, P, iopt, (b.ny - 1) * b.nz1);
+    kz_dct_pencils<double>(b, P, iopt, (b.ny - 1) * b.nz1, b.pld);
     return;
   }
   const int Lp = (b.ny - 1) * b.nz1;
   LAUNCH(b, iopt < 0 ? "dct2_rows" : "dct3_rows", (Lp + 1) / 2, TPB, dct_shm(b), dct_rows_kernel, P, P, Lp,
-         iopt, dct_args(b));
+         iopt, dct_args(b, b.pld));
 }
 
 static dim3 tgrid(const Box &b) {
@@ -972,12 +974,12 @@ static dim3 tgrid(const Box &b) {
 // cosftx on nlines contiguous lines of nx-1 doubles, in place
 void k_cosft_lines(Box &b, double *P, int nlines, int iopt) {
   if (b.dct_zfast) {
-    kz_dct_pencils<double>(b, P, iopt, nlines);
+    kz_dct_pencils<double>(b, P, iopt, nlines, b.nx - 1);
     return;
   }
   if (!b.dct_fft) throw std::invalid_argument("cosft on lines needs an FFT-based Poisson x transform");
   LAUNCH(b, iopt < 0 ? "dct2_rows" : "dct3_rows", (nlines + 1) / 2, TPB, dct_shm(b), dct_rows_kernel, P, P, nlines,
-         iopt, dct_args(b));
+         iopt, dct_args(b, b.nx - 1));
 }
 
 // cosftx on a planes-layout v-type field (rows 1..ny-1 of planes 2..nx).
@@ -990,13 +992,14 @@ void k_cosftx(Box &b, const double *in, double *out, int iopt) {
     if (b.esz != 8) {
       if (!b.op64) {
         OTBL_HIP(hipStreamSynchronize(b.s));
-        OTBL_HIP(hipMalloc((void **)&b.op64, b.nv * sizeof(double) + 64));
+        const size_t pen = std::max(b.nv, (size_t)(b.ny - 1) * b.nz1 * b.pld);
+        OTBL_HIP(hipMalloc((void **)&b.op64, pen * sizeof(double) + 64));
       }
       P = b.op64;
     }
-    LAUNCH(b, "to_pencils", tgrid(b), TPB, 0, planes_to_pencils_kernel, in, P, Lp, N, b.pv);
+    LAUNCH(b, "to_pencils", tgrid(b), TPB, 0, planes_to_pencils_kernel, in, P, Lp, N, b.pv, b.pld);
     dct_pencils(b, P, iopt);
-    LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, P, out, Lp, N, b.pv,
+    LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, P, out, Lp, N, b.pld, b.pv,
            (const double *)nullptr, b.nz1);
   } else {
     dim3 grid(nblk(Lp, 64), nblk(N, 64));
@@ -1797,11 +1800,11 @@ __global__ void poisson_y_kernel(double *rr, double *a2s, DevCoef c, int nx, int
 }
 
 // divergence (pois.f90:41-54) written straight into x-contiguous pencils
-// through an LDS tile (64 lines x 64 planes)
+// (line stride ld) through an LDS tile (64 lines x 64 planes)
 template <typename T>
 __global__ void __launch_bounds__(DIV_TPB)
 div_pencils_kernel(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w, T *__restrict__ P,
-                   DevCoefT<T> c, double idx_d, int nx, int ny, int nz1, double *__restrict__ P0, DevCoef cd,
+                   int ld, DevCoefT<T> c, double idx_d, int nx, int ny, int nz1, double *__restrict__ P0, DevCoef cd,
                    const double *__restrict__ um) {
   constexpr int XR = DIV_TPB / 64, NQ = 64 / XR; // planes per pass, passes
   __shared__ T tile[64][65];
@@ -1860,23 +1863,24 @@ div_pencils_kernel(const T *__restrict__ u, const T *__restrict__ v, const T *__
   __syncthreads();
   for (int ll = xr; ll < 64; ll += XR) {
     const int lw = l0 + ll, x = x0 + lane;
-    if (lw < Lp && x < N) P[(size_t)lw * N + x] = tile[lane][ll];
+    if (lw < Lp && x < N) P[(size_t)lw * ld + x] = tile[lane][ll];
   }
 }
 
 // per-mode y tridiagonal on pencils (pois.f90:102-117, soltriy
-// pois.f90:211-239): one thread per (kr, kx-mode x), rows at stride nz1*N.
+// pois.f90:211-239): one thread per (kr, kx-mode x), line (j-1, kr) at
+// P + ((j-1) nz1 + kr) ld, so rows at stride nz1 ld.
 __global__ void __launch_bounds__(TPB)
 poisson_y_pencil_kernel(double *__restrict__ P, double *__restrict__ S, DevCoef c, int nx, int ny, int nz1,
-                        int kr_end) {
+                        int kr_end, int ld) {
   constexpr int RB = 8; // rows whose loads are issued together
   const int N = nx - 1;
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)kr_end * N) return; // columns kr < kr_end
   const int x = (int)(t % N), kr = (int)(t / N), i = x + 2;
-  const size_t rs = (size_t)nz1 * N;
-  double *col = P + (size_t)kr * N + x;
-  double *d2 = S + (size_t)kr * N + x;
+  const size_t rs = (size_t)nz1 * ld;
+  double *col = P + (size_t)kr * ld + x;
+  double *d2 = S + (size_t)kr * ld + x;
   const int ny1 = ny - 1;
   if (kr == 1) { // the zero mode is real (pois.f90:102)
     for (int j = 0; j < ny1; ++j) col[(size_t)j * rs] = 0.0;
@@ -2099,14 +2103,14 @@ static void pois_mixed(Box &b, double varstep, bool use_m) {
   const int Lp = (ny - 1) * nz1, N = nx - 1;
   float *P = fld<float>(b.rest2), *F = fld<float>(b.rest);
   LAUNCH(b, "div", tgrid(b), DIV_TPB, 0, div_pencils_kernel<float>, fld<float>(b.f[OTBL_U]), fld<float>(b.f[OTBL_V]),
-         fld<float>(b.f[OTBL_W]), P, b.dcf, b.hc.idx, nx, ny, nz1, b.P0, b.dc, (const double *)b.um64);
-  kz_dct_pencils<float>(b, P, -1, Lp);
-  kz_dct_pencils<double>(b, b.P0, -1, 2 * (ny - 1));
+         fld<float>(b.f[OTBL_W]), P, b.pld, b.dcf, b.hc.idx, nx, ny, nz1, b.P0, b.dc, (const double *)b.um64);
+  kz_dct_pencils<float>(b, P, -1, Lp, b.pld);
+  kz_dct_pencils<double>(b, b.P0, -1, 2 * (ny - 1), N);
   kp_poisson_y<float, true>(b, P);
-  LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.P0, b.S0, b.dc, nx, ny, 2, 2);
-  kz_dct_pencils<float>(b, P, 1, Lp);
-  kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1));
-  LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pv, b.P0, nz1);
+  LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.P0, b.S0, b.dc, nx, ny, 2, 2, N);
+  kz_dct_pencils<float>(b, P, 1, Lp, b.pld);
+  kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1), N);
+  LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pld, b.pv, b.P0, nz1);
   project_launch(b, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.f[OTBL_P], (const double *)F, varstep, use_m, b.p0,
                  b.um64, prun<float>(b));
 }
@@ -2121,21 +2125,21 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
   }
   if (b.dct_fft) {
     LAUNCH(b, "div", tgrid(b), DIV_TPB, 0, div_pencils_kernel<double>, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest2,
-           b.dc, b.hc.idx, nx, ny, nz1, (double *)nullptr, b.dc, (const double *)nullptr);
+           b.pld, b.dc, b.hc.idx, nx, ny, nz1, (double *)nullptr, b.dc, (const double *)nullptr);
     dct_pencils(b, b.rest2, -1);
     if (b.pchunk) {
       // modes kz >= 1 partitioned in registers; the pinned mean-flow mode
       // (kr = 0, 1) marching, one thread per column
       kp_poisson_y<double, false>(b, b.rest2);
       LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.rest2, b.rest, b.dc, nx, ny,
-             nz1, 2);
+             nz1, 2, b.pld);
     } else {
       LAUNCH(b, "poisson_y", nblk((size_t)nz1 * N), TPB, 0, poisson_y_pencil_kernel, b.rest2, b.rest, b.dc, nx, ny,
-             nz1, nz1);
+             nz1, nz1, b.pld);
     }
     dct_pencils(b, b.rest2, 1);
-    LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, b.rest2, b.rest, Lp, N, b.pv,
-           (const double *)nullptr, nz1);
+    LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, b.rest2, b.rest, Lp, N, b.pld,
+           b.pv, (const double *)nullptr, nz1);
   } else {
     size_t ndiv = (size_t)(ny - 1) * nz1 * (nx - 1);
     LAUNCH(b, "div", nblk(ndiv), TPB, 0, div_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest, b.dc,

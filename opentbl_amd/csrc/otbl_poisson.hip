@@ -3,8 +3,8 @@
 // held in registers.
 //
 // The pencils hold the x-transformed divergence: row j of line (kz, kx) (Re
-// and Im parts, kr = 2 kz, 2 kz + 1) at P[(j-1) nz1 N + kr N + kx],
-// N = nx - 1.  Row j of the operator of mode (kz, kx):
+// and Im parts, kr = 2 kz, 2 kz + 1) at P[((j-1) nz1 + kr) ld + kx], kx <
+// N = nx - 1, ld the line stride.  Row j of the operator of mode (kz, kx):
 //   a(1,j) x(j-1) + (a(2,j) - kz^2 - kmod(kx)) x(j) + a(3,j) x(j+1) = b(j),
 // a = ayp, rows 1..ny-1 (no a(1,1), no a(3,ny-1) term).  For kz >= 1 the
 // operator is strictly diagonally dominant, so chunks factor without
@@ -45,7 +45,7 @@ template <typename T> __device__ __forceinline__ T prcp(T x) {
 
 template <int L, typename T, bool REF>
 __global__ void __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1)
-poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, int nz1) {
+poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, int nz1, int ld) {
   // interface system per chunk and lane column: yF, yL, VF, VL, WF, WL; then
   // the interface unknowns F, L
   __shared__ T red[PK][6][PK];
@@ -82,8 +82,8 @@ poisson_y_chunk(T *__restrict__ P, DevCoefT<T> c, DevCoef cd, int nx, int ny, in
   // 32-bit byte offsets from a wave-uniform base (the wave's first row, a
   // scalar 64-bit address): the two chunks of a wave span 2 L rows, which
   // stay below 4 GiB for any pencil set the launcher accepts
-  const unsigned rs = (unsigned)(nz1 * N * sizeof(T));
-  const unsigned base = (unsigned)(((2 * kz + (col & 1)) * N + x) * sizeof(T));
+  const unsigned rs = (unsigned)(nz1 * ld * sizeof(T));
+  const unsigned base = (unsigned)(((2 * kz + (col & 1)) * ld + x) * sizeof(T));
   const int r0 = k * L + 1;
   // rows q < nq of the chunk are operator rows (r0 + q <= ny - 1), the rest
   // identity rows: one count instead of L row indices kept live
@@ -324,14 +324,14 @@ template <typename T, bool REF> void kp_poisson_y(Box &b, T *P) {
   const size_t cols = (size_t)(nzh - 1) * N;
   if (cols == 0) return;
   // a wave addresses 2 L rows plus one row of offsets in 32 bits
-  if ((size_t)(2 * kp_chunk_rows(b.ny) + 1) * b.nz1 * N * sizeof(T) >= (size_t(1) << 32))
+  if ((size_t)(2 * kp_chunk_rows(b.ny) + 1) * b.nz1 * b.pld * sizeof(T) >= (size_t(1) << 32))
     throw std::invalid_argument("Poisson pencil rows too long for 32-bit offsets");
   const unsigned grid = (unsigned)((cols + PK / 2 - 1) / (PK / 2)); // 16 modes (Re, Im) per workgroup
   switch (kp_chunk_rows(b.ny)) {
 #define PCASE(LL)                                                                                          \
   case LL: {                                                                                               \
     auto kf = &poisson_y_chunk<LL, T, REF>;                                                                \
-    LAUNCH(b, "poisson_y", grid, 1024, 0, kf, P, coefs<T>(b), b.dc, b.nx, b.ny, b.nz1);                    \
+    LAUNCH(b, "poisson_y", grid, 1024, 0, kf, P, coefs<T>(b), b.dc, b.nx, b.ny, b.nz1, b.pld);             \
   } break;
     PCASE(2)
     PCASE(4)

@@ -376,7 +376,7 @@ template <typename T> __device__ __forceinline__ T ld32(const T *base, unsigned 
 // clamped indices and selects in place of branches.
 template <int M, typename R, int IOPT>
 __global__ void __launch_bounds__(ZTeam<M>::TPB_, sizeof(R) == 4 ? 6 : 1)
-dct_rows_z(const R *src, R *dst, int Lp, int N, const typename CT<R>::c *__restrict__ chirp,
+dct_rows_z(const R *src, R *dst, int Lp, int N, int ld, const typename CT<R>::c *__restrict__ chirp,
            const typename CT<R>::c *__restrict__ bf, const typename CT<R>::c *__restrict__ tw4,
            const typename CT<R>::c *__restrict__ tw) {
   using C = typename CT<R>::c;
@@ -390,7 +390,7 @@ dct_rows_z(const R *src, R *dst, int Lp, int N, const typename CT<R>::c *__restr
   const int l0 = 2 * pair, l1 = l0 + 1;
   if (l0 >= Lp) return; // a whole team
   const bool has1 = l1 < Lp;
-  const R *x0 = src + (size_t)l0 * N, *x1 = src + (size_t)(has1 ? l1 : l0) * N;
+  const R *x0 = src + (size_t)l0 * ld, *x1 = src + (size_t)(has1 ? l1 : l0) * ld;
   const int half = (N + 1) / 2;
   // live slots of butterfly m: r < RH (G m + r M/R0 < M/2 exactly then,
   // as G NB0 = M / R0)
@@ -458,7 +458,7 @@ dct_rows_z(const R *src, R *dst, int Lp, int N, const typename CT<R>::c *__restr
     }
   }
   zsync<Z::W>();
-  R *y0 = dst + (size_t)l0 * N, *y1 = dst + (size_t)l1 * N;
+  R *y0 = dst + (size_t)l0 * ld, *y1 = dst + (size_t)l1 * ld;
   if constexpr (IOPT < 0) {
     const R sc = R(2.0 / N);
     C tt[NO];
@@ -555,9 +555,9 @@ template <typename R> void kz_products2(Box &b) {
                    fld<R>(b.ya), fld<R>(b.yb), fld<R>(b.yc), ztw<R>(b), b.ny, b.nz1, b.nz2, nrows));
 }
 
-// Lp lines of N points from P (in place); the lines of the mean-flow mode
-// alone (Lp = 2 (ny-1)) are timed under their own names
-template <typename R> static void dct_launch(Box &b, R *P, int iopt, int Lp, const R *chirp, const R *bf,
+// Lp lines of N points from P (in place), line l at P + l ld; the lines of
+// the mean-flow mode alone (Lp = 2 (ny-1)) are timed under their own names
+template <typename R> static void dct_launch(Box &b, R *P, int iopt, int Lp, int ld, const R *chirp, const R *bf,
                                              const R *tw4, const R *tw) {
   using C = typename CT<R>::c;
   const int N = b.nx - 1, pairs = (Lp + 1) / 2;
@@ -567,8 +567,8 @@ template <typename R> static void dct_launch(Box &b, R *P, int iopt, int Lp, con
 #define ZDCT(MM)                                                                                              \
   case MM: {                                                                                                   \
     auto kf = iopt < 0 ? &dct_rows_z<MM, R, -1> : &dct_rows_z<MM, R, 1>;                                       \
-    LAUNCH(b, name, zgrid<MM>(pairs), ZTeam<MM>::TPB_, 0, kf, P, P, Lp, N, (const C *)chirp, (const C *)bf,    \
-           (const C *)tw4, (const C *)tw);                                                                     \
+    LAUNCH(b, name, zgrid<MM>(pairs), ZTeam<MM>::TPB_, 0, kf, P, P, Lp, N, ld, (const C *)chirp,              \
+           (const C *)bf, (const C *)tw4, (const C *)tw);                                                      \
   } break;
     ZDCT(512)
     ZDCT(1024)
@@ -581,11 +581,11 @@ template <typename R> static void dct_launch(Box &b, R *P, int iopt, int Lp, con
   }
 }
 
-template <> void kz_dct_pencils<double>(Box &b, double *P, int iopt, int Lp) {
-  dct_launch<double>(b, P, iopt, Lp, b.dct_chirp, b.dct_bf, b.dct_tw4, b.dct_ztw);
+template <> void kz_dct_pencils<double>(Box &b, double *P, int iopt, int Lp, int ld) {
+  dct_launch<double>(b, P, iopt, Lp, ld, b.dct_chirp, b.dct_bf, b.dct_tw4, b.dct_ztw);
 }
-template <> void kz_dct_pencils<float>(Box &b, float *P, int iopt, int Lp) {
-  dct_launch<float>(b, P, iopt, Lp, b.dct_chirp_f, b.dct_bf_f, b.dct_tw4_f, b.dct_ztw_f);
+template <> void kz_dct_pencils<float>(Box &b, float *P, int iopt, int Lp, int ld) {
+  dct_launch<float>(b, P, iopt, Lp, ld, b.dct_chirp_f, b.dct_bf_f, b.dct_tw4_f, b.dct_ztw_f);
 }
 
 template void kz_products1<double>(Box &);
