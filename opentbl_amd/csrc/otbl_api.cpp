@@ -560,10 +560,12 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   falloc(*b, b->wki1, b->nu); falloc(*b, b->wki2, b->nv); falloc(*b, b->wki3, b->nu);
   falloc(*b, b->ya, b->nu); falloc(*b, b->yb, b->nu); falloc(*b, b->yc, b->nu);
   b->pld = (b->nx - 1 + 31) / 32 * 32;
+  b->phs = (b->pv + 31) / 32 * 32;
   {
     // the pencils (and the FP64 solve's scratch in the same layout) live in
-    // rest2 / rest: (ny-1) nz1 lines of pld values, or a v-type field
-    const size_t pen = std::max(b->nv, (size_t)(b->ny - 1) * b->nz1 * b->pld);
+    // rest2 / rest: (ny-1) nz1 lines of pld values, phi nx planes of phs
+    // values, or a v-type field
+    const size_t pen = std::max({b->nv, (size_t)(b->ny - 1) * b->nz1 * b->pld, (size_t)b->nx * b->phs});
     falloc(*b, b->rest, pen); falloc(*b, b->rest2, pen);
   }
   dalloc(b->implu, 3 * (size_t)(b->ny + 2) + 8);

@@ -87,6 +87,9 @@ struct Box {
   // up to 32 values, so every line starts on a 128-byte boundary; the
   // values past nx-1 are never read or written
   int pld = 0;
+  // plane stride of the pressure correction phi the pencils are transposed
+  // back into (Box::rest): pv rounded up to 32 values
+  size_t phs = 0;
   hipStream_t s = nullptr;
   // plane loop 2 of rhsp on a stream of its own beside plane loop 1 (its y
   // interpolations need only the substep's initial v, w; its products wait

@@ -1941,7 +1941,7 @@ poisson_y_pencil_kernel(double *__restrict__ P, double *__restrict__ S, DevCoef 
 // mean-flow pressure and um the FP64 mean-flow u: their differences are
 // taken in FP64 (else all null).
 template <typename T>
-__device__ __forceinline__ void project_col(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T> c,
+__device__ __forceinline__ void project_col(T *u, T *v, T *w, T *p, const T *rest, size_t phs, DevCoefT<T> c,
                                             const DevScalars *ds, double vstep, int use_m, double idxx_d, int nx,
                                             int ny, int nz1, const double *__restrict__ P0, double *p0,
                                             const double *idyy_d, double *um, int pm, int kr, int j) {
@@ -1977,10 +1977,10 @@ __device__ __forceinline__ void project_col(T *u, T *v, T *w, T *p, const T *res
   const T kz = c.kaz[kr / 2];
   const T iy = j <= ny - 1 ? c.idyy[j] : T(0);
   const size_t o = (size_t)(j - 2) * s + kr; // phi row j-1 (v-type row j-2)
-  const T *ph = rest + (size_t)(i0 - 1) * pv;
+  const T *ph = rest + (size_t)(i0 - 1) * phs;
   T phc = i0 <= i1 ? ph[o] : T(0); // phi(i, row j-1), carried from plane to plane
-  for (int i = i0; i <= i1; ++i, e += pu, ov += pv, ph += pv) {
-    const T phn = i <= nx - 1 ? ph[pv + o] : T(0); // phi(i+1, row j-1)
+  for (int i = i0; i <= i1; ++i, e += pu, ov += pv, ph += phs) {
+    const T phn = i <= nx - 1 ? ph[phs + o] : T(0); // phi(i+1, row j-1)
     p[ov] = varstep * phc + p[ov];
     const T pp = ph[o - kr + (kr ^ 1)];
     const T t = (kr & 1) ? kz * pp : -kz * pp;
@@ -1992,14 +1992,14 @@ __device__ __forceinline__ void project_col(T *u, T *v, T *w, T *p, const T *res
 }
 
 template <typename T>
-__global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T> c, const DevScalars *ds,
+__global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, size_t phs, DevCoefT<T> c, const DevScalars *ds,
                                double vstep, int use_m, double idxx_d, int nx, int ny, int nz1,
                                const double *__restrict__ P0, double *p0, const double *idyy_d, double *um,
                                int pm) {
   int kr, j;
   // rows 2..ny of planes 2..nx change
   if (!pcol(nz1, ny + 1, kr, j) || j < 2 || j > ny) return;
-  project_col(u, v, w, p, rest, c, ds, vstep, use_m, idxx_d, nx, ny, nz1, P0, p0, idyy_d, um, pm, kr, j);
+  project_col(u, v, w, p, rest, phs, c, ds, vstep, use_m, idxx_d, nx, ny, nz1, P0, p0, idyy_d, um, pm, kr, j);
 }
 
 // The FP32 projection with one thread per mode (Re and Im, kr = 2 kz and
@@ -2007,7 +2007,8 @@ __global__ void project_kernel(T *u, T *v, T *w, T *p, const T *rest, DevCoefT<T
 // phi that the w update of the Re (Im) part needs is the thread's own.
 // Mode 0 (its Re part takes the FP64 mean-flow lines) runs project_col per
 // part.  nz1 is even, so every pair is 8-byte aligned.
-__global__ void project_pair_kernel(float *u, float *v, float *w, float *p, const float *rest, DevCoefT<float> c,
+__global__ void project_pair_kernel(float *u, float *v, float *w, float *p, const float *rest, size_t phs,
+                                    DevCoefT<float> c,
                                     const DevScalars *ds, double vstep, int use_m, double idxx_d, int nx, int ny,
                                     int nz1, const double *__restrict__ P0, double *p0, const double *idyy_d,
                                     double *um, int pm) {
@@ -2015,7 +2016,7 @@ __global__ void project_pair_kernel(float *u, float *v, float *w, float *p, cons
   if (!pcol(nz1 / 2, ny + 1, kp, j) || j < 2 || j > ny) return;
   if (kp == 0) {
     for (int kr = 0; kr < 2; ++kr)
-      project_col(u, v, w, p, rest, c, ds, vstep, use_m, idxx_d, nx, ny, nz1, P0, p0, idyy_d, um, pm, kr, j);
+      project_col(u, v, w, p, rest, phs, c, ds, vstep, use_m, idxx_d, nx, ny, nz1, P0, p0, idyy_d, um, pm, kr, j);
     return;
   }
   const int i0 = max(2, (int)blockIdx.y * pm + 1), i1 = min(nx, (int)blockIdx.y * pm + pm);
@@ -2025,14 +2026,14 @@ __global__ void project_pair_kernel(float *u, float *v, float *w, float *p, cons
   const float iy = j <= ny - 1 ? c.idyy[j] : 0.0f;
   float2 *u2 = reinterpret_cast<float2 *>(u), *v2 = reinterpret_cast<float2 *>(v);
   float2 *w2 = reinterpret_cast<float2 *>(w), *p2 = reinterpret_cast<float2 *>(p);
-  const float2 *ph = reinterpret_cast<const float2 *>(rest + (size_t)(i0 - 1) * pv);
+  const float2 *ph = reinterpret_cast<const float2 *>(rest + (size_t)(i0 - 1) * phs);
   // float2 indices: element offsets / 2
   size_t e = ((size_t)(i0 - 1) * pu + (size_t)(j - 1) * s) / 2 + kp;
   size_t ov = ((size_t)(i0 - 1) * pv + (size_t)(j - 1) * s) / 2 + kp;
-  const size_t o = (size_t)(j - 2) * s / 2 + kp, pv2 = pv / 2, pu2 = pu / 2, s2 = s / 2;
+  const size_t o = (size_t)(j - 2) * s / 2 + kp, pv2 = pv / 2, pu2 = pu / 2, s2 = s / 2, ph2 = phs / 2;
   float2 phc = i0 <= i1 ? ph[o] : make_float2(0.0f, 0.0f);
-  for (int i = i0; i <= i1; ++i, e += pu2, ov += pv2, ph += pv2) {
-    const float2 phn = i <= nx - 1 ? ph[pv2 + o] : make_float2(0.0f, 0.0f);
+  for (int i = i0; i <= i1; ++i, e += pu2, ov += pv2, ph += ph2) {
+    const float2 phn = i <= nx - 1 ? ph[ph2 + o] : make_float2(0.0f, 0.0f);
     float2 pp = p2[ov];
     pp.x = varstep * phc.x + pp.x;
     pp.y = varstep * phc.y + pp.y;
@@ -2078,18 +2079,19 @@ static void save_predict_launch(Box &b, const T *u, const T *v, const T *w, cons
   }
 }
 
-// The projection launch of pois (pois.f90:144-181): phi (planes layout)
-// applied to u, v, w and p in place; in the mixed substep the FP64 mean-flow
-// phi pencils P0 update the FP64 mean pressure p0 and mean-flow u um.
-static void project_launch(Box &b, double *u, double *v, double *w, double *p, const double *phi, double varstep,
-                           bool use_m, double *p0, double *um, int pm) {
+// The projection launch of pois (pois.f90:144-181): phi (planes layout,
+// plane stride phs) applied to u, v, w and p in place; in the mixed substep
+// the FP64 mean-flow phi pencils P0 update the FP64 mean pressure p0 and
+// mean-flow u um.
+static void project_launch(Box &b, double *u, double *v, double *w, double *p, const double *phi, size_t phs,
+                           double varstep, bool use_m, double *p0, double *um, int pm) {
   if (b.precision == OTBL_MIXED) {
     const dim3 pg(nblk(b.pu / 2), (unsigned)((b.nx + pm - 1) / pm)); // one thread per mode (Re, Im)
     LAUNCH(b, "project", pg, TPB, 0, project_pair_kernel, fld<float>(u), fld<float>(v), fld<float>(w), fld<float>(p),
-           (const float *)phi, b.dcf, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, b.nx, b.ny, b.nz1,
+           (const float *)phi, phs, b.dcf, b.ds, varstep, use_m ? 1 : 0, b.hc.idxx, b.nx, b.ny, b.nz1,
            (const double *)b.P0, p0, b.dc.idyy, um, pm);
   } else {
-    LAUNCH(b, "project", pgrid(b, pm), TPB, 0, project_kernel<double>, u, v, w, p, phi, b.dc, b.ds, varstep,
+    LAUNCH(b, "project", pgrid(b, pm), TPB, 0, project_kernel<double>, u, v, w, p, phi, phs, b.dc, b.ds, varstep,
            use_m ? 1 : 0, b.hc.idxx, b.nx, b.ny, b.nz1, (const double *)nullptr, (double *)nullptr, b.dc.idyy,
            (double *)nullptr, pm);
   }
@@ -2110,8 +2112,8 @@ static void pois_mixed(Box &b, double varstep, bool use_m) {
   LAUNCH(b, "poisson_y0", nblk(2 * (size_t)N), TPB, 0, poisson_y_pencil_kernel, b.P0, b.S0, b.dc, nx, ny, 2, 2, N);
   kz_dct_pencils<float>(b, P, 1, Lp, b.pld);
   kz_dct_pencils<double>(b, b.P0, 1, 2 * (ny - 1), N);
-  LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pld, b.pv, b.P0, nz1);
-  project_launch(b, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.f[OTBL_P], (const double *)F, varstep, use_m, b.p0,
+  LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<float>, P, F, Lp, N, b.pld, b.phs, b.P0, nz1);
+  project_launch(b, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.f[OTBL_P], (const double *)F, b.phs, varstep, use_m, b.p0,
                  b.um64, prun<float>(b));
 }
 
@@ -2139,7 +2141,7 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
     }
     dct_pencils(b, b.rest2, 1);
     LAUNCH(b, "to_planes", tgrid(b), DIV_TPB, 0, pencils_to_planes_kernel<double>, b.rest2, b.rest, Lp, N, b.pld,
-           b.pv, (const double *)nullptr, nz1);
+           b.phs, (const double *)nullptr, nz1);
   } else {
     size_t ndiv = (size_t)(ny - 1) * nz1 * (nx - 1);
     LAUNCH(b, "div", nblk(ndiv), TPB, 0, div_kernel, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.rest, b.dc,
@@ -2149,8 +2151,9 @@ void k_pois(Box &b, int m, double varstep, bool use_m) {
     LAUNCH(b, "poisson_y", nblk(ncol), TPB, 0, poisson_y_kernel, b.rest2, b.rest, b.dc, nx, ny, nz1);
     k_cosftx(b, b.rest2, b.rest, 1);
   }
-  project_launch(b, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.f[OTBL_P], b.rest, varstep, use_m, nullptr, nullptr,
-                 prun<double>(b));
+  // phi: padded planes from the pencils, the planes layout from the dense x transforms
+  project_launch(b, b.f[OTBL_U], b.f[OTBL_V], b.f[OTBL_W], b.f[OTBL_P], b.rest, b.dct_fft ? b.phs : b.pv, varstep,
+                 use_m, nullptr, nullptr, prun<double>(b));
 }
 
 // ---------------------------------------------------------------------------
@@ -2493,7 +2496,7 @@ template <typename T> static std::string autotune_t(Box &b, int reps) {
       for (int pm : {1, 2, 4, 8}) {
         const double ms = time_launches(b, reps, [&] {
           save_predict_launch<T>(b, u, v, w, p, fld<T>(b.resu), fld<T>(b.resv), fld<T>(b.resw), um, rm, pm);
-          project_launch(b, b.resu, b.resv, b.resw, b.rhsv, b.rest, 0.0, false, p0, um, pm);
+          project_launch(b, b.resu, b.resv, b.resw, b.rhsv, b.rest, b.phs, 0.0, false, p0, um, pm);
         });
         std::snprintf(buf, sizeof buf, "prun%d=%.4fms ", pm, ms);
         rep += buf;
