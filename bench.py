@@ -62,6 +62,10 @@ def kernel_model(cfg, s=8):
     m["rhs_loop2"] = ((X - 2) * z * s * (2 * Ru + 2 * Rv + 2 * Ru + Ru + Rv + Ru + Ru + Rv + Rv), 0.0, 1)
     # res, rhs, rhs_prev, u read; u written (rhs becomes rhs_prev by a role swap)
     m["rk_final"] = ((5 * Ru + 5 * Rv + 5 * Ru) * f / 3, 0.0, 3)
+    # the fused tail per component (FP64 / full-wave y): rhs and the field read
+    # and written, res and rhs_prev read, and for v and w the two products of
+    # plane loop 2 (planes 2..nx-1) read
+    m["rk_tail"] = (((6 * Rv + 12 * Ru) * X + 4 * Ru * (X - 2)) * z * s / 3, 0.0, 3)
     m["save_predict"] = ((4 * Ru + 3 * Rv) * f, 0.0, 1)  # u, v, w, p read; predicted u, v, w written
     N, L = cfg.nx - 1, (cfg.ny - 1) * z
     # Bluestein length: the shortest engine length >= 2N-1 (3 x 1024 or a power of two)
@@ -84,7 +88,7 @@ TRACE_NAMES = {"dct2_rows": "dct_rows_z", "dct3_rows": "dct_rows_z", "div": "div
                "poisson_y": "poisson_y_chunk", "to_planes": "pencils_to_planes_kernel",
                "products1": ("products1_zp", "products1_z"), "products2": "products2_z", "interpxx": "interpxx_w",
                "difvisxx": "difvisxx_w", "interpyy": "interpyy_y", "rhsu_loop1": "rhsu_loop1_y",
-               "rhs_loop2": "rhs_loop2_y", "rk_final": ("rk_final_y", "rk_final_yh"),
+               "rhs_loop2": "rhs_loop2_y", "rk_final": ("rk_final_y", "rk_final_yh"), "rk_tail": "rk_tail_y",
                "project": ("project_kernel", "project_pair_kernel"),
                "save_predict": ("save_predict_kernel", "save_predict_pair_kernel")}
 

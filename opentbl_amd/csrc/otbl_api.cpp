@@ -594,6 +594,7 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
   // for (OTBL_Z_STOCKHAM keeps the LDS Stockham kernels)
   b->zfast = zfft_supported(b->nz) && std::getenv("OTBL_Z_STOCKHAM") == nullptr;
   b->zpair = std::getenv("OTBL_Z_NOPAIR") ? 0 : (std::getenv("OTBL_ZP_OCC2") ? 2 : 1);
+  b->ytail = std::getenv("OTBL_Y_NOTAIL") == nullptr;
   dalloc(b->rowmax, 3 * (size_t)(b->ny + 1) * b->nx);
   if (b->zfast) {
     dalloc(b->ztw, 2 * (size_t)b->nz);
@@ -1219,11 +1220,12 @@ int otbl_get_paths(otbl_ctx *ctx, char *buf, size_t len) {
   Box &b = box_of(ctx);
   need(buf && len > 0, "otbl_get_paths: null buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof tmp, "y:%s%d%s z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d prun:%d tuned:%d streams:%d",
+  std::snprintf(tmp, sizeof tmp,
+                "y:%s%d%s z:%s%d dct:%s%d x:%s xseg:%d pois:%s%d prun:%d tuned:%d streams:%d tail:%d",
                 b.yL ? "chunk" : "march", b.yL, (b.ytabh_f && b.yhalf) ? "h" : "", b.zfast ? "engine" : "stockham", b.nz,
                 b.dct_zfast ? "engine" : (b.dct_fft ? "stockham" : "gemm"), b.dct_M, b.xwin ? "window" : "march", b.xseg,
                 b.pchunk ? "chunk" : "march", b.pchunk, b.prun > 0 ? b.prun : (b.esz == 4 ? 4 : 2), b.tuned ? 1 : 0,
-                b.s2 ? 2 : 1);
+                b.s2 ? 2 : 1, b.tail() ? 1 : 0);
   std::snprintf(buf, len, "%s", tmp);
   API_END
 }

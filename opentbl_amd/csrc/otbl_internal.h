@@ -150,6 +150,11 @@ struct Box {
   int prun = 0;
   // the FP32 rk_final on half-wave chunks when its tables exist
   bool yhalf = true;
+  // rhs_loop2, boun and rk_final fused per component (ky_rk_tail) where the
+  // y solves run on full-wave chunks (Box::tail()); OTBL_Y_NOTAIL at create
+  // keeps the three passes
+  bool ytail = true;
+  bool tail() const { return yL && ytail && !(esz == 4 && ytabh_f && yhalf); }
   // set by otbl_autotune: the choices above were timed on this box
   bool tuned = false;
   // chunked y-line solver: rows per chunk (0: one thread marches each

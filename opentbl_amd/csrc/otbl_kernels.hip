@@ -2201,6 +2201,7 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   // writes the other buffer of each pair, genflu plane 1 only); ya, yb are
   // untouched until then
   const bool side = b.s2 && !b.kprof && nx > 2;
+  const bool tail = b.tail(); // rhs_loop2, boun and rk_final fused per component
   auto on_side = [&](auto &&f) {
     std::swap(b.s, b.s2);
     try {
@@ -2299,7 +2300,7 @@ template <typename T> static void rhsp_t(Box &b, int m) {
     on_side([&] { kz_products2<T>(b); });
     OTBL_HIP(hipEventRecord(b.ev_join, b.s2));
     OTBL_HIP(hipStreamWaitEvent(b.s, b.ev_join, 0));
-    ky_rhs_loop2<T>(b, rhsu, rhsv, rhsw, resu, resv, resw, ya, yb, yc);
+    if (!tail) ky_rhs_loop2<T>(b, rhsu, rhsv, rhsw, resu, resv, resw, ya, yb, yc);
   } else if (nx > 2) {
     if constexpr (F64) {
       k_interpyy(b, resw, ya, 0, 2, nx - 2, b.pu, b.pu);
@@ -2313,11 +2314,29 @@ template <typename T> static void rhsp_t(Box &b, int m) {
     else
       LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel<T>, resv, resw, ya, yb, yc, plan, ny, nz1,
              b.nz2);
-    if (b.yL)
+    if (tail) {
+      // plane loop 2's y terms are formed in the fused tail below
+    } else if (b.yL)
       ky_rhs_loop2<T>(b, rhsu, rhsv, rhsw, resu, resv, resw, ya, yb, yc);
     else if constexpr (F64)
       LAUNCH(b, "rhs_loop2", nblk((size_t)nz1 * (nx - 2)), TPB, 0, rhs_loop2_kernel, rhsu, rhsv, rhsw, resu, resv,
              resw, ya, yb, yc, wki1, b.dc, ny, nz1, nx - 2);
+  }
+  if (tail) {
+    // plane loop 2's y terms, boun's rows and the final update per component
+    // in one pass each; the implicit factors for this dt first
+    launch_impl_factor(b, 0.0, 0);
+    ky_rk_tail<T>(b, 1, v, rhsv, resv, rp[1], ya, yb);
+    if (um) k_boun_mean(b);
+    ky_rk_tail<T>(b, 3, u, rhsu, resu, rp[0], (const T *)nullptr, (const T *)nullptr);
+    if constexpr (!F64) {
+      // the FP64 mean-flow u (kz = 0 real part)
+      ky_mean_rk_final<T>(b, um, b.rm64, rhsu, rp[0]);
+      LAUNCH(b, "rk_mean", nblk((size_t)nx * (ny + 1)), TPB, 0, mean_to_field_kernel<T>, u, (const double *)um, nx,
+             ny + 1, nz1);
+    }
+    ky_rk_tail<T>(b, 2, w, rhsw, resw, rp[2], yc, ya);
+    return;
   }
   // boundary conditions (rhsp.F90:421-445)
   LAUNCH(b, "boun", nblk((size_t)nz1 * nx), TPB, 0, boun_kernel<T>, u, v, w, rhsu, rhsv, rhsw, c, ny, nz1, nx, um,

@@ -211,6 +211,13 @@ template <typename T>
 void ky_rhs_loop2(Box &b, T *rhsu, T *rhsv, T *rhsw, const T *resu, const T *resv, const T *resw, const T *vw,
                   const T *vv, const T *ww);
 template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs, const T *rhsp, int jee);
+// plane loop 2's y terms, boun's rows and rk_final of one component in one
+// pass (k2 = 1 v, 2 w, 3 u); pa, pb: the loop's products (v: vw, vv; w: ww,
+// vw; u: none)
+template <typename T>
+void ky_rk_tail(Box &b, int k2, T *u, T *rhs, const T *res, const T *rhsp, const T *pa, const T *pb);
+// boun's boundary rows of the FP64 mean-flow u of the mixed substep
+void k_boun_mean(Box &b);
 // FP32 tables of the half-wave chunking of the six fixed systems
 // (Box::ytabh_f; the implicit systems' are written with each factorisation)
 void k_ytab_halve_fixed(Box &b);

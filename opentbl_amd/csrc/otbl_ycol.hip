@@ -102,6 +102,10 @@ __device__ __forceinline__ size_t yopaque(size_t x) {
   asm volatile("" : "+s"(x));
   return x;
 }
+__device__ __forceinline__ unsigned yopaque_u(unsigned x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
 
 // Scheduling fence: the unrolled sweeps are split into groups of YGRP rows
 // so the compiler cannot hoist every load and coefficient read of a sweep
@@ -684,6 +688,217 @@ rk_final_yh(float *u, const float *__restrict__ res, const float *__restrict__ r
   }
 }
 
+// ---------------------------------------------------------------------------
+// The tail of rhsp fused per component: plane loop 2's y terms
+// (rhsp.F90:355-415), the boundary rows of boun (rhsp.F90:421-445) and
+// rk_final (rhsp.F90:448-484) in one pass over the component's columns.  The
+// final rhs is formed in registers, written once (it is the next substep's
+// rhs_prev) and used at once by the RK combination, instead of being written
+// by rhs_loop2 and read back by rk_final.  K2 selects the component:
+//   1 v: rhsv  = rhsv - kv resv (rows 2..ny-1) + i kz vw (rows 1..ny)
+//                + d(vv)/dy;
+//   2 w: rhsw  = rhsw - kv resw (rows 2..ny) + d(vw)/dy + i kz ww (rows
+//                1..ny-1);
+//   3 u: rhsu  = rhsu - kv resu (rows 2..ny),
+// each on planes 2..nx-1 (vw, ww taken at the paired column kr ^ 1, pa the
+// explicit product, pb the one differentiated).  boun then zeroes the rhs of
+// plane 1, v's rows 1 and ny, u's and w's row ny+1, and extrapolates u's and
+// w's row 1 from rows 2..4; the field's boundary rows are set (v: 0 and the
+// inflow vmagic, u and w: the one-sided closures of inby and ccon; mode 0 of
+// a mixed box's u: its FP64 mean flow um, whose boundary rows
+// boun_mean_kernel set).  Same arithmetic as rhs_loop2_y, boun_kernel and
+// rk_final_y(h), in the same order.
+// ---------------------------------------------------------------------------
+
+// u / w field rows 1 and ny+1 of boun from the column's interior rows
+template <typename T, class At>
+__device__ __forceinline__ void tail_field_bnd(const DevCoefT<T> &c, int ny, At at, T &b0, T &bn) {
+  const T i21 = T(-1) / M4(c.inby, 2, 1), c11 = T(-1) / M2(c.ccon, 1, 1);
+  b0 = i21 * (M4(c.inby, 2, 2) * at(1) + M4(c.inby, 2, 3) * at(2) + M4(c.inby, 2, 4) * at(3));
+  bn = c11 * (M2(c.ccon, 1, 2) * at(ny - 1) + M2(c.ccon, 1, 3) * at(ny - 2) + M2(c.ccon, 1, 4) * at(ny - 3));
+}
+
+// The fused tail runs on the full-wave chunks (FP64, and FP32 fields of 2^30
+// values or more).  Each rhs row is written as soon as it is formed, with
+// rows 2..4 also to LDS for row 1, and the RK update reads it back from cache.
+// On the half-wave FP32 chunks, whose rk_final alone fills its 64 registers,
+// the fused kernel spilled and measured slower than the three passes (9.0
+// against 7.7 ms per substep at 1536 x 315 x 1024), so those keep them; on
+// the full-wave chunks FP64 took 12.7 ms against 14.6.
+template <int L, typename T, int K2>
+__global__ void __launch_bounds__(YTPB)
+rk_tail_y(T *u, T *rhs, const T *__restrict__ res, const T *__restrict__ rhsp, const T *__restrict__ pa,
+          const T *__restrict__ pb, const T *__restrict__ tabv, const T *__restrict__ tabi,
+          const T *__restrict__ tabd, DevCoefT<T> c, const double *__restrict__ f,
+          const DevScalars *__restrict__ ds, const double *__restrict__ um, int jee, int ny, int nz1, int nx) {
+  constexpr bool SOLVE = K2 != 3;
+  __shared__ T xch[2 * YNCH * 64];
+  __shared__ T hal[2 * YNCH * 64];
+  __shared__ T bx[3 * 64];
+  constexpr bool STAGE = sizeof(T) == 4;
+  __shared__ T tlv_s[STAGE ? ytab_size(L) : 1], tli_s[STAGE ? ytab_size(L) : 1];
+  __shared__ T tld_s[STAGE && SOLVE ? ytab_size(L) : 1];
+  const T *tlv = tabv, *tli = tabi, *tld = tabd;
+  if constexpr (STAGE) {
+    ytab_stage<L>(tlv_s, tabv);
+    ytab_stage<L>(tli_s, tabi);
+    tlv = tlv_s;
+    tli = tli_s;
+    if constexpr (SOLVE) {
+      ytab_stage<L>(tld_s, tabd);
+      tld = tld_s;
+    }
+  }
+  const YCol q = ycol((size_t)nz1 * nx, nz1, 0);
+  const size_t s = nz1, off = q.ip * s * jee + q.kr;
+  const size_t opu = q.ip * s * (ny + 1) + q.kr, opp = opu - q.kr + (q.kr ^ 1);
+  const bool act = q.ok && q.ip > 0;
+  const bool loop2 = q.ip >= 1 && q.ip + 2 <= (size_t)nx; // planes 2..nx-1
+  const int j0 = q.p * L;
+  const T kv = c.kvis[q.kr / 2], kz = c.kaz[q.kr / 2];
+  const T sg = (q.kr & 1) ? T(1) : T(-1);
+  constexpr int GG = yrows_group(L, sizeof(T) == 4 ? 5 : 4);
+  __syncthreads(); // tables staged
+  // the final rhs, row by row: written at once (the RK update reads it back
+  // from cache), rows 2..4 also to LDS for row 1
+  T d[SOLVE ? L : 1];
+  if constexpr (SOLVE) {
+    const YBnd<T> bd = differzy_bnd(K2 == 1 ? 1 : 0, j0, L, ny, c.dcby, pb + opu, s);
+    ysolve_col<L>(d, pb + opu, s, ny, tld, q.p, q.lane, jee, bd.bot, bd.top, xch);
+  }
+  {
+    const size_t s1 = yopaque(s);
+#pragma unroll
+    for (int g0 = 0; g0 < L; g0 += GG) {
+      T x[GG], sr[GG], pw[GG];
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const size_t jc = (size_t)yclamp(j0 + g0 + r, jee) * s1;
+        x[r] = rhs[off + jc];
+        sr[r] = res[off + jc];
+        if constexpr (SOLVE) pw[r] = pa[opp + (size_t)yclamp(j0 + g0 + r, ny) * s1];
+      }
+#pragma unroll
+      for (int r = 0; r < GG; ++r) {
+        const int j = j0 + g0 + r;
+        T e = x[r];
+        if constexpr (K2 == 1) {
+          if (j >= 1 && j <= ny - 2) e -= sr[r] * kv;
+          e += sg * pw[r] * kz;
+          e = e + d[g0 + r];
+        } else if constexpr (K2 == 2) {
+          if (j >= 1 && j <= ny - 1) e -= sr[r] * kv;
+          e += d[g0 + r];
+          if (j <= ny - 1) e += sg * pw[r] * kz;
+        } else {
+          if (j >= 1 && j <= ny - 1) e = x[r] - sr[r] * kv;
+        }
+        e = loop2 ? e : x[r];
+        if (q.ip == 0) e = T(0);
+        if constexpr (K2 == 1) {
+          if (j == 0 || j == ny - 1) e = T(0);
+        } else {
+          if (j == ny) e = T(0);
+          if (j >= 1 && j <= 3) bx[(j - 1) * 64 + q.lane] = e;
+        }
+        if (q.ok && j < jee) rhs[off + (size_t)j * s1] = e;
+      }
+      YFENCE();
+    }
+  }
+  if constexpr (K2 != 1) {
+    // row 1 from rows 2..4 (another chunk may hold them)
+    __syncthreads();
+    if (q.p == 0 && q.ok) {
+      const T i21 = T(-1) / M4(c.inby, 2, 1);
+      rhs[off] = i21 * (M4(c.inby, 2, 2) * bx[q.lane] + M4(c.inby, 2, 3) * bx[64 + q.lane] +
+                        M4(c.inby, 2, 4) * bx[128 + q.lane]);
+    }
+  }
+  YFENCE();
+  T u1, uj;
+  if constexpr (K2 == 1) {
+    u1 = T(0);
+    uj = q.kr == 0 ? c.vmagic[q.ip + 1] : T(0);
+  } else if (K2 == 3 && um && q.kr == 0) {
+    u1 = (T)um[q.ip];
+    uj = (T)um[(size_t)ny * nx + q.ip];
+  } else {
+    tail_field_bnd(c, ny, [&](int j) { return u[off + (size_t)j * s]; }, u1, uj);
+  }
+  // boun's boundary rows into the field now, by the lanes that hold them:
+  // the RK update below reads them back
+  if (q.ok && j0 == 0) u[off] = u1;
+  if (q.ok && jee - 1 >= j0 && jee - 1 < j0 + L) u[off + (size_t)(jee - 1) * s] = uj;
+  YFENCE();
+  const T a1b = (T)f[3 * (jee + 1)], a3b = (T)f[3 * (jee + 1) + 1];
+  const T var2 = (q.ip == (size_t)(nx - 1)) ? T(0) : (T)ds->var2;
+  const T var3 = (T)ds->var3, var4 = (T)ds->var4;
+  const YBnd<T> bd = vistyy_bnd(jee, j0, L, ny, c.cofvby, res + off, s);
+  T t[L];
+  if constexpr (sizeof(T) == 4) {
+    T w[L];
+    {
+      T rh[L], rp[L], uu[L];
+      const size_t s1 = yopaque(s);
+#pragma unroll
+      for (int r = 0; r < L; ++r) {
+        const size_t o = off + (size_t)yclamp(j0 + r, jee) * s1;
+        rh[r] = rhs[o];
+        rp[r] = rhsp[o];
+        uu[r] = u[o];
+      }
+#pragma unroll
+      for (int r = 0; r < L; ++r) w[r] = uu[r] - (var3 * rh[r] + var4 * rp[r]);
+    }
+    YFENCE();
+    ysolve_col<L>(t, res + off, s, jee, tlv, q.p, q.lane, jee, bd.bot, bd.top, xch);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+      if (act && j0 + r < jee) t[r] = w[r] - var2 * t[r];
+  } else {
+    YFENCE();
+    ysolve_col<L>(t, res + off, s, jee, tlv, q.p, q.lane, jee, bd.bot, bd.top, xch);
+    const size_t s1 = yopaque(s);
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+      if (r % YGRP == 0) YFENCE();
+      const int j = j0 + r;
+      const size_t o = off + (size_t)yclamp(j, jee) * s1;
+      const T rr = rhs[o], rp = rhsp[o], uu = u[o];
+      if (act && j < jee) t[r] = uu - (var3 * rr + var2 * t[r] + var4 * rp);
+    }
+  }
+  YFENCE();
+  hal[q.p * 64 + q.lane] = t[0];
+  hal[(YNCH + q.p) * 64 + q.lane] = t[L - 1];
+  __syncthreads();
+  const T tm = q.p > 0 ? hal[(YNCH + q.p - 1) * 64 + q.lane] : T(0);
+  const T tp = q.p < YNCH - 1 ? hal[(q.p + 1) * 64 + q.lane] : T(0);
+  const T yend = yfwd<L>(
+      t, [&](int k) { return k == 0 ? tm : (k == L + 1 ? tp : t[k - 1]); }, tli, q.p, jee, u1, uj, -a1b * u1,
+      -a3b * uj);
+  ybwd<L>(t, yend, tli, q.p, q.lane, xch);
+  const size_t s2 = yopaque(s);
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    const int j = j0 + r;
+    if (act && j >= 1 && j < jee - 1) u[off + (size_t)j * s2] = t[r];
+  }
+}
+
+// boun's rows 1 and ny+1 of the FP64 mean-flow u of the mixed substep
+// (um [ny+1][nx], plane fastest), from its own interior
+__global__ void boun_mean_kernel(double *um, DevCoef cd, int ny, int nx) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= nx) return;
+  double *U = um + ip;
+  auto UR = [&](int r) -> double & { return U[(size_t)r * nx]; }; // row r+1
+  UR(0) = -1.0 / M4(cd.inby, 2, 1) * (M4(cd.inby, 2, 2) * UR(1) + M4(cd.inby, 2, 3) * UR(2) + M4(cd.inby, 2, 4) * UR(3));
+  UR(ny) = -1.0 / M2(cd.ccon, 1, 1) *
+           (M2(cd.ccon, 1, 2) * UR(ny - 1) + M2(cd.ccon, 1, 3) * UR(ny - 2) + M2(cd.ccon, 1, 4) * UR(ny - 3));
+}
+
 // rk_final of the FP64 mean-flow u of the mixed substep (FP32 fields): the
 // explicit RK3 combination with vistyy (rhsp.F90:448-475) and the implicit y
 // solve (rhsp.F90:479-484) of the kz = 0 real part of u, every plane a
@@ -830,6 +1045,30 @@ template <typename T> void ky_rk_final(Box &b, T *u, const T *res, const T *rhs,
             LAUNCH(b, "rk_final", g, YTPB, 0, kf, u, res, rhs, rhsp, ytab<T>(b, uu ? YT_VIYU : YT_VIYV),
                    ytab<T>(b, uu ? YT_IMPU : YT_IMPV), coefs<T>(b), uu ? b.implu : b.implv, b.ds, jee, b.ny, b.nz1,
                    b.nx));
+}
+
+// the fused tail of one component (k2 = 1 v, 2 w, 3 u) on the full-wave
+// chunks; pa, pb the products of plane loop 2 (v: vw, vv; w: ww, vw; u:
+// unused)
+template <typename T>
+void ky_rk_tail(Box &b, int k2, T *u, T *rhs, const T *res, const T *rhsp, const T *pa, const T *pb) {
+  const int jee = k2 == 1 ? b.ny : b.ny + 1;
+  const bool uu = jee == b.ny + 1;
+  const int kv = uu ? YT_VIYU : YT_VIYV, ki = uu ? YT_IMPU : YT_IMPV, kd = k2 == 1 ? YT_DCYV : YT_DCYU;
+  const double *um = (k2 == 3 && b.precision == OTBL_MIXED) ? b.um64 : nullptr;
+  if (k2 == 3) pa = pb = nullptr;
+  const unsigned g = ygrid((size_t)b.nz1 * b.nx);
+  YDISPATCH(b.yL, auto kf = k2 == 1 ? &rk_tail_y<L, T, 1> : (k2 == 2 ? &rk_tail_y<L, T, 2> : &rk_tail_y<L, T, 3>);
+            LAUNCH(b, "rk_tail", g, YTPB, 0, kf, u, rhs, res, rhsp, pa, pb, ytab<T>(b, kv), ytab<T>(b, ki),
+                   ytab<T>(b, kd), coefs<T>(b), uu ? b.implu : b.implv, b.ds, um, jee, b.ny, b.nz1, b.nx));
+}
+template void ky_rk_tail<double>(Box &, int, double *, double *, const double *, const double *, const double *,
+                                 const double *);
+template void ky_rk_tail<float>(Box &, int, float *, float *, const float *, const float *, const float *,
+                                const float *);
+
+void k_boun_mean(Box &b) {
+  LAUNCH(b, "boun", (b.nx + 255) / 256, 256, 0, boun_mean_kernel, b.um64, b.dc, b.ny, b.nx);
 }
 
 template <typename T> void ky_mean_rk_final(Box &b, double *um, const double *rm, const T *rhs, const T *rhsp) {

@@ -290,6 +290,47 @@ def test_divergence_after_pois(tiny):
 
 
 @pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("ny", [40, 97], ids=["y40", "y97"])
+def test_fused_tail_matches_three_passes(mixed, ny, monkeypatch):
+    """rhsp's tail fused per component (plane loop 2's y terms, boun's rows
+    and rk_final in one pass; the default where the y solves run on
+    full-wave chunks: FP64, and FP32 without the half-wave tables) gives the
+    fields, rhs and rhs_prev of rhs_loop2 + boun + rk_final run apart
+    (OTBL_Y_NOTAIL), bit for bit: the same chunks and the same expressions.
+    ny = 40 gives chunks of 4 rows, ny = 97 of 8 (rows 2..4, from which boun
+    extrapolates row 1, are exchanged through LDS either way)."""
+    from opentbl_amd.lib import OTBL_MIXED
+    cfg = config.small(name="tail", nx=130, ny=ny, nz=256, Lx=8.0 * 130 / 32, kzmax=8)
+    st = ic.make_state(cfg)
+    prec = {"precision": OTBL_MIXED} if mixed else {}
+    if mixed:
+        monkeypatch.setenv("OTBL_Y_HALF", "0") # the full-wave FP32 y kernels
+    fused = GpuBox(cfg, st, graphs=True, **prec)
+    monkeypatch.setenv("OTBL_Y_NOTAIL", "1")
+    apart = GpuBox(cfg, st, graphs=True, **prec)
+    assert fused.paths()["tail"] == "1" and apart.paths()["tail"] == "0"
+    fused.step(3)
+    apart.step(3)
+    for n in ("u", "v", "w", "p", "rhsupa", "rhsvpa", "rhswpa"):
+        a, b = fused.get(n), apart.get(n)
+        assert np.isfinite(a).all(), n
+        assert np.array_equal(a, b), (n, np.abs(a - b).max())
+    fused.close()
+    apart.close()
+
+
+def test_half_wave_mixed_keeps_three_passes():
+    """The mixed substep on half-wave y chunks keeps rhs_loop2, boun and
+    rk_final apart (the fused kernel spilled there and was slower)."""
+    from opentbl_amd.lib import OTBL_MIXED
+    cfg = config.small(name="tailh", nx=130, ny=40, nz=256, Lx=8.0 * 130 / 32, kzmax=8)
+    b = GpuBox(cfg, ic.make_state(cfg), precision=OTBL_MIXED)
+    p = b.paths()
+    assert p["y"].endswith("h") and p["tail"] == "0", p
+    b.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
 def test_side_stream_matches_serial(mixed, monkeypatch):
     """rhsp's plane loop 2 on its own stream beside plane loop 1 (the
     default where ya / yb hold no scratch) gives the fields the serial
