@@ -52,12 +52,12 @@ def test_roofline_takes_the_dominant_kernel_and_its_bound():
     assert h["achieved"] == pytest.approx(m["rk_final"][0] / (4.3 / 3 * 1e-3) / 1e9)
 
 
-@pytest.mark.parametrize("kname,prec", [("rk_final", "mixed"), ("rk_final", "fp64"), ("project", "mixed"),
-                                        ("save_predict", "mixed"), ("products1", "mixed")])
+@pytest.mark.parametrize("kname,prec", [("rk_final", "mixed"), ("rk_tail", "fp64"), ("project", "mixed"),
+                                        ("save_predict", "mixed"), ("products1", "mixed"), ("products1", "fp64")])
 def test_profiled_traffic_finds_the_kernel(kname, prec):
     """The committed summaries name the HIP functions (rk_final_yh<10> in
-    FP32, rk_final_y<20, double> in FP64, the pair kernels of the FP32
-    element-wise steps); each lookup must find its row."""
+    FP32, the fused tail rk_tail_y<20, double, k> in FP64, the pair kernels
+    of the FP32 element-wise steps); each lookup must find its row."""
     t = bench.profiled_traffic(CFG.name, prec, kname)
     assert t is not None and t > 1e9, (kname, prec, t)
 
