@@ -183,6 +183,8 @@ struct Box {
   bool dct_zfast = false; // M a power of two run on the register engine
   double *dct_ztw = nullptr; // exp(2 pi i m / M)
   int dct_M = 0;
+  // rows per workgroup of the LDS products kernels (short z rows batched)
+  int zrb = 1;
   FFTPlan dct_plan{};
   double *dct_chirp = nullptr, *dct_bf = nullptr, *dct_tw4 = nullptr;
   // per-substep diagnostics ring (info_per_step records): diag_cap records

@@ -562,6 +562,9 @@ void k_implzy_op(Box &b, double *u, const double *res, int R, double rkk, int np
 // LDS layout of the transform kernels: lo[64], hi[128] twiddle tables, then
 // the signal buffers.
 static constexpr int TW_SLOTS = 64 + 128;
+// rows per block of the LDS products kernels: at most ZROWS_MAX, about 512
+// points per block (Box::zrb)
+static constexpr int ZROWS_MAX = 8;
 
 // Hermitian-extended spectrum of two real signals packed as z = x + i y,
 // x = c2r(X), y = c2r(Y).  X, Y rows hold modes 0..nz2 (re, im pairs); Im of
@@ -643,7 +646,10 @@ __device__ inline void block_rowmax3(double a, double b, double c, double *rowma
   __syncthreads();
 }
 
-// Plane loop 1 of rhsp (rhsp.F90:120-153) for one (j, i) row: with u, u_x,
+// Plane loop 1 of rhsp (rhsp.F90:120-153) for rb consecutive rows (j, i) of a
+// plane (their signals transformed together, sharing the block's twiddles
+// and barriers; rb rows of nz points fill the block better than one when nz
+// is short): with u, u_x,
 // w_x, v_x and u interpolated to v rows (uy) in spectral space, form
 // r2c(u_x^2) -> wki1, r2c(u w_x) -> wki3, r2c(v_x uy) -> wki2 (rows <= ny),
 // and the maxima |u|, |w_x|, |v_x|(j) for the CFL time step.  In FP32 the
@@ -653,88 +659,119 @@ __device__ inline void block_rowmax3(double a, double b, double c, double *rowma
 template <typename R>
 __global__ void __launch_bounds__(TPB)
 products1_kernel(const R *__restrict__ u, R *wki1, R *wki2, R *wki3, const R *__restrict__ uy, FFTPlan plan, int ny,
-                 int nz1, int nz2, double *rowmax, int nx) {
+                 int nz1, int nz2, double *rowmax, int nx, int rb) {
   using C = typename CT<R>::c;
   constexpr bool SPLIT = sizeof(R) == 4;
   extern __shared__ double2 sh_raw[];
+  __shared__ R sU[2][ZROWS_MAX]; // u_x and uy of mode 0 of each row, read before any row is written
   C *sh = reinterpret_cast<C *>(sh_raw);
   const int n = plan.n;
-  C *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n;
-  const int j = blockIdx.x + 1, i = blockIdx.y + 1;
-  const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
-  const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
-  const bool vrow = j <= ny;
-  const R Ux = SPLIT ? wki1[ou] : R(0), Uy = (SPLIT && vrow) ? uy[ou] : R(0);
+  // rows j0 .. j0+nr-1 of plane i; row r's two signals at Q0 + 2 n r
+  C *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n * rb;
+  const int j0 = blockIdx.x * rb + 1, i = blockIdx.y + 1;
+  const int nr = min(rb, ny + 2 - j0);
+  auto ou = [&](int r) { return ((size_t)(i - 1) * (ny + 1) + (j0 + r - 1)) * nz1; };
+  auto ov = [&](int r) { return ((size_t)(i - 1) * ny + (j0 + r - 1)) * nz1; };
+  auto vrow = [&](int r) { return j0 + r <= ny; };
+  if (SPLIT && threadIdx.x < nr) {
+    sU[0][threadIdx.x] = wki1[ou(threadIdx.x)];
+    sU[1][threadIdx.x] = vrow(threadIdx.x) ? uy[ou(threadIdx.x)] : R(0);
+  }
   fill_twiddles(lo, hi, n);
-  pack2(Q0, u + ou, wki1 + ou, n, nz2);
-  pack2(Q0 + n, wki3 + ou, vrow ? wki2 + ov : nullptr, n, nz2);
-  if (SPLIT && threadIdx.x == 0) Q0[0].y = R(0); // the mean of u_x (slot 0 was packed by this thread)
+  for (int r = 0; r < rb; ++r) {
+    C *q = Q0 + 2 * n * r;
+    const bool ok = r < nr;
+    pack2(q, ok ? u + ou(r) : (const R *)nullptr, ok ? wki1 + ou(r) : (const R *)nullptr, n, nz2);
+    pack2(q + n, ok ? wki3 + ou(r) : (const R *)nullptr, (ok && vrow(r)) ? wki2 + ov(r) : (const R *)nullptr, n,
+          nz2);
+    if (SPLIT && threadIdx.x == 0) q[0].y = R(0); // the mean of u_x (slot 0 was packed by this thread)
+  }
   __syncthreads();
-  const bool inA = fft_multi<+1>(Q0, Q1, n, 2, plan, lo, hi);
+  const bool inA = fft_multi<+1>(Q0, Q1, n, 2 * rb, plan, lo, hi);
   C *Rb = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
   const C *UY = nullptr;
-  if (vrow) {
-    pack2(F, uy + ou, (const R *)nullptr, n, nz2);
-    if (SPLIT && threadIdx.x == 0) F[0].x = R(0); // the mean of uy (slot 0 was packed by this thread)
-    __syncthreads();
-    UY = fft_multi<+1>(F, F + n, n, 1, plan, lo, hi) ? F : F + n;
-  }
-  double um = 0.0, wm = 0.0, vm = 0.0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    C a = Rb[q], b2 = Rb[n + q];
-    R uu = a.x, ux = a.y, wx = b2.x, vx = b2.y;
-    um = fmax(um, (double)fabs(uu));
-    wm = fmax(wm, (double)fabs(wx));
-    Rb[q] = mkc<C>(ux * ux, uu * wx);
-    if (vrow) {
-      vm = fmax(vm, (double)fabs(vx));
-      Rb[n + q] = mkc<C>(vx * UY[q].x, R(0));
+  if (vrow(0)) {
+    for (int r = 0; r < rb; ++r) {
+      pack2(F + n * r, (r < nr && vrow(r)) ? uy + ou(r) : (const R *)nullptr, (const R *)nullptr, n, nz2);
+      if (SPLIT && threadIdx.x == 0) F[n * r].x = R(0); // the mean of uy (slot 0 was packed by this thread)
     }
+    __syncthreads();
+    UY = fft_multi<+1>(F, F + n * rb, n, rb, plan, lo, hi) ? F : F + n * rb;
   }
-  __syncthreads();
-  // rows numbered (i-1)(ny+1) + (j-1), as rowmax_kernel reads them
-  block_rowmax3(um, wm, vrow ? vm : 0.0, rowmax, (size_t)(ny + 1) * nx, (size_t)(i - 1) * (ny + 1) + (j - 1));
-  const C *O = fft_multi<-1>(Rb, F, n, vrow ? 2 : 1, plan, lo, hi) ? Rb : F;
-  if (SPLIT) {
-    unpack2(O, wki1 + ou, wki3 + ou, n, nz2, (const R *)(wki1 + ou), Ux, R(2) * Ux);
-    if (vrow) unpack2(O + n, wki2 + ov, (R *)nullptr, n, nz2, (const R *)(wki2 + ov), Uy, Uy);
-  } else {
-    unpack2(O, wki1 + ou, wki3 + ou, n, nz2);
-    if (vrow) unpack2(O + n, wki2 + ov, (R *)nullptr, n, nz2);
+  for (int r = 0; r < nr; ++r) {
+    double um = 0.0, wm = 0.0, vm = 0.0;
+    const bool vr = vrow(r);
+    C *q = Rb + 2 * n * r;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      C a = q[k], b2 = q[n + k];
+      R uu = a.x, ux = a.y, wx = b2.x, vx = b2.y;
+      um = fmax(um, (double)fabs(uu));
+      wm = fmax(wm, (double)fabs(wx));
+      q[k] = mkc<C>(ux * ux, uu * wx);
+      if (vr) {
+        vm = fmax(vm, (double)fabs(vx));
+        q[n + k] = mkc<C>(vx * UY[n * r + k].x, R(0));
+      }
+    }
+    // rows numbered (i-1)(ny+1) + (j-1), as rowmax_kernel reads them
+    block_rowmax3(um, wm, vr ? vm : 0.0, rowmax, (size_t)(ny + 1) * nx, (size_t)(i - 1) * (ny + 1) + (j0 + r - 1));
+  }
+  const C *O = fft_multi<-1>(Rb, F, n, 2 * rb, plan, lo, hi) ? Rb : F;
+  for (int r = 0; r < nr; ++r) {
+    const C *o = O + 2 * n * r;
+    if (SPLIT) {
+      const R Ux = sU[0][r], Uy = sU[1][r];
+      unpack2(o, wki1 + ou(r), wki3 + ou(r), n, nz2, (const R *)(wki1 + ou(r)), Ux, R(2) * Ux);
+      if (vrow(r)) unpack2(o + n, wki2 + ov(r), (R *)nullptr, n, nz2, (const R *)(wki2 + ov(r)), Uy, Uy);
+    } else {
+      unpack2(o, wki1 + ou(r), wki3 + ou(r), n, nz2);
+      if (vrow(r)) unpack2(o + n, wki2 + ov(r), (R *)nullptr, n, nz2);
+    }
   }
 }
 
-// Plane loop 2 of rhsp (rhsp.F90:367-415) for one (j, i) row, j <= ny:
+// Plane loop 2 of rhsp (rhsp.F90:367-415) for rb consecutive rows (j, i), j <= ny:
 // vw = r2c(v * w_v) -> ya (w interpolated to v rows is read from ya),
 // vv = r2c(v_u^2) -> yb (v interpolated to u rows is read from yb),
 // ww = r2c(w^2) -> yc.
 template <typename R>
 __global__ void __launch_bounds__(TPB)
 products2_kernel(const R *__restrict__ resv, const R *__restrict__ resw, R *ya, R *yb, R *yc, FFTPlan plan, int ny,
-                 int nz1, int nz2) {
+                 int nz1, int nz2, int rb) {
   using C = typename CT<R>::c;
   extern __shared__ double2 sh_raw[];
   C *sh = reinterpret_cast<C *>(sh_raw);
   const int n = plan.n;
-  C *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n;
-  const int j = blockIdx.x + 1, i = blockIdx.y + 2;
-  const size_t ou = ((size_t)(i - 1) * (ny + 1) + (j - 1)) * nz1;
-  const size_t ov = ((size_t)(i - 1) * ny + (j - 1)) * nz1;
+  C *lo = sh, *hi = sh + 64, *Q0 = sh + TW_SLOTS, *Q1 = Q0 + 2 * n * rb;
+  const int j0 = blockIdx.x * rb + 1, i = blockIdx.y + 2;
+  const int nr = min(rb, ny + 1 - j0);
+  auto ou = [&](int r) { return ((size_t)(i - 1) * (ny + 1) + (j0 + r - 1)) * nz1; };
+  auto ov = [&](int r) { return ((size_t)(i - 1) * ny + (j0 + r - 1)) * nz1; };
   fill_twiddles(lo, hi, n);
-  pack2(Q0, resv + ov, ya + ou, n, nz2);
-  pack2(Q0 + n, yb + ou, resw + ou, n, nz2);
-  __syncthreads();
-  const bool inA = fft_multi<+1>(Q0, Q1, n, 2, plan, lo, hi);
-  C *Rb = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    C a = Rb[q], b2 = Rb[n + q];
-    Rb[q] = mkc<C>(a.x * a.y, b2.x * b2.x);
-    Rb[n + q] = mkc<C>(b2.y * b2.y, R(0));
+  for (int r = 0; r < rb; ++r) {
+    C *q = Q0 + 2 * n * r;
+    const bool ok = r < nr;
+    pack2(q, ok ? resv + ov(r) : (const R *)nullptr, ok ? ya + ou(r) : (const R *)nullptr, n, nz2);
+    pack2(q + n, ok ? yb + ou(r) : (const R *)nullptr, ok ? resw + ou(r) : (const R *)nullptr, n, nz2);
   }
   __syncthreads();
-  const C *O = fft_multi<-1>(Rb, F, n, 2, plan, lo, hi) ? Rb : F;
-  unpack2(O, ya + ou, yb + ou, n, nz2);
-  unpack2(O + n, yc + ou, (R *)nullptr, n, nz2);
+  const bool inA = fft_multi<+1>(Q0, Q1, n, 2 * rb, plan, lo, hi);
+  C *Rb = inA ? Q0 : Q1, *F = inA ? Q1 : Q0;
+  for (int k = threadIdx.x; k < 2 * n * rb; k += blockDim.x) {
+    const int r = k / (2 * n), q = k - r * 2 * n;
+    if (q < n) {
+      C a = Rb[k], b2 = Rb[k + n];
+      Rb[k] = mkc<C>(a.x * a.y, b2.x * b2.x);
+      Rb[k + n] = mkc<C>(b2.y * b2.y, R(0));
+    }
+  }
+  __syncthreads();
+  const C *O = fft_multi<-1>(Rb, F, n, 2 * rb, plan, lo, hi) ? Rb : F;
+  for (int r = 0; r < nr; ++r) {
+    const C *o = O + 2 * n * r;
+    unpack2(o, ya + ou(r), yb + ou(r), n, nz2);
+    unpack2(o + n, yc + ou(r), (R *)nullptr, n, nz2);
+  }
 }
 
 // fourxz (ffts.f90:27-67) on independent rows, for the operator-level API.
@@ -2192,7 +2229,7 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   T *ya = fld<T>(b.ya), *yb = fld<T>(b.yb), *yc = fld<T>(b.yc);
   const DevCoefT<T> &c = coefs<T>(b);
   const FFTPlan &plan = b.zplan;
-  const size_t shm = zfft_shm(b, 4);
+  const size_t shm = zfft_shm(b, 4 * b.zrb);
 
   LAUNCH(b, "reset_max", 1, TPB, 0, reset_max_kernel, b.ds, b.vmbits, ny);
   // plane loop 2 beside plane loop 1 (per-kernel timing runs them in order):
@@ -2247,8 +2284,8 @@ template <typename T> static void rhsp_t(Box &b, int m) {
   if (b.zfast) {
     kz_products1<T>(b);
   } else {
-    LAUNCH(b, "products1", dim3(ny + 1, nx), TPB, shm, products1_kernel<T>, u, wki1, wki2, wki3, yc, plan, ny, nz1,
-           b.nz2, b.rowmax, nx);
+    LAUNCH(b, "products1", dim3((ny + b.zrb) / b.zrb, nx), TPB, shm, products1_kernel<T>, u, wki1, wki2, wki3, yc,
+           plan, ny, nz1, b.nz2, b.rowmax, nx, b.zrb);
     kz_rowmax(b);
   }
   if (side) OTBL_HIP(hipEventRecord(b.ev_p1, b.s)); // yc read: plane loop 2's products may overwrite it
@@ -2312,8 +2349,8 @@ template <typename T> static void rhsp_t(Box &b, int m) {
     if (b.zfast)
       kz_products2<T>(b);
     else
-      LAUNCH(b, "products2", dim3(ny, nx - 2), TPB, shm, products2_kernel<T>, resv, resw, ya, yb, yc, plan, ny, nz1,
-             b.nz2);
+      LAUNCH(b, "products2", dim3((ny + b.zrb - 1) / b.zrb, nx - 2), TPB, shm, products2_kernel<T>, resv, resw, ya,
+             yb, yc, plan, ny, nz1, b.nz2, b.zrb);
     if (tail) {
       // plane loop 2's y terms are formed in the fused tail below
     } else if (b.yL)
@@ -2371,7 +2408,9 @@ void k_substep_rhsp(Box &b, int m) {
 }
 
 void k_setup_attrs(Box &b) {
-  const int shm = (int)zfft_shm(b, 4);
+  // rows per block of the LDS products kernels
+  b.zrb = std::max(1, std::min(ZROWS_MAX, 512 / b.nz));
+  const int shm = (int)zfft_shm(b, 4 * b.zrb);
   OTBL_HIP(hipFuncSetAttribute((const void *)products1_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
   OTBL_HIP(hipFuncSetAttribute((const void *)products2_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
   OTBL_HIP(hipFuncSetAttribute((const void *)products1_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
