@@ -619,23 +619,25 @@ int otbl_create(const otbl_params *prm, const double *y, otbl_ctx **out) {
     // the mixed substep has no marching fallback: short lines, which the
     // windowed kernel solves exactly, or any decaying x systems
     if (b->precision == OTBL_MIXED) b->xwin = b->nx <= XLINE_EXACT || worst < 1e-17;
-    // segmented lines: enough threads in flight (about 6e5) when the
-    // forward recurrences forget their start as well and the segments stay
-    // longer than their lead-in and backward window
+    // segmented lines when the forward recurrences forget their start as
+    // well: about 6e5 threads in flight, at most 8 segments of at least one
+    // group of XLINE_LEAD planes.  A segment marches its lead-in and the
+    // backward window past its end again; on short lines the parallelism
+    // pays for that (interpxx + difvisxx of a 256 x 96 x 64 box: 1.46 ms on
+    // one segment, 0.44 ms on eight; 1536 x 315 x 1024 keeps three)
     double lead = 0.0;
     for (const std::vector<double> *a : {&h.inxu, &h.inxv, &h.dcxu, &h.dcxv, &h.vixu, &h.vixv})
       lead = std::max(lead, lead_decay(*a, b->nx));
     const size_t lines = b->pu;
-    int nseg = (int)std::min<size_t>(4, std::max<size_t>(1, (600000 + lines / 2) / lines));
-    while (nseg > 1 && b->nx / nseg < 4 * (XLINE_LEAD + XLINE_WINDOW)) --nseg;
+    int nseg = (int)std::min<size_t>(8, std::max<size_t>(1, (600000 + lines / 2) / lines));
+    nseg = std::max(1, std::min(nseg, b->nx / XLINE_LEAD));
     // OTBL_XSEG=n forces n segments (at most one per group of XLINE_LEAD planes)
     if (const char *e = std::getenv("OTBL_XSEG"))
       nseg = std::max(1, std::min(std::atoi(e), (b->nx + XLINE_LEAD - 1) / XLINE_LEAD));
     b->xseg = (b->xwin && lead < 1e-17) ? nseg : 1;
-    // the most segments the autotuner may try: each at least four lead-ins
-    // and backward windows long
-    int smax = 1;
-    while (smax < 8 && b->nx / (smax + 1) >= 4 * (XLINE_LEAD + XLINE_WINDOW)) ++smax;
+    // the most segments the autotuner may try: at most 8, each at least one
+    // group of XLINE_LEAD planes
+    const int smax = std::max(1, std::min(8, b->nx / XLINE_LEAD));
     b->xseg_max = (b->xwin && lead < 1e-17) ? std::max(smax, b->xseg) : 1;
   }
   // chunked y-line solver unless the columns are longer than its
