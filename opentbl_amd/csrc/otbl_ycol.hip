@@ -102,10 +102,6 @@ __device__ __forceinline__ size_t yopaque(size_t x) {
   asm volatile("" : "+s"(x));
   return x;
 }
-__device__ __forceinline__ unsigned yopaque_u(unsigned x) {
-  asm volatile("" : "+s"(x));
-  return x;
-}
 
 // Scheduling fence: the unrolled sweeps are split into groups of YGRP rows
 // so the compiler cannot hoist every load and coefficient read of a sweep
@@ -692,9 +688,9 @@ rk_final_yh(float *u, const float *__restrict__ res, const float *__restrict__ r
 // The tail of rhsp fused per component: plane loop 2's y terms
 // (rhsp.F90:355-415), the boundary rows of boun (rhsp.F90:421-445) and
 // rk_final (rhsp.F90:448-484) in one pass over the component's columns.  The
-// final rhs is formed in registers, written once (it is the next substep's
-// rhs_prev) and used at once by the RK combination, instead of being written
-// by rhs_loop2 and read back by rk_final.  K2 selects the component:
+// final rhs is written once (it is the next substep's rhs_prev) and read back
+// at once by the RK combination, from cache, instead of being written by
+// rhs_loop2 and read back from HBM by rk_final.  K2 selects the component:
 //   1 v: rhsv  = rhsv - kv resv (rows 2..ny-1) + i kz vw (rows 1..ny)
 //                + d(vv)/dy;
 //   2 w: rhsw  = rhsw - kv resw (rows 2..ny) + d(vw)/dy + i kz ww (rows
@@ -706,8 +702,8 @@ rk_final_yh(float *u, const float *__restrict__ res, const float *__restrict__ r
 // w's row 1 from rows 2..4; the field's boundary rows are set (v: 0 and the
 // inflow vmagic, u and w: the one-sided closures of inby and ccon; mode 0 of
 // a mixed box's u: its FP64 mean flow um, whose boundary rows
-// boun_mean_kernel set).  Same arithmetic as rhs_loop2_y, boun_kernel and
-// rk_final_y(h), in the same order.
+// boun_mean_kernel set).  Same chunks and arithmetic as rhs_loop2_y,
+// boun_kernel and rk_final_y, in the same order: the same results bit for bit.
 // ---------------------------------------------------------------------------
 
 // u / w field rows 1 and ny+1 of boun from the column's interior rows
